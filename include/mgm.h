@@ -1,0 +1,147 @@
+/*
+ * mgm.h -- C ABI of libmgm, the MI355X-native RBF-FD meshless multigrid path.
+ *
+ * Plain C types only (pointers, sizes, scalars); no torch or HIP types.
+ * Every entry point returns 0 on success and a negative code on failure;
+ * mgm_last_error() describes the last failure on the calling thread.
+ * Host arrays are in the reference's node order (the order after its BFS
+ * reordering, hf/mesh_functions.h:774-872) and row-major (a reference
+ * `double** D` of N rows x m is D[i*m+j]).  Device data is owned by the
+ * problem object and never exposed.
+ *
+ * Reference interfaces replaced (file:line in the reference tree):
+ *   mgm_points_read_gmsh      read_PointStructure      header_files/mesh_functions.h:172
+ *   mgm_problem_setup         read_complete_mesh_data  header_files/mesh_functions.h:914
+ *                             + create_derivative_matrices header_files/rbf.h:401
+ *   mgm_relax                 relaxation               header_files/heat_conduction.h:25,
+ *                                                      header_files/timple_solver.h:235,
+ *                                                      header_files/fractional_step_solver.h:168
+ *   mgm_residual              calculate_residuals      header_files/heat_conduction.h:101 (+T :274, F :236)
+ *   mgm_restrict              restrict_residuals       header_files/heat_conduction.h:58 (+T :292, F :198)
+ *   mgm_prolong               prolongate_corrections   header_files/heat_conduction.h:79 (+T :308, F :214)
+ *   mgm_vcycle                V-cycle loop             mg_heat_conduction.c:64-75,
+ *                                                      header_files/timple_solver.h:191
+ *   mgm_spmv                  multiply_sparse_matrix_vector_gpu header_files/mat_lib.h:81
+ *   mgm_l2_norm_gen           l2_norm_gen              header_files/general_functions.h:181
+ */
+#ifndef MGM_H
+#define MGM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enums ------------------------------------------------------------ */
+enum { MGM_VAR_H = 0, MGM_VAR_T = 1, MGM_VAR_F = 2 };  /* heat / TIMPLE / fractional-step semantics */
+enum { MGM_ROWS_REFERENCE = 0, MGM_ROWS_BY_TAG = 1 };  /* transfer rows for i>=nb, or for !boundary */
+enum { MGM_ORDER_COLOUR = 0, MGM_ORDER_NATURAL = 1 };  /* SOR sweep order */
+enum {
+    MGM_MAT_DX = 0, MGM_MAT_DY = 1, MGM_MAT_DZ = 2, MGM_MAT_LAP = 3, MGM_MAT_LAPP = 4,
+    MGM_MAT_RESTR = 5, MGM_MAT_PROL = 6
+};
+enum {
+    MGM_F_P = 0, MGM_F_SOURCE = 1, MGM_F_RES = 2, MGM_F_PPRIME = 3, MGM_F_U = 4, MGM_F_V = 5,
+    MGM_F_W = 6, MGM_F_UNEW = 7, MGM_F_VNEW = 8, MGM_F_WNEW = 9, MGM_F_POLD = 10,
+    MGM_F_T = 11, MGM_F_DPDN = 12, MGM_F_DPDX = 13, MGM_F_DPDY = 14, MGM_F_DPDZ = 15,
+    MGM_F_UOLD = 16, MGM_F_VOLD = 17, MGM_F_WOLD = 18, MGM_NUM_FIELDS = 19
+};
+enum { MGM_PTS_RESTRICTION = 0, MGM_PTS_PROLONGATION = 1 };
+
+/* ---- configuration (mirror of struct parameters, header_files/structures.h:15-41) */
+typedef struct {
+    int dimension;                 /* 2 or 3 */
+    int poly_degree;               /* finest level */
+    int coarse_poly_degree;        /* other levels; the reference hard-codes 3 (mesh_functions.h:917) */
+    int phs_degree;
+    double cloud_size_multiplier;  /* m = floor(multiplier * n) */
+    float omega;                   /* SOR factor; float as in the reference */
+    int num_relax;
+    int transfer_rows;             /* MGM_ROWS_* */
+    int sweep_order;               /* MGM_ORDER_* */
+} mgm_config;
+
+typedef struct {
+    int N, nb, m, n, poly_degree, num_colours;
+    double d_avg;                  /* after calculate_avg_dx (mesh_functions.h:895) */
+    double d_avg_bbox;             /* reader's bbox estimate (mesh_functions.h:274-281) */
+    int knn_ties;                  /* queries whose stencil needed the kd-order tie-break */
+} mgm_level_info;
+
+typedef struct mgm_points mgm_points;
+typedef struct mgm_problem mgm_problem;
+
+/* ---- status / device ------------------------------------------------------ */
+const char *mgm_last_error(void);
+int mgm_device_count(void);        /* 0 when no GPU is visible */
+int mgm_init(int device);          /* select device, create the library stream */
+
+/* ---- host-only helpers (no GPU needed) ------------------------------------ */
+void mgm_config_default(mgm_config *cfg);
+int mgm_poly_terms(int dim, int deg, short *px, short *py, short *pz);
+int mgm_points_read_gmsh(const char *path, int dim, mgm_points **out);
+/* Jittered Cartesian cloud on [0,1]^dim, n per axis (seeded SplitMix64);
+ * optionally also written to gmsh_path as Gmsh v2. */
+int mgm_points_generate(int dim, int n, uint64_t seed, double jitter, int with_volume_elements,
+                        const char *gmsh_path, mgm_points **out);
+int mgm_points_info(const mgm_points *p, int *N, int *nb, int *num_elem, double *d_avg);
+int mgm_points_copy(const mgm_points *p, double *x, double *y, double *z, double *nx,
+                    double *ny, double *nz, uint8_t *btag, uint8_t *ctag);
+void mgm_points_free(mgm_points *p);
+uint64_t mgm_splitmix64(uint64_t *state);
+
+/* ---- problem: setup -------------------------------------------------------- */
+int mgm_problem_create(const mgm_config *cfg, int num_levels, mgm_problem **out);
+void mgm_problem_destroy(mgm_problem *p);
+int mgm_problem_set_level(mgm_problem *p, int lev, const mgm_points *pts);
+/* kNN stencils, BFS reordering, normals, spacing, transfer points, colouring,
+ * weights (Dx, Dy, Dz, Laplacian, lap_Poison), transfer rows, ELL slabs. */
+int mgm_problem_setup(mgm_problem *p);
+int mgm_level_info_get(const mgm_problem *p, int lev, mgm_level_info *out);
+/* seconds spent in setup phases: [0] kNN [1] reorder+colour [2] transfer points
+ * [3] weights+transfer rows [4] slab assembly [5] total */
+int mgm_setup_timings(const mgm_problem *p, double *t6);
+/* Rebuild the weight rows of one level (batched local solves) and return the
+ * kernel time in seconds and the number of systems solved. */
+int mgm_rebuild_weights(mgm_problem *p, int lev, double *seconds, long *systems);
+
+/* ---- problem: data exchange (reference node order) ----------------------- */
+int mgm_get_geometry(const mgm_problem *p, int lev, double *x, double *y, double *z,
+                     double *nx, double *ny, double *nz, uint8_t *btag);
+int mgm_get_cloud(const mgm_problem *p, int lev, int32_t *cloud);
+int mgm_get_points(const mgm_problem *p, int lev, int which, int32_t *out);
+int mgm_get_colours(const mgm_problem *p, int lev, int32_t *colour);
+/* width: m of this level (DX..LAPP), m of level lev-1 (RESTR), m of level lev+1 (PROL) */
+int mgm_get_matrix(const mgm_problem *p, int lev, int which, double *out);
+int mgm_set_matrix(mgm_problem *p, int lev, int which, const double *in);
+/* Identity rows (Dirichlet) in lap_Poison for every boundary node of every level. */
+int mgm_dirichlet_rows(mgm_problem *p);
+int mgm_get_field(const mgm_problem *p, int lev, int which, double *out);
+int mgm_set_field(mgm_problem *p, int lev, int which, const double *in);
+int mgm_zero_fields(mgm_problem *p);
+
+/* ---- problem: operators (reference variant semantics) --------------------- */
+int mgm_relax(mgm_problem *p, int lev, int variant);
+int mgm_residual(mgm_problem *p, int lev, int variant);
+int mgm_restrict(mgm_problem *p, int lev, int variant);      /* lev -> lev+1 */
+int mgm_prolong(mgm_problem *p, int lev, int variant);       /* lev -> lev-1 */
+int mgm_vcycle(mgm_problem *p, int variant, int ncycles);
+/* out = M * in over fixed-width rows (M: MGM_MAT_DX..MGM_MAT_LAPP). */
+int mgm_spmv(mgm_problem *p, int lev, int which_matrix, int in_field, int out_field);
+int mgm_l2_norm_gen(const mgm_problem *p, int lev, int field_a, int field_b, double *out);
+
+/* ---- resident solve -------------------------------------------------------- */
+/* V-cycles on level-0 (field P, SOURCE) until l2(r)/l2(f) <= tol (l2_norm_gen
+ * norms, r = source - lap_Poison*p on interior nodes) or max_cycles.  history
+ * (may be NULL) receives max_cycles+1 ratios (index 0 = before the first cycle).
+ * precision: 0 FP64; 1 FP32 inner V-cycle inside FP64 defect correction. */
+int mgm_solve(mgm_problem *p, int variant, int precision, double tol, int max_cycles,
+              int *cycles_done, double *history, double *seconds);
+/* Time `reps` applications of lap_Poison on level lev; returns seconds per apply. */
+int mgm_time_spmv(mgm_problem *p, int lev, int reps, double *seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,226 @@
+"""multigrid_meshless_amd -- MI355X-native RBF-FD meshless multigrid hot path.
+
+Python host mirror of the C ABI in include/mgm.h (libmgm.so, HIP kernels for
+gfx950).  Names follow the reference's operator interface:
+read_PointStructure/read_complete_mesh_data -> Points/Problem.setup,
+relaxation/calculate_residuals/restrict_residuals/prolongate_corrections ->
+Problem.relax/residual/restrict/prolong, the V-cycle loop -> Problem.vcycle,
+and the resident solve -> Problem.solve.  Arrays are numpy, in the reference's
+node order (after its BFS reordering), row-major.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import Config, LevelInfo, MgmError, check, lib
+
+VAR_H, VAR_T, VAR_F = 0, 1, 2
+ROWS_REFERENCE, ROWS_BY_TAG = 0, 1
+ORDER_COLOUR, ORDER_NATURAL = 0, 1
+MAT = dict(Dx=0, Dy=1, Dz=2, lap=3, lap_Poison=4, restr=5, prol=6)
+FIELD = dict(p=0, source=1, res=2, pprime=3, u=4, v=5, w=6, u_new=7, v_new=8, w_new=9,
+             p_old=10, T=11, dpdn=12, dpdx=13, dpdy=14, dpdz=15, u_old=16, v_old=17, w_old=18)
+
+__all__ = ["Config", "Points", "Problem", "init", "device_count", "MgmError", "config",
+           "VAR_H", "VAR_T", "VAR_F", "MAT", "FIELD"]
+
+
+def device_count() -> int:
+    return lib().mgm_device_count()
+
+
+def init(device: int = 0):
+    check(lib().mgm_init(device), "mgm_init")
+
+
+def config(**kw) -> Config:
+    c = Config()
+    lib().mgm_config_default(C.byref(c))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+class Points:
+    """A level's points as the Gmsh reader leaves them (hf/mesh_functions.h:172-476)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def read_gmsh(cls, path: str, dim: int) -> "Points":
+        h = C.c_void_p()
+        check(lib().mgm_points_read_gmsh(path.encode(), dim, C.byref(h)), "read_gmsh")
+        return cls(h)
+
+    @classmethod
+    def generate(cls, dim: int, n: int, seed: int, jitter: float = 0.2,
+                 with_volume_elements: bool = False, gmsh_path: str | None = None) -> "Points":
+        h = C.c_void_p()
+        check(lib().mgm_points_generate(dim, n, seed, jitter, int(with_volume_elements),
+                                        gmsh_path.encode() if gmsh_path else None, C.byref(h)),
+              "generate")
+        return cls(h)
+
+    def info(self):
+        N, nb, ne, d = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        check(lib().mgm_points_info(self._h, C.byref(N), C.byref(nb), C.byref(ne), C.byref(d)))
+        return dict(N=N.value, nb=nb.value, num_elem=ne.value, d_avg=d.value)
+
+    def arrays(self) -> dict:
+        N = self.info()["N"]
+        out = {k: np.zeros(N) for k in ("x", "y", "z", "nx", "ny", "nz")}
+        out["btag"] = np.zeros(N, np.uint8)
+        out["ctag"] = np.zeros(N, np.uint8)
+        check(lib().mgm_points_copy(self._h, *[out[k].ctypes.data for k in
+                                                ("x", "y", "z", "nx", "ny", "nz", "btag", "ctag")]))
+        return out
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().mgm_points_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class Problem:
+    """A multigrid hierarchy resident on the GPU."""
+
+    def __init__(self, cfg: Config, levels: list):
+        self.cfg = cfg
+        self.L = len(levels)
+        h = C.c_void_p()
+        check(lib().mgm_problem_create(C.byref(cfg), self.L, C.byref(h)), "problem_create")
+        self._h = h
+        self._levels = levels
+        for k, pts in enumerate(levels):
+            check(lib().mgm_problem_set_level(h, k, pts._h), "set_level")
+
+    def setup(self):
+        check(lib().mgm_problem_setup(self._h), "setup")
+        return self
+
+    def info(self, lev: int) -> dict:
+        o = LevelInfo()
+        check(lib().mgm_level_info_get(self._h, lev, C.byref(o)), "level_info")
+        return {k: getattr(o, k) for k, _ in o._fields_}
+
+    def timings(self) -> dict:
+        t = np.zeros(6)
+        check(lib().mgm_setup_timings(self._h, t.ctypes.data))
+        return dict(zip(("knn", "reorder_colour", "transfer_points", "weights", "slabs", "total"), t))
+
+    def rebuild_weights(self, lev: int):
+        s, n = C.c_double(), C.c_long()
+        check(lib().mgm_rebuild_weights(self._h, lev, C.byref(s), C.byref(n)), "rebuild_weights")
+        return s.value, n.value
+
+    def geometry(self, lev: int) -> dict:
+        N = self.info(lev)["N"]
+        out = {k: np.zeros(N) for k in ("x", "y", "z", "nx", "ny", "nz")}
+        out["btag"] = np.zeros(N, np.uint8)
+        check(lib().mgm_get_geometry(self._h, lev, *[out[k].ctypes.data for k in
+                                                     ("x", "y", "z", "nx", "ny", "nz", "btag")]))
+        return out
+
+    def cloud(self, lev: int) -> np.ndarray:
+        i = self.info(lev)
+        out = np.zeros((i["N"], i["m"]), np.int32)
+        check(lib().mgm_get_cloud(self._h, lev, out.ctypes.data))
+        return out
+
+    def points(self, lev: int, which: str) -> np.ndarray:
+        out = np.zeros(self.info(lev)["N"], np.int32)
+        check(lib().mgm_get_points(self._h, lev, 0 if which == "restriction" else 1, out.ctypes.data))
+        return out
+
+    def colours(self, lev: int) -> np.ndarray:
+        out = np.zeros(self.info(lev)["N"], np.int32)
+        check(lib().mgm_get_colours(self._h, lev, out.ctypes.data))
+        return out
+
+    def _width(self, lev, which):
+        if which == MAT["restr"]:
+            return self.info(lev - 1)["m"]
+        if which == MAT["prol"]:
+            return self.info(lev + 1)["m"]
+        return self.info(lev)["m"]
+
+    def matrix(self, lev: int, name: str) -> np.ndarray:
+        w = MAT[name]
+        out = np.zeros((self.info(lev)["N"], self._width(lev, w)))
+        check(lib().mgm_get_matrix(self._h, lev, w, out.ctypes.data), f"get_matrix {name}")
+        return out
+
+    def set_matrix(self, lev: int, name: str, a: np.ndarray):
+        w = MAT[name]
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == (self.info(lev)["N"], self._width(lev, w))
+        check(lib().mgm_set_matrix(self._h, lev, w, a.ctypes.data), f"set_matrix {name}")
+
+    def dirichlet_rows(self):
+        check(lib().mgm_dirichlet_rows(self._h), "dirichlet_rows")
+
+    def field(self, lev: int, name: str) -> np.ndarray:
+        out = np.zeros(self.info(lev)["N"])
+        check(lib().mgm_get_field(self._h, lev, FIELD[name], out.ctypes.data), f"get_field {name}")
+        return out
+
+    def set_field(self, lev: int, name: str, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == (self.info(lev)["N"],)
+        check(lib().mgm_set_field(self._h, lev, FIELD[name], a.ctypes.data), f"set_field {name}")
+
+    def zero_fields(self):
+        check(lib().mgm_zero_fields(self._h))
+
+    # ---- operators (reference variant semantics) ----
+    def relax(self, lev, variant=VAR_H):
+        check(lib().mgm_relax(self._h, lev, variant), "relax")
+
+    def residual(self, lev, variant=VAR_H):
+        check(lib().mgm_residual(self._h, lev, variant), "residual")
+
+    def restrict(self, lev, variant=VAR_H):
+        check(lib().mgm_restrict(self._h, lev, variant), "restrict")
+
+    def prolong(self, lev, variant=VAR_H):
+        check(lib().mgm_prolong(self._h, lev, variant), "prolong")
+
+    def vcycle(self, variant=VAR_H, ncycles=1):
+        check(lib().mgm_vcycle(self._h, variant, ncycles), "vcycle")
+
+    def spmv(self, lev, matrix="lap_Poison", src="p", dst="dpdn"):
+        check(lib().mgm_spmv(self._h, lev, MAT[matrix], FIELD[src], FIELD[dst]), "spmv")
+
+    def l2_norm_gen(self, lev, a, b=None) -> float:
+        out = C.c_double()
+        check(lib().mgm_l2_norm_gen(self._h, lev, FIELD[a], -1 if b is None else FIELD[b],
+                                    C.byref(out)), "l2_norm_gen")
+        return out.value
+
+    def solve(self, tol=1e-8, max_cycles=200, variant=VAR_H, precision=0):
+        hist = np.full(max_cycles + 1, np.nan)
+        nc, sec = C.c_int(), C.c_double()
+        check(lib().mgm_solve(self._h, variant, precision, tol, max_cycles, C.byref(nc),
+                              hist.ctypes.data, C.byref(sec)), "solve")
+        return dict(cycles=nc.value, seconds=sec.value, history=hist[:nc.value + 1])
+
+    def time_spmv(self, lev=0, reps=20) -> float:
+        s = C.c_double()
+        check(lib().mgm_time_spmv(self._h, lev, reps, C.byref(s)), "time_spmv")
+        return s.value
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().mgm_problem_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

@@ -1,0 +1,155 @@
+// device.hpp -- HIP helpers and kernel launcher declarations for libmgm.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mgm_internal.h"
+
+namespace mgm {
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess) {                                                           \
+            ::mgm::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));          \
+            return -10;                                                                   \
+        }                                                                                 \
+    } while (0)
+
+// Owning device buffer.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count)
+    {
+        if (count == n && p) return hipSuccess;
+        release();
+        if (count == 0) return hipSuccess;
+        hipError_t e = hipMalloc((void **)&p, sizeof(T) * count);
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    hipError_t upload(const T *h, size_t count, hipStream_t s)
+    {
+        hipError_t e = alloc(count);
+        if (e != hipSuccess || count == 0) return e;
+        return hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s);
+    }
+    hipError_t upload(const std::vector<T> &v, hipStream_t s) { return upload(v.data(), v.size(), s); }
+    hipError_t download(T *h, size_t count, hipStream_t s) const
+    {
+        if (count == 0) return hipSuccess;
+        return hipMemcpyAsync(h, p, sizeof(T) * count, hipMemcpyDeviceToHost, s);
+    }
+};
+
+hipStream_t stream();
+
+// ---- kNN (knn.hip) --------------------------------------------------------
+// Candidate set binned into a uniform grid (built on the host).
+struct KnnGrid {
+    double x0, y0, z0, inv_c, c;
+    int nx, ny, nz;
+    const int *cell_start;   // ncells+1
+    const int *cell_pts;     // candidate ids grouped by cell, ascending id in a cell
+    const double *px, *py, *pz;  // coordinates indexed by candidate id
+};
+// Optional insertion-ordered kd-tree used to order equal distances exactly
+// as the reference's range query reports them.
+struct KdOrder {
+    const int *node_of;      // candidate id -> tree node (-1 if not in tree)
+    const int *parent;       // tree node -> parent (-1 root)
+    const int *depth;
+    const uint8_t *side;     // 0 left child, 1 right child
+    const double *pos;       // 3 per tree node
+};
+// For each query q in [0,nq): the K nearest candidates by sqrt distance.
+// out[q*K_out + j], j < K_out, receives ids (K_out <= K).  When `query_ids`
+// is non-null only those queries run.  tie_flag[q] is set when equal
+// distances occur among the K+1 best (order then needs the kd rule).
+int launch_knn(const KnnGrid &g, const KdOrder *kd, int nq, const double *qx, const double *qy,
+               const double *qz, const int *query_ids, int K, int K_out, int slot_offset,
+               int *out, int out_stride, uint8_t *tie_flag, hipStream_t s);
+
+// ---- weights (weights.hip) -------------------------------------------------
+struct WeightsArgs {
+    int nsys;                // systems in this launch
+    const int *sys;          // node index of each system (reference order)
+    int m, n, dim, phs;
+    const short *px, *py, *pz;  // device, n entries
+    const double *x, *y, *z;    // level coordinates, reference order
+    const int *cloud;           // N*m row-major, reference order
+    double *Dx, *Dy, *Dz, *lap; // N*m outputs (nullable)
+    // interpolation targets evaluated with each system's factorisation
+    const int *rt_off;  const double *rt_xyz; const int *rt_row; double *R_out;  // restriction
+    const int *pt_off;  const double *pt_xyz; const int *pt_row; double *P_out;  // prolongation
+    double *ws;         // workspace: mpn*mpn*batch
+    double *rhs;        // 4*mpn*batch
+    int *piv;           // mpn*batch
+    int batch;
+    int *status;        // nonzero if a zero pivot was met
+};
+int launch_weights(const WeightsArgs &a, hipStream_t s);
+
+// ---- stencil kernels (stencil.hip) ----------------------------------------
+struct Ell {          // fixed-width rows, column-major: entry j of row r at [j*ld + r]
+    const int *col;
+    const double *w;
+    int m, ld;
+};
+int launch_spmv(Ell A, int r0, int r1, const double *x, double *y, hipStream_t s);
+int launch_sor_rows(Ell A, int r0, int r1, const double *src, double *p, double w, double c,
+                    hipStream_t s);
+int launch_sor_natural(Ell A, int N, const int *iperm, const double *src, double *p, double w,
+                       double c, hipStream_t s);
+// residual forms: 0 = interior, res = s - sum (boundary res = 0)            (H/T)
+//                 1 = interior, res = s; res -= ... sequentially (boundary 0) (T fused)
+//                 2 = all rows, res = s; res -= ... sequentially             (F fused)
+//                 3 = interior, res = s - sum (boundary untouched)           (F)
+int launch_residual(Ell A, int N, int form, const uint8_t *btag, const double *src,
+                    const double *p, double *res, hipStream_t s);
+// restrict: out[r] = btag ? (keep_boundary ? out[r] : 0) : sum_j w*in[col]
+int launch_restrict(Ell R, int N, const uint8_t *btag, int keep_boundary, const double *in,
+                    double *out, hipStream_t s);
+// prolong: p[r] += sum_j w*pc[col] on interior rows
+int launch_prolong(Ell P, int N, const uint8_t *btag, const double *pc, double *p, hipStream_t s);
+int launch_zero(int N, const uint8_t *btag, int interior_only, double *p, hipStream_t s);
+int launch_pin(int N, const int *ref0_dev, double *p, double *scratch, hipStream_t s);
+// sum over interior rows of (a-b)^2 into *out (device scalar); b nullable
+int launch_sqdiff_sum(int N, const uint8_t *btag, const double *a, const double *b,
+                      double *partial, double *out, hipStream_t s);
+int launch_gather_rows(int N, int m, const double *rows, const int *perm, const int *iperm_cols,
+                       const int *cloud, double *w_out, int *col_out, hipStream_t s);
+int launch_transfer_slab(int Nt, int m, const double *rows, const int *perm, const int *pts,
+                         const int *cloud_src, const int *iperm_src, double *w_out, int *col_out,
+                         hipStream_t s);
+int launch_lap_poisson(int N, int m, int dim, const double *Dx, const double *Dy, const double *Dz,
+                       const double *lap, const double *nx, const double *ny, const double *nz,
+                       const uint8_t *btag, double *lapP, hipStream_t s);
+int launch_dirichlet_rows(int N, int m, const uint8_t *btag, double *rows, hipStream_t s);
+int launch_permute(int N, const int *idx, const double *in, double *out, hipStream_t s);
+int launch_scatter(int N, const int *idx, const double *in, double *out, hipStream_t s);
+
+}  // namespace mgm

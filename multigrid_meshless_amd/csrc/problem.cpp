@@ -1,0 +1,1093 @@
+// problem.cpp -- setup pipeline, V-cycle operators, resident solve and the
+// extern "C" entry points declared in include/mgm.h.
+//
+// Setup (reference: read_complete_mesh_data, hf/mesh_functions.h:914-928,
+// and create_derivative_matrices, hf/rbf.h:401-421), per level:
+//   1. stencils: GPU kNN over a uniform grid (knn.hip), file order
+//   2. BFS reordering from node 0 (hf/mesh_functions.h:774-872), host
+//   3. normal orientation (:478-527) and mean spacing (:895-910), host
+//   4. restriction / prolongation points: GPU nearest neighbour
+//   5. greedy colouring -> device order (colour, then Morton code)
+//   6. weights + transfer rows: batched local solves (weights.hip)
+//   7. lap_Poison rows, then column-major device-order slabs
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "problem.hpp"
+
+using namespace mgm;
+
+namespace mgm {
+
+static hipStream_t g_stream = nullptr;
+static int g_device = -1;
+hipStream_t stream() { return g_stream; }
+
+static double now()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static int ensure_init()
+{
+    if (g_stream) return 0;
+    MGM_FAIL(-9, "mgm_init(device) has not been called (or no GPU is visible)");
+}
+
+// ---------------------------------------------------------------------------
+// Uniform grid over a candidate subset of a point set.
+struct HostGrid {
+    KnnGrid g{};
+    std::vector<int> start, pts;
+    DevBuf<int> d_start, d_pts;
+};
+
+static int build_grid(const double *x, const double *y, const double *z, int N,
+                      const std::vector<int> &cand, int K, int dim, HostGrid &G)
+{
+    if (cand.empty()) MGM_FAIL(-5, "kNN: empty candidate set");
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    const double *P[3] = {x, y, z};
+    for (int id : cand)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], P[d][id]);
+            hi[d] = std::max(hi[d], P[d][id]);
+        }
+    (void)N;
+    double ext[3], vol = 1;
+    int active = 0;
+    for (int d = 0; d < 3; d++) {
+        ext[d] = hi[d] - lo[d];
+        if (ext[d] > 0 && (d < 2 || dim == 3)) { vol *= ext[d]; active++; }
+    }
+    double c;
+    if (active == 0) c = 1;
+    else c = 0.8 * pow((double)K * vol / (double)cand.size(), 1.0 / active);
+    if (!(c > 0)) c = 1;
+    int dims[3];
+    long ncell = 1;
+    for (int d = 0; d < 3; d++) {
+        if (ext[d] > 0 && (d < 2 || dim == 3)) dims[d] = std::max(1, (int)ceil(ext[d] / c));
+        else dims[d] = 1;
+        ncell *= dims[d];
+    }
+    while (ncell > 8L * (long)cand.size() + 64) {  // guard degenerate extents
+        c *= 1.5;
+        ncell = 1;
+        for (int d = 0; d < 3; d++) {
+            dims[d] = (dims[d] == 1) ? 1 : std::max(1, (int)ceil(ext[d] / c));
+            ncell *= dims[d];
+        }
+    }
+    G.g.x0 = lo[0]; G.g.y0 = lo[1]; G.g.z0 = lo[2];
+    G.g.c = c; G.g.inv_c = 1.0 / c;
+    G.g.nx = dims[0]; G.g.ny = dims[1]; G.g.nz = dims[2];
+    G.start.assign(ncell + 1, 0);
+    std::vector<int> cellof(cand.size());
+    for (size_t e = 0; e < cand.size(); e++) {
+        int id = cand[e];
+        int ci[3];
+        for (int d = 0; d < 3; d++) {
+            double f = floor((P[d][id] - lo[d]) * G.g.inv_c);
+            ci[d] = (f < 0) ? 0 : (f > dims[d] - 1 ? dims[d] - 1 : (int)f);
+        }
+        int cell = (ci[2] * dims[1] + ci[1]) * dims[0] + ci[0];
+        cellof[e] = cell;
+        G.start[cell + 1]++;
+    }
+    for (long k = 0; k < ncell; k++) G.start[k + 1] += G.start[k];
+    G.pts.resize(cand.size());
+    std::vector<int> fill(G.start.begin(), G.start.end() - 1);
+    for (size_t e = 0; e < cand.size(); e++) G.pts[fill[cellof[e]]++] = cand[e];  // ascending id
+    HIP_TRY(G.d_start.upload(G.start, g_stream));
+    HIP_TRY(G.d_pts.upload(G.pts, g_stream));
+    G.g.cell_start = G.d_start.p;
+    G.g.cell_pts = G.d_pts.p;
+    return 0;
+}
+
+// Insertion-ordered kd-tree over a candidate subset (hf/kdtree.c:167-194),
+// used only to order equal distances as the reference's range query does.
+struct HostKd {
+    std::vector<int> node_of, parent, depth;
+    std::vector<uint8_t> side;
+    std::vector<double> pos;
+    DevBuf<int> d_node_of, d_parent, d_depth;
+    DevBuf<uint8_t> d_side;
+    DevBuf<double> d_pos;
+    KdOrder kd{};
+};
+
+static int build_kd(const double *x, const double *y, const double *z, int N,
+                    const std::vector<int> &cand, HostKd &T)
+{
+    T.node_of.assign(N, -1);
+    size_t nn = cand.size();
+    T.parent.assign(nn, -1); T.depth.assign(nn, 0); T.side.assign(nn, 0); T.pos.resize(3 * nn);
+    std::vector<int> left(nn, -1), right(nn, -1);
+    for (size_t e = 0; e < nn; e++) {
+        int id = cand[e];
+        int nd = (int)e;
+        T.node_of[id] = nd;
+        T.pos[3 * e] = x[id]; T.pos[3 * e + 1] = y[id]; T.pos[3 * e + 2] = z[id];
+        if (e == 0) continue;
+        const double p[3] = {x[id], y[id], z[id]};
+        int cur = 0;
+        for (;;) {
+            int d = T.depth[cur] % 3;
+            if (p[d] < T.pos[3 * cur + d]) {
+                if (left[cur] < 0) { left[cur] = nd; T.side[nd] = 0; break; }
+                cur = left[cur];
+            } else {
+                if (right[cur] < 0) { right[cur] = nd; T.side[nd] = 1; break; }
+                cur = right[cur];
+            }
+        }
+        T.parent[nd] = cur;
+        T.depth[nd] = T.depth[cur] + 1;
+    }
+    HIP_TRY(T.d_node_of.upload(T.node_of, g_stream));
+    HIP_TRY(T.d_parent.upload(T.parent, g_stream));
+    HIP_TRY(T.d_depth.upload(T.depth, g_stream));
+    HIP_TRY(T.d_side.upload(T.side, g_stream));
+    HIP_TRY(T.d_pos.upload(T.pos, g_stream));
+    T.kd = KdOrder{T.d_node_of.p, T.d_parent.p, T.d_depth.p, T.d_side.p, T.d_pos.p};
+    return 0;
+}
+
+// Runs one kNN pass; queries with tied distances are rerun with the kd order.
+// Returns the number of queries that needed the tie-break.
+static int knn_pass(const double *hx, const double *hy, const double *hz, int Ncand_set,
+                    const double *d_cx, const double *d_cy, const double *d_cz,
+                    const std::vector<int> &cand, int dim,
+                    const double *d_qx, const double *d_qy, const double *d_qz, int Nq,
+                    const std::vector<int> &queries, int K, int K_out, int slot, int *d_out,
+                    int stride, int *ties_out)
+{
+    if (queries.empty()) { *ties_out = 0; return 0; }
+    HostGrid G;
+    int rc = build_grid(hx, hy, hz, Ncand_set, cand, K, dim, G);
+    if (rc) return rc;
+    G.g.px = d_cx; G.g.py = d_cy; G.g.pz = d_cz;
+    DevBuf<int> d_q;
+    DevBuf<uint8_t> d_tie;
+    HIP_TRY(d_q.upload(queries, g_stream));
+    HIP_TRY(d_tie.alloc(Nq));
+    HIP_TRY(hipMemsetAsync(d_tie.p, 0, Nq, g_stream));
+    rc = launch_knn(G.g, nullptr, (int)queries.size(), d_qx, d_qy, d_qz, d_q.p, K, K_out, slot, d_out,
+                    stride, d_tie.p, g_stream);
+    if (rc) return rc;
+    std::vector<uint8_t> tie(Nq);
+    HIP_TRY(d_tie.download(tie.data(), Nq, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    std::vector<int> again;
+    for (int q : queries) {
+        if (tie[q] == 2) MGM_FAIL(-6, "kNN: fewer candidates than stencil points");
+        if (tie[q] == 1) again.push_back(q);
+    }
+    *ties_out = (int)again.size();
+    if (again.empty()) return 0;
+    HostKd T;
+    rc = build_kd(hx, hy, hz, Ncand_set, cand, T);
+    if (rc) return rc;
+    DevBuf<int> d_again;
+    HIP_TRY(d_again.upload(again, g_stream));
+    rc = launch_knn(G.g, &T.kd, (int)again.size(), d_qx, d_qy, d_qz, d_again.p, K, K_out, slot,
+                    d_out, stride, nullptr, g_stream);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+// find_cloud_index (hf/kdtree_functions.h:113-151) on the GPU, file order.
+static int level_knn(Level &L, int dim)
+{
+    const int N = L.N, m = L.m, nb = L.nb;
+    const PointSet &P = L.in;
+    DevBuf<double> dx, dy, dz;
+    DevBuf<int> dcloud;
+    HIP_TRY(dx.upload(P.x, g_stream));
+    HIP_TRY(dy.upload(P.y, g_stream));
+    HIP_TRY(dz.upload(P.z, g_stream));
+    HIP_TRY(dcloud.alloc((size_t)N * m));
+    HIP_TRY(hipMemsetAsync(dcloud.p, 0, sizeof(int) * (size_t)N * m, g_stream));
+    std::vector<int> candA, candB, qA, qB;
+    for (int i = 0; i < N; i++) {
+        if (!P.ctag[i]) candA.push_back(i);
+        if (!P.btag[i]) candB.push_back(i);
+    }
+    for (int i = nb; i < N; i++)
+        if (!P.ctag[i]) qA.push_back(i);
+    for (int i = 0; i < nb; i++) qB.push_back(i);
+    int tA = 0, tB = 0, rc;
+    rc = knn_pass(P.x.data(), P.y.data(), P.z.data(), N, dx.p, dy.p, dz.p, candA, dim, dx.p, dy.p,
+                  dz.p, N, qA, m + 1, m, 0, dcloud.p, m, &tA);
+    if (rc) return rc;
+    rc = knn_pass(P.x.data(), P.y.data(), P.z.data(), N, dx.p, dy.p, dz.p, candB, dim, dx.p, dy.p,
+                  dz.p, N, qB, m, m - 1, 1, dcloud.p, m, &tB);
+    if (rc) return rc;
+    L.knn_ties = tA + tB;
+    L.cloud.resize((size_t)N * m);
+    HIP_TRY(dcloud.download(L.cloud.data(), (size_t)N * m, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    for (int i = 0; i < nb; i++) L.cloud[(size_t)i * m] = i;
+    return 0;
+}
+
+// rcm_reordering_with_boundarynodes (hf/mesh_functions.h:774-872).
+static int level_bfs(Level &L)
+{
+    const int N = L.N, m = L.m;
+    std::vector<int> q1(N), q2(N);
+    std::vector<uint8_t> vis(N, 0);
+    int count = 1, head = 0;
+    q1[0] = 0; vis[0] = 1;
+    while (head < count) {
+        int cur = q1[head++];
+        const int *cr = &L.cloud[(size_t)cur * m];
+        for (int i = 1; i < m; i++) {
+            int nbr = cr[i];
+            if (!vis[nbr]) { q1[count++] = nbr; vis[nbr] = 1; }
+        }
+    }
+    if (count != N) MGM_FAIL(-7, "BFS reordering did not reach every node (disconnected stencil graph)");
+    for (int i = 0; i < N; i++) q2[q1[i]] = i;
+    const PointSet &P = L.in;
+    L.x.resize(N); L.y.resize(N); L.z.resize(N); L.nx.resize(N); L.ny.resize(N); L.nz.resize(N);
+    L.btag.resize(N);
+    for (int i = 0; i < N; i++) {
+        int s = q1[i];
+        L.x[i] = P.x[s]; L.y[i] = P.y[s]; L.z[i] = P.z[s];
+        L.nx[i] = P.nx[s]; L.ny[i] = P.ny[s]; L.nz[i] = P.nz[s];
+        L.btag[i] = P.btag[s];
+    }
+    L.ctag = P.ctag;
+    std::vector<int> nc((size_t)N * m);
+    for (int i = 0; i < N; i++) {
+        const int *src = &L.cloud[(size_t)q1[i] * m];
+        int *dst = &nc[(size_t)i * m];
+        for (int j = 0; j < m; j++) dst[j] = q2[src[j]];
+    }
+    L.cloud.swap(nc);
+    return 0;
+}
+
+// correct_normal_directions (hf/mesh_functions.h:478-527) and
+// calculate_avg_dx (:895-910).
+static void level_normals_spacing(Level &L, int dim)
+{
+    const int N = L.N, m = L.m;
+    for (int i = 0; i < N; i++) {
+        if (!L.btag[i]) continue;
+        double xa = 0, ya = 0, za = 0;
+        short count = 0;
+        for (int j = m - 1; j > 0; j--) {
+            int c = L.cloud[(size_t)i * m + j];
+            if (!L.btag[c]) {
+                xa += L.x[c]; ya += L.y[c]; za += L.z[c];
+                count++;
+                if (count == 1) break;
+            }
+        }
+        xa = xa / count; ya = ya / count; za = za / count;
+        double ddx = xa - L.x[i], ddy = ya - L.y[i], ddz = za - L.z[i];
+        double dot = ddx * L.nx[i] + ddy * L.ny[i] + ddz * L.nz[i];
+        if (dot > 0) { L.nx[i] = -L.nx[i]; L.ny[i] = -L.ny[i]; L.nz[i] = -L.nz[i]; }
+        double mag = sqrt(L.nx[i] * L.nx[i] + L.ny[i] * L.ny[i] + L.nz[i] * L.nz[i]);
+        L.nx[i] = L.nx[i] / mag; L.ny[i] = L.ny[i] / mag; L.nz[i] = L.nz[i] / mag;
+    }
+    double s = 0;
+    for (int i = 0; i < N; i++) {
+        if (L.btag[i]) continue;
+        int c = L.cloud[(size_t)i * m + 1];
+        double ddx = L.x[i] - L.x[c], ddy = L.y[i] - L.y[c];
+        double ddz = (dim == 3) ? L.z[i] - L.z[c] : 0;
+        s += sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+    }
+    L.d_avg = s / N;
+}
+
+// find_nearest_point (hf/kdtree_functions.h:153-175): for every node of A the
+// nearest non-corner node of B (reference order on both sides).
+static int nearest_points(const Level &A, const Level &B, int dim, std::vector<int> &out)
+{
+    DevBuf<double> ax, ay, az, bx, by, bz;
+    HIP_TRY(ax.upload(A.x, g_stream)); HIP_TRY(ay.upload(A.y, g_stream)); HIP_TRY(az.upload(A.z, g_stream));
+    HIP_TRY(bx.upload(B.x, g_stream)); HIP_TRY(by.upload(B.y, g_stream)); HIP_TRY(bz.upload(B.z, g_stream));
+    std::vector<int> cand, q;
+    for (int i = 0; i < B.N; i++)
+        if (!B.ctag[i]) cand.push_back(i);
+    for (int i = 0; i < A.N; i++)
+        if (!A.ctag[i]) q.push_back(i);
+    DevBuf<int> d_out;
+    HIP_TRY(d_out.alloc(A.N));
+    HIP_TRY(hipMemsetAsync(d_out.p, 0, sizeof(int) * A.N, g_stream));
+    int ties = 0;
+    int rc = knn_pass(B.x.data(), B.y.data(), B.z.data(), B.N, bx.p, by.p, bz.p, cand, dim, ax.p,
+                      ay.p, az.p, A.N, q, 2, 1, 0, d_out.p, 1, &ties);
+    if (rc) return rc;
+    out.resize(A.N);
+    HIP_TRY(d_out.download(out.data(), A.N, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+static uint64_t spread_bits(uint64_t v, int stride)
+{
+    uint64_t r = 0;
+    for (int b = 0; b < 21; b++)
+        if (v & (1ull << b)) r |= 1ull << (b * stride);
+    return r;
+}
+
+// Greedy colouring on the symmetrised stencil graph, in reference order, then
+// the device order: colour classes contiguous, Morton order inside a class.
+static void level_colour_order(Level &L, int dim)
+{
+    const int N = L.N, m = L.m;
+    std::vector<int> deg(N + 1, 0);
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < m; j++) {
+            int c = L.cloud[(size_t)i * m + j];
+            if (c != i) deg[c + 1]++;
+        }
+    for (int i = 0; i < N; i++) deg[i + 1] += deg[i];
+    std::vector<int> radj(deg[N]), fill(deg.begin(), deg.end() - 1);
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < m; j++) {
+            int c = L.cloud[(size_t)i * m + j];
+            if (c != i) radj[fill[c]++] = i;
+        }
+    L.colour.assign(N, -1);
+    std::vector<int> mark(4096, 0);
+    int maxc = 0;
+    for (int i = 0; i < N; i++) {
+        int stamp = i + 1;
+        for (int j = 0; j < m; j++) {
+            int c = L.cloud[(size_t)i * m + j];
+            if (c != i && L.colour[c] >= 0) mark[L.colour[c]] = stamp;
+        }
+        for (int k = deg[i]; k < deg[i + 1]; k++) {
+            int c = radj[k];
+            if (L.colour[c] >= 0) mark[L.colour[c]] = stamp;
+        }
+        int col = 0;
+        while (col < 4095 && mark[col] == stamp) col++;
+        L.colour[i] = col;
+        maxc = std::max(maxc, col + 1);
+    }
+    L.ncol = maxc;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    const std::vector<double> *P[3] = {&L.x, &L.y, &L.z};
+    for (int d = 0; d < 3; d++)
+        for (int i = 0; i < N; i++) { lo[d] = std::min(lo[d], (*P[d])[i]); hi[d] = std::max(hi[d], (*P[d])[i]); }
+    std::vector<uint64_t> key(N);
+    for (int i = 0; i < N; i++) {
+        uint64_t k = 0;
+        for (int d = 0; d < dim; d++) {
+            double e = hi[d] - lo[d];
+            double t = e > 0 ? ((*P[d])[i] - lo[d]) / e : 0;
+            uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0));
+            k |= spread_bits(q, dim) << d;
+        }
+        key[i] = k;
+    }
+    L.perm.resize(N);
+    std::iota(L.perm.begin(), L.perm.end(), 0);
+    std::sort(L.perm.begin(), L.perm.end(), [&](int a, int b) {
+        if (L.colour[a] != L.colour[b]) return L.colour[a] < L.colour[b];
+        if (key[a] != key[b]) return key[a] < key[b];
+        return a < b;
+    });
+    L.iperm.resize(N);
+    for (int d = 0; d < N; d++) L.iperm[L.perm[d]] = d;
+    L.col_off.assign(L.ncol + 1, 0);
+    for (int i = 0; i < N; i++) L.col_off[L.colour[i] + 1]++;
+    for (int c = 0; c < L.ncol; c++) L.col_off[c + 1] += L.col_off[c];
+}
+
+static int upload_level(Level &L)
+{
+    HIP_TRY(L.rx.upload(L.x, g_stream)); HIP_TRY(L.ry.upload(L.y, g_stream)); HIP_TRY(L.rz.upload(L.z, g_stream));
+    HIP_TRY(L.rnx.upload(L.nx, g_stream)); HIP_TRY(L.rny.upload(L.ny, g_stream)); HIP_TRY(L.rnz.upload(L.nz, g_stream));
+    HIP_TRY(L.rbtag.upload(L.btag, g_stream));
+    HIP_TRY(L.rcloud.upload(L.cloud, g_stream));
+    HIP_TRY(L.d_perm.upload(L.perm, g_stream));
+    HIP_TRY(L.d_iperm.upload(L.iperm, g_stream));
+    HIP_TRY(L.d_rp.upload(L.rp, g_stream));
+    HIP_TRY(L.d_pp.upload(L.pp, g_stream));
+    HIP_TRY(L.d_px.upload(L.px, g_stream)); HIP_TRY(L.d_py.upload(L.py, g_stream)); HIP_TRY(L.d_pz.upload(L.pz, g_stream));
+    std::vector<uint8_t> db(L.N);
+    for (int d = 0; d < L.N; d++) db[d] = L.btag[L.perm[d]];
+    HIP_TRY(L.dbtag.upload(db, g_stream));
+    int r0 = L.iperm[0];
+    HIP_TRY(L.ref0.upload(&r0, 1, g_stream));
+    for (auto &f : L.f) {
+        HIP_TRY(f.alloc(L.N));
+        HIP_TRY(hipMemsetAsync(f.p, 0, sizeof(double) * L.N, g_stream));
+    }
+    return 0;
+}
+
+static bool row_valid(const Level &L, int i, int mode)
+{
+    return mode == MGM_ROWS_REFERENCE ? (i >= L.nb) : (L.btag[i] == 0);
+}
+
+// Batched weight + transfer-row solves for the systems of level l.
+static int level_weights(mgm_problem *pb, int l, double *kernel_seconds)
+{
+    Level &L = pb->lv[l];
+    const int dim = pb->cfg.dimension, mode = pb->cfg.transfer_rows;
+    const int N = L.N, m = L.m, mpn = L.m + L.n;
+    for (int o = 0; o < NUM_OPS; o++) {
+        if (o == OP_DZ && dim != 3) continue;
+        HIP_TRY(L.rows[o].alloc((size_t)N * m));
+        HIP_TRY(hipMemsetAsync(L.rows[o].p, 0, sizeof(double) * N * m, g_stream));
+    }
+    std::vector<int> sys;
+    for (int i = 0; i < N; i++)
+        if (!L.ctag[i]) sys.push_back(i);
+    // interpolation targets mapped onto this level's systems
+    std::vector<int> roff(N + 1, 0), poff(N + 1, 0), rrow, prow;
+    std::vector<double> rxyz, pxyz;
+    if (l + 1 < pb->L) {
+        Level &C = pb->lv[l + 1];
+        for (int i = 0; i < C.N; i++)
+            if (row_valid(C, i, mode)) roff[C.rp[i] + 1]++;
+        for (int s = 0; s < N; s++) roff[s + 1] += roff[s];
+        rrow.resize(roff[N]); rxyz.resize(3 * (size_t)roff[N]);
+        std::vector<int> fill(roff.begin(), roff.end() - 1);
+        for (int i = 0; i < C.N; i++)
+            if (row_valid(C, i, mode)) {
+                int k = fill[C.rp[i]]++;
+                rrow[k] = i; rxyz[3 * k] = C.x[i]; rxyz[3 * k + 1] = C.y[i]; rxyz[3 * k + 2] = C.z[i];
+            }
+        HIP_TRY(C.Rrows.alloc((size_t)C.N * m));
+        HIP_TRY(hipMemsetAsync(C.Rrows.p, 0, sizeof(double) * C.N * m, g_stream));
+    }
+    if (l > 0) {
+        Level &F = pb->lv[l - 1];
+        for (int i = 0; i < F.N; i++)
+            if (row_valid(F, i, mode)) poff[F.pp[i] + 1]++;
+        for (int s = 0; s < N; s++) poff[s + 1] += poff[s];
+        prow.resize(poff[N]); pxyz.resize(3 * (size_t)poff[N]);
+        std::vector<int> fill(poff.begin(), poff.end() - 1);
+        for (int i = 0; i < F.N; i++)
+            if (row_valid(F, i, mode)) {
+                int k = fill[F.pp[i]]++;
+                prow[k] = i; pxyz[3 * k] = F.x[i]; pxyz[3 * k + 1] = F.y[i]; pxyz[3 * k + 2] = F.z[i];
+            }
+        HIP_TRY(F.Prows.alloc((size_t)F.N * m));
+        HIP_TRY(hipMemsetAsync(F.Prows.p, 0, sizeof(double) * F.N * m, g_stream));
+    }
+    DevBuf<int> d_sys, d_roff, d_poff, d_rrow, d_prow, d_status;
+    DevBuf<double> d_rxyz, d_pxyz, ws, rhs;
+    DevBuf<int> piv;
+    HIP_TRY(d_sys.upload(sys, g_stream));
+    HIP_TRY(d_roff.upload(roff, g_stream)); HIP_TRY(d_poff.upload(poff, g_stream));
+    if (!rrow.empty()) { HIP_TRY(d_rrow.upload(rrow, g_stream)); HIP_TRY(d_rxyz.upload(rxyz, g_stream)); }
+    if (!prow.empty()) { HIP_TRY(d_prow.upload(prow, g_stream)); HIP_TRY(d_pxyz.upload(pxyz, g_stream)); }
+    HIP_TRY(d_status.alloc(1));
+    HIP_TRY(hipMemsetAsync(d_status.p, 0, sizeof(int), g_stream));
+    // batch sized for ~4 GiB of factor workspace
+    long batch = (long)((4.0 * 1024 * 1024 * 1024) / (8.0 * mpn * mpn));
+    batch = std::max(64L, std::min(batch, (long)sys.size()));
+    batch = (batch + 63) / 64 * 64;
+    HIP_TRY(ws.alloc((size_t)mpn * mpn * batch));
+    HIP_TRY(rhs.alloc((size_t)4 * mpn * batch));
+    HIP_TRY(piv.alloc((size_t)mpn * batch));
+    WeightsArgs a{};
+    a.nsys = (int)sys.size(); a.sys = d_sys.p; a.m = m; a.n = L.n; a.dim = dim; a.phs = pb->cfg.phs_degree;
+    a.px = L.d_px.p; a.py = L.d_py.p; a.pz = L.d_pz.p;
+    a.x = L.rx.p; a.y = L.ry.p; a.z = L.rz.p; a.cloud = L.rcloud.p;
+    a.Dx = L.rows[OP_DX].p; a.Dy = L.rows[OP_DY].p; a.Dz = (dim == 3) ? L.rows[OP_DZ].p : nullptr;
+    a.lap = L.rows[OP_LAP].p;
+    if (l + 1 < pb->L) { a.rt_off = d_roff.p; a.rt_xyz = d_rxyz.p; a.rt_row = d_rrow.p; a.R_out = pb->lv[l + 1].Rrows.p; }
+    if (l > 0) { a.pt_off = d_poff.p; a.pt_xyz = d_pxyz.p; a.pt_row = d_prow.p; a.P_out = pb->lv[l - 1].Prows.p; }
+    a.ws = ws.p; a.rhs = rhs.p; a.piv = piv.p; a.batch = (int)batch; a.status = d_status.p;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0)); HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, g_stream));
+    int rc = launch_weights(a, g_stream);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(e1, g_stream));
+    int status = 0;
+    HIP_TRY(hipMemcpyAsync(&status, d_status.p, sizeof(int), hipMemcpyDeviceToHost, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    if (kernel_seconds) *kernel_seconds = ms * 1e-3;
+    if (status) MGM_FAIL(-8, "weights: singular local system (zero pivot)");
+    return 0;
+}
+
+static int level_lap_poisson(mgm_problem *pb, int l)
+{
+    Level &L = pb->lv[l];
+    int dim = pb->cfg.dimension;
+    return launch_lap_poisson(L.N, L.m, dim, L.rows[OP_DX].p, L.rows[OP_DY].p,
+                              dim == 3 ? L.rows[OP_DZ].p : nullptr, L.rows[OP_LAP].p, L.rnx.p,
+                              L.rny.p, L.rnz.p, L.rbtag.p, L.rows[OP_LAPP].p, g_stream);
+}
+
+static int level_slab(mgm_problem *pb, int l, int which)
+{
+    Level &L = pb->lv[l];
+    if (!L.rows[which].p) return 0;
+    HIP_TRY(L.slab[which].alloc((size_t)L.N * L.m));
+    return launch_gather_rows(L.N, L.m, L.rows[which].p, L.d_perm.p, L.d_iperm.p, L.rcloud.p,
+                              L.slab[which].p, nullptr, g_stream);
+}
+
+static int level_slabs(mgm_problem *pb, int l)
+{
+    Level &L = pb->lv[l];
+    HIP_TRY(L.col.alloc((size_t)L.N * L.m));
+    int rc = launch_gather_rows(L.N, L.m, L.rows[OP_LAP].p, L.d_perm.p, L.d_iperm.p, L.rcloud.p,
+                                nullptr, L.col.p, g_stream);
+    if (rc) return rc;
+    for (int o = 0; o < NUM_OPS; o++)
+        if ((rc = level_slab(pb, l, o))) return rc;
+    if (l > 0) {
+        Level &F = pb->lv[l - 1];
+        L.mr = F.m;
+        HIP_TRY(L.rcol.alloc((size_t)L.N * L.mr)); HIP_TRY(L.rw.alloc((size_t)L.N * L.mr));
+        rc = launch_transfer_slab(L.N, L.mr, L.Rrows.p, L.d_perm.p, L.d_rp.p, F.rcloud.p, F.d_iperm.p,
+                                  L.rw.p, L.rcol.p, g_stream);
+        if (rc) return rc;
+    }
+    if (l + 1 < pb->L) {
+        Level &C = pb->lv[l + 1];
+        L.mp = C.m;
+        HIP_TRY(L.pcol.alloc((size_t)L.N * L.mp)); HIP_TRY(L.pw.alloc((size_t)L.N * L.mp));
+        rc = launch_transfer_slab(L.N, L.mp, L.Prows.p, L.d_perm.p, L.d_pp.p, C.rcloud.p, C.d_iperm.p,
+                                  L.pw.p, L.pcol.p, g_stream);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+}  // namespace mgm
+
+mgm_problem::~mgm_problem()
+{
+    if (host_scal) (void)hipHostFree(host_scal);
+}
+
+// ===========================================================================
+// extern "C" API
+// ===========================================================================
+extern "C" {
+
+const char *mgm_last_error(void) { return mgm::last_error(); }
+
+int mgm_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int mgm_init(int device)
+{
+    if (g_stream && device == g_device) return 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) MGM_FAIL(-9, "no GPU visible to HIP");
+    if (device < 0 || device >= n) MGM_FAIL(-1, "mgm_init: bad device index");
+    HIP_TRY(hipSetDevice(device));
+    if (g_stream) (void)hipStreamDestroy(g_stream);
+    HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    g_device = device;
+    return 0;
+}
+
+void mgm_config_default(mgm_config *c)
+{
+    c->dimension = 2;
+    c->poly_degree = 3;
+    c->coarse_poly_degree = 3;
+    c->phs_degree = 3;
+    c->cloud_size_multiplier = 2.0;
+    c->omega = 1.0f;
+    c->num_relax = 3;
+    c->transfer_rows = MGM_ROWS_REFERENCE;
+    c->sweep_order = MGM_ORDER_COLOUR;
+}
+
+int mgm_poly_terms(int dim, int deg, short *px, short *py, short *pz)
+{
+    return mgm::poly_terms(dim, deg, px, py, pz);
+}
+
+uint64_t mgm_splitmix64(uint64_t *state) { return mgm::splitmix64(*state); }
+
+int mgm_points_read_gmsh(const char *path, int dim, mgm_points **out)
+{
+    if (!path || !out) MGM_FAIL(-1, "null argument");
+    if (dim != 2 && dim != 3) MGM_FAIL(-1, "dimension must be 2 or 3");
+    auto *p = new mgm_points();
+    int rc = mgm::read_gmsh(path, dim, p->ps);
+    if (rc) { delete p; return rc; }
+    *out = p;
+    return 0;
+}
+
+int mgm_points_generate(int dim, int n, uint64_t seed, double jitter, int with_volume_elements,
+                        const char *gmsh_path, mgm_points **out)
+{
+    if (!out) MGM_FAIL(-1, "null argument");
+    auto *p = new mgm_points();
+    int rc = mgm::generate_cloud(dim, n, seed, jitter, with_volume_elements, p->ps, gmsh_path);
+    if (rc) { delete p; return rc; }
+    *out = p;
+    return 0;
+}
+
+int mgm_points_info(const mgm_points *p, int *N, int *nb, int *num_elem, double *d_avg)
+{
+    if (!p) MGM_FAIL(-1, "null points");
+    if (N) *N = p->ps.N;
+    if (nb) *nb = p->ps.nb;
+    if (num_elem) *num_elem = p->ps.num_elem;
+    if (d_avg) *d_avg = p->ps.d_avg;
+    return 0;
+}
+
+int mgm_points_copy(const mgm_points *p, double *x, double *y, double *z, double *nx, double *ny,
+                    double *nz, uint8_t *btag, uint8_t *ctag)
+{
+    if (!p) MGM_FAIL(-1, "null points");
+    const auto &s = p->ps;
+    size_t N = s.N;
+    if (x) memcpy(x, s.x.data(), 8 * N);
+    if (y) memcpy(y, s.y.data(), 8 * N);
+    if (z) memcpy(z, s.z.data(), 8 * N);
+    if (nx) memcpy(nx, s.nx.data(), 8 * N);
+    if (ny) memcpy(ny, s.ny.data(), 8 * N);
+    if (nz) memcpy(nz, s.nz.data(), 8 * N);
+    if (btag) memcpy(btag, s.btag.data(), N);
+    if (ctag) memcpy(ctag, s.ctag.data(), N);
+    return 0;
+}
+
+void mgm_points_free(mgm_points *p) { delete p; }
+
+int mgm_problem_create(const mgm_config *cfg, int num_levels, mgm_problem **out)
+{
+    if (!cfg || !out) MGM_FAIL(-1, "null argument");
+    if (num_levels < 1 || num_levels > 16) MGM_FAIL(-1, "num_levels must be in [1,16]");
+    if (cfg->dimension != 2 && cfg->dimension != 3) MGM_FAIL(-1, "dimension must be 2 or 3");
+    if (cfg->phs_degree < 1 || cfg->phs_degree > 15) MGM_FAIL(-1, "phs_degree out of range");
+    if (cfg->poly_degree < 1 || cfg->coarse_poly_degree < 1) MGM_FAIL(-1, "poly degree must be >= 1");
+    auto *p = new mgm_problem();
+    p->cfg = *cfg;
+    p->L = num_levels;
+    p->lv.resize(num_levels);
+    *out = p;
+    return 0;
+}
+
+void mgm_problem_destroy(mgm_problem *p) { delete p; }
+
+int mgm_problem_set_level(mgm_problem *p, int lev, const mgm_points *pts)
+{
+    if (!p || !pts) MGM_FAIL(-1, "null argument");
+    if (lev < 0 || lev >= p->L) MGM_FAIL(-1, "level out of range");
+    p->lv[lev].in = pts->ps;
+    p->lv[lev].have_input = true;
+    p->ready = false;
+    return 0;
+}
+
+int mgm_problem_setup(mgm_problem *pb)
+{
+    if (!pb) MGM_FAIL(-1, "null problem");
+    int rc = ensure_init();
+    if (rc) return rc;
+    const int dim = pb->cfg.dimension;
+    double t0 = now(), tk = 0, tr = 0, tp = 0, tw = 0, ts = 0;
+    for (int l = 0; l < pb->L; l++) {
+        Level &L = pb->lv[l];
+        if (!L.have_input) MGM_FAIL(-1, "level without points");
+        L.poly = (l == 0) ? pb->cfg.poly_degree : pb->cfg.coarse_poly_degree;
+        L.n = poly_terms(dim, L.poly, nullptr, nullptr, nullptr);
+        L.px.resize(L.n); L.py.resize(L.n); L.pz.resize(L.n);
+        poly_terms(dim, L.poly, L.px.data(), L.py.data(), L.pz.data());
+        L.m = (int)floor(pb->cfg.cloud_size_multiplier * L.n + 1e-9);
+        L.N = L.in.N; L.nb = L.in.nb; L.d_avg_bbox = L.in.d_avg;
+        if (L.m < 2 || L.m > 127) MGM_FAIL(-1, "stencil size must be in [2,127]");
+        if (L.m + L.n > 256) MGM_FAIL(-1, "local system larger than 256");
+        if (L.N < L.m + 1) MGM_FAIL(-1, "level has fewer nodes than the stencil size");
+        double a = now();
+        if ((rc = level_knn(L, dim))) return rc;
+        double b = now();
+        if ((rc = level_bfs(L))) return rc;
+        level_normals_spacing(L, dim);
+        tk += b - a;
+        tr += now() - b;
+    }
+    double a = now();
+    for (int l = 0; l < pb->L; l++) {
+        Level &L = pb->lv[l];
+        if ((rc = nearest_points(L, (l == pb->L - 1) ? L : pb->lv[l + 1], dim, L.pp))) return rc;
+        if ((rc = nearest_points(L, (l == 0) ? L : pb->lv[l - 1], dim, L.rp))) return rc;
+    }
+    tp = now() - a;
+    a = now();
+    for (int l = 0; l < pb->L; l++) {
+        level_colour_order(pb->lv[l], dim);
+        if ((rc = upload_level(pb->lv[l]))) return rc;
+    }
+    tr += now() - a;
+    a = now();
+    for (int l = 0; l < pb->L; l++) {
+        if ((rc = level_weights(pb, l, nullptr))) return rc;
+        if ((rc = level_lap_poisson(pb, l))) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    tw = now() - a;
+    a = now();
+    for (int l = 0; l < pb->L; l++)
+        if ((rc = level_slabs(pb, l))) return rc;
+    HIP_TRY(pb->partial.alloc(1024));
+    HIP_TRY(pb->scal.alloc(16));
+    if (!pb->host_scal) HIP_TRY(hipHostMalloc((void **)&pb->host_scal, 16 * sizeof(double), 0));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    ts = now() - a;
+    pb->timings[0] = tk; pb->timings[1] = tr; pb->timings[2] = tp; pb->timings[3] = tw;
+    pb->timings[4] = ts; pb->timings[5] = now() - t0;
+    pb->ready = true;
+    return 0;
+}
+
+#define CHECK_READY(pb, lev)                                                      \
+    do {                                                                          \
+        if (!(pb)) MGM_FAIL(-1, "null problem");                                  \
+        if (!(pb)->ready) MGM_FAIL(-1, "problem is not set up");                  \
+        if ((lev) < 0 || (lev) >= (pb)->L) MGM_FAIL(-1, "level out of range");    \
+    } while (0)
+
+int mgm_level_info_get(const mgm_problem *pb, int lev, mgm_level_info *o)
+{
+    CHECK_READY(pb, lev);
+    const Level &L = pb->lv[lev];
+    o->N = L.N; o->nb = L.nb; o->m = L.m; o->n = L.n; o->poly_degree = L.poly;
+    o->num_colours = L.ncol; o->d_avg = L.d_avg; o->d_avg_bbox = L.d_avg_bbox; o->knn_ties = L.knn_ties;
+    return 0;
+}
+
+int mgm_setup_timings(const mgm_problem *pb, double *t6)
+{
+    if (!pb) MGM_FAIL(-1, "null problem");
+    for (int i = 0; i < 6; i++) t6[i] = pb->timings[i];
+    return 0;
+}
+
+int mgm_rebuild_weights(mgm_problem *pb, int lev, double *seconds, long *systems)
+{
+    CHECK_READY(pb, lev);
+    int rc = level_weights(pb, lev, seconds);
+    if (rc) return rc;
+    if ((rc = level_lap_poisson(pb, lev))) return rc;
+    for (int l = std::max(0, lev - 1); l <= std::min(pb->L - 1, lev + 1); l++)
+        if ((rc = level_slabs(pb, l))) return rc;
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    if (systems) {
+        long s = 0;
+        for (int i = 0; i < pb->lv[lev].N; i++) s += !pb->lv[lev].ctag[i];
+        *systems = s;
+    }
+    return 0;
+}
+
+int mgm_get_geometry(const mgm_problem *pb, int lev, double *x, double *y, double *z, double *nx,
+                     double *ny, double *nz, uint8_t *btag)
+{
+    CHECK_READY(pb, lev);
+    const Level &L = pb->lv[lev];
+    size_t N = L.N;
+    if (x) memcpy(x, L.x.data(), 8 * N);
+    if (y) memcpy(y, L.y.data(), 8 * N);
+    if (z) memcpy(z, L.z.data(), 8 * N);
+    if (nx) memcpy(nx, L.nx.data(), 8 * N);
+    if (ny) memcpy(ny, L.ny.data(), 8 * N);
+    if (nz) memcpy(nz, L.nz.data(), 8 * N);
+    if (btag) memcpy(btag, L.btag.data(), N);
+    return 0;
+}
+
+int mgm_get_cloud(const mgm_problem *pb, int lev, int32_t *cloud)
+{
+    CHECK_READY(pb, lev);
+    memcpy(cloud, pb->lv[lev].cloud.data(), sizeof(int) * pb->lv[lev].cloud.size());
+    return 0;
+}
+
+int mgm_get_points(const mgm_problem *pb, int lev, int which, int32_t *out)
+{
+    CHECK_READY(pb, lev);
+    const auto &v = (which == MGM_PTS_RESTRICTION) ? pb->lv[lev].rp : pb->lv[lev].pp;
+    memcpy(out, v.data(), sizeof(int) * v.size());
+    return 0;
+}
+
+int mgm_get_colours(const mgm_problem *pb, int lev, int32_t *colour)
+{
+    CHECK_READY(pb, lev);
+    memcpy(colour, pb->lv[lev].colour.data(), sizeof(int) * pb->lv[lev].N);
+    return 0;
+}
+
+static int matrix_buf(const mgm_problem *pb, int lev, int which, double **buf, size_t *count, int *slab_op)
+{
+    Level &L = const_cast<Level &>(pb->lv[lev]);
+    *slab_op = -1;
+    if (which >= MGM_MAT_DX && which <= MGM_MAT_LAPP) {
+        if (which == MGM_MAT_DZ && pb->cfg.dimension != 3) MGM_FAIL(-1, "Dz exists only in 3D");
+        *buf = L.rows[which].p;
+        *count = (size_t)L.N * L.m;
+        *slab_op = which;
+    } else if (which == MGM_MAT_RESTR) {
+        if (lev == 0) MGM_FAIL(-1, "level 0 has no restriction rows");
+        *buf = L.Rrows.p;
+        *count = (size_t)L.N * pb->lv[lev - 1].m;
+    } else if (which == MGM_MAT_PROL) {
+        if (lev == pb->L - 1) MGM_FAIL(-1, "the coarsest level has no prolongation rows");
+        *buf = L.Prows.p;
+        *count = (size_t)L.N * pb->lv[lev + 1].m;
+    } else {
+        MGM_FAIL(-1, "unknown matrix id");
+    }
+    return 0;
+}
+
+int mgm_get_matrix(const mgm_problem *pb, int lev, int which, double *out)
+{
+    CHECK_READY(pb, lev);
+    double *buf; size_t cnt; int op;
+    int rc = matrix_buf(pb, lev, which, &buf, &cnt, &op);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(out, buf, 8 * cnt, hipMemcpyDeviceToHost, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int mgm_set_matrix(mgm_problem *pb, int lev, int which, const double *in)
+{
+    CHECK_READY(pb, lev);
+    double *buf; size_t cnt; int op;
+    int rc = matrix_buf(pb, lev, which, &buf, &cnt, &op);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(buf, in, 8 * cnt, hipMemcpyHostToDevice, g_stream));
+    if (op >= 0) rc = level_slab(pb, lev, op);
+    else rc = level_slabs(pb, lev);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int mgm_dirichlet_rows(mgm_problem *pb)
+{
+    CHECK_READY(pb, 0);
+    for (int l = 0; l < pb->L; l++) {
+        Level &L = pb->lv[l];
+        int rc = launch_dirichlet_rows(L.N, L.m, L.rbtag.p, L.rows[OP_LAPP].p, g_stream);
+        if (rc) return rc;
+        if ((rc = level_slab(pb, l, OP_LAPP))) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int mgm_get_field(const mgm_problem *pb, int lev, int which, double *out)
+{
+    CHECK_READY(pb, lev);
+    if (which < 0 || which >= MGM_NUM_FIELDS) MGM_FAIL(-1, "unknown field");
+    const Level &L = pb->lv[lev];
+    std::vector<double> tmp(L.N);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), L.f[which].p, 8 * (size_t)L.N, hipMemcpyDeviceToHost, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    for (int i = 0; i < L.N; i++) out[i] = tmp[L.iperm[i]];
+    return 0;
+}
+
+int mgm_set_field(mgm_problem *pb, int lev, int which, const double *in)
+{
+    CHECK_READY(pb, lev);
+    if (which < 0 || which >= MGM_NUM_FIELDS) MGM_FAIL(-1, "unknown field");
+    Level &L = pb->lv[lev];
+    std::vector<double> tmp(L.N);
+    for (int d = 0; d < L.N; d++) tmp[d] = in[L.perm[d]];
+    HIP_TRY(hipMemcpyAsync(L.f[which].p, tmp.data(), 8 * (size_t)L.N, hipMemcpyHostToDevice, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int mgm_zero_fields(mgm_problem *pb)
+{
+    CHECK_READY(pb, 0);
+    for (auto &L : pb->lv)
+        for (auto &f : L.f) HIP_TRY(hipMemsetAsync(f.p, 0, 8 * (size_t)L.N, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Operators, stream-ordered (no host synchronisation) so a V-cycle can be
+// enqueued back to back or captured into a graph.
+namespace mgm {
+
+static int pfield(int variant) { return variant == MGM_VAR_T ? MGM_F_PPRIME : MGM_F_P; }
+
+int op_relax(mgm_problem *pb, int l, int variant)
+{
+    Level &L = pb->lv[l];
+    const double w = pb->cfg.omega;
+    const double c = (float)(1 - pb->cfg.omega);  // (1-omega) evaluated in float, as the reference
+    double *p = L.f[pfield(variant)].p, *src = L.f[MGM_F_SOURCE].p, *res = L.f[MGM_F_RES].p;
+    Ell A = L.op(OP_LAPP);
+    for (int it = 0; it < pb->cfg.num_relax; it++) {
+        int rc;
+        if (pb->cfg.sweep_order == MGM_ORDER_NATURAL) {
+            rc = launch_sor_natural(A, L.N, L.d_iperm.p, src, p, w, c, g_stream);
+            if (rc) return rc;
+        } else {
+            for (int k = 0; k < L.ncol; k++)
+                if ((rc = launch_sor_rows(A, L.col_off[k], L.col_off[k + 1], src, p, w, c, g_stream))) return rc;
+        }
+        if (variant == MGM_VAR_H) continue;
+        if ((rc = launch_pin(L.N, L.ref0.p, p, pb->scal.p + 8, g_stream))) return rc;
+        if (variant == MGM_VAR_T) rc = launch_residual(A, L.N, 1, L.dbtag.p, src, p, res, g_stream);
+        else rc = launch_residual(L.op(OP_LAP), L.N, 2, L.dbtag.p, src, p, res, g_stream);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int op_residual(mgm_problem *pb, int l, int variant)
+{
+    Level &L = pb->lv[l];
+    if (variant == MGM_VAR_F)
+        return launch_residual(L.op(OP_LAP), L.N, 3, L.dbtag.p, L.f[MGM_F_SOURCE].p, L.f[MGM_F_P].p,
+                               L.f[MGM_F_RES].p, g_stream);
+    return launch_residual(L.op(OP_LAPP), L.N, 0, L.dbtag.p, L.f[MGM_F_SOURCE].p,
+                           L.f[pfield(variant)].p, L.f[MGM_F_RES].p, g_stream);
+}
+
+int op_restrict(mgm_problem *pb, int l, int variant)
+{
+    Level &F = pb->lv[l], &C = pb->lv[l + 1];
+    return launch_restrict(C.restr(), C.N, C.dbtag.p, variant == MGM_VAR_F, F.f[MGM_F_RES].p,
+                           C.f[MGM_F_SOURCE].p, g_stream);
+}
+
+int op_prolong(mgm_problem *pb, int l, int variant)
+{
+    Level &C = pb->lv[l], &F = pb->lv[l - 1];
+    int pf = pfield(variant);
+    int rc = launch_prolong(F.prol(), F.N, F.dbtag.p, C.f[pf].p, F.f[pf].p, g_stream);
+    if (rc) return rc;
+    return launch_zero(C.N, C.dbtag.p, variant != MGM_VAR_T, C.f[pf].p, g_stream);
+}
+
+int op_vcycle(mgm_problem *pb, int variant)
+{
+    int rc;
+    const int L = pb->L;
+    for (int l = 0; l < L; l++) {
+        if ((rc = op_relax(pb, l, variant))) return rc;
+        if ((rc = op_residual(pb, l, variant))) return rc;
+        if (l != L - 1 && (rc = op_restrict(pb, l, variant))) return rc;
+    }
+    for (int l = L - 1; l > 0; l--) {
+        if ((rc = op_prolong(pb, l, variant))) return rc;
+        if (l != 1 && (rc = op_relax(pb, l - 1, variant))) return rc;
+    }
+    return 0;
+}
+
+}  // namespace mgm
+
+extern "C" {
+
+#define OP_SYNC(expr)                                     \
+    do {                                                  \
+        int _rc = (expr);                                 \
+        if (_rc) return _rc;                              \
+        HIP_TRY(hipStreamSynchronize(g_stream));          \
+        return 0;                                         \
+    } while (0)
+
+int mgm_relax(mgm_problem *pb, int lev, int variant)
+{
+    CHECK_READY(pb, lev);
+    OP_SYNC(op_relax(pb, lev, variant));
+}
+
+int mgm_residual(mgm_problem *pb, int lev, int variant)
+{
+    CHECK_READY(pb, lev);
+    OP_SYNC(op_residual(pb, lev, variant));
+}
+
+int mgm_restrict(mgm_problem *pb, int lev, int variant)
+{
+    CHECK_READY(pb, lev);
+    if (lev >= pb->L - 1) MGM_FAIL(-1, "restrict: no coarser level");
+    OP_SYNC(op_restrict(pb, lev, variant));
+}
+
+int mgm_prolong(mgm_problem *pb, int lev, int variant)
+{
+    CHECK_READY(pb, lev);
+    if (lev < 1) MGM_FAIL(-1, "prolong: no finer level");
+    OP_SYNC(op_prolong(pb, lev, variant));
+}
+
+int mgm_vcycle(mgm_problem *pb, int variant, int ncycles)
+{
+    CHECK_READY(pb, 0);
+    for (int k = 0; k < ncycles; k++) {
+        int rc = op_vcycle(pb, variant);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int mgm_spmv(mgm_problem *pb, int lev, int which_matrix, int in_field, int out_field)
+{
+    CHECK_READY(pb, lev);
+    if (which_matrix < MGM_MAT_DX || which_matrix > MGM_MAT_LAPP) MGM_FAIL(-1, "spmv: unknown matrix");
+    if (in_field < 0 || in_field >= MGM_NUM_FIELDS || out_field < 0 || out_field >= MGM_NUM_FIELDS ||
+        in_field == out_field)
+        MGM_FAIL(-1, "spmv: bad field ids");
+    Level &L = pb->lv[lev];
+    if (!L.slab[which_matrix].p) MGM_FAIL(-1, "spmv: matrix not built");
+    OP_SYNC(launch_spmv(L.op(which_matrix), 0, L.N, L.f[in_field].p, L.f[out_field].p, g_stream));
+}
+
+int mgm_l2_norm_gen(const mgm_problem *pbc, int lev, int fa, int fb, double *out)
+{
+    CHECK_READY(pbc, lev);
+    mgm_problem *pb = const_cast<mgm_problem *>(pbc);
+    Level &L = pb->lv[lev];
+    int rc = launch_sqdiff_sum(L.N, L.dbtag.p, L.f[fa].p, fb >= 0 ? L.f[fb].p : nullptr, pb->partial.p,
+                               pb->scal.p, g_stream);
+    if (rc) return rc;
+    double s = 0;
+    HIP_TRY(hipMemcpyAsync(&s, pb->scal.p, 8, hipMemcpyDeviceToHost, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    *out = sqrt(s / L.N);
+    return 0;
+}
+
+}  // extern "C"

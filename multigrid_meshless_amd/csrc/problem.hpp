@@ -1,0 +1,69 @@
+// problem.hpp -- the multigrid hierarchy held by an mgm_problem.
+#pragma once
+
+#include <array>
+#include <vector>
+
+#include "device.hpp"
+
+struct mgm_points {
+    mgm::PointSet ps;
+};
+
+namespace mgm {
+
+enum { OP_DX = 0, OP_DY = 1, OP_DZ = 2, OP_LAP = 3, OP_LAPP = 4, NUM_OPS = 5 };
+
+struct Level {
+    // ---- input (file order, after corner removal) ----
+    PointSet in;
+    bool have_input = false;
+    // ---- sizes ----
+    int N = 0, nb = 0, m = 0, n = 0, poly = 0;
+    double d_avg_bbox = 0, d_avg = 0;
+    std::vector<short> px, py, pz;
+    // ---- reference node order (after the BFS reordering) ----
+    std::vector<double> x, y, z, nx, ny, nz;
+    std::vector<uint8_t> btag, ctag;   // ctag is NOT permuted, as in the reference
+    std::vector<int> cloud;            // N*m
+    std::vector<int> rp, pp;           // restriction / prolongation points
+    std::vector<int> colour;
+    int ncol = 0;
+    std::vector<int> col_off;          // colour c occupies device rows [col_off[c], col_off[c+1])
+    std::vector<int> perm, iperm;      // device -> reference, reference -> device
+    int knn_ties = 0;
+    // ---- device, reference order ----
+    DevBuf<double> rx, ry, rz, rnx, rny, rnz;
+    DevBuf<uint8_t> rbtag;
+    DevBuf<int> rcloud, d_perm, d_iperm, d_rp, d_pp;
+    DevBuf<short> d_px, d_py, d_pz;
+    std::array<DevBuf<double>, NUM_OPS> rows;  // row-major N*m
+    DevBuf<double> Rrows;                      // this level as coarse: N * m_{l-1}
+    DevBuf<double> Prows;                      // this level as fine:   N * m_{l+1}
+    // ---- device, device order ----
+    DevBuf<uint8_t> dbtag;
+    DevBuf<int> col;                           // m*N
+    std::array<DevBuf<double>, NUM_OPS> slab;  // m*N
+    DevBuf<int> rcol, pcol;
+    DevBuf<double> rw, pw;
+    int mr = 0, mp = 0;                        // widths of restriction / prolongation rows
+    std::array<DevBuf<double>, MGM_NUM_FIELDS> f;
+    DevBuf<int> ref0;                          // device row of reference node 0
+
+    Ell op(int which) const { return Ell{col.p, slab[which].p, m, N}; }
+    Ell restr() const { return Ell{rcol.p, rw.p, mr, N}; }
+    Ell prol() const { return Ell{pcol.p, pw.p, mp, N}; }
+};
+
+}  // namespace mgm
+
+struct mgm_problem {
+    mgm_config cfg{};
+    int L = 0;
+    std::vector<mgm::Level> lv;
+    bool ready = false;
+    double timings[6] = {0, 0, 0, 0, 0, 0};
+    mgm::DevBuf<double> partial, scal;
+    double *host_scal = nullptr;   // pinned
+    ~mgm_problem();
+};

@@ -1,0 +1,132 @@
+// solve.cpp -- resident solve-to-tolerance and stencil-apply timing.
+//
+// The solve runs the reference V-cycle (mg_heat_conduction.c:64-75) with the
+// operators resident in HBM; after every cycle the finest-level residual
+// r = source - lap_Poison * p on interior nodes is formed and its
+// l2_norm_gen norm (hf/general_functions.h:181-192) is compared with the
+// source's.  The per-cycle launch sequence is captured once into a hipGraph
+// and replayed.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "problem.hpp"
+
+namespace mgm {
+int op_vcycle(mgm_problem *pb, int variant);
+int op_relax(mgm_problem *pb, int l, int variant);
+}  // namespace mgm
+
+using namespace mgm;
+
+namespace {
+
+double wall()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// residual of level 0 into RES and its squared interior norm into scal[0]
+int enqueue_fine_residual(mgm_problem *pb, int variant)
+{
+    Level &L = pb->lv[0];
+    int pf = variant == MGM_VAR_T ? MGM_F_PPRIME : MGM_F_P;
+    int rc = launch_residual(L.op(OP_LAPP), L.N, 0, L.dbtag.p, L.f[MGM_F_SOURCE].p, L.f[pf].p,
+                             L.f[MGM_F_RES].p, stream());
+    if (rc) return rc;
+    return launch_sqdiff_sum(L.N, L.dbtag.p, L.f[MGM_F_RES].p, nullptr, pb->partial.p, pb->scal.p, stream());
+}
+
+int read_scalar(mgm_problem *pb, double *v)
+{
+    HIP_TRY(hipMemcpyAsync(pb->host_scal, pb->scal.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    HIP_TRY(hipStreamSynchronize(stream()));
+    *v = pb->host_scal[0];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_cycles,
+              int *cycles_done, double *history, double *seconds)
+{
+    if (!pb || !pb->ready) MGM_FAIL(-1, "problem is not set up");
+    if (precision != 0) MGM_FAIL(-1, "mgm_solve: precision mode not available in this build");
+    if (variant != MGM_VAR_H && variant != MGM_VAR_T && variant != MGM_VAR_F) MGM_FAIL(-1, "bad variant");
+    if (max_cycles < 0) MGM_FAIL(-1, "max_cycles < 0");
+    Level &L0 = pb->lv[0];
+    int rc;
+    // |f| over interior nodes
+    rc = launch_sqdiff_sum(L0.N, L0.dbtag.p, L0.f[MGM_F_SOURCE].p, nullptr, pb->partial.p, pb->scal.p, stream());
+    if (rc) return rc;
+    double fs;
+    if ((rc = read_scalar(pb, &fs))) return rc;
+    double fnorm = sqrt(fs / L0.N);
+    if (!(fnorm > 0)) fnorm = 1.0;
+    if ((rc = enqueue_fine_residual(pb, variant))) return rc;
+    double rs;
+    if ((rc = read_scalar(pb, &rs))) return rc;
+    if (history) history[0] = sqrt(rs / L0.N) / fnorm;
+
+    // capture one cycle (V-cycle + fine residual + norm) into a graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIP_TRY(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+    rc = op_vcycle(pb, variant);
+    if (!rc) rc = enqueue_fine_residual(pb, variant);
+    hipError_t ce = hipStreamEndCapture(stream(), &graph);
+    if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (ce != hipSuccess) MGM_FAIL(-10, std::string("graph capture failed: ") + hipGetErrorString(ce));
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        MGM_FAIL(-10, std::string("graph instantiate failed: ") + hipGetErrorString(ie));
+    }
+    HIP_TRY(hipStreamSynchronize(stream()));
+    double t0 = wall();
+    int k = 0;
+    double ratio = history ? history[0] : 1.0;
+    for (k = 1; k <= max_cycles; k++) {
+        hipError_t le = hipGraphLaunch(exec, stream());
+        if (le != hipSuccess) { rc = -10; set_error(hipGetErrorString(le)); break; }
+        if ((rc = read_scalar(pb, &rs))) break;
+        ratio = sqrt(rs / L0.N) / fnorm;
+        if (history) history[k] = ratio;
+        if (!(ratio > tol)) break;  // also stops on NaN
+    }
+    double t1 = wall();
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    if (cycles_done) *cycles_done = (k > max_cycles) ? max_cycles : k;
+    if (seconds) *seconds = t1 - t0;
+    if (std::isnan(ratio)) MGM_FAIL(-11, "solve diverged (NaN residual)");
+    return 0;
+}
+
+int mgm_time_spmv(mgm_problem *pb, int lev, int reps, double *seconds)
+{
+    if (!pb || !pb->ready) MGM_FAIL(-1, "problem is not set up");
+    if (lev < 0 || lev >= pb->L || reps < 1) MGM_FAIL(-1, "bad arguments");
+    Level &L = pb->lv[lev];
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int rc = launch_spmv(L.op(OP_LAPP), 0, L.N, L.f[MGM_F_P].p, L.f[MGM_F_DPDN].p, stream());
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(e0, stream()));
+    for (int r = 0; r < reps; r++)
+        if ((rc = launch_spmv(L.op(OP_LAPP), 0, L.N, L.f[MGM_F_P].p, L.f[MGM_F_DPDN].p, stream()))) return rc;
+    HIP_TRY(hipEventRecord(e1, stream()));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *seconds = ms * 1e-3 / reps;
+    return 0;
+}
+
+}  // extern "C"
