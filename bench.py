@@ -1,0 +1,195 @@
+"""Benchmark: time to 1e-8 relative residual, 3D 1M-point multigrid Poisson.
+
+Workload (one "step" = one solve to tolerance from p = 0, operators resident in
+HBM): jittered 100^3 cube (seed 1; 998,816 nodes after corner/edge removal),
+5 levels 100^3/50^3/25^3/13^3/7^3 (seeds 1..5), PHS r^3 + cubic polynomials,
+m = 70 stencils (multiplier 3.5), Dirichlet-zero identity rows on every level,
+RHS = sum of 16 Fourier modes (amplitudes N(0,1), SplitMix64 seed 2), V-cycles
+of multicolour SOR (omega 1, 3 sweeps) until l2_norm_gen(r)/l2_norm_gen(f) <= 1e-8.
+
+Prints one JSON line (rank 0, one GPU).  `roofline` is the FP64 stencil apply
+(lap_Poison over level 0) timed with HIP events on the library stream;
+`cpu_baseline` is the oracle's sequential V-cycle in the same colour order on
+the same operators, timed for a bounded number of cycles on the host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def fourier_rhs(x, y, z, seed=2, modes=16):
+    """f = sum_q a_q sin(pi (k_q . x)), k_q in {1..4}^3, a_q ~ N(0,1) (Box-Muller on SplitMix64)."""
+    import ctypes as C
+    from multigrid_meshless_amd._lib import lib
+    s = C.c_uint64(seed)
+
+    def u01():
+        return (lib().mgm_splitmix64(C.byref(s)) >> 11) * 2.0 ** -53
+
+    f = np.zeros_like(x)
+    for _ in range(modes):
+        k = [1 + (lib().mgm_splitmix64(C.byref(s)) % 4) for _ in range(3)]
+        u1, u2 = u01(), u01()
+        a = math.sqrt(-2.0 * math.log(max(u1, 1e-300))) * math.cos(2 * math.pi * u2)
+        f += a * np.sin(math.pi * (k[0] * x + k[1] * y + k[2] * z))
+    return f
+
+
+def build_problem(mgm, n, levels, seed0=1):
+    sizes = []
+    v = n
+    for _ in range(levels):
+        sizes.append(v)
+        v = (v + 1) // 2
+    pts = [mgm.Points.generate(3, s, seed0 + k, 0.2) for k, s in enumerate(sizes)]
+    cfg = mgm.config(dimension=3, poly_degree=3, coarse_poly_degree=3, phs_degree=3,
+                     cloud_size_multiplier=3.5, omega=1.0, num_relax=3,
+                     transfer_rows=mgm.ROWS_BY_TAG, sweep_order=mgm.ORDER_COLOUR)
+    P = mgm.Problem(cfg, pts)
+    t0 = time.perf_counter()
+    P.setup()
+    t_setup = time.perf_counter() - t0
+    P.dirichlet_rows()
+    return P, sizes, t_setup
+
+
+def set_rhs(P):
+    g = P.geometry(0)
+    f = fourier_rhs(g["x"], g["y"], g["z"])
+    f[g["btag"] != 0] = 0.0
+    return f
+
+
+def reset(P, f):
+    P.zero_fields()
+    P.set_field(0, "source", f)
+
+
+def cpu_baseline(P, cycles_needed, sample_cycles, f):
+    """Oracle V-cycles (sequential sweep in the GPU's colour order) on the same operators."""
+    import oracle
+    L = P.L
+    lv = []
+    for k in range(L):
+        col = P.colours(k)
+        d = dict(cloud=P.cloud(k), Lp=P.matrix(k, "lap_Poison"), btag=P.geometry(k)["btag"],
+                 rp=P.points(k, "restriction"), pp=P.points(k, "prolongation"),
+                 order=np.lexsort((np.arange(len(col)), col)).astype(np.int32))
+        if k > 0:
+            d["R"] = P.matrix(k, "restr")
+        if k < L - 1:
+            d["P"] = P.matrix(k, "prol")
+        lv.append(d)
+    H = oracle.Hierarchy(lv)
+    H.lv[0]["s"][:] = f
+    fnorm = oracle.l2_norm_gen(lv[0]["btag"], f)
+    t0 = time.perf_counter()
+    for _ in range(sample_cycles):
+        H.vcycle(oracle.VAR_H, 1.0, 3)
+        H.residual(oracle.VAR_H, 0)
+        oracle.l2_norm_gen(lv[0]["btag"], H.lv[0]["res"]) / fnorm
+    dt = (time.perf_counter() - t0) / sample_cycles
+    return dt * cycles_needed, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=100, help="points per axis on the finest level")
+    ap.add_argument("--levels", type=int, default=5)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--max-cycles", type=int, default=400)
+    ap.add_argument("--cpu-cycles", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(the sharding design is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import multigrid_meshless_amd as mgm
+    mgm.init(0)
+    P, sizes, t_setup = build_problem(mgm, args.n, args.levels)
+    info0 = P.info(0)
+    N, m = info0["N"], info0["m"]
+    f = set_rhs(P)
+    print(f"[bench] setup {t_setup:.2f}s levels={sizes} N0={N} m={m} colours0={info0['num_colours']} "
+          f"phases={ {k: round(v, 3) for k, v in P.timings().items()} }", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        reset(P, f)
+        r = P.solve(args.tol, args.max_cycles)
+        print(f"[bench] warmup solve: {r['cycles']} cycles {r['seconds']:.4f}s "
+              f"final {r['history'][-1]:.3e}", file=sys.stderr, flush=True)
+    times, cycles, finals = [], [], []
+    t_all = 0.0
+    for _ in range(args.steps):
+        reset(P, f)
+        t0 = time.perf_counter()
+        r = P.solve(args.tol, args.max_cycles)
+        t_all += time.perf_counter() - t0
+        times.append(r["seconds"])
+        cycles.append(r["cycles"])
+        finals.append(float(r["history"][-1]))
+        print(f"[bench] solve: {r['cycles']} cycles {r['seconds']:.4f}s final {finals[-1]:.3e}",
+              file=sys.stderr, flush=True)
+    converged = all(v <= args.tol for v in finals)
+    t_solve = float(np.median(times))
+
+    # stencil apply roofline: lap_Poison over level 0, FP64 weights + int32 columns
+    t_apply = P.time_spmv(0, args.spmv_reps)
+    bytes_apply = N * (m * (8 + 4) + 8 + 8)
+    achieved = bytes_apply / t_apply / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        t_cpu, t_cyc = cpu_baseline(P, cycles[0], args.cpu_cycles, f)
+        import oracle
+        cpu = {"value": round(t_cpu, 3), "unit": "s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_cycles} V-cycles (+ fine residual norm) of the oracle's sequential "
+                         f"colour-ordered sweep on the same 1M-point operators, {t_cyc:.3f} s/cycle, "
+                         f"x {cycles[0]} cycles the GPU needed; threads={oracle.num_threads()} available, 1 used"}
+
+    out = {
+        "metric": "time-to-1e-8 residual, 3D 1M-pt MG Poisson (s); stencil-apply HBM GB/s",
+        "value": round(t_solve, 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_solve * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3D cube Poisson, {sizes[0]}^3 jittered cloud ({N} nodes), "
+                               f"{len(sizes)} levels {sizes}, m={m}, FP64 multicolour SOR V-cycle, "
+                               f"tol {args.tol}",
+                   "cycles": cycles[0], "converged": converged,
+                   "final_ratio": finals[0], "setup_s": round(t_setup, 2),
+                   "stencil_apply_GBps": round(achieved, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

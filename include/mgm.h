@@ -85,6 +85,11 @@ int mgm_points_read_gmsh(const char *path, int dim, mgm_points **out);
  * optionally also written to gmsh_path as Gmsh v2. */
 int mgm_points_generate(int dim, int n, uint64_t seed, double jitter, int with_volume_elements,
                         const char *gmsh_path, mgm_points **out);
+/* Points from arrays laid out as the reader leaves them (after corner removal). */
+int mgm_points_from_arrays(int N, int nb, int num_elem, double d_avg, const double *x,
+                           const double *y, const double *z, const double *nx, const double *ny,
+                           const double *nz, const uint8_t *btag, const uint8_t *ctag,
+                           mgm_points **out);
 int mgm_points_info(const mgm_points *p, int *N, int *nb, int *num_elem, double *d_avg);
 int mgm_points_copy(const mgm_points *p, double *x, double *y, double *z, double *nx,
                     double *ny, double *nz, uint8_t *btag, uint8_t *ctag);

@@ -66,6 +66,16 @@ class Points:
               "generate")
         return cls(h)
 
+    @classmethod
+    def from_arrays(cls, N, nb, num_elem, d_avg, x, y, z, nx, ny, nz, btag, ctag) -> "Points":
+        h = C.c_void_p()
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, nx, ny, nz)]
+        tags = [np.ascontiguousarray(a, dtype=np.uint8) for a in (btag, ctag)]
+        check(lib().mgm_points_from_arrays(int(N), int(nb), int(num_elem), float(d_avg),
+                                           *[a.ctypes.data for a in arrs + tags], C.byref(h)),
+              "points_from_arrays")
+        return cls(h)
+
     def info(self):
         N, nb, ne, d = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         check(lib().mgm_points_info(self._h, C.byref(N), C.byref(nb), C.byref(ne), C.byref(d)))

@@ -37,6 +37,7 @@ SIGNATURES = [
     ("mgm_poly_terms", _i, [_i, _i, _vp, _vp, _vp]),
     ("mgm_points_read_gmsh", _i, [_cp, _i, C.POINTER(_vp)]),
     ("mgm_points_generate", _i, [_i, _i, C.c_uint64, _d, _i, _cp, C.POINTER(_vp)]),
+    ("mgm_points_from_arrays", _i, [_i, _i, _i, _d] + [_vp] * 8 + [C.POINTER(_vp)]),
     ("mgm_points_info", _i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_d)]),
     ("mgm_points_copy", _i, [_vp] + [_vp] * 8),
     ("mgm_points_free", None, [_vp]),

@@ -151,7 +151,7 @@ int launch_knn(const KnnGrid &g, const KdOrder *kd, int nq, const double *qx, co
 {
     if (nq <= 0) return 0;
     size_t shm = (size_t)K * KNN_BLOCK * (sizeof(double) + sizeof(int));
-    if (shm > 160 * 1024) MGM_FAIL(-1, "knn: stencil too large for LDS");
+    if (shm > 64 * 1024) MGM_FAIL(-1, "knn: stencil larger than 84 points does not fit the 64 KiB LDS budget");
     KdOrder k0{};
     int blocks = (nq + KNN_BLOCK - 1) / KNN_BLOCK;
     hipLaunchKernelGGL(knn_kernel, dim3(blocks), dim3(KNN_BLOCK), shm, s, g, kd ? *kd : k0,

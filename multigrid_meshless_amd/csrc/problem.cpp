@@ -649,6 +649,23 @@ int mgm_points_generate(int dim, int n, uint64_t seed, double jitter, int with_v
     return 0;
 }
 
+int mgm_points_from_arrays(int N, int nb, int num_elem, double d_avg, const double *x,
+                           const double *y, const double *z, const double *nx, const double *ny,
+                           const double *nz, const uint8_t *btag, const uint8_t *ctag,
+                           mgm_points **out)
+{
+    if (!out || !x || !y || !z || !nx || !ny || !nz || !btag || !ctag) MGM_FAIL(-1, "null argument");
+    if (N <= 0 || nb < 0 || nb > N) MGM_FAIL(-1, "bad sizes");
+    auto *p = new mgm_points();
+    auto &s = p->ps;
+    s.N = N; s.nb = nb; s.num_elem = num_elem; s.d_avg = d_avg;
+    s.x.assign(x, x + N); s.y.assign(y, y + N); s.z.assign(z, z + N);
+    s.nx.assign(nx, nx + N); s.ny.assign(ny, ny + N); s.nz.assign(nz, nz + N);
+    s.btag.assign(btag, btag + N); s.ctag.assign(ctag, ctag + N);
+    *out = p;
+    return 0;
+}
+
 int mgm_points_info(const mgm_points *p, int *N, int *nb, int *num_elem, double *d_avg)
 {
     if (!p) MGM_FAIL(-1, "null points");
