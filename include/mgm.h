@@ -135,6 +135,26 @@ int mgm_vcycle(mgm_problem *p, int variant, int ncycles);
 /* out = M * in over fixed-width rows (M: MGM_MAT_DX..MGM_MAT_LAPP). */
 int mgm_spmv(mgm_problem *p, int lev, int which_matrix, int in_field, int out_field);
 int mgm_l2_norm_gen(const mgm_problem *p, int lev, int field_a, int field_b, double *out);
+/* out = D f over caller-supplied fixed-width rows (uploaded for the call):
+ * D and cloud are n_rows x n_cols row-major, f has ncol entries.
+ * (multiply_sparse_matrix_vector_gpu for row sets that are not resident.) */
+int mgm_spmv_rows(int n_rows, int n_cols, const double *D, const int32_t *cloud, int ncol,
+                  const double *f, double *out);
+
+/* ---- time steps that call the V-cycle (level-0 fields, resident) ----------- */
+typedef struct {
+    double dt, rho, mu, nu, facRe, facdt;
+    double field_rho;             /* FieldVariables.rho read by timple_solver.h:389 */
+    int iter_momentum, iter_timple, num_vcycles;
+} mgm_ns_params;
+int mgm_ns_set_params(mgm_problem *p, const mgm_ns_params *ns);
+/* fractional_step_explicit (header_files/fractional_step_solver.h:47): predictor,
+ * divergence source, boundary dp/dn, F-variant V-cycles on P, projection. */
+int mgm_fs_step(mgm_problem *p, double *steady_error);
+/* time_implicit_solver (header_files/timple_solver.h:40): implicit momentum
+ * Gauss-Seidel (sweep order as configured), divergence source, T-variant
+ * V-cycles on PPRIME, velocity/pressure update, Neumann boundary pressure. */
+int mgm_timple_step(mgm_problem *p, double *steady_error);
 
 /* ---- resident solve -------------------------------------------------------- */
 /* V-cycles on level-0 (field P, SOURCE) until l2(r)/l2(f) <= tol (l2_norm_gen

@@ -424,6 +424,16 @@ static int upload_level(Level &L)
     std::vector<uint8_t> db(L.N);
     for (int d = 0; d < L.N; d++) db[d] = L.btag[L.perm[d]];
     HIP_TRY(L.dbtag.upload(db, g_stream));
+    {
+        std::vector<double> t(L.N);
+        const std::vector<double> *src[3] = {&L.nx, &L.ny, &L.nz};
+        DevBuf<double> *dst[3] = {&L.dnx, &L.dny, &L.dnz};
+        for (int k = 0; k < 3; k++) {
+            for (int d = 0; d < L.N; d++) t[d] = (*src[k])[L.perm[d]];
+            HIP_TRY(dst[k]->upload(t, g_stream));
+            HIP_TRY(hipStreamSynchronize(g_stream));  // t is reused
+        }
+    }
     int r0 = L.iperm[0];
     HIP_TRY(L.ref0.upload(&r0, 1, g_stream));
     for (auto &f : L.f) {
@@ -1090,6 +1100,34 @@ int mgm_spmv(mgm_problem *pb, int lev, int which_matrix, int in_field, int out_f
     Level &L = pb->lv[lev];
     if (!L.slab[which_matrix].p) MGM_FAIL(-1, "spmv: matrix not built");
     OP_SYNC(launch_spmv(L.op(which_matrix), 0, L.N, L.f[in_field].p, L.f[out_field].p, g_stream));
+}
+
+int mgm_spmv_rows(int n_rows, int n_cols, const double *D, const int32_t *cloud, int ncol,
+                  const double *f, double *out)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (n_rows <= 0 || n_cols <= 0 || ncol <= 0 || !D || !cloud || !f || !out) MGM_FAIL(-1, "spmv_rows: bad arguments");
+    std::vector<double> w((size_t)n_rows * n_cols);
+    std::vector<int> c((size_t)n_rows * n_cols);
+    for (int i = 0; i < n_rows; i++)
+        for (int j = 0; j < n_cols; j++) {
+            int v = cloud[(size_t)i * n_cols + j];
+            if (v < 0 || v >= ncol) MGM_FAIL(-1, "spmv_rows: column index out of range");
+            w[(size_t)j * n_rows + i] = D[(size_t)i * n_cols + j];
+            c[(size_t)j * n_rows + i] = v;
+        }
+    DevBuf<double> dw, dx, dy;
+    DevBuf<int> dc;
+    HIP_TRY(dw.upload(w, g_stream));
+    HIP_TRY(dc.upload(c, g_stream));
+    HIP_TRY(dx.upload(f, ncol, g_stream));
+    HIP_TRY(dy.alloc(n_rows));
+    rc = launch_spmv(Ell{dc.p, dw.p, n_cols, n_rows}, 0, n_rows, dx.p, dy.p, g_stream);
+    if (rc) return rc;
+    HIP_TRY(dy.download(out, n_rows, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    return 0;
 }
 
 int mgm_l2_norm_gen(const mgm_problem *pbc, int lev, int fa, int fb, double *out)

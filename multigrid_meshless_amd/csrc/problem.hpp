@@ -42,6 +42,7 @@ struct Level {
     DevBuf<double> Prows;                      // this level as fine:   N * m_{l+1}
     // ---- device, device order ----
     DevBuf<uint8_t> dbtag;
+    DevBuf<double> dnx, dny, dnz;              // unit normals, device order
     DevBuf<int> col;                           // m*N
     std::array<DevBuf<double>, NUM_OPS> slab;  // m*N
     DevBuf<int> rcol, pcol;
@@ -65,5 +66,7 @@ struct mgm_problem {
     double timings[6] = {0, 0, 0, 0, 0, 0};
     mgm::DevBuf<double> partial, scal;
     double *host_scal = nullptr;   // pinned
+    mgm_ns_params ns{};
+    bool ns_set = false;
     ~mgm_problem();
 };

@@ -336,6 +336,12 @@ int launch_sqdiff_sum(int N, const uint8_t *btag, const double *a, const double 
     HIP_TRY(hipGetLastError());
     return 0;
 }
+int launch_sum_partials(int n, const double *partial, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(sum_partial_kernel, dim3(1), dim3(BLK), 0, s, n, partial, out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
 int launch_gather_rows(int N, int m, const double *rows, const int *perm, const int *iperm,
                        const int *cloud, double *w_out, int *col_out, hipStream_t s)
 {
