@@ -126,6 +126,10 @@ int mgm_get_field(const mgm_problem *p, int lev, int which, double *out);
 int mgm_set_field(mgm_problem *p, int lev, int which, const double *in);
 int mgm_zero_fields(mgm_problem *p);
 
+/* Smoother parameters used by subsequent relaxations (parameters.omega and
+ * parameters.num_relax in the reference, header_files/structures.h:26,31). */
+int mgm_set_smoother(mgm_problem *p, float omega, int num_relax);
+
 /* ---- problem: operators (reference variant semantics) --------------------- */
 int mgm_relax(mgm_problem *p, int lev, int variant);
 int mgm_residual(mgm_problem *p, int lev, int variant);

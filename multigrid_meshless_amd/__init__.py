@@ -190,6 +190,9 @@ class Problem:
     def zero_fields(self):
         check(lib().mgm_zero_fields(self._h))
 
+    def set_smoother(self, omega: float, num_relax: int):
+        check(lib().mgm_set_smoother(self._h, omega, num_relax), "set_smoother")
+
     # ---- operators (reference variant semantics) ----
     def relax(self, lev, variant=VAR_H):
         check(lib().mgm_relax(self._h, lev, variant), "relax")

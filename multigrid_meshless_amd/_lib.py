@@ -59,6 +59,7 @@ SIGNATURES = [
     ("mgm_get_field", _i, [_vp, _i, _i, _vp]),
     ("mgm_set_field", _i, [_vp, _i, _i, _vp]),
     ("mgm_zero_fields", _i, [_vp]),
+    ("mgm_set_smoother", _i, [_vp, _f, _i]),
     ("mgm_relax", _i, [_vp, _i, _i]),
     ("mgm_residual", _i, [_vp, _i, _i]),
     ("mgm_restrict", _i, [_vp, _i, _i]),

@@ -114,11 +114,22 @@ struct WeightsArgs {
 int launch_weights(const WeightsArgs &a, hipStream_t s);
 
 // ---- stencil kernels (stencil.hip) ----------------------------------------
-struct Ell {          // fixed-width rows, column-major: entry j of row r at [j*ld + r]
+// Fixed-width rows in sliced-ELL layout: rows are grouped in slices of S
+// consecutive rows (S = 8 or 16), a slice stores its m columns one after the
+// other, each column holding the slice's S entries contiguously.  Entry j of
+// row r lives at ell_at(m, S, r, j).  A wave covers one slice with 64/S lanes
+// per row, so every load instruction of the wave reads 64 consecutive values.
+struct Ell {
     const int *col;
     const double *w;
-    int m, ld;
+    int m, S;
 };
+__host__ __device__ inline size_t ell_at(int m, int S, int r, int j)
+{
+    return (size_t)(r / S) * (size_t)S * (size_t)m + (size_t)j * S + (size_t)(r % S);
+}
+inline int slice_height(int m) { return m > 36 ? 8 : 16; }
+inline size_t ell_size(int N, int m, int S) { return (size_t)((N + S - 1) / S) * S * (size_t)m; }
 int launch_spmv(Ell A, int r0, int r1, const double *x, double *y, hipStream_t s);
 int launch_sor_rows(Ell A, int r0, int r1, const double *src, double *p, double w, double c,
                     hipStream_t s);
@@ -140,9 +151,10 @@ int launch_pin(int N, const int *ref0_dev, double *p, double *scratch, hipStream
 // sum over interior rows of (a-b)^2 into *out (device scalar); b nullable
 int launch_sqdiff_sum(int N, const uint8_t *btag, const double *a, const double *b,
                       double *partial, double *out, hipStream_t s);
-int launch_gather_rows(int N, int m, const double *rows, const int *perm, const int *iperm_cols,
-                       const int *cloud, double *w_out, int *col_out, hipStream_t s);
-int launch_transfer_slab(int Nt, int m, const double *rows, const int *perm, const int *pts,
+int launch_gather_rows(int N, int m, int S, const double *rows, const int *perm,
+                       const int *iperm_cols, const int *cloud, double *w_out, int *col_out,
+                       hipStream_t s);
+int launch_transfer_slab(int Nt, int m, int S, const double *rows, const int *perm, const int *pts,
                          const int *cloud_src, const int *iperm_src, double *w_out, int *col_out,
                          hipStream_t s);
 int launch_lap_poisson(int N, int m, int dim, const double *Dx, const double *Dy, const double *Dz,

@@ -124,64 +124,65 @@ __device__ inline void momentum_node(const NsDev &d, int r)
 {
     // calculate_intermediate_velocity_implicit, one node (hf/timple_solver.h:92-160)
     const int m = d.m;
-    const size_t ld = d.ld;
+    const int S = d.S;
     const double rho = d.rho, mu = d.mu, dt = d.dt, facRe = d.facRe, facdt = d.facdt;
+    const size_t q0 = ell_at(m, S, r, 0);
     double t1 = 0, t2 = 0, t3 = 0, t4 = 0;
     // u
-    double adx = d.u[r] * d.Dx[r], ady = d.v[r] * d.Dy[r];
-    double diff = -facRe * mu * d.lap[r];
+    double adx = d.u[r] * d.Dx[q0], ady = d.v[r] * d.Dy[q0];
+    double diff = -facRe * mu * d.lap[q0];
     double uns = 1. / (dt * facdt);
     double den = rho * (adx + ady + uns) + diff;
     for (int j = 1; j < m; j++) {
-        size_t q = (size_t)j * ld + r;
+        size_t q = ell_at(m, S, r, j);
         double x = d.un[d.col[q]];
         t1 += d.Dx[q] * x;
         t2 += d.Dy[q] * x;
         t4 += d.lap[q] * x;
     }
     if (d.dim == 3) {
-        den += rho * d.w[r] * d.Dz[r];
+        den += rho * d.w[r] * d.Dz[q0];
         for (int j = 1; j < m; j++) {
-            size_t q = (size_t)j * ld + r;
+            size_t q = ell_at(m, S, r, j);
             t3 += d.Dz[q] * d.un[d.col[q]];
         }
     }
     double adv = d.u[r] * t1 + d.v[r] * t2 + d.w[r] * t3;
     uns = d.uo[r] / (dt * facdt);
-    d.un[r] = (rho * (uns - adv) - (facRe - 1) * mu * d.lap[r] * d.un[r] + mu * t4 - d.dpdx[r]) / den;
+    d.un[r] = (rho * (uns - adv) - (facRe - 1) * mu * d.lap[q0] * d.un[r] + mu * t4 - d.dpdx[r]) / den;
     // v
     t1 = 0; t2 = 0; t3 = 0; t4 = 0;
-    adx = d.u[r] * d.Dx[r]; ady = d.v[r] * d.Dy[r];
-    diff = -facRe * mu * d.lap[r];
+    adx = d.u[r] * d.Dx[q0]; ady = d.v[r] * d.Dy[q0];
+    diff = -facRe * mu * d.lap[q0];
     uns = 1. / (dt * facdt);
     den = rho * (adx + ady + uns) + diff;
     for (int j = 1; j < m; j++) {
-        size_t q = (size_t)j * ld + r;
+        size_t q = ell_at(m, S, r, j);
         double x = d.vn[d.col[q]];
         t1 += d.Dx[q] * x;
         t2 += d.Dy[q] * x;
         t4 += d.lap[q] * x;
     }
     if (d.dim == 3) {
-        den += rho * d.w[r] * d.Dz[r];
+        den += rho * d.w[r] * d.Dz[q0];
         for (int j = 1; j < m; j++) {
-            size_t q = (size_t)j * ld + r;
+            size_t q = ell_at(m, S, r, j);
             t3 += d.Dz[q] * d.vn[d.col[q]];
         }
     }
     adv = d.u[r] * t1 + d.v[r] * t2 + d.w[r] * t3;
     uns = d.vo[r] / (dt * facdt);
-    d.vn[r] = (rho * (uns - adv) - (facRe - 1) * mu * d.lap[r] * d.vn[r] + mu * t4 - d.dpdy[r]) / den;
+    d.vn[r] = (rho * (uns - adv) - (facRe - 1) * mu * d.lap[q0] * d.vn[r] + mu * t4 - d.dpdy[r]) / den;
     if (d.dim == 3) {
         // w
         t1 = 0; t2 = 0; t3 = 0; t4 = 0;
-        adx = d.u[r] * d.Dx[r]; ady = d.v[r] * d.Dy[r];
-        double adz = d.w[r] * d.Dz[r];
-        diff = -facRe * mu * d.lap[r];
+        adx = d.u[r] * d.Dx[q0]; ady = d.v[r] * d.Dy[q0];
+        double adz = d.w[r] * d.Dz[q0];
+        diff = -facRe * mu * d.lap[q0];
         uns = 1. / (dt * facdt);
         den = rho * (adx + ady + adz + uns) + diff;
         for (int j = 1; j < m; j++) {
-            size_t q = (size_t)j * ld + r;
+            size_t q = ell_at(m, S, r, j);
             double x = d.wn[d.col[q]];
             t1 += d.Dx[q] * x;
             t2 += d.Dy[q] * x;
@@ -190,7 +191,7 @@ __device__ inline void momentum_node(const NsDev &d, int r)
         }
         adv = d.u[r] * t1 + d.v[r] * t2 + d.w[r] * t3;
         uns = d.wo[r] / (dt * facdt);
-        d.wn[r] = (rho * (uns - adv) - (facRe - 1) * mu * d.lap[r] * d.wn[r] + mu * t4 - d.dpdz[r]) / den;
+        d.wn[r] = (rho * (uns - adv) - (facRe - 1) * mu * d.lap[q0] * d.wn[r] + mu * t4 - d.dpdz[r]) / den;
     }
 }
 
@@ -219,11 +220,12 @@ __global__ void __launch_bounds__(BLK) timple_boundary_pressure_kernel(NsDev d, 
     int r = blockIdx.x * BLK + threadIdx.x;
     if (r >= N || !d.btag[r]) return;
     const int m = d.m;
-    const size_t ld = d.ld;
+    const int S = d.S;
     const double nu = d.mu / d.rho;
+    const size_t q0 = ell_at(m, S, r, 0);
     double t1 = 0, t2 = 0, t3 = 0;
     {
-        size_t q = (size_t)(m - 1) * ld + r;
+        size_t q = ell_at(m, S, r, m - 1);
         int c = d.col[q];
         t1 = nu * d.lap[q] * d.un[c];
         t1 -= d.u[r] * d.un[c] * d.Dx[q];
@@ -244,15 +246,15 @@ __global__ void __launch_bounds__(BLK) timple_boundary_pressure_kernel(NsDev d, 
     d.dpdn[r] = dn;
     double sx = 0, sy = 0, sz = 0, Ap = 0.0;
     for (int j = 1; j < m; j++) {
-        size_t q = (size_t)j * ld + r;
+        size_t q = ell_at(m, S, r, j);
         double pj = d.pold[d.col[q]];
         sx += d.Dx[q] * pj;
         sy += d.Dy[q] * pj;
         if (d.dim == 3) sz += d.Dz[q] * pj;
     }
-    Ap += d.Dx[r] * d.nx[r];
-    Ap += d.Dy[r] * d.ny[r];
-    if (d.dim == 3) Ap += d.Dz[r] * d.nz[r];
+    Ap += d.Dx[q0] * d.nx[r];
+    Ap += d.Dy[q0] * d.ny[r];
+    if (d.dim == 3) Ap += d.Dz[q0] * d.nz[r];
     d.pold[r] = (dn - sx * d.nx[r] - sy * d.ny[r] - sz * d.nz[r]) / Ap;
 }
 

@@ -6,10 +6,9 @@
 namespace mgm {
 
 // Device view of level 0 for the momentum / boundary-pressure kernels
-// (device node order, ELL slabs with leading dimension ld).
+// (device node order, sliced-ELL slabs of slice height S).
 struct NsDev {
-    int m, dim;
-    size_t ld;
+    int m, dim, S;
     const int *col;
     const double *Dx, *Dy, *Dz, *lap;
     const uint8_t *btag;

@@ -22,7 +22,7 @@ NsDev ns_view(mgm_problem *pb)
     NsDev d{};
     d.m = L.m;
     d.dim = pb->cfg.dimension;
-    d.ld = (size_t)L.N;
+    d.S = L.S;
     d.col = L.col.p;
     d.Dx = L.slab[OP_DX].p; d.Dy = L.slab[OP_DY].p; d.Dz = L.slab[OP_DZ].p; d.lap = L.slab[OP_LAP].p;
     d.btag = L.dbtag.p;

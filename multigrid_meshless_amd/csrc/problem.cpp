@@ -550,16 +550,28 @@ static int level_slab(mgm_problem *pb, int l, int which)
 {
     Level &L = pb->lv[l];
     if (!L.rows[which].p) return 0;
-    HIP_TRY(L.slab[which].alloc((size_t)L.N * L.m));
-    return launch_gather_rows(L.N, L.m, L.rows[which].p, L.d_perm.p, L.d_iperm.p, L.rcloud.p,
+    size_t sz = ell_size(L.N, L.m, L.S);
+    if (L.slab[which].n != sz) {
+        HIP_TRY(L.slab[which].alloc(sz));
+        HIP_TRY(hipMemsetAsync(L.slab[which].p, 0, sizeof(double) * sz, g_stream));
+    }
+    return launch_gather_rows(L.N, L.m, L.S, L.rows[which].p, L.d_perm.p, L.d_iperm.p, L.rcloud.p,
                               L.slab[which].p, nullptr, g_stream);
 }
 
+// (Re)build every slab of level l: stencil columns, the five operators and
+// the transfer rows living on this level.  Padding rows of the last slice
+// hold weight 0 and column 0.
 static int level_slabs(mgm_problem *pb, int l)
 {
     Level &L = pb->lv[l];
-    HIP_TRY(L.col.alloc((size_t)L.N * L.m));
-    int rc = launch_gather_rows(L.N, L.m, L.rows[OP_LAP].p, L.d_perm.p, L.d_iperm.p, L.rcloud.p,
+    L.S = slice_height(L.m);
+    size_t sz = ell_size(L.N, L.m, L.S);
+    if (L.col.n != sz) {
+        HIP_TRY(L.col.alloc(sz));
+        HIP_TRY(hipMemsetAsync(L.col.p, 0, sizeof(int) * sz, g_stream));
+    }
+    int rc = launch_gather_rows(L.N, L.m, L.S, L.rows[OP_LAP].p, L.d_perm.p, L.d_iperm.p, L.rcloud.p,
                                 nullptr, L.col.p, g_stream);
     if (rc) return rc;
     for (int o = 0; o < NUM_OPS; o++)
@@ -567,16 +579,28 @@ static int level_slabs(mgm_problem *pb, int l)
     if (l > 0) {
         Level &F = pb->lv[l - 1];
         L.mr = F.m;
-        HIP_TRY(L.rcol.alloc((size_t)L.N * L.mr)); HIP_TRY(L.rw.alloc((size_t)L.N * L.mr));
-        rc = launch_transfer_slab(L.N, L.mr, L.Rrows.p, L.d_perm.p, L.d_rp.p, F.rcloud.p, F.d_iperm.p,
+        L.Sr = slice_height(L.mr);
+        size_t n = ell_size(L.N, L.mr, L.Sr);
+        if (L.rcol.n != n) {
+            HIP_TRY(L.rcol.alloc(n)); HIP_TRY(L.rw.alloc(n));
+            HIP_TRY(hipMemsetAsync(L.rcol.p, 0, sizeof(int) * n, g_stream));
+            HIP_TRY(hipMemsetAsync(L.rw.p, 0, sizeof(double) * n, g_stream));
+        }
+        rc = launch_transfer_slab(L.N, L.mr, L.Sr, L.Rrows.p, L.d_perm.p, L.d_rp.p, F.rcloud.p, F.d_iperm.p,
                                   L.rw.p, L.rcol.p, g_stream);
         if (rc) return rc;
     }
     if (l + 1 < pb->L) {
         Level &C = pb->lv[l + 1];
         L.mp = C.m;
-        HIP_TRY(L.pcol.alloc((size_t)L.N * L.mp)); HIP_TRY(L.pw.alloc((size_t)L.N * L.mp));
-        rc = launch_transfer_slab(L.N, L.mp, L.Prows.p, L.d_perm.p, L.d_pp.p, C.rcloud.p, C.d_iperm.p,
+        L.Sp = slice_height(L.mp);
+        size_t n = ell_size(L.N, L.mp, L.Sp);
+        if (L.pcol.n != n) {
+            HIP_TRY(L.pcol.alloc(n)); HIP_TRY(L.pw.alloc(n));
+            HIP_TRY(hipMemsetAsync(L.pcol.p, 0, sizeof(int) * n, g_stream));
+            HIP_TRY(hipMemsetAsync(L.pw.p, 0, sizeof(double) * n, g_stream));
+        }
+        rc = launch_transfer_slab(L.N, L.mp, L.Sp, L.Prows.p, L.d_perm.p, L.d_pp.p, C.rcloud.p, C.d_iperm.p,
                                   L.pw.p, L.pcol.p, g_stream);
         if (rc) return rc;
     }
@@ -956,6 +980,16 @@ int mgm_set_field(mgm_problem *pb, int lev, int which, const double *in)
     return 0;
 }
 
+int mgm_set_smoother(mgm_problem *pb, float omega, int num_relax)
+{
+    if (!pb) MGM_FAIL(-1, "null problem");
+    if (!(omega > 0.0f && omega < 2.0f)) MGM_FAIL(-1, "omega must be in (0, 2)");
+    if (num_relax < 0 || num_relax > 64) MGM_FAIL(-1, "num_relax must be in [0, 64]");
+    pb->cfg.omega = omega;
+    pb->cfg.num_relax = num_relax;
+    return 0;
+}
+
 int mgm_zero_fields(mgm_problem *pb)
 {
     CHECK_READY(pb, 0);
@@ -1108,14 +1142,16 @@ int mgm_spmv_rows(int n_rows, int n_cols, const double *D, const int32_t *cloud,
     int rc = ensure_init();
     if (rc) return rc;
     if (n_rows <= 0 || n_cols <= 0 || ncol <= 0 || !D || !cloud || !f || !out) MGM_FAIL(-1, "spmv_rows: bad arguments");
-    std::vector<double> w((size_t)n_rows * n_cols);
-    std::vector<int> c((size_t)n_rows * n_cols);
+    const int S = slice_height(n_cols);
+    size_t sz = ell_size(n_rows, n_cols, S);
+    std::vector<double> w(sz, 0.0);
+    std::vector<int> c(sz, 0);
     for (int i = 0; i < n_rows; i++)
         for (int j = 0; j < n_cols; j++) {
             int v = cloud[(size_t)i * n_cols + j];
             if (v < 0 || v >= ncol) MGM_FAIL(-1, "spmv_rows: column index out of range");
-            w[(size_t)j * n_rows + i] = D[(size_t)i * n_cols + j];
-            c[(size_t)j * n_rows + i] = v;
+            w[ell_at(n_cols, S, i, j)] = D[(size_t)i * n_cols + j];
+            c[ell_at(n_cols, S, i, j)] = v;
         }
     DevBuf<double> dw, dx, dy;
     DevBuf<int> dc;
@@ -1123,7 +1159,7 @@ int mgm_spmv_rows(int n_rows, int n_cols, const double *D, const int32_t *cloud,
     HIP_TRY(dc.upload(c, g_stream));
     HIP_TRY(dx.upload(f, ncol, g_stream));
     HIP_TRY(dy.alloc(n_rows));
-    rc = launch_spmv(Ell{dc.p, dw.p, n_cols, n_rows}, 0, n_rows, dx.p, dy.p, g_stream);
+    rc = launch_spmv(Ell{dc.p, dw.p, n_cols, S}, 0, n_rows, dx.p, dy.p, g_stream);
     if (rc) return rc;
     HIP_TRY(dy.download(out, n_rows, g_stream));
     HIP_TRY(hipStreamSynchronize(g_stream));

@@ -48,12 +48,13 @@ struct Level {
     DevBuf<int> rcol, pcol;
     DevBuf<double> rw, pw;
     int mr = 0, mp = 0;                        // widths of restriction / prolongation rows
+    int S = 8, Sr = 8, Sp = 8;                 // slice heights of the operator / transfer slabs
     std::array<DevBuf<double>, MGM_NUM_FIELDS> f;
     DevBuf<int> ref0;                          // device row of reference node 0
 
-    Ell op(int which) const { return Ell{col.p, slab[which].p, m, N}; }
-    Ell restr() const { return Ell{rcol.p, rw.p, mr, N}; }
-    Ell prol() const { return Ell{pcol.p, pw.p, mp, N}; }
+    Ell op(int which) const { return Ell{col.p, slab[which].p, m, S}; }
+    Ell restr() const { return Ell{rcol.p, rw.p, mr, Sr}; }
+    Ell prol() const { return Ell{pcol.p, pw.p, mp, Sp}; }
 };
 
 }  // namespace mgm

@@ -8,6 +8,7 @@
 // and replayed.
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "problem.hpp"
@@ -70,35 +71,46 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
     if ((rc = read_scalar(pb, &rs))) return rc;
     if (history) history[0] = sqrt(rs / L0.N) / fnorm;
 
-    // capture one cycle (V-cycle + fine residual + norm) into a graph
+    // capture one cycle (V-cycle + fine residual + norm) into a graph, unless
+    // MGM_GRAPH=0 asks for plain stream launches (profilers that do not follow
+    // graph launches)
+    const char *ge = getenv("MGM_GRAPH");
+    const bool use_graph = !(ge && ge[0] == '0');
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    HIP_TRY(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-    rc = op_vcycle(pb, variant);
-    if (!rc) rc = enqueue_fine_residual(pb, variant);
-    hipError_t ce = hipStreamEndCapture(stream(), &graph);
-    if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
-    if (ce != hipSuccess) MGM_FAIL(-10, std::string("graph capture failed: ") + hipGetErrorString(ce));
-    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ie != hipSuccess) {
-        (void)hipGraphDestroy(graph);
-        MGM_FAIL(-10, std::string("graph instantiate failed: ") + hipGetErrorString(ie));
+    if (use_graph) {
+        HIP_TRY(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+        rc = op_vcycle(pb, variant);
+        if (!rc) rc = enqueue_fine_residual(pb, variant);
+        hipError_t ce = hipStreamEndCapture(stream(), &graph);
+        if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+        if (ce != hipSuccess) MGM_FAIL(-10, std::string("graph capture failed: ") + hipGetErrorString(ce));
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            MGM_FAIL(-10, std::string("graph instantiate failed: ") + hipGetErrorString(ie));
+        }
     }
     HIP_TRY(hipStreamSynchronize(stream()));
     double t0 = wall();
     int k = 0;
     double ratio = history ? history[0] : 1.0;
     for (k = 1; k <= max_cycles; k++) {
-        hipError_t le = hipGraphLaunch(exec, stream());
-        if (le != hipSuccess) { rc = -10; set_error(hipGetErrorString(le)); break; }
+        if (use_graph) {
+            hipError_t le = hipGraphLaunch(exec, stream());
+            if (le != hipSuccess) { rc = -10; set_error(hipGetErrorString(le)); break; }
+        } else {
+            if ((rc = op_vcycle(pb, variant))) break;
+            if ((rc = enqueue_fine_residual(pb, variant))) break;
+        }
         if ((rc = read_scalar(pb, &rs))) break;
         ratio = sqrt(rs / L0.N) / fnorm;
         if (history) history[k] = ratio;
         if (!(ratio > tol)) break;  // also stops on NaN
     }
     double t1 = wall();
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (rc) return rc;
     if (cycles_done) *cycles_done = (k > max_cycles) ? max_cycles : k;
     if (seconds) *seconds = t1 - t0;

@@ -1,15 +1,21 @@
-// stencil.hip -- FP64 fixed-width (ELL) stencil kernels of the V-cycle.
+// stencil.hip -- FP64 fixed-width (sliced-ELL) stencil kernels of the V-cycle.
 //
-// Every operator row has exactly m entries stored column-major over the
-// level's device node order (entry j of row r at [j*ld + r]), so the 64 lanes
-// of a wave read 64 consecutive weights and indices per entry.  Device order
-// groups nodes by colour and, within a colour, by Morton code, so the rows of
-// one SOR colour are contiguous and neighbour gathers stay local.
+// Layout: rows in slices of S consecutive device rows; within a slice the m
+// columns follow each other and each column holds the slice's S entries
+// contiguously (ell_at in device.hpp).  Device order groups nodes by colour
+// and, within a colour, by Morton code, so the rows of one SOR colour are
+// contiguous and neighbour gathers stay local.
 //
-// Arithmetic is the reference's, operation for operation: sums start from 0.0
-// and add entries in stencil order j = 0..m-1, no FMA contraction, IEEE
-// division.  With the same operators and the same sweep order the results are
-// bitwise those of the reference loops cited at each kernel.
+// Work split: one wave per slice, 64/S lanes per row.  At step t the lanes of
+// a wave load entries j = t*(64/S) + lane/S of the S rows -- 64 consecutive
+// weights and 64 consecutive column indices -- and gather the corresponding
+// x values, so a row's m loads are in flight together instead of forming one
+// lane's dependent chain.  Each product w*x is rounded exactly as the
+// reference rounds it and staged in LDS; one lane per row then adds the
+// products in stencil order j = 0..m-1 starting from 0.0 (or subtracts them
+// in order, for the residual forms that do).  No FMA contraction, IEEE
+// division: with the same operators and sweep order the results are bitwise
+// those of the reference loops cited at each mode.
 #pragma clang fp contract(off)
 
 #include "device.hpp"
@@ -19,37 +25,103 @@ namespace mgm {
 namespace {
 
 constexpr int BLK = 256;
+constexpr int WAVES = BLK / 64;
 
 inline int nblk(long n) { return (int)((n + BLK - 1) / BLK); }
 
-// multiply_sparse_matrix_vector(_gpu) (hf/mat_lib.h:70-89)
-__global__ void __launch_bounds__(BLK)
-spmv_kernel(Ell A, int r0, int r1, const double *__restrict__ x, double *__restrict__ y)
+enum Mode {
+    M_SPMV = 0,      // y = sum_j w x                 hf/mat_lib.h:70-89
+    M_SOR = 1,       // one SOR update                 hf/heat_conduction.h:29-34
+    M_RES0 = 2,      // interior s - sum, bnd 0        hf/heat_conduction.h:101-112, timple_solver.h:274-290
+    M_RES1 = 3,      // interior s -= in order, bnd 0  hf/timple_solver.h:254-266
+    M_RES2 = 4,      // all rows s -= in order         hf/fractional_step_solver.h:185-191
+    M_RES3 = 5,      // interior s - sum, bnd kept     hf/fractional_step_solver.h:236-246
+    M_RESTRICT = 6,  // interior sum, bnd 0 or kept    hf/heat_conduction.h:58-77 (+T, F)
+    M_PROLONG = 7,   // interior p += sum              hf/heat_conduction.h:82-92
+};
+
+__device__ inline bool skips_boundary(int mode)
 {
-    int r = r0 + blockIdx.x * BLK + threadIdx.x;
-    if (r >= r1) return;
-    double acc = 0.0;
-    for (int j = 0; j < A.m; j++) {
-        size_t q = (size_t)j * A.ld + r;
-        acc += A.w[q] * x[A.col[q]];
-    }
-    y[r] = acc;
+    return mode == M_RES0 || mode == M_RES1 || mode == M_RES3 || mode == M_RESTRICT || mode == M_PROLONG;
 }
 
-// One SOR update of rows [r0, r1) (one colour class): the loop body of
-// relaxation (hf/heat_conduction.h:29-34, hf/timple_solver.h:240-247,
-// hf/fractional_step_solver.h:172-179).  Rows of a class never read each other.
+template <int S>
 __global__ void __launch_bounds__(BLK)
-sor_rows_kernel(Ell A, int r0, int r1, const double *__restrict__ src, double *p, double w, double c)
+rows_kernel(Ell A, int r0, int r1, int mode, const uint8_t *__restrict__ btag,
+            const double *__restrict__ src, const double *x, double *y, double w, double c,
+            int keep_boundary)
 {
-    int r = r0 + blockIdx.x * BLK + threadIdx.x;
-    if (r >= r1) return;
-    double sum = 0.0;
-    for (int j = 1; j < A.m; j++) {
-        size_t q = (size_t)j * A.ld + r;
-        sum += A.w[q] * p[A.col[q]];
+    extern __shared__ double prod[];          // [WAVES][m][S]
+    constexpr int LPR = 64 / S;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ri = lane % S, jo = lane / S;
+    const int m = A.m;
+    double *P = prod + (size_t)wave * m * S;
+    const long slice = (long)(r0 / S) + (long)blockIdx.x * WAVES + wave;
+    const long r = slice * S + ri;
+    const bool act = (r >= r0 && r < r1);
+    const bool bnd = act && btag && btag[r];
+    const bool need = act && !(bnd && skips_boundary(mode));
+    const size_t base = (size_t)slice * S * m;
+    if (need) {
+#pragma unroll 4
+        for (int j = jo; j < m; j += LPR) {
+            size_t q = base + (size_t)j * S + ri;
+            P[j * S + ri] = A.w[q] * x[A.col[q]];
+        }
     }
-    p[r] = w * ((src[r] - sum) / A.w[r]) + c * p[r];
+    __syncthreads();
+    if (jo != 0 || !act) return;
+    switch (mode) {
+    case M_SPMV: {
+        double acc = 0.0;
+        for (int j = 0; j < m; j++) acc += P[j * S + ri];
+        y[r] = acc;
+        break;
+    }
+    case M_SOR: {
+        double sum = 0.0;
+        for (int j = 1; j < m; j++) sum += P[j * S + ri];
+        y[r] = w * ((src[r] - sum) / A.w[base + ri]) + c * y[r];
+        break;
+    }
+    case M_RES0:
+    case M_RES3: {
+        if (bnd) {
+            if (mode == M_RES0) y[r] = 0.0;
+            break;
+        }
+        double sum = 0;
+        for (int j = 0; j < m; j++) sum += P[j * S + ri];
+        y[r] = src[r] - sum;
+        break;
+    }
+    case M_RES1:
+    case M_RES2: {
+        if (mode == M_RES1 && bnd) { y[r] = 0; break; }
+        double v = src[r];
+        for (int j = 0; j < m; j++) v = v - P[j * S + ri];
+        y[r] = v;
+        break;
+    }
+    case M_RESTRICT: {
+        if (bnd) {
+            if (!keep_boundary) y[r] = 0.0;
+            break;
+        }
+        double acc = 0.0;
+        for (int j = 0; j < m; j++) acc += P[j * S + ri];
+        y[r] = acc;
+        break;
+    }
+    case M_PROLONG: {
+        if (bnd) break;
+        double acc = 0.0;
+        for (int j = 0; j < m; j++) acc += P[j * S + ri];
+        y[r] = y[r] + acc;
+        break;
+    }
+    }
 }
 
 // Sequential sweep in reference node order (the reference's host loop); one
@@ -62,81 +134,11 @@ __global__ void sor_natural_kernel(Ell A, int N, const int *__restrict__ iperm,
         int r = iperm[i];
         double sum = 0.0;
         for (int j = 1; j < A.m; j++) {
-            size_t q = (size_t)j * A.ld + r;
+            size_t q = ell_at(A.m, A.S, r, j);
             sum += A.w[q] * p[A.col[q]];
         }
-        p[r] = w * ((src[r] - sum) / A.w[r]) + c * p[r];
+        p[r] = w * ((src[r] - sum) / A.w[ell_at(A.m, A.S, r, 0)]) + c * p[r];
     }
-}
-
-// calculate_residuals and the residual inside relaxation, four forms.
-__global__ void __launch_bounds__(BLK)
-residual_kernel(Ell A, int N, int form, const uint8_t *__restrict__ btag,
-                const double *__restrict__ src, const double *__restrict__ p, double *__restrict__ res)
-{
-    int r = blockIdx.x * BLK + threadIdx.x;
-    if (r >= N) return;
-    bool bnd = btag[r] != 0;
-    if (form == 0) {            // hf/heat_conduction.h:101-112, hf/timple_solver.h:274-290
-        if (bnd) { res[r] = 0.0; return; }
-        double sum = 0;
-        for (int j = 0; j < A.m; j++) {
-            size_t q = (size_t)j * A.ld + r;
-            sum += A.w[q] * p[A.col[q]];
-        }
-        res[r] = src[r] - sum;
-    } else if (form == 3) {     // hf/fractional_step_solver.h:236-246
-        if (bnd) return;
-        double sum = 0;
-        for (int j = 0; j < A.m; j++) {
-            size_t q = (size_t)j * A.ld + r;
-            sum += A.w[q] * p[A.col[q]];
-        }
-        res[r] = src[r] - sum;
-    } else {                    // form 1: hf/timple_solver.h:254-266; form 2: hf/fractional_step_solver.h:185-191
-        if (form == 1 && bnd) { res[r] = 0; return; }
-        double v = src[r];
-        for (int j = 0; j < A.m; j++) {
-            size_t q = (size_t)j * A.ld + r;
-            v = v - A.w[q] * p[A.col[q]];
-        }
-        res[r] = v;
-    }
-}
-
-// restrict_residuals (hf/heat_conduction.h:58-77; T :292-306; F :198-212);
-// the restriction point's fine cloud is pre-composed into the column indices.
-__global__ void __launch_bounds__(BLK)
-restrict_kernel(Ell R, int N, const uint8_t *__restrict__ btag, int keep_boundary,
-                const double *__restrict__ in, double *__restrict__ out)
-{
-    int r = blockIdx.x * BLK + threadIdx.x;
-    if (r >= N) return;
-    if (btag[r]) {
-        if (!keep_boundary) out[r] = 0.0;
-        return;
-    }
-    double acc = 0.0;
-    for (int j = 0; j < R.m; j++) {
-        size_t q = (size_t)j * R.ld + r;
-        acc += R.w[q] * in[R.col[q]];
-    }
-    out[r] = acc;
-}
-
-// prolongate_corrections, first loop (hf/heat_conduction.h:82-92).
-__global__ void __launch_bounds__(BLK)
-prolong_kernel(Ell P, int N, const uint8_t *__restrict__ btag, const double *__restrict__ pc,
-               double *__restrict__ p)
-{
-    int r = blockIdx.x * BLK + threadIdx.x;
-    if (r >= N || btag[r]) return;
-    double acc = 0.0;
-    for (int j = 0; j < P.m; j++) {
-        size_t q = (size_t)j * P.ld + r;
-        acc += P.w[q] * pc[P.col[q]];
-    }
-    p[r] = p[r] + acc;
 }
 
 __global__ void __launch_bounds__(BLK)
@@ -197,34 +199,36 @@ sum_partial_kernel(int n, const double *__restrict__ partial, double *__restrict
     if (threadIdx.x == 0) *out = sh[0];
 }
 
-// Row-major reference-order rows -> column-major device-order slab.
+// Row-major reference-order rows -> sliced-ELL device-order slab.
 __global__ void __launch_bounds__(BLK)
-gather_rows_kernel(int N, int m, const double *__restrict__ rows, const int *__restrict__ perm,
+gather_rows_kernel(int N, int m, int S, const double *__restrict__ rows, const int *__restrict__ perm,
                    const int *__restrict__ iperm, const int *__restrict__ cloud,
                    double *__restrict__ w_out, int *__restrict__ col_out)
 {
     long t = (long)blockIdx.x * BLK + threadIdx.x;
     if (t >= (long)N * m) return;
-    int dev = (int)(t % N), j = (int)(t / N);
+    int dev = (int)(t / m), j = (int)(t % m);
     int ref = perm[dev];
-    if (w_out) w_out[t] = rows[(size_t)ref * m + j];
-    if (col_out) col_out[t] = iperm[cloud[(size_t)ref * m + j]];
+    size_t o = ell_at(m, S, dev, j);
+    if (w_out) w_out[o] = rows[(size_t)ref * m + j];
+    if (col_out) col_out[o] = iperm[cloud[(size_t)ref * m + j]];
 }
 
 // Transfer rows (reference order, width m of the source level) -> slab over the
 // target level's device order, columns pre-composed through the source cloud.
 __global__ void __launch_bounds__(BLK)
-transfer_slab_kernel(int Nt, int m, const double *__restrict__ rows, const int *__restrict__ perm,
+transfer_slab_kernel(int Nt, int m, int S, const double *__restrict__ rows, const int *__restrict__ perm,
                      const int *__restrict__ pts, const int *__restrict__ cloud_src,
                      const int *__restrict__ iperm_src, double *__restrict__ w_out,
                      int *__restrict__ col_out)
 {
     long t = (long)blockIdx.x * BLK + threadIdx.x;
     if (t >= (long)Nt * m) return;
-    int dev = (int)(t % Nt), j = (int)(t / Nt);
+    int dev = (int)(t / m), j = (int)(t % m);
     int ref = perm[dev];
-    w_out[t] = rows[(size_t)ref * m + j];
-    col_out[t] = iperm_src[cloud_src[(size_t)pts[ref] * m + j]];
+    size_t o = ell_at(m, S, dev, j);
+    w_out[o] = rows[(size_t)ref * m + j];
+    col_out[o] = iperm_src[cloud_src[(size_t)pts[ref] * m + j]];
 }
 
 // create_laplacian_for_Poisson (hf/rbf.h:382-399) on row-major rows.
@@ -267,6 +271,25 @@ scatter_kernel(int N, const int *__restrict__ idx, const double *__restrict__ in
     if (t < N) out[idx[t]] = in[t];
 }
 
+int launch_rows(Ell A, int r0, int r1, int mode, const uint8_t *btag, const double *src,
+                const double *x, double *y, double w, double c, int keep, hipStream_t s)
+{
+    if (r1 <= r0) return 0;
+    long s0 = r0 / A.S, s1 = (r1 - 1) / A.S;
+    int blocks = (int)((s1 - s0 + 1 + WAVES - 1) / WAVES);
+    size_t shm = (size_t)WAVES * A.m * A.S * sizeof(double);
+    if (A.S == 8)
+        hipLaunchKernelGGL(rows_kernel<8>, dim3(blocks), dim3(BLK), shm, s, A, r0, r1, mode, btag, src,
+                           x, y, w, c, keep);
+    else if (A.S == 16)
+        hipLaunchKernelGGL(rows_kernel<16>, dim3(blocks), dim3(BLK), shm, s, A, r0, r1, mode, btag, src,
+                           x, y, w, c, keep);
+    else
+        MGM_FAIL(-1, "unsupported slice height");
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 }  // namespace
 
 #define LAUNCH(kern, n, ...)                                                         \
@@ -279,14 +302,12 @@ scatter_kernel(int N, const int *__restrict__ idx, const double *__restrict__ in
 
 int launch_spmv(Ell A, int r0, int r1, const double *x, double *y, hipStream_t s)
 {
-    LAUNCH(spmv_kernel, r1 - r0, A, r0, r1, x, y);
-    return 0;
+    return launch_rows(A, r0, r1, M_SPMV, nullptr, nullptr, x, y, 0, 0, 0, s);
 }
 int launch_sor_rows(Ell A, int r0, int r1, const double *src, double *p, double w, double c,
                     hipStream_t s)
 {
-    LAUNCH(sor_rows_kernel, r1 - r0, A, r0, r1, src, p, w, c);
-    return 0;
+    return launch_rows(A, r0, r1, M_SOR, nullptr, src, p, p, w, c, 0, s);
 }
 int launch_sor_natural(Ell A, int N, const int *iperm, const double *src, double *p, double w,
                        double c, hipStream_t s)
@@ -298,19 +319,18 @@ int launch_sor_natural(Ell A, int N, const int *iperm, const double *src, double
 int launch_residual(Ell A, int N, int form, const uint8_t *btag, const double *src,
                     const double *p, double *res, hipStream_t s)
 {
-    LAUNCH(residual_kernel, N, A, N, form, btag, src, p, res);
-    return 0;
+    static const int modes[4] = {M_RES0, M_RES1, M_RES2, M_RES3};
+    if (form < 0 || form > 3) MGM_FAIL(-1, "bad residual form");
+    return launch_rows(A, 0, N, modes[form], btag, src, p, res, 0, 0, 0, s);
 }
 int launch_restrict(Ell R, int N, const uint8_t *btag, int keep_boundary, const double *in,
                     double *out, hipStream_t s)
 {
-    LAUNCH(restrict_kernel, N, R, N, btag, keep_boundary, in, out);
-    return 0;
+    return launch_rows(R, 0, N, M_RESTRICT, btag, nullptr, in, out, 0, 0, keep_boundary, s);
 }
 int launch_prolong(Ell P, int N, const uint8_t *btag, const double *pc, double *p, hipStream_t s)
 {
-    LAUNCH(prolong_kernel, N, P, N, btag, pc, p);
-    return 0;
+    return launch_rows(P, 0, N, M_PROLONG, btag, nullptr, pc, p, 0, 0, 0, s);
 }
 int launch_zero(int N, const uint8_t *btag, int interior_only, double *p, hipStream_t s)
 {
@@ -342,17 +362,17 @@ int launch_sum_partials(int n, const double *partial, double *out, hipStream_t s
     HIP_TRY(hipGetLastError());
     return 0;
 }
-int launch_gather_rows(int N, int m, const double *rows, const int *perm, const int *iperm,
+int launch_gather_rows(int N, int m, int S, const double *rows, const int *perm, const int *iperm,
                        const int *cloud, double *w_out, int *col_out, hipStream_t s)
 {
-    LAUNCH(gather_rows_kernel, (long)N * m, N, m, rows, perm, iperm, cloud, w_out, col_out);
+    LAUNCH(gather_rows_kernel, (long)N * m, N, m, S, rows, perm, iperm, cloud, w_out, col_out);
     return 0;
 }
-int launch_transfer_slab(int Nt, int m, const double *rows, const int *perm, const int *pts,
+int launch_transfer_slab(int Nt, int m, int S, const double *rows, const int *perm, const int *pts,
                          const int *cloud_src, const int *iperm_src, double *w_out, int *col_out,
                          hipStream_t s)
 {
-    LAUNCH(transfer_slab_kernel, (long)Nt * m, Nt, m, rows, perm, pts, cloud_src, iperm_src, w_out,
+    LAUNCH(transfer_slab_kernel, (long)Nt * m, Nt, m, S, rows, perm, pts, cloud_src, iperm_src, w_out,
            col_out);
     return 0;
 }
