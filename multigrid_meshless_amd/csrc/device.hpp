@@ -128,11 +128,25 @@ __host__ __device__ inline size_t ell_at(int m, int S, int r, int j)
 {
     return (size_t)(r / S) * (size_t)S * (size_t)m + (size_t)j * S + (size_t)(r % S);
 }
-inline int slice_height(int m) { return m > 36 ? 8 : 16; }
+// Rows per slice for stencils of width m: 8 above 36 columns, else 16.  The
+// environment variable MGM_SLICE_HEIGHT (4, 8 or 16) overrides it.
+int slice_height(int m);
 inline size_t ell_size(int N, int m, int S) { return (size_t)((N + S - 1) / S) * S * (size_t)m; }
-int launch_spmv(Ell A, int r0, int r1, const double *x, double *y, hipStream_t s);
-int launch_sor_rows(Ell A, int r0, int r1, const double *src, double *p, double w, double c,
-                    hipStream_t s);
+// Device row tags (dbtag): 0 interior, 1 boundary, 2 padding row.  Row counts
+// passed to the full-pass launchers are device row counts (padding included).
+enum { ROW_INTERIOR = 0, ROW_BOUNDARY = 1, ROW_PAD = 2 };
+// y = A x over rows [0, nrows); rows tagged ROW_PAD in btag (nullable) are skipped.
+int launch_spmv(Ell A, int nrows, const uint8_t *btag, const double *x, double *y, hipStream_t s);
+// One SOR update of the rows in the given slices (one colour class).
+int launch_sor_slices(Ell A, const int *slices, int nslices, const uint8_t *btag,
+                      const double *src, double *p, double w, double c, hipStream_t s);
+// The whole relaxation of a small level (all sweeps, all colour classes, and
+// for the T / F variants the pin and residual after each sweep, A2 being the
+// operator of the F residual) in one single-workgroup launch.
+int launch_relax_small(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol, int nrows,
+                       const uint8_t *btag, const int *ref0, const double *src, double *p, double *res,
+                       double w, double c, int num_relax, int variant, hipStream_t s);
+bool relax_small_fits(int m, int m2, int S);
 int launch_sor_natural(Ell A, int N, const int *iperm, const double *src, double *p, double w,
                        double c, hipStream_t s);
 // residual forms: 0 = interior, res = s - sum (boundary res = 0)            (H/T)
@@ -147,21 +161,23 @@ int launch_restrict(Ell R, int N, const uint8_t *btag, int keep_boundary, const 
 // prolong: p[r] += sum_j w*pc[col] on interior rows
 int launch_prolong(Ell P, int N, const uint8_t *btag, const double *pc, double *p, hipStream_t s);
 int launch_zero(int N, const uint8_t *btag, int interior_only, double *p, hipStream_t s);
-int launch_pin(int N, const int *ref0_dev, double *p, double *scratch, hipStream_t s);
+int launch_pin(int N, const uint8_t *btag, const int *ref0_dev, double *p, double *scratch,
+               hipStream_t s);
 // sum over interior rows of (a-b)^2 into *out (device scalar); b nullable
 int launch_sqdiff_sum(int N, const uint8_t *btag, const double *a, const double *b,
                       double *partial, double *out, hipStream_t s);
-int launch_gather_rows(int N, int m, int S, const double *rows, const int *perm,
-                       const int *iperm_cols, const int *cloud, double *w_out, int *col_out,
-                       hipStream_t s);
-int launch_transfer_slab(int Nt, int m, int S, const double *rows, const int *perm, const int *pts,
-                         const int *cloud_src, const int *iperm_src, double *w_out, int *col_out,
-                         hipStream_t s);
+// Row-major reference-order rows (N x m) -> slab in device order (iperm:
+// reference -> device row); columns mapped through cloud and iperm.
+int launch_gather_rows(int N, int m, int S, const double *rows, const int *iperm,
+                       const int *cloud, double *w_out, int *col_out, hipStream_t s);
+// Transfer rows of the target level (Nt x m, reference order) -> slab over the
+// target's device order; column j = iperm_src[cloud_src[pts[i]*m + j]].
+int launch_transfer_slab(int Nt, int m, int S, const double *rows, const int *iperm_t,
+                         const int *pts, const int *cloud_src, const int *iperm_src,
+                         double *w_out, int *col_out, hipStream_t s);
 int launch_lap_poisson(int N, int m, int dim, const double *Dx, const double *Dy, const double *Dz,
                        const double *lap, const double *nx, const double *ny, const double *nz,
                        const uint8_t *btag, double *lapP, hipStream_t s);
 int launch_dirichlet_rows(int N, int m, const uint8_t *btag, double *rows, hipStream_t s);
-int launch_permute(int N, const int *idx, const double *in, double *out, hipStream_t s);
-int launch_scatter(int N, const int *idx, const double *in, double *out, hipStream_t s);
 
 }  // namespace mgm

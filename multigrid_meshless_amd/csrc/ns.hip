@@ -41,7 +41,7 @@ mass_source_kernel(int N, const uint8_t *__restrict__ btag, const double *__rest
 {
     int i = blockIdx.x * BLK + threadIdx.x;
     if (i >= N) return;
-    if (!btag[i]) src[i] = rho * (dx[i] + dy[i] + dz[i]) / dt;
+    if (btag[i] == ROW_INTERIOR) src[i] = rho * (dx[i] + dy[i] + dz[i]) / dt;
     else if (zero_boundary) src[i] = 0;
 }
 
@@ -55,7 +55,7 @@ fs_boundary_dpdn_kernel(int N, int dim, const uint8_t *__restrict__ btag, const 
                         double dt, double *__restrict__ dpdn)
 {
     int i = blockIdx.x * BLK + threadIdx.x;
-    if (i >= N || !btag[i]) return;
+    if (i >= N || btag[i] != ROW_BOUNDARY) return;
     double dpdx = (un[i] - u[i]) * rho / dt;
     double dpdy = (vn[i] - v[i]) * rho / dt;
     double r = dpdx * nx[i] + dpdy * ny[i];
@@ -195,10 +195,15 @@ __device__ inline void momentum_node(const NsDev &d, int r)
     }
 }
 
-__global__ void __launch_bounds__(BLK) momentum_rows_kernel(NsDev d, int r0, int r1)
+// Interior rows of the listed slices (one colour class), one thread per row.
+__global__ void __launch_bounds__(BLK) momentum_slices_kernel(NsDev d, const int *__restrict__ slices,
+                                                              int nslices)
 {
-    int r = r0 + blockIdx.x * BLK + threadIdx.x;
-    if (r >= r1 || d.btag[r]) return;
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    long k = t / d.S;
+    if (k >= nslices) return;
+    int r = slices[k] * d.S + (int)(t % d.S);
+    if (d.btag[r] != ROW_INTERIOR) return;
     momentum_node(d, r);
 }
 
@@ -207,7 +212,7 @@ __global__ void momentum_natural_kernel(NsDev d, int N, const int *__restrict__ 
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     for (int i = 0; i < N; i++) {
         int r = iperm[i];
-        if (!d.btag[r]) momentum_node(d, r);
+        if (d.btag[r] == ROW_INTERIOR) momentum_node(d, r);
     }
 }
 
@@ -218,7 +223,7 @@ __global__ void momentum_natural_kernel(NsDev d, int N, const int *__restrict__ 
 __global__ void __launch_bounds__(BLK) timple_boundary_pressure_kernel(NsDev d, int N)
 {
     int r = blockIdx.x * BLK + threadIdx.x;
-    if (r >= N || !d.btag[r]) return;
+    if (r >= N || d.btag[r] != ROW_BOUNDARY) return;
     const int m = d.m;
     const int S = d.S;
     const double nu = d.mu / d.rho;
@@ -320,9 +325,9 @@ int launch_vel_change(int N, int dim, const double *a0, const double *a1, const 
     HIP_TRY(hipGetLastError());
     return launch_sum_partials(nb, partial, out, s);
 }
-int launch_momentum_rows(const NsDev &d, int r0, int r1, hipStream_t s)
+int launch_momentum_slices(const NsDev &d, const int *slices, int nslices, hipStream_t s)
 {
-    LAUNCH(momentum_rows_kernel, r1 - r0, d, r0, r1);
+    LAUNCH(momentum_slices_kernel, (long)nslices * d.S, d, slices, nslices);
     return 0;
 }
 int launch_momentum_natural(const NsDev &d, int N, const int *iperm, hipStream_t s)

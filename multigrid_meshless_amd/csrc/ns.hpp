@@ -39,7 +39,7 @@ int launch_timple_correct(int N, int dim, const uint8_t *btag, const double *dx,
 int launch_vel_change(int N, int dim, const double *a0, const double *a1, const double *b0,
                       const double *b1, const double *c0, const double *c1, double *partial,
                       double *out, hipStream_t s);
-int launch_momentum_rows(const NsDev &d, int r0, int r1, hipStream_t s);
+int launch_momentum_slices(const NsDev &d, const int *slices, int nslices, hipStream_t s);
 int launch_momentum_natural(const NsDev &d, int N, const int *iperm, hipStream_t s);
 int launch_timple_boundary_pressure(const NsDev &d, int N, hipStream_t s);
 

@@ -41,13 +41,13 @@ NsDev ns_view(mgm_problem *pb)
 int spmv(mgm_problem *pb, int op, int in, int out)
 {
     Level &L = pb->lv[0];
-    return launch_spmv(L.op(op), 0, L.N, L.f[in].p, L.f[out].p, stream());
+    return launch_spmv(L.op(op), L.Nd, L.dbtag.p, L.f[in].p, L.f[out].p, stream());
 }
 
 int finish_error(mgm_problem *pb, int a0, int a1, int b0, int b1, int c0, int c1, double *err)
 {
     Level &L = pb->lv[0];
-    int rc = launch_vel_change(L.N, pb->cfg.dimension, L.f[a0].p, L.f[a1].p, L.f[b0].p, L.f[b1].p,
+    int rc = launch_vel_change(L.Nd, pb->cfg.dimension, L.f[a0].p, L.f[a1].p, L.f[b0].p, L.f[b1].p,
                                L.f[c0].p, L.f[c1].p, pb->partial.p, pb->scal.p, stream());
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(pb->host_scal, pb->scal.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
@@ -77,13 +77,13 @@ int mgm_fs_step(mgm_problem *pb, double *steady_error)
     if (!pb->ns_set) MGM_FAIL(-1, "mgm_ns_set_params has not been called");
     Level &L = pb->lv[0];
     hipStream_t s = stream();
-    const int N = L.N, dim = pb->cfg.dimension;
+    const int Nd = L.Nd, dim = pb->cfg.dimension;  // device rows, padding included
     auto F = [&](int k) { return L.f[k].p; };
     const double dt = pb->ns.dt, rho = pb->ns.rho, nu = pb->ns.nu;
     // hf/fractional_step_solver.h:50-55
-    RUN(launch_copy(N, F(MGM_F_U), F(MGM_F_UOLD), s));
-    RUN(launch_copy(N, F(MGM_F_V), F(MGM_F_VOLD), s));
-    if (dim == 3) RUN(launch_copy(N, F(MGM_F_W), F(MGM_F_WOLD), s));
+    RUN(launch_copy(Nd, F(MGM_F_U), F(MGM_F_UOLD), s));
+    RUN(launch_copy(Nd, F(MGM_F_V), F(MGM_F_VOLD), s));
+    if (dim == 3) RUN(launch_copy(Nd, F(MGM_F_W), F(MGM_F_WOLD), s));
     // calculate_intermediate_velocity (:73-114)
     const int comps[3] = {MGM_F_U, MGM_F_V, MGM_F_W}, outs[3] = {MGM_F_UNEW, MGM_F_VNEW, MGM_F_WNEW};
     for (int c = 0; c < dim; c++) {
@@ -91,17 +91,17 @@ int mgm_fs_step(mgm_problem *pb, double *steady_error)
         RUN(spmv(pb, OP_DY, comps[c], MGM_F_DPDY));
         if (dim == 3) RUN(spmv(pb, OP_DZ, comps[c], MGM_F_DPDZ));
         RUN(spmv(pb, OP_LAP, comps[c], MGM_F_DPDN));
-        RUN(launch_fs_predict(N, F(comps[c]), F(MGM_F_U), F(MGM_F_V), F(MGM_F_W), F(MGM_F_DPDX),
+        RUN(launch_fs_predict(Nd, F(comps[c]), F(MGM_F_U), F(MGM_F_V), F(MGM_F_W), F(MGM_F_DPDX),
                               F(MGM_F_DPDY), F(MGM_F_DPDZ), F(MGM_F_DPDN), dt, nu, F(outs[c]), s));
     }
     // calculate_mass_residual (:116-131)
     RUN(spmv(pb, OP_DX, MGM_F_UNEW, MGM_F_DPDX));
     RUN(spmv(pb, OP_DY, MGM_F_VNEW, MGM_F_DPDY));
     if (dim == 3) RUN(spmv(pb, OP_DZ, MGM_F_WNEW, MGM_F_DPDZ));
-    RUN(launch_mass_source(N, L.dbtag.p, F(MGM_F_DPDX), F(MGM_F_DPDY), F(MGM_F_DPDZ), rho, dt, 0,
+    RUN(launch_mass_source(Nd, L.dbtag.p, F(MGM_F_DPDX), F(MGM_F_DPDY), F(MGM_F_DPDZ), rho, dt, 0,
                            F(MGM_F_SOURCE), s));
     // calculate_boundary_dpdn (:133-148)
-    RUN(launch_fs_boundary_dpdn(N, dim, L.dbtag.p, F(MGM_F_U), F(MGM_F_V), F(MGM_F_W), F(MGM_F_UNEW),
+    RUN(launch_fs_boundary_dpdn(Nd, dim, L.dbtag.p, F(MGM_F_U), F(MGM_F_V), F(MGM_F_W), F(MGM_F_UNEW),
                                 F(MGM_F_VNEW), F(MGM_F_WNEW), L.dnx.p, L.dny.p, L.dnz.p, rho,
                                 dt, F(MGM_F_DPDN), s));
     // multigrid_Poisson_solver (:150-166)
@@ -110,7 +110,7 @@ int mgm_fs_step(mgm_problem *pb, double *steady_error)
     RUN(spmv(pb, OP_DX, MGM_F_P, MGM_F_DPDX));
     RUN(spmv(pb, OP_DY, MGM_F_P, MGM_F_DPDY));
     if (dim == 3) RUN(spmv(pb, OP_DZ, MGM_F_P, MGM_F_DPDZ));
-    RUN(launch_fs_correct(N, dim, L.dbtag.p, F(MGM_F_UNEW), F(MGM_F_VNEW), F(MGM_F_WNEW), F(MGM_F_DPDX),
+    RUN(launch_fs_correct(Nd, dim, L.dbtag.p, F(MGM_F_UNEW), F(MGM_F_VNEW), F(MGM_F_WNEW), F(MGM_F_DPDX),
                           F(MGM_F_DPDY), F(MGM_F_DPDZ), dt, rho, F(MGM_F_U), F(MGM_F_V), F(MGM_F_W), s));
     RUN(spmv(pb, OP_DX, MGM_F_U, MGM_F_DPDX));
     RUN(spmv(pb, OP_DY, MGM_F_V, MGM_F_DPDY));
@@ -124,39 +124,40 @@ int mgm_timple_step(mgm_problem *pb, double *steady_error)
     if (!pb->ns_set) MGM_FAIL(-1, "mgm_ns_set_params has not been called");
     Level &L = pb->lv[0];
     hipStream_t s = stream();
-    const int N = L.N, dim = pb->cfg.dimension;
+    const int Nd = L.Nd, dim = pb->cfg.dimension;  // device rows, padding included
     auto F = [&](int k) { return L.f[k].p; };
     const double dt = pb->ns.dt, rho = pb->ns.rho;
     NsDev d = ns_view(pb);
     // hf/timple_solver.h:43-48
-    RUN(launch_copy(N, F(MGM_F_UNEW), F(MGM_F_UOLD), s));
-    RUN(launch_copy(N, F(MGM_F_UNEW), F(MGM_F_U), s));
-    RUN(launch_copy(N, F(MGM_F_VNEW), F(MGM_F_VOLD), s));
-    RUN(launch_copy(N, F(MGM_F_VNEW), F(MGM_F_V), s));
-    RUN(launch_copy(N, F(MGM_F_WNEW), F(MGM_F_WOLD), s));
-    RUN(launch_copy(N, F(MGM_F_WNEW), F(MGM_F_W), s));
-    RUN(launch_zero(N, L.dbtag.p, 0, F(MGM_F_PPRIME), s));
+    RUN(launch_copy(Nd, F(MGM_F_UNEW), F(MGM_F_UOLD), s));
+    RUN(launch_copy(Nd, F(MGM_F_UNEW), F(MGM_F_U), s));
+    RUN(launch_copy(Nd, F(MGM_F_VNEW), F(MGM_F_VOLD), s));
+    RUN(launch_copy(Nd, F(MGM_F_VNEW), F(MGM_F_V), s));
+    RUN(launch_copy(Nd, F(MGM_F_WNEW), F(MGM_F_WOLD), s));
+    RUN(launch_copy(Nd, F(MGM_F_WNEW), F(MGM_F_W), s));
+    RUN(launch_zero(Nd, L.dbtag.p, 0, F(MGM_F_PPRIME), s));
     for (int it = 0; it < pb->ns.iter_timple; it++) {
-        RUN(launch_copy(N, F(MGM_F_UNEW), F(MGM_F_U), s));
-        RUN(launch_copy(N, F(MGM_F_VNEW), F(MGM_F_V), s));
-        RUN(launch_copy(N, F(MGM_F_WNEW), F(MGM_F_W), s));
-        RUN(launch_zero(N, L.dbtag.p, 0, F(MGM_F_PPRIME), s));
+        RUN(launch_copy(Nd, F(MGM_F_UNEW), F(MGM_F_U), s));
+        RUN(launch_copy(Nd, F(MGM_F_VNEW), F(MGM_F_V), s));
+        RUN(launch_copy(Nd, F(MGM_F_WNEW), F(MGM_F_W), s));
+        RUN(launch_zero(Nd, L.dbtag.p, 0, F(MGM_F_PPRIME), s));
         // calculate_intermediate_velocity_implicit (:72-164)
         RUN(spmv(pb, OP_DX, MGM_F_POLD, MGM_F_DPDX));
         RUN(spmv(pb, OP_DY, MGM_F_POLD, MGM_F_DPDY));
         if (dim == 3) RUN(spmv(pb, OP_DZ, MGM_F_POLD, MGM_F_DPDZ));
         for (int k = 0; k < pb->ns.iter_momentum; k++) {
             if (pb->cfg.sweep_order == MGM_ORDER_NATURAL) {
-                RUN(launch_momentum_natural(d, N, L.d_iperm.p, s));
+                RUN(launch_momentum_natural(d, L.N, L.d_iperm.p, s));
             } else {
-                for (int c = 0; c < L.ncol; c++) RUN(launch_momentum_rows(d, L.col_off[c], L.col_off[c + 1], s));
+                for (int c = 0; c < L.ncol; c++)
+                    RUN(launch_momentum_slices(d, L.d_col_slices.p + L.col_soff[c], L.col_soff[c + 1] - L.col_soff[c], s));
             }
         }
         // calculate_mass_residual_implicit (:166-189)
         RUN(spmv(pb, OP_DX, MGM_F_UNEW, MGM_F_DPDX));
         RUN(spmv(pb, OP_DY, MGM_F_VNEW, MGM_F_DPDY));
         if (dim == 3) RUN(spmv(pb, OP_DZ, MGM_F_WNEW, MGM_F_DPDZ));
-        RUN(launch_mass_source(N, L.dbtag.p, F(MGM_F_DPDX), F(MGM_F_DPDY), F(MGM_F_DPDZ), rho, dt, 1,
+        RUN(launch_mass_source(Nd, L.dbtag.p, F(MGM_F_DPDX), F(MGM_F_DPDY), F(MGM_F_DPDZ), rho, dt, 1,
                                F(MGM_F_SOURCE), s));
         // multigrid_Poisson_solver (:191-209)
         for (int k = 0; k < pb->ns.num_vcycles; k++) RUN(op_vcycle(pb, MGM_VAR_T));
@@ -164,12 +165,12 @@ int mgm_timple_step(mgm_problem *pb, double *steady_error)
         RUN(spmv(pb, OP_DX, MGM_F_PPRIME, MGM_F_DPDX));
         RUN(spmv(pb, OP_DY, MGM_F_PPRIME, MGM_F_DPDY));
         if (dim == 3) RUN(spmv(pb, OP_DZ, MGM_F_PPRIME, MGM_F_DPDZ));
-        RUN(launch_timple_correct(N, dim, L.dbtag.p, F(MGM_F_DPDX), F(MGM_F_DPDY), F(MGM_F_DPDZ),
+        RUN(launch_timple_correct(Nd, dim, L.dbtag.p, F(MGM_F_DPDX), F(MGM_F_DPDY), F(MGM_F_DPDZ),
                                   F(MGM_F_PPRIME), dt, rho, F(MGM_F_UNEW), F(MGM_F_VNEW), F(MGM_F_WNEW),
                                   F(MGM_F_POLD), s));
-        RUN(launch_pin(N, L.ref0.p, F(MGM_F_POLD), pb->scal.p + 8, s));
+        RUN(launch_pin(Nd, L.dbtag.p, L.ref0.p, F(MGM_F_POLD), pb->scal.p + 8, s));
         // update_boundary_pressure (:358-410)
-        RUN(launch_timple_boundary_pressure(d, N, s));
+        RUN(launch_timple_boundary_pressure(d, Nd, s));
     }
     // steady-state error (:63-69)
     return finish_error(pb, MGM_F_UOLD, MGM_F_UNEW, MGM_F_VOLD, MGM_F_VNEW, MGM_F_WOLD, MGM_F_WNEW,

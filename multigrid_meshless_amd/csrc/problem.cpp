@@ -396,18 +396,44 @@ static void level_colour_order(Level &L, int dim)
         }
         key[i] = k;
     }
-    L.perm.resize(N);
-    std::iota(L.perm.begin(), L.perm.end(), 0);
-    std::sort(L.perm.begin(), L.perm.end(), [&](int a, int b) {
-        if (L.colour[a] != L.colour[b]) return L.colour[a] < L.colour[b];
-        if (key[a] != key[b]) return key[a] < key[b];
-        return a < b;
+    // Morton rank -> block of kBlock rows; inside a block, colour classes
+    std::vector<int> byk(N);
+    std::iota(byk.begin(), byk.end(), 0);
+    std::sort(byk.begin(), byk.end(), [&](int a, int b) { return key[a] != key[b] ? key[a] < key[b] : a < b; });
+    constexpr int kBlock = 4096;
+    std::vector<int> block(N);
+    for (int r = 0; r < N; r++) block[byk[r]] = r / kBlock;
+    std::vector<int> ord(byk);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+        if (block[a] != block[b]) return block[a] < block[b];
+        return L.colour[a] < L.colour[b];
     });
-    L.iperm.resize(N);
-    for (int d = 0; d < N; d++) L.iperm[L.perm[d]] = d;
-    L.col_off.assign(L.ncol + 1, 0);
-    for (int i = 0; i < N; i++) L.col_off[L.colour[i] + 1]++;
-    for (int c = 0; c < L.ncol; c++) L.col_off[c + 1] += L.col_off[c];
+    L.S = slice_height(L.m);
+    const int S = L.S;
+    std::vector<std::vector<int>> slices(L.ncol);
+    L.perm.clear();
+    L.perm.reserve((size_t)N + (size_t)S * L.ncol * (N / kBlock + 1));
+    L.iperm.assign(N, -1);
+    for (size_t q = 0; q < ord.size();) {
+        size_t e = q;
+        while (e < ord.size() && block[ord[e]] == block[ord[q]] && L.colour[ord[e]] == L.colour[ord[q]]) e++;
+        const int c = L.colour[ord[q]];
+        const int first = (int)L.perm.size();
+        for (size_t t = q; t < e; t++) {
+            L.iperm[ord[t]] = (int)L.perm.size();
+            L.perm.push_back(ord[t]);
+        }
+        while (L.perm.size() % S) L.perm.push_back(-1);
+        for (int s = first / S; s < (int)(L.perm.size() / S); s++) slices[c].push_back(s);
+        q = e;
+    }
+    L.Nd = (int)L.perm.size();
+    L.col_soff.assign(L.ncol + 1, 0);
+    L.col_slices.clear();
+    for (int c = 0; c < L.ncol; c++) {
+        L.col_slices.insert(L.col_slices.end(), slices[c].begin(), slices[c].end());
+        L.col_soff[c + 1] = (int)L.col_slices.size();
+    }
 }
 
 static int upload_level(Level &L)
@@ -416,30 +442,32 @@ static int upload_level(Level &L)
     HIP_TRY(L.rnx.upload(L.nx, g_stream)); HIP_TRY(L.rny.upload(L.ny, g_stream)); HIP_TRY(L.rnz.upload(L.nz, g_stream));
     HIP_TRY(L.rbtag.upload(L.btag, g_stream));
     HIP_TRY(L.rcloud.upload(L.cloud, g_stream));
-    HIP_TRY(L.d_perm.upload(L.perm, g_stream));
     HIP_TRY(L.d_iperm.upload(L.iperm, g_stream));
     HIP_TRY(L.d_rp.upload(L.rp, g_stream));
     HIP_TRY(L.d_pp.upload(L.pp, g_stream));
     HIP_TRY(L.d_px.upload(L.px, g_stream)); HIP_TRY(L.d_py.upload(L.py, g_stream)); HIP_TRY(L.d_pz.upload(L.pz, g_stream));
-    std::vector<uint8_t> db(L.N);
-    for (int d = 0; d < L.N; d++) db[d] = L.btag[L.perm[d]];
+    HIP_TRY(L.d_col_slices.upload(L.col_slices, g_stream));
+    HIP_TRY(L.d_col_soff.upload(L.col_soff, g_stream));
+    const int Nd = L.Nd;
+    std::vector<uint8_t> db(Nd);
+    for (int d = 0; d < Nd; d++) db[d] = L.perm[d] < 0 ? (uint8_t)ROW_PAD : (L.btag[L.perm[d]] ? 1 : 0);
     HIP_TRY(L.dbtag.upload(db, g_stream));
-    {
-        std::vector<double> t(L.N);
-        const std::vector<double> *src[3] = {&L.nx, &L.ny, &L.nz};
-        DevBuf<double> *dst[3] = {&L.dnx, &L.dny, &L.dnz};
-        for (int k = 0; k < 3; k++) {
-            for (int d = 0; d < L.N; d++) t[d] = (*src[k])[L.perm[d]];
-            HIP_TRY(dst[k]->upload(t, g_stream));
-            HIP_TRY(hipStreamSynchronize(g_stream));  // t is reused
-        }
+    std::vector<double> t[3];
+    const std::vector<double> *src[3] = {&L.nx, &L.ny, &L.nz};
+    DevBuf<double> *dst[3] = {&L.dnx, &L.dny, &L.dnz};
+    for (int k = 0; k < 3; k++) {
+        t[k].assign(Nd, 0.0);
+        for (int d = 0; d < Nd; d++)
+            if (L.perm[d] >= 0) t[k][d] = (*src[k])[L.perm[d]];
+        HIP_TRY(dst[k]->upload(t[k], g_stream));
     }
     int r0 = L.iperm[0];
     HIP_TRY(L.ref0.upload(&r0, 1, g_stream));
     for (auto &f : L.f) {
-        HIP_TRY(f.alloc(L.N));
-        HIP_TRY(hipMemsetAsync(f.p, 0, sizeof(double) * L.N, g_stream));
+        HIP_TRY(f.alloc(Nd));
+        HIP_TRY(hipMemsetAsync(f.p, 0, sizeof(double) * Nd, g_stream));
     }
+    HIP_TRY(hipStreamSynchronize(g_stream));  // host staging vectors die here
     return 0;
 }
 
@@ -550,28 +578,27 @@ static int level_slab(mgm_problem *pb, int l, int which)
 {
     Level &L = pb->lv[l];
     if (!L.rows[which].p) return 0;
-    size_t sz = ell_size(L.N, L.m, L.S);
+    size_t sz = ell_size(L.Nd, L.m, L.S);
     if (L.slab[which].n != sz) {
         HIP_TRY(L.slab[which].alloc(sz));
         HIP_TRY(hipMemsetAsync(L.slab[which].p, 0, sizeof(double) * sz, g_stream));
     }
-    return launch_gather_rows(L.N, L.m, L.S, L.rows[which].p, L.d_perm.p, L.d_iperm.p, L.rcloud.p,
+    return launch_gather_rows(L.N, L.m, L.S, L.rows[which].p, L.d_iperm.p, L.rcloud.p,
                               L.slab[which].p, nullptr, g_stream);
 }
 
 // (Re)build every slab of level l: stencil columns, the five operators and
-// the transfer rows living on this level.  Padding rows of the last slice
-// hold weight 0 and column 0.
+// the transfer rows living on this level.  Padding rows hold weight 0 and
+// column 0.
 static int level_slabs(mgm_problem *pb, int l)
 {
     Level &L = pb->lv[l];
-    L.S = slice_height(L.m);
-    size_t sz = ell_size(L.N, L.m, L.S);
+    size_t sz = ell_size(L.Nd, L.m, L.S);
     if (L.col.n != sz) {
         HIP_TRY(L.col.alloc(sz));
         HIP_TRY(hipMemsetAsync(L.col.p, 0, sizeof(int) * sz, g_stream));
     }
-    int rc = launch_gather_rows(L.N, L.m, L.S, L.rows[OP_LAP].p, L.d_perm.p, L.d_iperm.p, L.rcloud.p,
+    int rc = launch_gather_rows(L.N, L.m, L.S, L.rows[OP_LAP].p, L.d_iperm.p, L.rcloud.p,
                                 nullptr, L.col.p, g_stream);
     if (rc) return rc;
     for (int o = 0; o < NUM_OPS; o++)
@@ -580,13 +607,13 @@ static int level_slabs(mgm_problem *pb, int l)
         Level &F = pb->lv[l - 1];
         L.mr = F.m;
         L.Sr = slice_height(L.mr);
-        size_t n = ell_size(L.N, L.mr, L.Sr);
+        size_t n = ell_size(L.Nd, L.mr, L.Sr);
         if (L.rcol.n != n) {
             HIP_TRY(L.rcol.alloc(n)); HIP_TRY(L.rw.alloc(n));
             HIP_TRY(hipMemsetAsync(L.rcol.p, 0, sizeof(int) * n, g_stream));
             HIP_TRY(hipMemsetAsync(L.rw.p, 0, sizeof(double) * n, g_stream));
         }
-        rc = launch_transfer_slab(L.N, L.mr, L.Sr, L.Rrows.p, L.d_perm.p, L.d_rp.p, F.rcloud.p, F.d_iperm.p,
+        rc = launch_transfer_slab(L.N, L.mr, L.Sr, L.Rrows.p, L.d_iperm.p, L.d_rp.p, F.rcloud.p, F.d_iperm.p,
                                   L.rw.p, L.rcol.p, g_stream);
         if (rc) return rc;
     }
@@ -594,13 +621,13 @@ static int level_slabs(mgm_problem *pb, int l)
         Level &C = pb->lv[l + 1];
         L.mp = C.m;
         L.Sp = slice_height(L.mp);
-        size_t n = ell_size(L.N, L.mp, L.Sp);
+        size_t n = ell_size(L.Nd, L.mp, L.Sp);
         if (L.pcol.n != n) {
             HIP_TRY(L.pcol.alloc(n)); HIP_TRY(L.pw.alloc(n));
             HIP_TRY(hipMemsetAsync(L.pcol.p, 0, sizeof(int) * n, g_stream));
             HIP_TRY(hipMemsetAsync(L.pw.p, 0, sizeof(double) * n, g_stream));
         }
-        rc = launch_transfer_slab(L.N, L.mp, L.Sp, L.Prows.p, L.d_perm.p, L.d_pp.p, C.rcloud.p, C.d_iperm.p,
+        rc = launch_transfer_slab(L.N, L.mp, L.Sp, L.Prows.p, L.d_iperm.p, L.d_pp.p, C.rcloud.p, C.d_iperm.p,
                                   L.pw.p, L.pcol.p, g_stream);
         if (rc) return rc;
     }
@@ -961,8 +988,8 @@ int mgm_get_field(const mgm_problem *pb, int lev, int which, double *out)
     CHECK_READY(pb, lev);
     if (which < 0 || which >= MGM_NUM_FIELDS) MGM_FAIL(-1, "unknown field");
     const Level &L = pb->lv[lev];
-    std::vector<double> tmp(L.N);
-    HIP_TRY(hipMemcpyAsync(tmp.data(), L.f[which].p, 8 * (size_t)L.N, hipMemcpyDeviceToHost, g_stream));
+    std::vector<double> tmp(L.Nd);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), L.f[which].p, 8 * (size_t)L.Nd, hipMemcpyDeviceToHost, g_stream));
     HIP_TRY(hipStreamSynchronize(g_stream));
     for (int i = 0; i < L.N; i++) out[i] = tmp[L.iperm[i]];
     return 0;
@@ -973,9 +1000,9 @@ int mgm_set_field(mgm_problem *pb, int lev, int which, const double *in)
     CHECK_READY(pb, lev);
     if (which < 0 || which >= MGM_NUM_FIELDS) MGM_FAIL(-1, "unknown field");
     Level &L = pb->lv[lev];
-    std::vector<double> tmp(L.N);
-    for (int d = 0; d < L.N; d++) tmp[d] = in[L.perm[d]];
-    HIP_TRY(hipMemcpyAsync(L.f[which].p, tmp.data(), 8 * (size_t)L.N, hipMemcpyHostToDevice, g_stream));
+    std::vector<double> tmp(L.Nd);
+    for (int d = 0; d < L.Nd; d++) tmp[d] = L.perm[d] < 0 ? 0.0 : in[L.perm[d]];
+    HIP_TRY(hipMemcpyAsync(L.f[which].p, tmp.data(), 8 * (size_t)L.Nd, hipMemcpyHostToDevice, g_stream));
     HIP_TRY(hipStreamSynchronize(g_stream));
     return 0;
 }
@@ -994,7 +1021,7 @@ int mgm_zero_fields(mgm_problem *pb)
 {
     CHECK_READY(pb, 0);
     for (auto &L : pb->lv)
-        for (auto &f : L.f) HIP_TRY(hipMemsetAsync(f.p, 0, 8 * (size_t)L.N, g_stream));
+        for (auto &f : L.f) HIP_TRY(hipMemsetAsync(f.p, 0, 8 * (size_t)L.Nd, g_stream));
     HIP_TRY(hipStreamSynchronize(g_stream));
     return 0;
 }
@@ -1006,6 +1033,15 @@ int mgm_zero_fields(mgm_problem *pb)
 // enqueued back to back or captured into a graph.
 namespace mgm {
 
+// Levels with at most MGM_SMALL_ROWS device rows relax in one single-workgroup
+// launch.  Off by default: at 2k rows one CU's gather rate makes a colour step
+// slower than a separate launch over many CUs.
+static int small_level_rows()
+{
+    const char *e = getenv("MGM_SMALL_ROWS");
+    return e ? atoi(e) : 0;
+}
+
 static int pfield(int variant) { return variant == MGM_VAR_T ? MGM_F_PPRIME : MGM_F_P; }
 
 int op_relax(mgm_problem *pb, int l, int variant)
@@ -1015,6 +1051,11 @@ int op_relax(mgm_problem *pb, int l, int variant)
     const double c = (float)(1 - pb->cfg.omega);  // (1-omega) evaluated in float, as the reference
     double *p = L.f[pfield(variant)].p, *src = L.f[MGM_F_SOURCE].p, *res = L.f[MGM_F_RES].p;
     Ell A = L.op(OP_LAPP);
+    if (pb->cfg.sweep_order != MGM_ORDER_NATURAL && L.Nd <= small_level_rows() &&
+        relax_small_fits(L.m, L.m, L.S))
+        return launch_relax_small(A, L.op(OP_LAP), L.d_col_slices.p, L.d_col_soff.p, L.ncol, L.Nd,
+                                  L.dbtag.p, L.ref0.p, src, p, res, w, c, pb->cfg.num_relax, variant,
+                                  g_stream);
     for (int it = 0; it < pb->cfg.num_relax; it++) {
         int rc;
         if (pb->cfg.sweep_order == MGM_ORDER_NATURAL) {
@@ -1022,12 +1063,14 @@ int op_relax(mgm_problem *pb, int l, int variant)
             if (rc) return rc;
         } else {
             for (int k = 0; k < L.ncol; k++)
-                if ((rc = launch_sor_rows(A, L.col_off[k], L.col_off[k + 1], src, p, w, c, g_stream))) return rc;
+                if ((rc = launch_sor_slices(A, L.d_col_slices.p + L.col_soff[k], L.col_soff[k + 1] - L.col_soff[k],
+                                            L.dbtag.p, src, p, w, c, g_stream)))
+                    return rc;
         }
         if (variant == MGM_VAR_H) continue;
-        if ((rc = launch_pin(L.N, L.ref0.p, p, pb->scal.p + 8, g_stream))) return rc;
-        if (variant == MGM_VAR_T) rc = launch_residual(A, L.N, 1, L.dbtag.p, src, p, res, g_stream);
-        else rc = launch_residual(L.op(OP_LAP), L.N, 2, L.dbtag.p, src, p, res, g_stream);
+        if ((rc = launch_pin(L.Nd, L.dbtag.p, L.ref0.p, p, pb->scal.p + 8, g_stream))) return rc;
+        if (variant == MGM_VAR_T) rc = launch_residual(A, L.Nd, 1, L.dbtag.p, src, p, res, g_stream);
+        else rc = launch_residual(L.op(OP_LAP), L.Nd, 2, L.dbtag.p, src, p, res, g_stream);
         if (rc) return rc;
     }
     return 0;
@@ -1037,16 +1080,16 @@ int op_residual(mgm_problem *pb, int l, int variant)
 {
     Level &L = pb->lv[l];
     if (variant == MGM_VAR_F)
-        return launch_residual(L.op(OP_LAP), L.N, 3, L.dbtag.p, L.f[MGM_F_SOURCE].p, L.f[MGM_F_P].p,
+        return launch_residual(L.op(OP_LAP), L.Nd, 3, L.dbtag.p, L.f[MGM_F_SOURCE].p, L.f[MGM_F_P].p,
                                L.f[MGM_F_RES].p, g_stream);
-    return launch_residual(L.op(OP_LAPP), L.N, 0, L.dbtag.p, L.f[MGM_F_SOURCE].p,
+    return launch_residual(L.op(OP_LAPP), L.Nd, 0, L.dbtag.p, L.f[MGM_F_SOURCE].p,
                            L.f[pfield(variant)].p, L.f[MGM_F_RES].p, g_stream);
 }
 
 int op_restrict(mgm_problem *pb, int l, int variant)
 {
     Level &F = pb->lv[l], &C = pb->lv[l + 1];
-    return launch_restrict(C.restr(), C.N, C.dbtag.p, variant == MGM_VAR_F, F.f[MGM_F_RES].p,
+    return launch_restrict(C.restr(), C.Nd, C.dbtag.p, variant == MGM_VAR_F, F.f[MGM_F_RES].p,
                            C.f[MGM_F_SOURCE].p, g_stream);
 }
 
@@ -1054,9 +1097,9 @@ int op_prolong(mgm_problem *pb, int l, int variant)
 {
     Level &C = pb->lv[l], &F = pb->lv[l - 1];
     int pf = pfield(variant);
-    int rc = launch_prolong(F.prol(), F.N, F.dbtag.p, C.f[pf].p, F.f[pf].p, g_stream);
+    int rc = launch_prolong(F.prol(), F.Nd, F.dbtag.p, C.f[pf].p, F.f[pf].p, g_stream);
     if (rc) return rc;
-    return launch_zero(C.N, C.dbtag.p, variant != MGM_VAR_T, C.f[pf].p, g_stream);
+    return launch_zero(C.Nd, C.dbtag.p, variant != MGM_VAR_T, C.f[pf].p, g_stream);
 }
 
 int op_vcycle(mgm_problem *pb, int variant)
@@ -1133,7 +1176,7 @@ int mgm_spmv(mgm_problem *pb, int lev, int which_matrix, int in_field, int out_f
         MGM_FAIL(-1, "spmv: bad field ids");
     Level &L = pb->lv[lev];
     if (!L.slab[which_matrix].p) MGM_FAIL(-1, "spmv: matrix not built");
-    OP_SYNC(launch_spmv(L.op(which_matrix), 0, L.N, L.f[in_field].p, L.f[out_field].p, g_stream));
+    OP_SYNC(launch_spmv(L.op(which_matrix), L.Nd, L.dbtag.p, L.f[in_field].p, L.f[out_field].p, g_stream));
 }
 
 int mgm_spmv_rows(int n_rows, int n_cols, const double *D, const int32_t *cloud, int ncol,
@@ -1159,7 +1202,7 @@ int mgm_spmv_rows(int n_rows, int n_cols, const double *D, const int32_t *cloud,
     HIP_TRY(dc.upload(c, g_stream));
     HIP_TRY(dx.upload(f, ncol, g_stream));
     HIP_TRY(dy.alloc(n_rows));
-    rc = launch_spmv(Ell{dc.p, dw.p, n_cols, S}, 0, n_rows, dx.p, dy.p, g_stream);
+    rc = launch_spmv(Ell{dc.p, dw.p, n_cols, S}, n_rows, nullptr, dx.p, dy.p, g_stream);
     if (rc) return rc;
     HIP_TRY(dy.download(out, n_rows, g_stream));
     HIP_TRY(hipStreamSynchronize(g_stream));
@@ -1171,7 +1214,7 @@ int mgm_l2_norm_gen(const mgm_problem *pbc, int lev, int fa, int fb, double *out
     CHECK_READY(pbc, lev);
     mgm_problem *pb = const_cast<mgm_problem *>(pbc);
     Level &L = pb->lv[lev];
-    int rc = launch_sqdiff_sum(L.N, L.dbtag.p, L.f[fa].p, fb >= 0 ? L.f[fb].p : nullptr, pb->partial.p,
+    int rc = launch_sqdiff_sum(L.Nd, L.dbtag.p, L.f[fa].p, fb >= 0 ? L.f[fb].p : nullptr, pb->partial.p,
                                pb->scal.p, g_stream);
     if (rc) return rc;
     double s = 0;

@@ -29,13 +29,19 @@ struct Level {
     std::vector<int> rp, pp;           // restriction / prolongation points
     std::vector<int> colour;
     int ncol = 0;
-    std::vector<int> col_off;          // colour c occupies device rows [col_off[c], col_off[c+1])
-    std::vector<int> perm, iperm;      // device -> reference, reference -> device
+    // Device order: Morton-ranked blocks of rows; inside a block, colour
+    // classes; each (block, colour) chunk padded to whole slices with dummy
+    // rows (device tag 2, zero weights, never referenced).  Colour c is the
+    // slice list col_slices[col_soff[c] .. col_soff[c+1]).
+    int Nd = 0;                        // device rows including padding
+    std::vector<int> col_slices, col_soff;
+    DevBuf<int> d_col_slices, d_col_soff;
+    std::vector<int> perm, iperm;      // device -> reference (-1 = padding), reference -> device
     int knn_ties = 0;
     // ---- device, reference order ----
     DevBuf<double> rx, ry, rz, rnx, rny, rnz;
     DevBuf<uint8_t> rbtag;
-    DevBuf<int> rcloud, d_perm, d_iperm, d_rp, d_pp;
+    DevBuf<int> rcloud, d_iperm, d_rp, d_pp;
     DevBuf<short> d_px, d_py, d_pz;
     std::array<DevBuf<double>, NUM_OPS> rows;  // row-major N*m
     DevBuf<double> Rrows;                      // this level as coarse: N * m_{l-1}

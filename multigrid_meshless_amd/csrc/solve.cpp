@@ -32,10 +32,10 @@ int enqueue_fine_residual(mgm_problem *pb, int variant)
 {
     Level &L = pb->lv[0];
     int pf = variant == MGM_VAR_T ? MGM_F_PPRIME : MGM_F_P;
-    int rc = launch_residual(L.op(OP_LAPP), L.N, 0, L.dbtag.p, L.f[MGM_F_SOURCE].p, L.f[pf].p,
+    int rc = launch_residual(L.op(OP_LAPP), L.Nd, 0, L.dbtag.p, L.f[MGM_F_SOURCE].p, L.f[pf].p,
                              L.f[MGM_F_RES].p, stream());
     if (rc) return rc;
-    return launch_sqdiff_sum(L.N, L.dbtag.p, L.f[MGM_F_RES].p, nullptr, pb->partial.p, pb->scal.p, stream());
+    return launch_sqdiff_sum(L.Nd, L.dbtag.p, L.f[MGM_F_RES].p, nullptr, pb->partial.p, pb->scal.p, stream());
 }
 
 int read_scalar(mgm_problem *pb, double *v)
@@ -60,7 +60,7 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
     Level &L0 = pb->lv[0];
     int rc;
     // |f| over interior nodes
-    rc = launch_sqdiff_sum(L0.N, L0.dbtag.p, L0.f[MGM_F_SOURCE].p, nullptr, pb->partial.p, pb->scal.p, stream());
+    rc = launch_sqdiff_sum(L0.Nd, L0.dbtag.p, L0.f[MGM_F_SOURCE].p, nullptr, pb->partial.p, pb->scal.p, stream());
     if (rc) return rc;
     double fs;
     if ((rc = read_scalar(pb, &fs))) return rc;
@@ -126,11 +126,11 @@ int mgm_time_spmv(mgm_problem *pb, int lev, int reps, double *seconds)
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    int rc = launch_spmv(L.op(OP_LAPP), 0, L.N, L.f[MGM_F_P].p, L.f[MGM_F_DPDN].p, stream());
+    int rc = launch_spmv(L.op(OP_LAPP), L.Nd, L.dbtag.p, L.f[MGM_F_P].p, L.f[MGM_F_DPDN].p, stream());
     if (rc) return rc;
     HIP_TRY(hipEventRecord(e0, stream()));
     for (int r = 0; r < reps; r++)
-        if ((rc = launch_spmv(L.op(OP_LAPP), 0, L.N, L.f[MGM_F_P].p, L.f[MGM_F_DPDN].p, stream()))) return rc;
+        if ((rc = launch_spmv(L.op(OP_LAPP), L.Nd, L.dbtag.p, L.f[MGM_F_P].p, L.f[MGM_F_DPDN].p, stream()))) return rc;
     HIP_TRY(hipEventRecord(e1, stream()));
     HIP_TRY(hipEventSynchronize(e1));
     float ms = 0;

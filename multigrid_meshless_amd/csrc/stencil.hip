@@ -2,9 +2,10 @@
 //
 // Layout: rows in slices of S consecutive device rows; within a slice the m
 // columns follow each other and each column holds the slice's S entries
-// contiguously (ell_at in device.hpp).  Device order groups nodes by colour
-// and, within a colour, by Morton code, so the rows of one SOR colour are
-// contiguous and neighbour gathers stay local.
+// contiguously (ell_at in device.hpp).  Device order: Morton-ranked blocks of
+// rows, colour classes inside a block, Morton order inside a class, each
+// (block, colour) chunk padded to whole slices; a colour step walks that
+// colour's slice list and its neighbour gathers stay inside a few blocks.
 //
 // Work split: one wave per slice, 64/S lanes per row.  At step t the lanes of
 // a wave load entries j = t*(64/S) + lane/S of the S rows -- 64 consecutive
@@ -18,6 +19,8 @@
 // those of the reference loops cited at each mode.
 #pragma clang fp contract(off)
 
+#include <cstdlib>
+
 #include "device.hpp"
 
 namespace mgm {
@@ -26,6 +29,8 @@ namespace {
 
 constexpr int BLK = 256;
 constexpr int WAVES = BLK / 64;
+constexpr int SMALL_BLK = 512;
+constexpr size_t kSmallLdsMax = 64 * 1024;
 
 inline int nblk(long n) { return (int)((n + BLK - 1) / BLK); }
 
@@ -45,82 +50,164 @@ __device__ inline bool skips_boundary(int mode)
     return mode == M_RES0 || mode == M_RES1 || mode == M_RES3 || mode == M_RESTRICT || mode == M_PROLONG;
 }
 
-template <int S>
-__global__ void __launch_bounds__(BLK)
-rows_kernel(Ell A, int r0, int r1, int mode, const uint8_t *__restrict__ btag,
-            const double *__restrict__ src, const double *x, double *y, double w, double c,
-            int keep_boundary)
+// Wave-scope barrier: LDS written by some lanes of a wave is visible to the
+// other lanes of that wave afterwards.  Each wave stages into its own LDS
+// region, so waves never wait for each other.
+__device__ inline void wave_sync()
 {
-    extern __shared__ double prod[];          // [WAVES][m][S]
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Renumber them so XCD i takes a contiguous range of slices: neighbouring
+// rows (and the x entries their stencils gather) then stay in one L2.
+__device__ inline long xcd_block(long b, long nb)
+{
+    const long xcd = b % 8, q = nb / 8, rem = nb % 8, idx = b / 8;
+    return (xcd < rem) ? xcd * (q + 1) + idx : rem * (q + 1) + (xcd - rem) * q + idx;
+}
+
+// One wave applies the rows of one slice: lanes (ri, jo) stage the products
+// w*x of entries j = jo, jo+LPR, ... of row ri in LDS at P[j*S + ri]; after a
+// wave barrier lane jo == 0 combines its row in stencil order.
+template <int S>
+__device__ inline void slice_rows(const Ell &A, long slice, int nrows, int mode,
+                                  const uint8_t *__restrict__ btag, const double *__restrict__ src,
+                                  const double *x, double *y, double w, double c, int keep_boundary,
+                                  double *P)
+{
     constexpr int LPR = 64 / S;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63;
     const int ri = lane % S, jo = lane / S;
     const int m = A.m;
-    double *P = prod + (size_t)wave * m * S;
-    const long slice = (long)(r0 / S) + (long)blockIdx.x * WAVES + wave;
     const long r = slice * S + ri;
-    const bool act = (r >= r0 && r < r1);
-    const bool bnd = act && btag && btag[r];
+    const bool inr = r < nrows;
+    const uint8_t tag = (inr && btag) ? btag[r] : (uint8_t)(inr ? ROW_INTERIOR : ROW_PAD);
+    const bool act = tag != ROW_PAD;
+    const bool bnd = tag == ROW_BOUNDARY;
     const bool need = act && !(bnd && skips_boundary(mode));
-    const size_t base = (size_t)slice * S * m;
+    const size_t base = (size_t)slice * S * m + ri;
     if (need) {
 #pragma unroll 4
         for (int j = jo; j < m; j += LPR) {
-            size_t q = base + (size_t)j * S + ri;
+            const size_t q = base + (size_t)j * S;
             P[j * S + ri] = A.w[q] * x[A.col[q]];
         }
     }
-    __syncthreads();
-    if (jo != 0 || !act) return;
-    switch (mode) {
-    case M_SPMV: {
-        double acc = 0.0;
-        for (int j = 0; j < m; j++) acc += P[j * S + ri];
-        y[r] = acc;
-        break;
-    }
-    case M_SOR: {
-        double sum = 0.0;
-        for (int j = 1; j < m; j++) sum += P[j * S + ri];
-        y[r] = w * ((src[r] - sum) / A.w[base + ri]) + c * y[r];
-        break;
-    }
-    case M_RES0:
-    case M_RES3: {
-        if (bnd) {
-            if (mode == M_RES0) y[r] = 0.0;
+    wave_sync();
+    if (jo == 0 && act) {
+        switch (mode) {
+        case M_SPMV: {
+            double acc = 0.0;
+            for (int j = 0; j < m; j++) acc += P[j * S + ri];
+            y[r] = acc;
             break;
         }
-        double sum = 0;
-        for (int j = 0; j < m; j++) sum += P[j * S + ri];
-        y[r] = src[r] - sum;
-        break;
-    }
-    case M_RES1:
-    case M_RES2: {
-        if (mode == M_RES1 && bnd) { y[r] = 0; break; }
-        double v = src[r];
-        for (int j = 0; j < m; j++) v = v - P[j * S + ri];
-        y[r] = v;
-        break;
-    }
-    case M_RESTRICT: {
-        if (bnd) {
-            if (!keep_boundary) y[r] = 0.0;
+        case M_SOR: {
+            double sum = 0.0;
+            for (int j = 1; j < m; j++) sum += P[j * S + ri];
+            y[r] = w * ((src[r] - sum) / A.w[base]) + c * y[r];
             break;
         }
-        double acc = 0.0;
-        for (int j = 0; j < m; j++) acc += P[j * S + ri];
-        y[r] = acc;
-        break;
+        case M_RES0:
+        case M_RES3: {
+            if (bnd) {
+                if (mode == M_RES0) y[r] = 0.0;
+                break;
+            }
+            double sum = 0;
+            for (int j = 0; j < m; j++) sum += P[j * S + ri];
+            y[r] = src[r] - sum;
+            break;
+        }
+        case M_RES1:
+        case M_RES2: {
+            if (mode == M_RES1 && bnd) { y[r] = 0; break; }
+            double v = src[r];
+            for (int j = 0; j < m; j++) v = v - P[j * S + ri];
+            y[r] = v;
+            break;
+        }
+        case M_RESTRICT: {
+            if (bnd) {
+                if (!keep_boundary) y[r] = 0.0;
+                break;
+            }
+            double acc = 0.0;
+            for (int j = 0; j < m; j++) acc += P[j * S + ri];
+            y[r] = acc;
+            break;
+        }
+        case M_PROLONG: {
+            if (bnd) break;
+            double acc = 0.0;
+            for (int j = 0; j < m; j++) acc += P[j * S + ri];
+            y[r] = y[r] + acc;
+            break;
+        }
+        }
     }
-    case M_PROLONG: {
-        if (bnd) break;
-        double acc = 0.0;
-        for (int j = 0; j < m; j++) acc += P[j * S + ri];
-        y[r] = y[r] + acc;
-        break;
-    }
+    wave_sync();  // P is reused by the wave's next slice
+}
+
+// Slice k of the launch is slices[k] when a slice list is given (one colour
+// class), else k itself (a full pass).  Rows at or beyond nrows and padding
+// rows (tag ROW_PAD) are neither read nor written.
+template <int S>
+__global__ void __launch_bounds__(BLK)
+rows_kernel(Ell A, const int *__restrict__ slices, int nslices, int nrows, int mode,
+            const uint8_t *__restrict__ btag, const double *__restrict__ src, const double *x,
+            double *y, double w, double c, int keep_boundary)
+{
+    extern __shared__ double prod[];          // [WAVES][m][S]
+    const int wave = threadIdx.x >> 6;
+    const long k = xcd_block(blockIdx.x, gridDim.x) * WAVES + wave;
+    if (k >= nslices) return;
+    const long slice = slices ? (long)slices[k] : k;
+    slice_rows<S>(A, slice, nrows, mode, btag, src, x, y, w, c, keep_boundary,
+                  prod + (size_t)wave * A.m * S);
+}
+
+// Whole relaxation of a small level in one workgroup: num_relax sweeps over
+// the colour classes in order (workgroup barrier between classes), each sweep
+// followed, for the T and F variants, by p -= p[node 0] and the residual form
+// 1 (T) or 2 (F, with the lap operator L2).  Identical arithmetic to the
+// per-colour launches.
+template <int S>
+__global__ void __launch_bounds__(SMALL_BLK)
+relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int *__restrict__ col_soff,
+                   int ncol, int nrows, const uint8_t *__restrict__ btag, const int *__restrict__ ref0,
+                   const double *__restrict__ src, double *p, double *res, double w, double c,
+                   int num_relax, int variant)
+{
+    extern __shared__ double prod[];          // [SMALL_BLK/64][max(m, m2)][S]
+    constexpr int NW = SMALL_BLK / 64;
+    const int wave = threadIdx.x >> 6;
+    const int mw = A.m > A2.m ? A.m : A2.m;
+    double *P = prod + (size_t)wave * mw * S;
+    __shared__ double pref;
+    const long nsl = (nrows + S - 1) / S;
+    for (int it = 0; it < num_relax; it++) {
+        for (int cl = 0; cl < ncol; cl++) {
+            const int s0 = col_soff[cl], s1 = col_soff[cl + 1];
+            for (int k = s0 + wave; k < s1; k += NW)
+                slice_rows<S>(A, col_slices[k], nrows, M_SOR, btag, src, p, p, w, c, 0, P);
+            __syncthreads();
+        }
+        if (variant == MGM_VAR_H) continue;
+        if (threadIdx.x == 0) pref = p[*ref0];
+        __syncthreads();
+        const double pr = pref;
+        for (long r = threadIdx.x; r < nrows; r += SMALL_BLK)
+            if (btag[r] != ROW_PAD) p[r] = p[r] - pr;
+        __syncthreads();
+        const Ell &R = (variant == MGM_VAR_T) ? A : A2;
+        const int form = (variant == MGM_VAR_T) ? M_RES1 : M_RES2;
+        for (long k = wave; k < nsl; k += NW)
+            slice_rows<S>(R, k, nrows, form, btag, src, p, res, 0, 0, 0, P);
+        __syncthreads();
     }
 }
 
@@ -145,7 +232,7 @@ __global__ void __launch_bounds__(BLK)
 zero_kernel(int N, const uint8_t *__restrict__ btag, int interior_only, double *p)
 {
     int r = blockIdx.x * BLK + threadIdx.x;
-    if (r >= N) return;
+    if (r >= N || btag[r] == ROW_PAD) return;
     if (!interior_only || !btag[r]) p[r] = 0.0;
 }
 
@@ -156,10 +243,10 @@ __global__ void copy_scalar_kernel(const int *ref0, const double *p, double *dst
 
 // p -= p[node 0] (hf/timple_solver.h:248-252, hf/fractional_step_solver.h:180-183)
 __global__ void __launch_bounds__(BLK)
-pin_kernel(int N, const double *__restrict__ pref, double *p)
+pin_kernel(int N, const uint8_t *__restrict__ btag, const double *__restrict__ pref, double *p)
 {
     int r = blockIdx.x * BLK + threadIdx.x;
-    if (r >= N) return;
+    if (r >= N || btag[r] == ROW_PAD) return;
     p[r] = p[r] - *pref;
 }
 
@@ -201,33 +288,31 @@ sum_partial_kernel(int n, const double *__restrict__ partial, double *__restrict
 
 // Row-major reference-order rows -> sliced-ELL device-order slab.
 __global__ void __launch_bounds__(BLK)
-gather_rows_kernel(int N, int m, int S, const double *__restrict__ rows, const int *__restrict__ perm,
-                   const int *__restrict__ iperm, const int *__restrict__ cloud,
-                   double *__restrict__ w_out, int *__restrict__ col_out)
+gather_rows_kernel(int N, int m, int S, const double *__restrict__ rows, const int *__restrict__ iperm,
+                   const int *__restrict__ cloud, double *__restrict__ w_out,
+                   int *__restrict__ col_out)
 {
     long t = (long)blockIdx.x * BLK + threadIdx.x;
     if (t >= (long)N * m) return;
-    int dev = (int)(t / m), j = (int)(t % m);
-    int ref = perm[dev];
-    size_t o = ell_at(m, S, dev, j);
-    if (w_out) w_out[o] = rows[(size_t)ref * m + j];
-    if (col_out) col_out[o] = iperm[cloud[(size_t)ref * m + j]];
+    int ref = (int)(t / m), j = (int)(t % m);
+    size_t o = ell_at(m, S, iperm[ref], j);
+    if (w_out) w_out[o] = rows[t];
+    if (col_out) col_out[o] = iperm[cloud[t]];
 }
 
 // Transfer rows (reference order, width m of the source level) -> slab over the
 // target level's device order, columns pre-composed through the source cloud.
 __global__ void __launch_bounds__(BLK)
-transfer_slab_kernel(int Nt, int m, int S, const double *__restrict__ rows, const int *__restrict__ perm,
-                     const int *__restrict__ pts, const int *__restrict__ cloud_src,
-                     const int *__restrict__ iperm_src, double *__restrict__ w_out,
-                     int *__restrict__ col_out)
+transfer_slab_kernel(int Nt, int m, int S, const double *__restrict__ rows,
+                     const int *__restrict__ iperm_t, const int *__restrict__ pts,
+                     const int *__restrict__ cloud_src, const int *__restrict__ iperm_src,
+                     double *__restrict__ w_out, int *__restrict__ col_out)
 {
     long t = (long)blockIdx.x * BLK + threadIdx.x;
     if (t >= (long)Nt * m) return;
-    int dev = (int)(t / m), j = (int)(t % m);
-    int ref = perm[dev];
-    size_t o = ell_at(m, S, dev, j);
-    w_out[o] = rows[(size_t)ref * m + j];
+    int ref = (int)(t / m), j = (int)(t % m);
+    size_t o = ell_at(m, S, iperm_t[ref], j);
+    w_out[o] = rows[t];
     col_out[o] = iperm_src[cloud_src[(size_t)pts[ref] * m + j]];
 }
 
@@ -257,33 +342,24 @@ dirichlet_rows_kernel(int N, int m, const uint8_t *__restrict__ btag, double *__
     if (btag[i]) rows[t] = (j == 0) ? 1.0 : 0.0;
 }
 
-__global__ void __launch_bounds__(BLK)
-permute_kernel(int N, const int *__restrict__ idx, const double *__restrict__ in, double *__restrict__ out)
+// slices == nullptr: all slices covering rows [0, nrows)
+int launch_rows(Ell A, const int *slices, int nslices, int nrows, int mode, const uint8_t *btag,
+                const double *src, const double *x, double *y, double w, double c, int keep,
+                hipStream_t s)
 {
-    int t = blockIdx.x * BLK + threadIdx.x;
-    if (t < N) out[t] = in[idx[t]];
-}
-
-__global__ void __launch_bounds__(BLK)
-scatter_kernel(int N, const int *__restrict__ idx, const double *__restrict__ in, double *__restrict__ out)
-{
-    int t = blockIdx.x * BLK + threadIdx.x;
-    if (t < N) out[idx[t]] = in[t];
-}
-
-int launch_rows(Ell A, int r0, int r1, int mode, const uint8_t *btag, const double *src,
-                const double *x, double *y, double w, double c, int keep, hipStream_t s)
-{
-    if (r1 <= r0) return 0;
-    long s0 = r0 / A.S, s1 = (r1 - 1) / A.S;
-    int blocks = (int)((s1 - s0 + 1 + WAVES - 1) / WAVES);
+    if (!slices) nslices = (nrows + A.S - 1) / A.S;
+    if (nslices <= 0 || nrows <= 0) return 0;
+    int blocks = (nslices + WAVES - 1) / WAVES;
     size_t shm = (size_t)WAVES * A.m * A.S * sizeof(double);
-    if (A.S == 8)
-        hipLaunchKernelGGL(rows_kernel<8>, dim3(blocks), dim3(BLK), shm, s, A, r0, r1, mode, btag, src,
-                           x, y, w, c, keep);
+    if (A.S == 4)
+        hipLaunchKernelGGL(rows_kernel<4>, dim3(blocks), dim3(BLK), shm, s, A, slices, nslices, nrows,
+                           mode, btag, src, x, y, w, c, keep);
+    else if (A.S == 8)
+        hipLaunchKernelGGL(rows_kernel<8>, dim3(blocks), dim3(BLK), shm, s, A, slices, nslices, nrows,
+                           mode, btag, src, x, y, w, c, keep);
     else if (A.S == 16)
-        hipLaunchKernelGGL(rows_kernel<16>, dim3(blocks), dim3(BLK), shm, s, A, r0, r1, mode, btag, src,
-                           x, y, w, c, keep);
+        hipLaunchKernelGGL(rows_kernel<16>, dim3(blocks), dim3(BLK), shm, s, A, slices, nslices, nrows,
+                           mode, btag, src, x, y, w, c, keep);
     else
         MGM_FAIL(-1, "unsupported slice height");
     HIP_TRY(hipGetLastError());
@@ -300,14 +376,53 @@ int launch_rows(Ell A, int r0, int r1, int mode, const uint8_t *btag, const doub
         }                                                                            \
     } while (0)
 
-int launch_spmv(Ell A, int r0, int r1, const double *x, double *y, hipStream_t s)
+int slice_height(int m)
 {
-    return launch_rows(A, r0, r1, M_SPMV, nullptr, nullptr, x, y, 0, 0, 0, s);
+    static const int forced = [] {
+        const char *e = getenv("MGM_SLICE_HEIGHT");
+        int v = e ? atoi(e) : 0;
+        return (v == 4 || v == 8 || v == 16) ? v : 0;
+    }();
+    if (forced) return forced;
+    return m > 36 ? 8 : 16;
 }
-int launch_sor_rows(Ell A, int r0, int r1, const double *src, double *p, double w, double c,
-                    hipStream_t s)
+
+int launch_spmv(Ell A, int nrows, const uint8_t *btag, const double *x, double *y, hipStream_t s)
 {
-    return launch_rows(A, r0, r1, M_SOR, nullptr, src, p, p, w, c, 0, s);
+    return launch_rows(A, nullptr, 0, nrows, M_SPMV, btag, nullptr, x, y, 0, 0, 0, s);
+}
+int launch_sor_slices(Ell A, const int *slices, int nslices, const uint8_t *btag,
+                      const double *src, double *p, double w, double c, hipStream_t s)
+{
+    if (nslices <= 0) return 0;
+    return launch_rows(A, slices, nslices, 0x7fffffff, M_SOR, btag, src, p, p, w, c, 0, s);
+}
+int launch_relax_small(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol, int nrows,
+                       const uint8_t *btag, const int *ref0, const double *src, double *p, double *res,
+                       double w, double c, int num_relax, int variant, hipStream_t s)
+{
+    if (A.S != A2.S) MGM_FAIL(-1, "relax_small: operators differ in slice height");
+    const int mw = A.m > A2.m ? A.m : A2.m;
+    const size_t shm = (size_t)(SMALL_BLK / 64) * mw * A.S * sizeof(double);
+    if (shm > kSmallLdsMax) MGM_FAIL(-1, "relax_small: stencil too wide");
+    if (A.S == 4)
+        hipLaunchKernelGGL(relax_small_kernel<4>, dim3(1), dim3(SMALL_BLK), shm, s, A, A2, col_slices,
+                           col_soff, ncol, nrows, btag, ref0, src, p, res, w, c, num_relax, variant);
+    else if (A.S == 8)
+        hipLaunchKernelGGL(relax_small_kernel<8>, dim3(1), dim3(SMALL_BLK), shm, s, A, A2, col_slices,
+                           col_soff, ncol, nrows, btag, ref0, src, p, res, w, c, num_relax, variant);
+    else if (A.S == 16)
+        hipLaunchKernelGGL(relax_small_kernel<16>, dim3(1), dim3(SMALL_BLK), shm, s, A, A2, col_slices,
+                           col_soff, ncol, nrows, btag, ref0, src, p, res, w, c, num_relax, variant);
+    else
+        MGM_FAIL(-1, "unsupported slice height");
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+bool relax_small_fits(int m, int m2, int S)
+{
+    const int mw = m > m2 ? m : m2;
+    return (size_t)(SMALL_BLK / 64) * mw * S * sizeof(double) <= kSmallLdsMax;
 }
 int launch_sor_natural(Ell A, int N, const int *iperm, const double *src, double *p, double w,
                        double c, hipStream_t s)
@@ -321,27 +436,28 @@ int launch_residual(Ell A, int N, int form, const uint8_t *btag, const double *s
 {
     static const int modes[4] = {M_RES0, M_RES1, M_RES2, M_RES3};
     if (form < 0 || form > 3) MGM_FAIL(-1, "bad residual form");
-    return launch_rows(A, 0, N, modes[form], btag, src, p, res, 0, 0, 0, s);
+    return launch_rows(A, nullptr, 0, N, modes[form], btag, src, p, res, 0, 0, 0, s);
 }
 int launch_restrict(Ell R, int N, const uint8_t *btag, int keep_boundary, const double *in,
                     double *out, hipStream_t s)
 {
-    return launch_rows(R, 0, N, M_RESTRICT, btag, nullptr, in, out, 0, 0, keep_boundary, s);
+    return launch_rows(R, nullptr, 0, N, M_RESTRICT, btag, nullptr, in, out, 0, 0, keep_boundary, s);
 }
 int launch_prolong(Ell P, int N, const uint8_t *btag, const double *pc, double *p, hipStream_t s)
 {
-    return launch_rows(P, 0, N, M_PROLONG, btag, nullptr, pc, p, 0, 0, 0, s);
+    return launch_rows(P, nullptr, 0, N, M_PROLONG, btag, nullptr, pc, p, 0, 0, 0, s);
 }
 int launch_zero(int N, const uint8_t *btag, int interior_only, double *p, hipStream_t s)
 {
     LAUNCH(zero_kernel, N, N, btag, interior_only, p);
     return 0;
 }
-int launch_pin(int N, const int *ref0_dev, double *p, double *scratch, hipStream_t s)
+int launch_pin(int N, const uint8_t *btag, const int *ref0_dev, double *p, double *scratch,
+               hipStream_t s)
 {
     hipLaunchKernelGGL(copy_scalar_kernel, dim3(1), dim3(64), 0, s, ref0_dev, p, scratch);
     HIP_TRY(hipGetLastError());
-    LAUNCH(pin_kernel, N, N, scratch, p);
+    LAUNCH(pin_kernel, N, N, btag, scratch, p);
     return 0;
 }
 int launch_sqdiff_sum(int N, const uint8_t *btag, const double *a, const double *b,
@@ -362,18 +478,18 @@ int launch_sum_partials(int n, const double *partial, double *out, hipStream_t s
     HIP_TRY(hipGetLastError());
     return 0;
 }
-int launch_gather_rows(int N, int m, int S, const double *rows, const int *perm, const int *iperm,
+int launch_gather_rows(int N, int m, int S, const double *rows, const int *iperm,
                        const int *cloud, double *w_out, int *col_out, hipStream_t s)
 {
-    LAUNCH(gather_rows_kernel, (long)N * m, N, m, S, rows, perm, iperm, cloud, w_out, col_out);
+    LAUNCH(gather_rows_kernel, (long)N * m, N, m, S, rows, iperm, cloud, w_out, col_out);
     return 0;
 }
-int launch_transfer_slab(int Nt, int m, int S, const double *rows, const int *perm, const int *pts,
-                         const int *cloud_src, const int *iperm_src, double *w_out, int *col_out,
-                         hipStream_t s)
+int launch_transfer_slab(int Nt, int m, int S, const double *rows, const int *iperm_t,
+                         const int *pts, const int *cloud_src, const int *iperm_src,
+                         double *w_out, int *col_out, hipStream_t s)
 {
-    LAUNCH(transfer_slab_kernel, (long)Nt * m, Nt, m, S, rows, perm, pts, cloud_src, iperm_src, w_out,
-           col_out);
+    LAUNCH(transfer_slab_kernel, (long)Nt * m, Nt, m, S, rows, iperm_t, pts, cloud_src, iperm_src,
+           w_out, col_out);
     return 0;
 }
 int launch_lap_poisson(int N, int m, int dim, const double *Dx, const double *Dy, const double *Dz,
@@ -386,16 +502,6 @@ int launch_lap_poisson(int N, int m, int dim, const double *Dx, const double *Dy
 int launch_dirichlet_rows(int N, int m, const uint8_t *btag, double *rows, hipStream_t s)
 {
     LAUNCH(dirichlet_rows_kernel, (long)N * m, N, m, btag, rows);
-    return 0;
-}
-int launch_permute(int N, const int *idx, const double *in, double *out, hipStream_t s)
-{
-    LAUNCH(permute_kernel, N, N, idx, in, out);
-    return 0;
-}
-int launch_scatter(int N, const int *idx, const double *in, double *out, hipStream_t s)
-{
-    LAUNCH(scatter_kernel, N, N, idx, in, out);
     return 0;
 }
 

@@ -147,11 +147,15 @@ def test_ns_poisson_vcycles_bitwise(name, var, gpu):
         np.testing.assert_array_equal(P.field(k, "source"), g[f"mg_L{k}_source"])
 
 
-@pytest.mark.parametrize("name", ["gen2d", "gen3d"])
-def test_colour_order_vcycles_match_oracle(name, gpu, oracle):
-    """Parallel multicolour SOR == the oracle's sequential sweep in colour order, bit for bit."""
+@pytest.mark.parametrize("small_rows", ["0", "1000000"], ids=["per_colour_launches", "fused_small_level"])
+@pytest.mark.parametrize("name,var", [("gen2d", 0), ("gen3d", 0), ("gen2d", 1), ("gen2d", 2), ("gen3d", 2)])
+def test_colour_order_vcycles_match_oracle(name, var, small_rows, gpu, oracle, monkeypatch):
+    """Parallel multicolour SOR == the oracle's sequential sweep in colour order, bit for bit,
+    both with one launch per colour class and with a level's whole relaxation in one
+    single-workgroup launch (MGM_SMALL_ROWS sets the level size below which the latter runs)."""
+    monkeypatch.setenv("MGM_SMALL_ROWS", small_rows)
     g = load_golden(name)
-    print(f"[gpu colour vcycle] {name}", flush=True)
+    print(f"[gpu colour vcycle] {name} variant {var} small_rows {small_rows}", flush=True)
     P = gpu_problem(gpu, g, order=gpu.ORDER_COLOUR)
     L = num_levels(g)
     lv = []
@@ -178,11 +182,13 @@ def test_colour_order_vcycles_match_oracle(name, gpu, oracle):
     P.zero_fields()
     P.set_field(0, "source", src)
     for _ in range(3):
-        H.vcycle(oracle.VAR_H, 1.0, 3)
-        P.vcycle(gpu.VAR_H, 1)
+        H.vcycle(var, 1.0, 3)
+        P.vcycle(var, 1)
+    pf = "pprime" if var == gpu.VAR_T else "p"
     for k in range(L):
-        np.testing.assert_array_equal(P.field(k, "p"), H.lv[k]["p"], err_msg=f"{name} L{k}")
+        np.testing.assert_array_equal(P.field(k, pf), H.lv[k]["p"], err_msg=f"{name} L{k}")
         np.testing.assert_array_equal(P.field(k, "res"), H.lv[k]["res"])
+        np.testing.assert_array_equal(P.field(k, "source"), H.lv[k]["s"])
 
 
 def test_knn_ties_follow_reference_order(gpu, oracle):
