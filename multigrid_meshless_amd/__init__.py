@@ -225,6 +225,22 @@ class Problem:
                               hist.ctypes.data, C.byref(sec)), "solve")
         return dict(cycles=nc.value, seconds=sec.value, history=hist[:nc.value + 1])
 
+    # ---- time steps (level-0 fields, resident) ----
+    def ns_set_params(self, **kw):
+        """dt, rho, mu, nu, facRe, facdt, field_rho, iter_momentum, iter_timple, num_vcycles."""
+        from ._lib import NsParams
+        check(lib().mgm_ns_set_params(self._h, C.byref(NsParams(**kw))), "ns_set_params")
+
+    def timple_step(self) -> float:
+        err = C.c_double()
+        check(lib().mgm_timple_step(self._h, C.byref(err)), "timple_step")
+        return err.value
+
+    def fs_step(self) -> float:
+        err = C.c_double()
+        check(lib().mgm_fs_step(self._h, C.byref(err)), "fs_step")
+        return err.value
+
     def time_spmv(self, lev=0, reps=20) -> float:
         s = C.c_double()
         check(lib().mgm_time_spmv(self._h, lev, reps, C.byref(s)), "time_spmv")

@@ -27,6 +27,13 @@ class LevelInfo(C.Structure):
                 ("d_avg_bbox", C.c_double), ("knn_ties", C.c_int)]
 
 
+class NsParams(C.Structure):
+    """mgm_ns_params: the scalars the two time steps read (struct parameters and FieldVariables.rho)."""
+    _fields_ = [("dt", C.c_double), ("rho", C.c_double), ("mu", C.c_double), ("nu", C.c_double),
+                ("facRe", C.c_double), ("facdt", C.c_double), ("field_rho", C.c_double),
+                ("iter_momentum", C.c_int), ("iter_timple", C.c_int), ("num_vcycles", C.c_int)]
+
+
 # (name, restype, argtypes) for every symbol declared in include/mgm.h
 _vp, _i, _d, _f, _cp = C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_char_p
 SIGNATURES = [

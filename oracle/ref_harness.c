@@ -102,7 +102,9 @@ static double *field_ptr(int lev, int which)
     case 3: return F[lev].pprime; case 4: return F[lev].u;      case 5: return F[lev].v;
     case 6: return F[lev].w;      case 7: return F[lev].u_new;  case 8: return F[lev].v_new;
     case 9: return F[lev].w_new;  case 10: return F[lev].p_old; case 11: return F[lev].T;
-    case 12: return F[lev].dpdn;
+    case 12: return F[lev].dpdn;  case 13: return F[lev].dpdx;  case 14: return F[lev].dpdy;
+    case 15: return F[lev].dpdz;  case 16: return F[lev].u_old; case 17: return F[lev].v_old;
+    case 18: return F[lev].w_old;
     }
     return 0;
 }
@@ -169,6 +171,15 @@ void ref_ns_init(void)
     parameters.dt = calculate_dt(&P[0]);
 }
 double ref_dt(void) { return parameters.dt; }
+/* d: dt, rho, mu, nu, facRe, facdt, F[0].rho; i: iter_momentum, iter_timple, num_vcycles, num_relax */
+void ref_ns_params(double *d, int *i, float *omega)
+{
+    d[0] = parameters.dt; d[1] = parameters.rho; d[2] = parameters.mu; d[3] = parameters.nu;
+    d[4] = parameters.facRe; d[5] = parameters.facdt; d[6] = F[0].rho;
+    i[0] = parameters.iter_momentum; i[1] = parameters.iter_timple; i[2] = parameters.num_vcycles;
+    i[3] = parameters.num_relax;
+    *omega = parameters.omega;
+}
 void ref_mg_poisson(void) { multigrid_Poisson_solver(P, F); }
 #endif
 

@@ -168,6 +168,40 @@ def run_case(name: str, workdir: str, out: str):
             lib.ref_set_omega_relax(C.c_float(case["omega"]), 3, nv)
             lib.ref_mg_poisson()
             fields("mg", [(pf, "p"), (1, "source"), (2, "res")])
+            # whole time steps from the LDC initial state (mg_NS_solver.c:51-72):
+            # every field of every level before the first step, level-0 fields
+            # and the steady-state error after each of two steps
+            lib.ref_ns_init()
+            if case["lib"] == "timple":
+                # time_implicit_solver starts from u_new/v_new, which the LDC
+                # initialisation leaves at zero (the step would then be a no-op):
+                # start from the lid velocity plus seeded interior perturbations
+                rng = np.random.default_rng(13)
+                N0 = dims[0][0]
+                inner = res["L0_btag"] == 0
+                u = np.zeros(N0)
+                lib.ref_get_field(0, 4, u.ctypes.data_as(C.c_void_p))
+                un = u.copy()
+                un[inner] += 0.1 * rng.standard_normal(inner.sum())
+                vn = np.zeros(N0)
+                vn[inner] = 0.1 * rng.standard_normal(inner.sum())
+                po = 0.1 * rng.standard_normal(N0)
+                for fid, a in ((7, un), (8, vn), (10, po)):
+                    lib.ref_set_field(0, fid, a.ctypes.data_as(C.c_void_p))
+            fields("st_init", [(k, str(k)) for k in range(19)])
+            d, iv, om = (C.c_double * 7)(), (C.c_int * 4)(), C.c_float()
+            lib.ref_ns_params(d, iv, C.byref(om))
+            res["st_params_d"] = np.array(list(d))
+            res["st_params_i"] = np.array(list(iv), np.int32)
+            res["st_omega"] = np.float32(om.value)
+            lib.ref_step.restype = C.c_double
+            for k in range(2):
+                err = lib.ref_step()
+                res[f"st{k}_err"] = np.float64(err)
+                for fid in range(19):
+                    a = np.zeros(dims[0][0])
+                    lib.ref_get_field(0, fid, a.ctypes.data_as(C.c_void_p))
+                    res[f"st{k}_L0_{fid}"] = a
     finally:
         os.dup2(saved, 1)
         os.close(devnull)

@@ -215,3 +215,47 @@ def test_spmv_matches_oracle(built, oracle):
     P.spmv(0, "lap_Poison", "p", "dpdn")
     ref = oracle.spmv(P.matrix(0, "lap_Poison"), P.cloud(0), f)
     np.testing.assert_array_equal(P.field(0, "dpdn"), ref)
+
+
+FIELD_NAMES = {v: k for k, v in __import__("multigrid_meshless_amd").FIELD.items()}
+# Fields each time step defines (reference state after the step); the rest
+# (dpdx/dpdy/dpdz/dpdn interiors, T) are scratch the reference leaves in
+# whatever state its loops did and the step's outputs do not read.
+STEP_OUTPUTS = {
+    "timple": ["u", "v", "u_new", "v_new", "u_old", "v_old", "p_old", "pprime", "res", "source"],
+    "fs": ["u", "v", "u_new", "v_new", "u_old", "v_old", "p", "res", "source"],
+}
+
+
+@pytest.mark.parametrize("name,step", [("timple2d", "timple"), ("fs2d", "fs")])
+def test_time_steps_match_reference(name, step, gpu):
+    """Two whole time steps (time_implicit_solver, hf/timple_solver.h:40; fractional_step_explicit,
+    hf/fractional_step_solver.h:47) from the reference's own state, with its operators and its
+    natural sweep order: the step's fields equal the reference's bit for bit, its steady-state
+    error to 1e-14 relative."""
+    g = load_golden(name)
+    print(f"[gpu time step] {name}", flush=True)
+    P = gpu_problem(gpu, g)
+    _load_reference_operators(P, g)
+    dim = case_meta(g)[0]
+    for k in range(num_levels(g)):
+        for key in ("Dx", "Dy") + (("Dz",) if dim == 3 else ()):
+            P.set_matrix(k, key, g[f"L{k}_{key}"])
+        for fid in range(19):
+            P.set_field(k, FIELD_NAMES[fid], g[f"st_init_L{k}_{fid}"])
+    d, iv = g["st_params_d"], g["st_params_i"]
+    P.ns_set_params(dt=d[0], rho=d[1], mu=d[2], nu=d[3], facRe=d[4], facdt=d[5], field_rho=d[6],
+                    iter_momentum=int(iv[0]), iter_timple=int(iv[1]), num_vcycles=int(iv[2]))
+    for s in range(2):
+        err = P.timple_step() if step == "timple" else P.fs_step()
+        bad = []
+        for fid in range(19):
+            a, b = P.field(0, FIELD_NAMES[fid]), g[f"st{s}_L0_{fid}"]
+            if not np.array_equal(a, b):
+                bad.append((FIELD_NAMES[fid], float(np.abs(a - b).max()), float(np.abs(b).max())))
+        print(f"  step {s}: err {err!r} ref {float(g[f'st{s}_err'])!r} mismatched {bad}", flush=True)
+        assert not [b for b in bad if b[0] in STEP_OUTPUTS[step]], (s, bad)
+        # the steady-state error is a sum of squares over all nodes: the GPU adds
+        # it as a tree, the reference left to right, so it may differ in the last bit
+        ref_err = float(g[f"st{s}_err"])
+        assert abs(err - ref_err) <= 1e-14 * abs(ref_err), (s, err, ref_err)
