@@ -116,6 +116,7 @@ def main():
     ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the reduced-precision solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -150,6 +151,25 @@ def main():
               file=sys.stderr, flush=True)
     converged = all(v <= args.tol for v in finals)
     t_solve = float(np.median(times))
+    p64 = P.field(0, "p")
+
+    # reduced-precision inner V-cycles inside FP64 defect correction
+    mixed = {}
+    if not args.no_mixed:
+        for prec, tag in ((1, "fp32"), (2, "fp32_bf16_coarse")):
+            reset(P, f)
+            P.solve(args.tol, args.max_cycles, precision=prec)      # builds the narrowed copies
+            ts = []
+            for _ in range(args.steps):
+                reset(P, f)
+                r = P.solve(args.tol, args.max_cycles, precision=prec)
+                ts.append(r["seconds"])
+            err = float(np.abs(P.field(0, "p") - p64).max() / np.abs(p64).max())
+            mixed[tag] = {"s": round(float(np.median(ts)), 5), "cycles": r["cycles"],
+                          "final_ratio": float(r["history"][-1]),
+                          "max_rel_diff_vs_fp64_solution": err}
+            print(f"[bench] precision {prec} ({tag}): {r['cycles']} cycles {np.median(ts):.4f}s "
+                  f"final {r['history'][-1]:.3e} field diff vs FP64 {err:.2e}", file=sys.stderr, flush=True)
 
     # stencil apply roofline: lap_Poison over level 0, FP64 weights + int32 columns
     t_apply = P.time_spmv(0, args.spmv_reps)
@@ -183,7 +203,8 @@ def main():
                                f"tol {args.tol}",
                    "cycles": cycles[0], "converged": converged,
                    "final_ratio": finals[0], "setup_s": round(t_setup, 2),
-                   "stencil_apply_GBps": round(achieved, 1)},
+                   "stencil_apply_GBps": round(achieved, 1),
+                   "reduced_precision": mixed},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,

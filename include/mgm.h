@@ -164,7 +164,11 @@ int mgm_timple_step(mgm_problem *p, double *steady_error);
 /* V-cycles on level-0 (field P, SOURCE) until l2(r)/l2(f) <= tol (l2_norm_gen
  * norms, r = source - lap_Poison*p on interior nodes) or max_cycles.  history
  * (may be NULL) receives max_cycles+1 ratios (index 0 = before the first cycle).
- * precision: 0 FP64; 1 FP32 inner V-cycle inside FP64 defect correction. */
+ * precision: 0 FP64 V-cycles (any variant, either sweep order);
+ *            1 defect correction: FP64 residual and update, each correction one
+ *              FP32 V-cycle (FP32 weights, vectors and sums) from zero;
+ *            2 as 1 with BF16-stored weights on levels >= 1.
+ *            1 and 2 support the H variant in the multicolour order. */
 int mgm_solve(mgm_problem *p, int variant, int precision, double tol, int max_cycles,
               int *cycles_done, double *history, double *seconds);
 /* Time `reps` applications of lap_Poison on level lev; returns seconds per apply. */

@@ -119,11 +119,18 @@ int launch_weights(const WeightsArgs &a, hipStream_t s);
 // other, each column holding the slice's S entries contiguously.  Entry j of
 // row r lives at ell_at(m, S, r, j).  A wave covers one slice with 64/S lanes
 // per row, so every load instruction of the wave reads 64 consecutive values.
-struct Ell {
+// TW is the stored weight type: double, float, or bf16 (the upper 16 bits of
+// an IEEE float, widened exactly to float when used).
+struct bf16 {
+    uint16_t bits;
+};
+template <typename TW>
+struct EllT {
     const int *col;
-    const double *w;
+    const TW *w;
     int m, S;
 };
+using Ell = EllT<double>;
 __host__ __device__ inline size_t ell_at(int m, int S, int r, int j)
 {
     return (size_t)(r / S) * (size_t)S * (size_t)m + (size_t)j * S + (size_t)(r % S);
@@ -135,11 +142,15 @@ inline size_t ell_size(int N, int m, int S) { return (size_t)((N + S - 1) / S) *
 // Device row tags (dbtag): 0 interior, 1 boundary, 2 padding row.  Row counts
 // passed to the full-pass launchers are device row counts (padding included).
 enum { ROW_INTERIOR = 0, ROW_BOUNDARY = 1, ROW_PAD = 2 };
+// The stencil launchers are instantiated for (TW, TV) = (double, double),
+// (float, float) and (bf16, float); TV is the vector/accumulation type.
 // y = A x over rows [0, nrows); rows tagged ROW_PAD in btag (nullable) are skipped.
-int launch_spmv(Ell A, int nrows, const uint8_t *btag, const double *x, double *y, hipStream_t s);
+template <typename TW, typename TV>
+int launch_spmv(EllT<TW> A, int nrows, const uint8_t *btag, const TV *x, TV *y, hipStream_t s);
 // One SOR update of the rows in the given slices (one colour class).
-int launch_sor_slices(Ell A, const int *slices, int nslices, const uint8_t *btag,
-                      const double *src, double *p, double w, double c, hipStream_t s);
+template <typename TW, typename TV>
+int launch_sor_slices(EllT<TW> A, const int *slices, int nslices, const uint8_t *btag, const TV *src,
+                      TV *p, TV w, TV c, hipStream_t s);
 // The whole relaxation of a small level (all sweeps, all colour classes, and
 // for the T / F variants the pin and residual after each sweep, A2 being the
 // operator of the F residual) in one single-workgroup launch.
@@ -153,19 +164,27 @@ int launch_sor_natural(Ell A, int N, const int *iperm, const double *src, double
 //                 1 = interior, res = s; res -= ... sequentially (boundary 0) (T fused)
 //                 2 = all rows, res = s; res -= ... sequentially             (F fused)
 //                 3 = interior, res = s - sum (boundary untouched)           (F)
-int launch_residual(Ell A, int N, int form, const uint8_t *btag, const double *src,
-                    const double *p, double *res, hipStream_t s);
+template <typename TW, typename TV>
+int launch_residual(EllT<TW> A, int N, int form, const uint8_t *btag, const TV *src, const TV *p,
+                    TV *res, hipStream_t s);
 // restrict: out[r] = btag ? (keep_boundary ? out[r] : 0) : sum_j w*in[col]
-int launch_restrict(Ell R, int N, const uint8_t *btag, int keep_boundary, const double *in,
-                    double *out, hipStream_t s);
+template <typename TW, typename TV>
+int launch_restrict(EllT<TW> R, int N, const uint8_t *btag, int keep_boundary, const TV *in, TV *out,
+                    hipStream_t s);
 // prolong: p[r] += sum_j w*pc[col] on interior rows
-int launch_prolong(Ell P, int N, const uint8_t *btag, const double *pc, double *p, hipStream_t s);
-int launch_zero(int N, const uint8_t *btag, int interior_only, double *p, hipStream_t s);
+template <typename TW, typename TV>
+int launch_prolong(EllT<TW> P, int N, const uint8_t *btag, const TV *pc, TV *p, hipStream_t s);
+template <typename TV>
+int launch_zero(int N, const uint8_t *btag, int interior_only, TV *p, hipStream_t s);
 int launch_pin(int N, const uint8_t *btag, const int *ref0_dev, double *p, double *scratch,
                hipStream_t s);
 // sum over interior rows of (a-b)^2 into *out (device scalar); b nullable
 int launch_sqdiff_sum(int N, const uint8_t *btag, const double *a, const double *b,
                       double *partial, double *out, hipStream_t s);
+// Mixed precision: element-wise narrowing (n values) and p += e on non-padding rows.
+int launch_narrow_f32(long n, const double *in, float *out, hipStream_t s);
+int launch_narrow_bf16(long n, const double *in, bf16 *out, hipStream_t s);
+int launch_add_correction(int N, const uint8_t *btag, const float *e, double *p, hipStream_t s);
 // Row-major reference-order rows (N x m) -> slab in device order (iperm:
 // reference -> device row); columns mapped through cloud and iperm.
 int launch_gather_rows(int N, int m, int S, const double *rows, const int *iperm,

@@ -1,0 +1,129 @@
+// mixed.cpp -- the reduced-precision inner V-cycle of mgm_solve's defect
+// correction.  The FP64 operators of every level are narrowed once to FP32
+// (precision 1) or, on levels >= 1, to BF16 (precision 2; level 0 stays
+// FP32); vectors, products and sums are FP32.  The cycle is the H variant of
+// the FP64 one (hf/heat_conduction.h:25-112 order and boundary handling) in
+// multicolour order, started from a zero correction.  It has no reference
+// counterpart: the reference solves in FP64 only.
+#include "problem.hpp"
+
+namespace mgm {
+
+namespace {
+
+template <typename F>
+int with_op(const Level &L, F &&f)
+{
+    if (L.w16) return f(EllT<bf16>{L.col.p, L.lapp16.p, L.m, L.S});
+    return f(EllT<float>{L.col.p, L.lapp32.p, L.m, L.S});
+}
+
+int relax_mp(mgm_problem *pb, Level &L)
+{
+    const float w = pb->cfg.omega;
+    const float c = 1.0f - pb->cfg.omega;
+    return with_op(L, [&](auto A) {
+        for (int it = 0; it < pb->cfg.num_relax; it++)
+            for (int k = 0; k < L.ncol; k++) {
+                int rc = launch_sor_slices(A, L.d_col_slices.p + L.col_soff[k], L.col_soff[k + 1] - L.col_soff[k],
+                                           L.dbtag.p, (const float *)L.src32.p, L.p32.p, w, c, stream());
+                if (rc) return rc;
+            }
+        return 0;
+    });
+}
+
+int residual_mp(Level &L)
+{
+    return with_op(L, [&](auto A) {
+        return launch_residual(A, L.Nd, 0, L.dbtag.p, (const float *)L.src32.p, (const float *)L.p32.p,
+                               L.res32.p, stream());
+    });
+}
+
+int restrict_mp(Level &F, Level &C)
+{
+    if (C.w16)
+        return launch_restrict(EllT<bf16>{C.rcol.p, C.rw16.p, C.mr, C.Sr}, C.Nd, C.dbtag.p, 0,
+                               (const float *)F.res32.p, C.src32.p, stream());
+    return launch_restrict(EllT<float>{C.rcol.p, C.rw32.p, C.mr, C.Sr}, C.Nd, C.dbtag.p, 0,
+                           (const float *)F.res32.p, C.src32.p, stream());
+}
+
+int prolong_mp(Level &F, Level &C)
+{
+    int rc;
+    if (F.w16)
+        rc = launch_prolong(EllT<bf16>{F.pcol.p, F.pw16.p, F.mp, F.Sp}, F.Nd, F.dbtag.p,
+                            (const float *)C.p32.p, F.p32.p, stream());
+    else
+        rc = launch_prolong(EllT<float>{F.pcol.p, F.pw32.p, F.mp, F.Sp}, F.Nd, F.dbtag.p,
+                            (const float *)C.p32.p, F.p32.p, stream());
+    if (rc) return rc;
+    return launch_zero(C.Nd, C.dbtag.p, 1, C.p32.p, stream());
+}
+
+template <typename T>
+int narrow(const DevBuf<double> &src, DevBuf<T> &dst)
+{
+    HIP_TRY(dst.alloc(src.n));
+    if constexpr (sizeof(T) == 4) return launch_narrow_f32((long)src.n, src.p, dst.p, stream());
+    else return launch_narrow_bf16((long)src.n, src.p, dst.p, stream());
+}
+
+}  // namespace
+
+// Build the narrowed operators and the FP32 fields for `precision` (1 or 2).
+int mp_prepare(mgm_problem *pb, int precision)
+{
+    if (pb->mp_built == precision) return 0;
+    for (int l = 0; l < pb->L; l++) {
+        Level &L = pb->lv[l];
+        L.w16 = (precision == 2 && l > 0);
+        int rc;
+        if (L.w16) {
+            L.lapp32.release(); L.rw32.release(); L.pw32.release();
+            if ((rc = narrow(L.slab[OP_LAPP], L.lapp16))) return rc;
+            if (l > 0 && (rc = narrow(L.rw, L.rw16))) return rc;
+            if (l + 1 < pb->L && (rc = narrow(L.pw, L.pw16))) return rc;
+        } else {
+            L.lapp16.release(); L.rw16.release(); L.pw16.release();
+            if ((rc = narrow(L.slab[OP_LAPP], L.lapp32))) return rc;
+            if (l > 0 && (rc = narrow(L.rw, L.rw32))) return rc;
+            if (l + 1 < pb->L && (rc = narrow(L.pw, L.pw32))) return rc;
+        }
+        for (DevBuf<float> *f : {&L.p32, &L.src32, &L.res32}) {
+            if (f->n != (size_t)L.Nd) {
+                HIP_TRY(f->alloc(L.Nd));
+                HIP_TRY(hipMemsetAsync(f->p, 0, sizeof(float) * L.Nd, stream()));
+            }
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(stream()));
+    pb->mp_built = precision;
+    return 0;
+}
+
+// One defect-correction step: e = V-cycle(A, r) in reduced precision from
+// e = 0, then p += e.  RES of level 0 must hold r (FP64) on entry.
+int op_correction_mp(mgm_problem *pb)
+{
+    Level &L0 = pb->lv[0];
+    int rc;
+    if ((rc = launch_narrow_f32(L0.Nd, L0.f[MGM_F_RES].p, L0.src32.p, stream()))) return rc;
+    for (auto &L : pb->lv)
+        if ((rc = launch_zero(L.Nd, L.dbtag.p, 0, L.p32.p, stream()))) return rc;
+    const int nl = pb->L;
+    for (int l = 0; l < nl; l++) {
+        if ((rc = relax_mp(pb, pb->lv[l]))) return rc;
+        if ((rc = residual_mp(pb->lv[l]))) return rc;
+        if (l != nl - 1 && (rc = restrict_mp(pb->lv[l], pb->lv[l + 1]))) return rc;
+    }
+    for (int l = nl - 1; l > 0; l--) {
+        if ((rc = prolong_mp(pb->lv[l - 1], pb->lv[l]))) return rc;
+        if (l != 1 && (rc = relax_mp(pb, pb->lv[l - 1]))) return rc;
+    }
+    return launch_add_correction(L0.Nd, L0.dbtag.p, L0.p32.p, L0.f[MGM_F_P].p, stream());
+}
+
+}  // namespace mgm

@@ -57,6 +57,11 @@ struct Level {
     int S = 8, Sr = 8, Sp = 8;                 // slice heights of the operator / transfer slabs
     std::array<DevBuf<double>, MGM_NUM_FIELDS> f;
     DevBuf<int> ref0;                          // device row of reference node 0
+    // ---- mixed-precision copies (built on demand, device order) ----
+    bool w16 = false;                          // this level's weights are stored as BF16
+    DevBuf<float> lapp32, rw32, pw32;
+    DevBuf<bf16> lapp16, rw16, pw16;
+    DevBuf<float> p32, src32, res32;
 
     Ell op(int which) const { return Ell{col.p, slab[which].p, m, S}; }
     Ell restr() const { return Ell{rcol.p, rw.p, mr, Sr}; }
@@ -75,5 +80,6 @@ struct mgm_problem {
     double *host_scal = nullptr;   // pinned
     mgm_ns_params ns{};
     bool ns_set = false;
+    int mp_built = 0;              // precision mode the FP32/BF16 copies hold (0: none / stale)
     ~mgm_problem();
 };

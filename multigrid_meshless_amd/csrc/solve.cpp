@@ -15,6 +15,8 @@
 
 namespace mgm {
 int op_vcycle(mgm_problem *pb, int variant);
+int mp_prepare(mgm_problem *pb, int precision);
+int op_correction_mp(mgm_problem *pb);
 int op_relax(mgm_problem *pb, int l, int variant);
 }  // namespace mgm
 
@@ -54,11 +56,17 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
               int *cycles_done, double *history, double *seconds)
 {
     if (!pb || !pb->ready) MGM_FAIL(-1, "problem is not set up");
-    if (precision != 0) MGM_FAIL(-1, "mgm_solve: precision mode not available in this build");
+    if (precision < 0 || precision > 2) MGM_FAIL(-1, "mgm_solve: precision must be 0, 1 or 2");
     if (variant != MGM_VAR_H && variant != MGM_VAR_T && variant != MGM_VAR_F) MGM_FAIL(-1, "bad variant");
+    if (precision && variant != MGM_VAR_H) MGM_FAIL(-1, "mgm_solve: reduced precision supports the H variant");
+    if (precision && pb->cfg.sweep_order != MGM_ORDER_COLOUR)
+        MGM_FAIL(-1, "mgm_solve: reduced precision needs the multicolour sweep order");
     if (max_cycles < 0) MGM_FAIL(-1, "max_cycles < 0");
     Level &L0 = pb->lv[0];
     int rc;
+    if (precision && (rc = mp_prepare(pb, precision))) return rc;
+    // one cycle: FP64 V-cycle, or reduced-precision correction of the FP64 iterate
+    auto cycle = [&]() { return precision ? op_correction_mp(pb) : op_vcycle(pb, variant); };
     // |f| over interior nodes
     rc = launch_sqdiff_sum(L0.Nd, L0.dbtag.p, L0.f[MGM_F_SOURCE].p, nullptr, pb->partial.p, pb->scal.p, stream());
     if (rc) return rc;
@@ -80,7 +88,7 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
     hipGraphExec_t exec = nullptr;
     if (use_graph) {
         HIP_TRY(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-        rc = op_vcycle(pb, variant);
+        rc = cycle();
         if (!rc) rc = enqueue_fine_residual(pb, variant);
         hipError_t ce = hipStreamEndCapture(stream(), &graph);
         if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
@@ -100,7 +108,7 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
             hipError_t le = hipGraphLaunch(exec, stream());
             if (le != hipSuccess) { rc = -10; set_error(hipGetErrorString(le)); break; }
         } else {
-            if ((rc = op_vcycle(pb, variant))) break;
+            if ((rc = cycle())) break;
             if ((rc = enqueue_fine_residual(pb, variant))) break;
         }
         if ((rc = read_scalar(pb, &rs))) break;

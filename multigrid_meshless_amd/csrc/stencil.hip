@@ -69,14 +69,20 @@ __device__ inline long xcd_block(long b, long nb)
     return (xcd < rem) ? xcd * (q + 1) + idx : rem * (q + 1) + (xcd - rem) * q + idx;
 }
 
+// Weight conversion: FP64 and FP32 weights are used as stored, BF16 weights
+// (upper half of an FP32) widen exactly to FP32.
+__device__ inline double wv(double v) { return v; }
+__device__ inline float wv(float v) { return v; }
+__device__ inline float wv(bf16 v) { return __uint_as_float((unsigned)v.bits << 16); }
+
 // One wave applies the rows of one slice: lanes (ri, jo) stage the products
 // w*x of entries j = jo, jo+LPR, ... of row ri in LDS at P[j*S + ri]; after a
-// wave barrier lane jo == 0 combines its row in stencil order.
-template <int S>
-__device__ inline void slice_rows(const Ell &A, long slice, int nrows, int mode,
-                                  const uint8_t *__restrict__ btag, const double *__restrict__ src,
-                                  const double *x, double *y, double w, double c, int keep_boundary,
-                                  double *P)
+// wave barrier lane jo == 0 combines its row in stencil order.  TW is the
+// stored weight type, TV the type of vectors, products and sums.
+template <int S, typename TW, typename TV>
+__device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int mode,
+                                  const uint8_t *__restrict__ btag, const TV *__restrict__ src,
+                                  const TV *x, TV *y, TV w, TV c, int keep_boundary, TV *P)
 {
     constexpr int LPR = 64 / S;
     const int lane = threadIdx.x & 63;
@@ -93,31 +99,31 @@ __device__ inline void slice_rows(const Ell &A, long slice, int nrows, int mode,
 #pragma unroll 4
         for (int j = jo; j < m; j += LPR) {
             const size_t q = base + (size_t)j * S;
-            P[j * S + ri] = A.w[q] * x[A.col[q]];
+            P[j * S + ri] = wv(A.w[q]) * x[A.col[q]];
         }
     }
     wave_sync();
     if (jo == 0 && act) {
         switch (mode) {
         case M_SPMV: {
-            double acc = 0.0;
+            TV acc = 0;
             for (int j = 0; j < m; j++) acc += P[j * S + ri];
             y[r] = acc;
             break;
         }
         case M_SOR: {
-            double sum = 0.0;
+            TV sum = 0;
             for (int j = 1; j < m; j++) sum += P[j * S + ri];
-            y[r] = w * ((src[r] - sum) / A.w[base]) + c * y[r];
+            y[r] = w * ((src[r] - sum) / wv(A.w[base])) + c * y[r];
             break;
         }
         case M_RES0:
         case M_RES3: {
             if (bnd) {
-                if (mode == M_RES0) y[r] = 0.0;
+                if (mode == M_RES0) y[r] = 0;
                 break;
             }
-            double sum = 0;
+            TV sum = 0;
             for (int j = 0; j < m; j++) sum += P[j * S + ri];
             y[r] = src[r] - sum;
             break;
@@ -125,24 +131,24 @@ __device__ inline void slice_rows(const Ell &A, long slice, int nrows, int mode,
         case M_RES1:
         case M_RES2: {
             if (mode == M_RES1 && bnd) { y[r] = 0; break; }
-            double v = src[r];
+            TV v = src[r];
             for (int j = 0; j < m; j++) v = v - P[j * S + ri];
             y[r] = v;
             break;
         }
         case M_RESTRICT: {
             if (bnd) {
-                if (!keep_boundary) y[r] = 0.0;
+                if (!keep_boundary) y[r] = 0;
                 break;
             }
-            double acc = 0.0;
+            TV acc = 0;
             for (int j = 0; j < m; j++) acc += P[j * S + ri];
             y[r] = acc;
             break;
         }
         case M_PROLONG: {
             if (bnd) break;
-            double acc = 0.0;
+            TV acc = 0;
             for (int j = 0; j < m; j++) acc += P[j * S + ri];
             y[r] = y[r] + acc;
             break;
@@ -155,19 +161,20 @@ __device__ inline void slice_rows(const Ell &A, long slice, int nrows, int mode,
 // Slice k of the launch is slices[k] when a slice list is given (one colour
 // class), else k itself (a full pass).  Rows at or beyond nrows and padding
 // rows (tag ROW_PAD) are neither read nor written.
-template <int S>
+template <int S, typename TW, typename TV>
 __global__ void __launch_bounds__(BLK)
-rows_kernel(Ell A, const int *__restrict__ slices, int nslices, int nrows, int mode,
-            const uint8_t *__restrict__ btag, const double *__restrict__ src, const double *x,
-            double *y, double w, double c, int keep_boundary)
+rows_kernel(EllT<TW> A, const int *__restrict__ slices, int nslices, int nrows, int mode,
+            const uint8_t *__restrict__ btag, const TV *__restrict__ src, const TV *x, TV *y, TV w,
+            TV c, int keep_boundary)
 {
-    extern __shared__ double prod[];          // [WAVES][m][S]
+    extern __shared__ unsigned char lds_raw[];
+    TV *prod = reinterpret_cast<TV *>(lds_raw);   // [WAVES][m][S]
     const int wave = threadIdx.x >> 6;
     const long k = xcd_block(blockIdx.x, gridDim.x) * WAVES + wave;
     if (k >= nslices) return;
     const long slice = slices ? (long)slices[k] : k;
-    slice_rows<S>(A, slice, nrows, mode, btag, src, x, y, w, c, keep_boundary,
-                  prod + (size_t)wave * A.m * S);
+    slice_rows<S, TW, TV>(A, slice, nrows, mode, btag, src, x, y, w, c, keep_boundary,
+                          prod + (size_t)wave * A.m * S);
 }
 
 // Whole relaxation of a small level in one workgroup: num_relax sweeps over
@@ -182,7 +189,8 @@ relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int 
                    const double *__restrict__ src, double *p, double *res, double w, double c,
                    int num_relax, int variant)
 {
-    extern __shared__ double prod[];          // [SMALL_BLK/64][max(m, m2)][S]
+    extern __shared__ unsigned char lds_raw[];
+    double *prod = reinterpret_cast<double *>(lds_raw);   // [SMALL_BLK/64][max(m, m2)][S]
     constexpr int NW = SMALL_BLK / 64;
     const int wave = threadIdx.x >> 6;
     const int mw = A.m > A2.m ? A.m : A2.m;
@@ -193,7 +201,7 @@ relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int 
         for (int cl = 0; cl < ncol; cl++) {
             const int s0 = col_soff[cl], s1 = col_soff[cl + 1];
             for (int k = s0 + wave; k < s1; k += NW)
-                slice_rows<S>(A, col_slices[k], nrows, M_SOR, btag, src, p, p, w, c, 0, P);
+                slice_rows<S, double, double>(A, col_slices[k], nrows, M_SOR, btag, src, p, p, w, c, 0, P);
             __syncthreads();
         }
         if (variant == MGM_VAR_H) continue;
@@ -206,7 +214,7 @@ relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int 
         const Ell &R = (variant == MGM_VAR_T) ? A : A2;
         const int form = (variant == MGM_VAR_T) ? M_RES1 : M_RES2;
         for (long k = wave; k < nsl; k += NW)
-            slice_rows<S>(R, k, nrows, form, btag, src, p, res, 0, 0, 0, P);
+            slice_rows<S, double, double>(R, k, nrows, form, btag, src, p, res, 0.0, 0.0, 0, P);
         __syncthreads();
     }
 }
@@ -228,12 +236,13 @@ __global__ void sor_natural_kernel(Ell A, int N, const int *__restrict__ iperm,
     }
 }
 
+template <typename TV>
 __global__ void __launch_bounds__(BLK)
-zero_kernel(int N, const uint8_t *__restrict__ btag, int interior_only, double *p)
+zero_kernel(int N, const uint8_t *__restrict__ btag, int interior_only, TV *p)
 {
     int r = blockIdx.x * BLK + threadIdx.x;
     if (r >= N || btag[r] == ROW_PAD) return;
-    if (!interior_only || !btag[r]) p[r] = 0.0;
+    if (!interior_only || !btag[r]) p[r] = 0;
 }
 
 __global__ void copy_scalar_kernel(const int *ref0, const double *p, double *dst)
@@ -284,6 +293,33 @@ sum_partial_kernel(int n, const double *__restrict__ partial, double *__restrict
         __syncthreads();
     }
     if (threadIdx.x == 0) *out = sh[0];
+}
+
+// Narrowing copies for the mixed-precision V-cycle.  BF16 rounds to nearest,
+// ties to even (weights are finite).
+__global__ void __launch_bounds__(BLK) narrow_f32_kernel(long n, const double *__restrict__ in,
+                                                          float *__restrict__ out)
+{
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t < n) out[t] = (float)in[t];
+}
+__global__ void __launch_bounds__(BLK) narrow_bf16_kernel(long n, const double *__restrict__ in,
+                                                           bf16 *__restrict__ out)
+{
+    long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= n) return;
+    unsigned u = __float_as_uint((float)in[t]);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    out[t].bits = (uint16_t)(u >> 16);
+}
+// p += e on every non-padding row
+__global__ void __launch_bounds__(BLK) add_correction_kernel(int N, const uint8_t *__restrict__ btag,
+                                                              const float *__restrict__ e,
+                                                              double *__restrict__ p)
+{
+    int r = blockIdx.x * BLK + threadIdx.x;
+    if (r >= N || btag[r] == ROW_PAD) return;
+    p[r] = p[r] + (double)e[r];
 }
 
 // Row-major reference-order rows -> sliced-ELL device-order slab.
@@ -343,23 +379,23 @@ dirichlet_rows_kernel(int N, int m, const uint8_t *__restrict__ btag, double *__
 }
 
 // slices == nullptr: all slices covering rows [0, nrows)
-int launch_rows(Ell A, const int *slices, int nslices, int nrows, int mode, const uint8_t *btag,
-                const double *src, const double *x, double *y, double w, double c, int keep,
-                hipStream_t s)
+template <typename TW, typename TV>
+int launch_rows(EllT<TW> A, const int *slices, int nslices, int nrows, int mode, const uint8_t *btag,
+                const TV *src, const TV *x, TV *y, TV w, TV c, int keep, hipStream_t s)
 {
     if (!slices) nslices = (nrows + A.S - 1) / A.S;
     if (nslices <= 0 || nrows <= 0) return 0;
     int blocks = (nslices + WAVES - 1) / WAVES;
-    size_t shm = (size_t)WAVES * A.m * A.S * sizeof(double);
+    size_t shm = (size_t)WAVES * A.m * A.S * sizeof(TV);
     if (A.S == 4)
-        hipLaunchKernelGGL(rows_kernel<4>, dim3(blocks), dim3(BLK), shm, s, A, slices, nslices, nrows,
-                           mode, btag, src, x, y, w, c, keep);
+        hipLaunchKernelGGL((rows_kernel<4, TW, TV>), dim3(blocks), dim3(BLK), shm, s, A, slices, nslices,
+                           nrows, mode, btag, src, x, y, w, c, keep);
     else if (A.S == 8)
-        hipLaunchKernelGGL(rows_kernel<8>, dim3(blocks), dim3(BLK), shm, s, A, slices, nslices, nrows,
-                           mode, btag, src, x, y, w, c, keep);
+        hipLaunchKernelGGL((rows_kernel<8, TW, TV>), dim3(blocks), dim3(BLK), shm, s, A, slices, nslices,
+                           nrows, mode, btag, src, x, y, w, c, keep);
     else if (A.S == 16)
-        hipLaunchKernelGGL(rows_kernel<16>, dim3(blocks), dim3(BLK), shm, s, A, slices, nslices, nrows,
-                           mode, btag, src, x, y, w, c, keep);
+        hipLaunchKernelGGL((rows_kernel<16, TW, TV>), dim3(blocks), dim3(BLK), shm, s, A, slices, nslices,
+                           nrows, mode, btag, src, x, y, w, c, keep);
     else
         MGM_FAIL(-1, "unsupported slice height");
     HIP_TRY(hipGetLastError());
@@ -387,15 +423,17 @@ int slice_height(int m)
     return m > 36 ? 8 : 16;
 }
 
-int launch_spmv(Ell A, int nrows, const uint8_t *btag, const double *x, double *y, hipStream_t s)
+template <typename TW, typename TV>
+int launch_spmv(EllT<TW> A, int nrows, const uint8_t *btag, const TV *x, TV *y, hipStream_t s)
 {
-    return launch_rows(A, nullptr, 0, nrows, M_SPMV, btag, nullptr, x, y, 0, 0, 0, s);
+    return launch_rows<TW, TV>(A, nullptr, 0, nrows, M_SPMV, btag, nullptr, x, y, TV(0), TV(0), 0, s);
 }
-int launch_sor_slices(Ell A, const int *slices, int nslices, const uint8_t *btag,
-                      const double *src, double *p, double w, double c, hipStream_t s)
+template <typename TW, typename TV>
+int launch_sor_slices(EllT<TW> A, const int *slices, int nslices, const uint8_t *btag, const TV *src,
+                      TV *p, TV w, TV c, hipStream_t s)
 {
     if (nslices <= 0) return 0;
-    return launch_rows(A, slices, nslices, 0x7fffffff, M_SOR, btag, src, p, p, w, c, 0, s);
+    return launch_rows<TW, TV>(A, slices, nslices, 0x7fffffff, M_SOR, btag, src, p, p, w, c, 0, s);
 }
 int launch_relax_small(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol, int nrows,
                        const uint8_t *btag, const int *ref0, const double *src, double *p, double *res,
@@ -431,27 +469,48 @@ int launch_sor_natural(Ell A, int N, const int *iperm, const double *src, double
     HIP_TRY(hipGetLastError());
     return 0;
 }
-int launch_residual(Ell A, int N, int form, const uint8_t *btag, const double *src,
-                    const double *p, double *res, hipStream_t s)
+template <typename TW, typename TV>
+int launch_residual(EllT<TW> A, int N, int form, const uint8_t *btag, const TV *src, const TV *p,
+                    TV *res, hipStream_t s)
 {
     static const int modes[4] = {M_RES0, M_RES1, M_RES2, M_RES3};
     if (form < 0 || form > 3) MGM_FAIL(-1, "bad residual form");
-    return launch_rows(A, nullptr, 0, N, modes[form], btag, src, p, res, 0, 0, 0, s);
+    return launch_rows<TW, TV>(A, nullptr, 0, N, modes[form], btag, src, p, res, TV(0), TV(0), 0, s);
 }
-int launch_restrict(Ell R, int N, const uint8_t *btag, int keep_boundary, const double *in,
-                    double *out, hipStream_t s)
+template <typename TW, typename TV>
+int launch_restrict(EllT<TW> R, int N, const uint8_t *btag, int keep_boundary, const TV *in, TV *out,
+                    hipStream_t s)
 {
-    return launch_rows(R, nullptr, 0, N, M_RESTRICT, btag, nullptr, in, out, 0, 0, keep_boundary, s);
+    return launch_rows<TW, TV>(R, nullptr, 0, N, M_RESTRICT, btag, nullptr, in, out, TV(0), TV(0),
+                               keep_boundary, s);
 }
-int launch_prolong(Ell P, int N, const uint8_t *btag, const double *pc, double *p, hipStream_t s)
+template <typename TW, typename TV>
+int launch_prolong(EllT<TW> P, int N, const uint8_t *btag, const TV *pc, TV *p, hipStream_t s)
 {
-    return launch_rows(P, nullptr, 0, N, M_PROLONG, btag, nullptr, pc, p, 0, 0, 0, s);
+    return launch_rows<TW, TV>(P, nullptr, 0, N, M_PROLONG, btag, nullptr, pc, p, TV(0), TV(0), 0, s);
 }
-int launch_zero(int N, const uint8_t *btag, int interior_only, double *p, hipStream_t s)
+template <typename TV>
+int launch_zero(int N, const uint8_t *btag, int interior_only, TV *p, hipStream_t s)
 {
-    LAUNCH(zero_kernel, N, N, btag, interior_only, p);
+    LAUNCH(zero_kernel<TV>, N, N, btag, interior_only, p);
     return 0;
 }
+
+#define MGM_INSTANTIATE(TW, TV)                                                                      \
+    template int launch_spmv<TW, TV>(EllT<TW>, int, const uint8_t *, const TV *, TV *, hipStream_t); \
+    template int launch_sor_slices<TW, TV>(EllT<TW>, const int *, int, const uint8_t *, const TV *,  \
+                                           TV *, TV, TV, hipStream_t);                               \
+    template int launch_residual<TW, TV>(EllT<TW>, int, int, const uint8_t *, const TV *,            \
+                                         const TV *, TV *, hipStream_t);                             \
+    template int launch_restrict<TW, TV>(EllT<TW>, int, const uint8_t *, int, const TV *, TV *,      \
+                                         hipStream_t);                                               \
+    template int launch_prolong<TW, TV>(EllT<TW>, int, const uint8_t *, const TV *, TV *, hipStream_t);
+MGM_INSTANTIATE(double, double)
+MGM_INSTANTIATE(float, float)
+MGM_INSTANTIATE(bf16, float)
+#undef MGM_INSTANTIATE
+template int launch_zero<double>(int, const uint8_t *, int, double *, hipStream_t);
+template int launch_zero<float>(int, const uint8_t *, int, float *, hipStream_t);
 int launch_pin(int N, const uint8_t *btag, const int *ref0_dev, double *p, double *scratch,
                hipStream_t s)
 {
@@ -476,6 +535,21 @@ int launch_sum_partials(int n, const double *partial, double *out, hipStream_t s
 {
     hipLaunchKernelGGL(sum_partial_kernel, dim3(1), dim3(BLK), 0, s, n, partial, out);
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+int launch_narrow_f32(long n, const double *in, float *out, hipStream_t s)
+{
+    LAUNCH(narrow_f32_kernel, n, n, in, out);
+    return 0;
+}
+int launch_narrow_bf16(long n, const double *in, bf16 *out, hipStream_t s)
+{
+    LAUNCH(narrow_bf16_kernel, n, n, in, out);
+    return 0;
+}
+int launch_add_correction(int N, const uint8_t *btag, const float *e, double *p, hipStream_t s)
+{
+    LAUNCH(add_correction_kernel, N, N, btag, e, p);
     return 0;
 }
 int launch_gather_rows(int N, int m, int S, const double *rows, const int *iperm,
