@@ -157,7 +157,9 @@ int launch_sor_slices(EllT<TW> A, const int *slices, int nslices, const uint8_t 
 int launch_relax_small(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol, int nrows,
                        const uint8_t *btag, const int *ref0, const double *src, double *p, double *res,
                        double w, double c, int num_relax, int variant, hipStream_t s);
-bool relax_small_fits(int m, int m2, int S);
+// Whether a level of nrows device rows and stencil widths m, m2 fits the
+// single-workgroup relax (its p vector and staging in one workgroup's LDS).
+bool relax_small_fits(int nrows, int m, int m2, int S);
 int launch_sor_natural(Ell A, int N, const int *iperm, const double *src, double *p, double w,
                        double c, hipStream_t s);
 // residual forms: 0 = interior, res = s - sum (boundary res = 0)            (H/T)

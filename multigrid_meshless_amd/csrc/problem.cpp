@@ -1035,13 +1035,12 @@ int mgm_zero_fields(mgm_problem *pb)
 // enqueued back to back or captured into a graph.
 namespace mgm {
 
-// Levels with at most MGM_SMALL_ROWS device rows relax in one single-workgroup
-// launch.  Off by default: at 2k rows one CU's gather rate makes a colour step
-// slower than a separate launch over many CUs.
+// Levels with at most this many device rows (and whose p vector fits in LDS)
+// relax in one single-workgroup launch; MGM_SMALL_ROWS overrides, 0 disables.
 static int small_level_rows()
 {
     const char *e = getenv("MGM_SMALL_ROWS");
-    return e ? atoi(e) : 0;
+    return e ? atoi(e) : 12000;
 }
 
 static int pfield(int variant) { return variant == MGM_VAR_T ? MGM_F_PPRIME : MGM_F_P; }
@@ -1054,7 +1053,7 @@ int op_relax(mgm_problem *pb, int l, int variant)
     double *p = L.f[pfield(variant)].p, *src = L.f[MGM_F_SOURCE].p, *res = L.f[MGM_F_RES].p;
     Ell A = L.op(OP_LAPP);
     if (pb->cfg.sweep_order != MGM_ORDER_NATURAL && L.Nd <= small_level_rows() &&
-        relax_small_fits(L.m, L.m, L.S))
+        relax_small_fits(L.Nd, L.m, L.m, L.S))
         return launch_relax_small(A, L.op(OP_LAP), L.d_col_slices.p, L.d_col_soff.p, L.ncol, L.Nd,
                                   L.dbtag.p, L.ref0.p, src, p, res, w, c, pb->cfg.num_relax, variant,
                                   g_stream);

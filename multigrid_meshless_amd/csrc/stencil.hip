@@ -29,8 +29,9 @@ namespace {
 
 constexpr int BLK = 256;
 constexpr int WAVES = BLK / 64;
-constexpr int SMALL_BLK = 512;
-constexpr size_t kSmallLdsMax = 64 * 1024;
+constexpr int SMALL_BLK_MAX = 1024;
+// dynamic LDS of the single-workgroup relax: gfx950 gives a workgroup 160 KB
+constexpr size_t kSmallLdsMax = 160 * 1024 - 1024;
 
 inline int nblk(long n) { return (int)((n + BLK - 1) / BLK); }
 
@@ -75,16 +76,76 @@ __device__ inline double wv(double v) { return v; }
 __device__ inline float wv(float v) { return v; }
 __device__ inline float wv(bf16 v) { return __uint_as_float((unsigned)v.bits << 16); }
 
+// Stage the products w*x of a lane's stencil entries j = jo, jo+LPR, ... of
+// its row in LDS at P[j*S + ri] (base = the row's entry 0 in the slab).
+// NIT == 0: loop unrolled by 4, loads and gathers interleaved -- best for
+// streaming passes, many waves per CU hide the latency.  NIT > 0: all weight
+// and column loads of a chunk of NIT entries first, then all gathers -- one
+// memory round trip per chunk, for latency-bound single-workgroup sweeps.
+template <int S, int NIT, typename TW, typename TV>
+__device__ inline void stage_products(const EllT<TW> &A, size_t base, int ri, int jo, const TV *x, TV *P)
+{
+    constexpr int LPR = 64 / S;
+    const int m = A.m;
+    if constexpr (NIT == 0) {
+#pragma unroll 4
+        for (int j = jo; j < m; j += LPR) {
+            const size_t q = base + (size_t)j * S;
+            P[j * S + ri] = wv(A.w[q]) * x[A.col[q]];
+        }
+    } else {
+        for (int j0 = 0; j0 < m; j0 += NIT * LPR) {
+            TW wq[NIT];
+            int cq[NIT];
+#pragma unroll
+            for (int t = 0; t < NIT; t++) {
+                const int j = j0 + t * LPR + jo;
+                if (j < m) {
+                    wq[t] = A.w[base + (size_t)j * S];
+                    cq[t] = A.col[base + (size_t)j * S];
+                }
+            }
+            TV xq[NIT];
+#pragma unroll
+            for (int t = 0; t < NIT; t++) {
+                const int j = j0 + t * LPR + jo;
+                if (j < m) xq[t] = x[cq[t]];
+            }
+#pragma unroll
+            for (int t = 0; t < NIT; t++) {
+                const int j = j0 + t * LPR + jo;
+                if (j < m) P[j * S + ri] = wv(wq[t]) * xq[t];
+            }
+        }
+    }
+}
+
+// acc (op) P[j*S + ri] for j = j0, j0+1, ..., m-1 in that order, op = + or -:
+// the reference's left-to-right row sum.  Four products are read per step so
+// four LDS reads are in flight per dependent wait; a longer unroll grows the
+// code that every launch fetches cold and measured slower.
+template <bool SUBTRACT, typename TV>
+__device__ inline TV ordered_sum(const TV *P, int S, int ri, int j0, int m, TV acc)
+{
+    int j = j0;
+    for (; j + 4 <= m; j += 4) {
+        const TV a = P[j * S + ri], b = P[(j + 1) * S + ri], c = P[(j + 2) * S + ri], d = P[(j + 3) * S + ri];
+        if (SUBTRACT) { acc = acc - a; acc = acc - b; acc = acc - c; acc = acc - d; }
+        else { acc = acc + a; acc = acc + b; acc = acc + c; acc = acc + d; }
+    }
+    for (; j < m; j++) acc = SUBTRACT ? acc - P[j * S + ri] : acc + P[j * S + ri];
+    return acc;
+}
+
 // One wave applies the rows of one slice: lanes (ri, jo) stage the products
-// w*x of entries j = jo, jo+LPR, ... of row ri in LDS at P[j*S + ri]; after a
-// wave barrier lane jo == 0 combines its row in stencil order.  TW is the
-// stored weight type, TV the type of vectors, products and sums.
-template <int S, typename TW, typename TV>
+// of row ri (stage_products); after a wave barrier lane jo == 0 combines its
+// row in stencil order.  TW is the stored weight type, TV the type of
+// vectors, products and sums.
+template <int S, typename TW, typename TV, int NIT = 0>
 __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int mode,
                                   const uint8_t *__restrict__ btag, const TV *__restrict__ src,
                                   const TV *x, TV *y, TV w, TV c, int keep_boundary, TV *P)
 {
-    constexpr int LPR = 64 / S;
     const int lane = threadIdx.x & 63;
     const int ri = lane % S, jo = lane / S;
     const int m = A.m;
@@ -95,25 +156,15 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
     const bool bnd = tag == ROW_BOUNDARY;
     const bool need = act && !(bnd && skips_boundary(mode));
     const size_t base = (size_t)slice * S * m + ri;
-    if (need) {
-#pragma unroll 4
-        for (int j = jo; j < m; j += LPR) {
-            const size_t q = base + (size_t)j * S;
-            P[j * S + ri] = wv(A.w[q]) * x[A.col[q]];
-        }
-    }
+    if (need) stage_products<S, NIT, TW, TV>(A, base, ri, jo, x, P);
     wave_sync();
     if (jo == 0 && act) {
         switch (mode) {
-        case M_SPMV: {
-            TV acc = 0;
-            for (int j = 0; j < m; j++) acc += P[j * S + ri];
-            y[r] = acc;
+        case M_SPMV:
+            y[r] = ordered_sum<false>(P, S, ri, 0, m, TV(0));
             break;
-        }
         case M_SOR: {
-            TV sum = 0;
-            for (int j = 1; j < m; j++) sum += P[j * S + ri];
+            const TV sum = ordered_sum<false>(P, S, ri, 1, m, TV(0));
             y[r] = w * ((src[r] - sum) / wv(A.w[base])) + c * y[r];
             break;
         }
@@ -123,17 +174,13 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
                 if (mode == M_RES0) y[r] = 0;
                 break;
             }
-            TV sum = 0;
-            for (int j = 0; j < m; j++) sum += P[j * S + ri];
-            y[r] = src[r] - sum;
+            y[r] = src[r] - ordered_sum<false>(P, S, ri, 0, m, TV(0));
             break;
         }
         case M_RES1:
         case M_RES2: {
             if (mode == M_RES1 && bnd) { y[r] = 0; break; }
-            TV v = src[r];
-            for (int j = 0; j < m; j++) v = v - P[j * S + ri];
-            y[r] = v;
+            y[r] = ordered_sum<true>(P, S, ri, 0, m, src[r]);
             break;
         }
         case M_RESTRICT: {
@@ -141,18 +188,13 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
                 if (!keep_boundary) y[r] = 0;
                 break;
             }
-            TV acc = 0;
-            for (int j = 0; j < m; j++) acc += P[j * S + ri];
-            y[r] = acc;
+            y[r] = ordered_sum<false>(P, S, ri, 0, m, TV(0));
             break;
         }
-        case M_PROLONG: {
+        case M_PROLONG:
             if (bnd) break;
-            TV acc = 0;
-            for (int j = 0; j < m; j++) acc += P[j * S + ri];
-            y[r] = y[r] + acc;
+            y[r] = y[r] + ordered_sum<false>(P, S, ri, 0, m, TV(0));
             break;
-        }
         }
     }
     wave_sync();  // P is reused by the wave's next slice
@@ -177,46 +219,52 @@ rows_kernel(EllT<TW> A, const int *__restrict__ slices, int nslices, int nrows, 
                           prod + (size_t)wave * A.m * S);
 }
 
-// Whole relaxation of a small level in one workgroup: num_relax sweeps over
-// the colour classes in order (workgroup barrier between classes), each sweep
-// followed, for the T and F variants, by p -= p[node 0] and the residual form
-// 1 (T) or 2 (F, with the lap operator L2).  Identical arithmetic to the
-// per-colour launches.
+// Whole relaxation of a small level in one workgroup, with the level's p held
+// in LDS: num_relax sweeps over the colour classes in order (workgroup barrier
+// between classes), each sweep followed, for the T and F variants, by
+// p -= p[node 0] and the residual form 1 (T) or 2 (F, with the lap operator
+// A2).  Weights and columns stream from L2; neighbour gathers and updates hit
+// LDS, so a colour step costs a barrier instead of a launch.  Identical
+// arithmetic to the per-colour launches.
 template <int S>
-__global__ void __launch_bounds__(SMALL_BLK)
+__global__ void __launch_bounds__(SMALL_BLK_MAX)
 relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int *__restrict__ col_soff,
                    int ncol, int nrows, const uint8_t *__restrict__ btag, const int *__restrict__ ref0,
                    const double *__restrict__ src, double *p, double *res, double w, double c,
                    int num_relax, int variant)
 {
     extern __shared__ unsigned char lds_raw[];
-    double *prod = reinterpret_cast<double *>(lds_raw);   // [SMALL_BLK/64][max(m, m2)][S]
-    constexpr int NW = SMALL_BLK / 64;
+    const int NW = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6;
     const int mw = A.m > A2.m ? A.m : A2.m;
-    double *P = prod + (size_t)wave * mw * S;
+    double *xs = reinterpret_cast<double *>(lds_raw);                 // [nrows]
+    double *P = xs + nrows + (size_t)wave * mw * S;                   // [NW][mw][S]
     __shared__ double pref;
+    for (int r = threadIdx.x; r < nrows; r += blockDim.x) xs[r] = p[r];
+    __syncthreads();
     const long nsl = (nrows + S - 1) / S;
     for (int it = 0; it < num_relax; it++) {
         for (int cl = 0; cl < ncol; cl++) {
             const int s0 = col_soff[cl], s1 = col_soff[cl + 1];
             for (int k = s0 + wave; k < s1; k += NW)
-                slice_rows<S, double, double>(A, col_slices[k], nrows, M_SOR, btag, src, p, p, w, c, 0, P);
+                slice_rows<S, double, double, 72 * S / 64>(A, col_slices[k], nrows, M_SOR, btag, src, xs, xs,
+                                                            w, c, 0, P);
             __syncthreads();
         }
         if (variant == MGM_VAR_H) continue;
-        if (threadIdx.x == 0) pref = p[*ref0];
+        if (threadIdx.x == 0) pref = xs[*ref0];
         __syncthreads();
         const double pr = pref;
-        for (long r = threadIdx.x; r < nrows; r += SMALL_BLK)
-            if (btag[r] != ROW_PAD) p[r] = p[r] - pr;
+        for (int r = threadIdx.x; r < nrows; r += blockDim.x)
+            if (btag[r] != ROW_PAD) xs[r] = xs[r] - pr;
         __syncthreads();
         const Ell &R = (variant == MGM_VAR_T) ? A : A2;
         const int form = (variant == MGM_VAR_T) ? M_RES1 : M_RES2;
         for (long k = wave; k < nsl; k += NW)
-            slice_rows<S, double, double>(R, k, nrows, form, btag, src, p, res, 0.0, 0.0, 0, P);
+            slice_rows<S, double, double, 72 * S / 64>(R, k, nrows, form, btag, src, xs, res, 0.0, 0.0, 0, P);
         __syncthreads();
     }
+    for (int r = threadIdx.x; r < nrows; r += blockDim.x) p[r] = xs[r];
 }
 
 // Sequential sweep in reference node order (the reference's host loop); one
@@ -435,32 +483,57 @@ int launch_sor_slices(EllT<TW> A, const int *slices, int nslices, const uint8_t 
     if (nslices <= 0) return 0;
     return launch_rows<TW, TV>(A, slices, nslices, 0x7fffffff, M_SOR, btag, src, p, p, w, c, 0, s);
 }
+// LDS bytes of relax_small_kernel with nw waves; 0 if it does not fit
+static size_t relax_small_lds(int nrows, int mw, int S, int nw)
+{
+    size_t b = ((size_t)nrows + (size_t)nw * mw * S) * sizeof(double);
+    return b <= kSmallLdsMax ? b : 0;
+}
+static int relax_small_waves(int nrows, int mw, int S)
+{
+    for (int nw = SMALL_BLK_MAX / 64; nw >= 4; nw /= 2)
+        if (relax_small_lds(nrows, mw, S, nw)) return nw;
+    return 0;
+}
+bool relax_small_fits(int nrows, int m, int m2, int S)
+{
+    return relax_small_waves(nrows, m > m2 ? m : m2, S) > 0;
+}
+template <int S>
+static int launch_relax_small_t(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol,
+                                int nrows, const uint8_t *btag, const int *ref0, const double *src,
+                                double *p, double *res, double w, double c, int num_relax, int variant,
+                                int nw, size_t shm, hipStream_t s)
+{
+    static bool attr = false;
+    if (!attr) {
+        HIP_TRY(hipFuncSetAttribute((const void *)relax_small_kernel<S>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSmallLdsMax));
+        attr = true;
+    }
+    hipLaunchKernelGGL(relax_small_kernel<S>, dim3(1), dim3(64 * nw), shm, s, A, A2, col_slices, col_soff,
+                       ncol, nrows, btag, ref0, src, p, res, w, c, num_relax, variant);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
 int launch_relax_small(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol, int nrows,
                        const uint8_t *btag, const int *ref0, const double *src, double *p, double *res,
                        double w, double c, int num_relax, int variant, hipStream_t s)
 {
     if (A.S != A2.S) MGM_FAIL(-1, "relax_small: operators differ in slice height");
     const int mw = A.m > A2.m ? A.m : A2.m;
-    const size_t shm = (size_t)(SMALL_BLK / 64) * mw * A.S * sizeof(double);
-    if (shm > kSmallLdsMax) MGM_FAIL(-1, "relax_small: stencil too wide");
-    if (A.S == 4)
-        hipLaunchKernelGGL(relax_small_kernel<4>, dim3(1), dim3(SMALL_BLK), shm, s, A, A2, col_slices,
-                           col_soff, ncol, nrows, btag, ref0, src, p, res, w, c, num_relax, variant);
-    else if (A.S == 8)
-        hipLaunchKernelGGL(relax_small_kernel<8>, dim3(1), dim3(SMALL_BLK), shm, s, A, A2, col_slices,
-                           col_soff, ncol, nrows, btag, ref0, src, p, res, w, c, num_relax, variant);
-    else if (A.S == 16)
-        hipLaunchKernelGGL(relax_small_kernel<16>, dim3(1), dim3(SMALL_BLK), shm, s, A, A2, col_slices,
-                           col_soff, ncol, nrows, btag, ref0, src, p, res, w, c, num_relax, variant);
-    else
-        MGM_FAIL(-1, "unsupported slice height");
-    HIP_TRY(hipGetLastError());
-    return 0;
-}
-bool relax_small_fits(int m, int m2, int S)
-{
-    const int mw = m > m2 ? m : m2;
-    return (size_t)(SMALL_BLK / 64) * mw * S * sizeof(double) <= kSmallLdsMax;
+    const int nw = relax_small_waves(nrows, mw, A.S);
+    if (!nw) MGM_FAIL(-1, "relax_small: level does not fit in LDS");
+    const size_t shm = relax_small_lds(nrows, mw, A.S, nw);
+    switch (A.S) {
+    case 4: return launch_relax_small_t<4>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p, res,
+                                           w, c, num_relax, variant, nw, shm, s);
+    case 8: return launch_relax_small_t<8>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p, res,
+                                           w, c, num_relax, variant, nw, shm, s);
+    case 16: return launch_relax_small_t<16>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p,
+                                             res, w, c, num_relax, variant, nw, shm, s);
+    }
+    MGM_FAIL(-1, "unsupported slice height");
 }
 int launch_sor_natural(Ell A, int N, const int *iperm, const double *src, double *p, double w,
                        double c, hipStream_t s)
