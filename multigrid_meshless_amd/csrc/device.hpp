@@ -147,10 +147,11 @@ enum { ROW_INTERIOR = 0, ROW_BOUNDARY = 1, ROW_PAD = 2 };
 // y = A x over rows [0, nrows); rows tagged ROW_PAD in btag (nullable) are skipped.
 template <typename TW, typename TV>
 int launch_spmv(EllT<TW> A, int nrows, const uint8_t *btag, const TV *x, TV *y, hipStream_t s);
-// One SOR update of the rows in the given slices (one colour class).
+// One SOR update of the rows of nslices slices (one colour class): the slices
+// listed in `slices`, or first, first+1, ... when slices is null.
 template <typename TW, typename TV>
-int launch_sor_slices(EllT<TW> A, const int *slices, int nslices, const uint8_t *btag, const TV *src,
-                      TV *p, TV w, TV c, hipStream_t s);
+int launch_sor_slices(EllT<TW> A, const int *slices, int first, int nslices, const uint8_t *btag,
+                      const TV *src, TV *p, TV w, TV c, hipStream_t s);
 // The whole relaxation of a small level (all sweeps, all colour classes, and
 // for the T / F variants the pin and residual after each sweep, A2 being the
 // operator of the F residual) in one single-workgroup launch.

@@ -25,8 +25,11 @@ int relax_mp(mgm_problem *pb, Level &L)
     return with_op(L, [&](auto A) {
         for (int it = 0; it < pb->cfg.num_relax; it++)
             for (int k = 0; k < L.ncol; k++) {
-                int rc = launch_sor_slices(A, L.d_col_slices.p + L.col_soff[k], L.col_soff[k + 1] - L.col_soff[k],
-                                           L.dbtag.p, (const float *)L.src32.p, L.p32.p, w, c, stream());
+                const int *list;
+                int first, n;
+                L.colour_slices(k, &list, &first, &n);
+                int rc = launch_sor_slices(A, list, first, n, L.dbtag.p, (const float *)L.src32.p, L.p32.p, w, c,
+                                           stream());
                 if (rc) return rc;
             }
         return 0;

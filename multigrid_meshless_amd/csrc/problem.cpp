@@ -344,6 +344,9 @@ static uint64_t spread_bits(uint64_t v, int stride)
     return r;
 }
 
+// Levels above this many rows are ordered in Morton blocks of 4096 rows.
+constexpr int kBlockedRows = 262144;
+
 // Greedy colouring on the symmetrised stencil graph, in reference order, then
 // the device order: colour classes contiguous, Morton order inside a class.
 static void level_colour_order(Level &L, int dim)
@@ -396,11 +399,13 @@ static void level_colour_order(Level &L, int dim)
         }
         key[i] = k;
     }
-    // Morton rank -> block of kBlock rows; inside a block, colour classes
+    // Morton rank -> block of kBlock rows; inside a block, colour classes.
+    // Levels whose vectors fit one L2 many times over stay one block, so each
+    // colour class is one contiguous run of slices.
     std::vector<int> byk(N);
     std::iota(byk.begin(), byk.end(), 0);
     std::sort(byk.begin(), byk.end(), [&](int a, int b) { return key[a] != key[b] ? key[a] < key[b] : a < b; });
-    constexpr int kBlock = 4096;
+    const int kBlock = N > kBlockedRows ? 4096 : N;
     std::vector<int> block(N);
     for (int r = 0; r < N; r++) block[byk[r]] = r / kBlock;
     std::vector<int> ord(byk);
@@ -430,7 +435,10 @@ static void level_colour_order(Level &L, int dim)
     L.Nd = (int)L.perm.size();
     L.col_soff.assign(L.ncol + 1, 0);
     L.col_slices.clear();
+    L.col_contig = true;
     for (int c = 0; c < L.ncol; c++) {
+        for (size_t q = 1; q < slices[c].size(); q++)
+            if (slices[c][q] != slices[c][q - 1] + 1) L.col_contig = false;
         L.col_slices.insert(L.col_slices.end(), slices[c].begin(), slices[c].end());
         L.col_soff[c + 1] = (int)L.col_slices.size();
     }
@@ -1063,10 +1071,12 @@ int op_relax(mgm_problem *pb, int l, int variant)
             rc = launch_sor_natural(A, L.N, L.d_iperm.p, src, p, w, c, g_stream);
             if (rc) return rc;
         } else {
-            for (int k = 0; k < L.ncol; k++)
-                if ((rc = launch_sor_slices(A, L.d_col_slices.p + L.col_soff[k], L.col_soff[k + 1] - L.col_soff[k],
-                                            L.dbtag.p, src, p, w, c, g_stream)))
-                    return rc;
+            for (int k = 0; k < L.ncol; k++) {
+                const int *list;
+                int first, n;
+                L.colour_slices(k, &list, &first, &n);
+                if ((rc = launch_sor_slices(A, list, first, n, L.dbtag.p, src, p, w, c, g_stream))) return rc;
+            }
         }
         if (variant == MGM_VAR_H) continue;
         if ((rc = launch_pin(L.Nd, L.dbtag.p, L.ref0.p, p, pb->scal.p + 8, g_stream))) return rc;

@@ -36,6 +36,7 @@ struct Level {
     int Nd = 0;                        // device rows including padding
     std::vector<int> col_slices, col_soff;
     DevBuf<int> d_col_slices, d_col_soff;
+    bool col_contig = false;           // every colour's slices are consecutive (no list needed)
     std::vector<int> perm, iperm;      // device -> reference (-1 = padding), reference -> device
     int knn_ties = 0;
     // ---- device, reference order ----
@@ -64,6 +65,13 @@ struct Level {
     DevBuf<float> p32, src32, res32;
 
     Ell op(int which) const { return Ell{col.p, slab[which].p, m, S}; }
+    // slices of colour k for launch_sor_slices: a device list, or (null, first)
+    void colour_slices(int k, const int **list, int *first, int *n) const
+    {
+        *n = col_soff[k + 1] - col_soff[k];
+        *list = col_contig ? nullptr : d_col_slices.p + col_soff[k];
+        *first = (col_contig && *n) ? col_slices[col_soff[k]] : 0;
+    }
     Ell restr() const { return Ell{rcol.p, rw.p, mr, Sr}; }
     Ell prol() const { return Ell{pcol.p, pw.p, mp, Sp}; }
 };

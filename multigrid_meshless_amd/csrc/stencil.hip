@@ -87,53 +87,52 @@ __device__ inline void stage_products(const EllT<TW> &A, size_t base, int ri, in
 {
     constexpr int LPR = 64 / S;
     const int m = A.m;
+    // entry j = t*LPR + jo sits at base + j*S = lane base + t*64
+    const TW *wp = A.w + base + (size_t)jo * S;
+    const int *cp = A.col + base + (size_t)jo * S;
+    TV *pp = P + jo * S + ri;
+    const int nt = (m - jo + LPR - 1) / LPR;   // entries of this lane
     if constexpr (NIT == 0) {
 #pragma unroll 4
-        for (int j = jo; j < m; j += LPR) {
-            const size_t q = base + (size_t)j * S;
-            P[j * S + ri] = wv(A.w[q]) * x[A.col[q]];
-        }
+        for (int t = 0; t < nt; t++) pp[t * 64] = wv(wp[t * 64]) * x[cp[t * 64]];
     } else {
-        for (int j0 = 0; j0 < m; j0 += NIT * LPR) {
+        for (int t0 = 0; t0 < nt; t0 += NIT) {
             TW wq[NIT];
             int cq[NIT];
-#pragma unroll
-            for (int t = 0; t < NIT; t++) {
-                const int j = j0 + t * LPR + jo;
-                if (j < m) {
-                    wq[t] = A.w[base + (size_t)j * S];
-                    cq[t] = A.col[base + (size_t)j * S];
-                }
-            }
             TV xq[NIT];
 #pragma unroll
-            for (int t = 0; t < NIT; t++) {
-                const int j = j0 + t * LPR + jo;
-                if (j < m) xq[t] = x[cq[t]];
-            }
+            for (int t = 0; t < NIT; t++)
+                if (t0 + t < nt) {
+                    wq[t] = wp[(t0 + t) * 64];
+                    cq[t] = cp[(t0 + t) * 64];
+                }
 #pragma unroll
-            for (int t = 0; t < NIT; t++) {
-                const int j = j0 + t * LPR + jo;
-                if (j < m) P[j * S + ri] = wv(wq[t]) * xq[t];
-            }
+            for (int t = 0; t < NIT; t++)
+                if (t0 + t < nt) xq[t] = x[cq[t]];
+#pragma unroll
+            for (int t = 0; t < NIT; t++)
+                if (t0 + t < nt) pp[(t0 + t) * 64] = wv(wq[t]) * xq[t];
         }
     }
 }
 
 // acc (op) P[j*S + ri] for j = j0, j0+1, ..., m-1 in that order, op = + or -:
-// the reference's left-to-right row sum.  Four products are read per step so
-// four LDS reads are in flight per dependent wait; a longer unroll grows the
-// code that every launch fetches cold and measured slower.
+// the reference's left-to-right row sum.  Products are read eight at a time
+// (one LDS wait per eight dependent adds, no selects on the chain), the tail
+// one by one.
 template <bool SUBTRACT, typename TV>
 __device__ inline TV ordered_sum(const TV *P, int S, int ri, int j0, int m, TV acc)
 {
+    const TV *q = P + ri;
     int j = j0;
-    for (; j + 4 <= m; j += 4) {
-        const TV a = P[j * S + ri], b = P[(j + 1) * S + ri], c = P[(j + 2) * S + ri], d = P[(j + 3) * S + ri];
-        if (SUBTRACT) { acc = acc - a; acc = acc - b; acc = acc - c; acc = acc - d; }
-        else { acc = acc + a; acc = acc + b; acc = acc + c; acc = acc + d; }
+    for (; j + 8 <= m; j += 8) {
+        TV v[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) v[t] = q[(j + t) * S];
+#pragma unroll
+        for (int t = 0; t < 8; t++) acc = SUBTRACT ? acc - v[t] : acc + v[t];
     }
-    for (; j < m; j++) acc = SUBTRACT ? acc - P[j * S + ri] : acc + P[j * S + ri];
+    for (; j < m; j++) acc = SUBTRACT ? acc - q[j * S] : acc + q[j * S];
     return acc;
 }
 
@@ -154,8 +153,18 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
     const uint8_t tag = (inr && btag) ? btag[r] : (uint8_t)(inr ? ROW_INTERIOR : ROW_PAD);
     const bool act = tag != ROW_PAD;
     const bool bnd = tag == ROW_BOUNDARY;
-    const bool need = act && !(bnd && skips_boundary(mode));
+    // Modes that skip boundary rows wait for the tag before loading (saves the
+    // boundary rows' traffic); the others load at once -- padding rows hold
+    // weight 0 and column 0, so their loads are harmless and never combined.
+    const bool need = skips_boundary(mode) ? (act && !bnd) : inr;
     const size_t base = (size_t)slice * S * m + ri;
+    // the combining lane's row operands, loaded alongside the stencil
+    const bool comb = jo == 0 && act;
+    TV sv = 0, yv = 0, dg = 1;
+    if (comb && mode != M_SPMV && mode != M_RESTRICT && mode != M_PROLONG && !(bnd && skips_boundary(mode)))
+        sv = src[r];
+    if (comb && (mode == M_SOR || (mode == M_PROLONG && !bnd))) yv = y[r];
+    if (comb && mode == M_SOR) dg = wv(A.w[base]);
     if (need) stage_products<S, NIT, TW, TV>(A, base, ri, jo, x, P);
     wave_sync();
     if (jo == 0 && act) {
@@ -165,7 +174,7 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
             break;
         case M_SOR: {
             const TV sum = ordered_sum<false>(P, S, ri, 1, m, TV(0));
-            y[r] = w * ((src[r] - sum) / wv(A.w[base])) + c * y[r];
+            y[r] = w * ((sv - sum) / dg) + c * yv;
             break;
         }
         case M_RES0:
@@ -174,13 +183,13 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
                 if (mode == M_RES0) y[r] = 0;
                 break;
             }
-            y[r] = src[r] - ordered_sum<false>(P, S, ri, 0, m, TV(0));
+            y[r] = sv - ordered_sum<false>(P, S, ri, 0, m, TV(0));
             break;
         }
         case M_RES1:
         case M_RES2: {
             if (mode == M_RES1 && bnd) { y[r] = 0; break; }
-            y[r] = ordered_sum<true>(P, S, ri, 0, m, src[r]);
+            y[r] = ordered_sum<true>(P, S, ri, 0, m, sv);
             break;
         }
         case M_RESTRICT: {
@@ -193,7 +202,7 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
         }
         case M_PROLONG:
             if (bnd) break;
-            y[r] = y[r] + ordered_sum<false>(P, S, ri, 0, m, TV(0));
+            y[r] = yv + ordered_sum<false>(P, S, ri, 0, m, TV(0));
             break;
         }
     }
@@ -201,11 +210,11 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
 }
 
 // Slice k of the launch is slices[k] when a slice list is given (one colour
-// class), else k itself (a full pass).  Rows at or beyond nrows and padding
+// class), else first + k (a contiguous colour class, or a full pass).  Rows at or beyond nrows and padding
 // rows (tag ROW_PAD) are neither read nor written.
 template <int S, typename TW, typename TV>
 __global__ void __launch_bounds__(BLK)
-rows_kernel(EllT<TW> A, const int *__restrict__ slices, int nslices, int nrows, int mode,
+rows_kernel(EllT<TW> A, const int *__restrict__ slices, int first, int nslices, int nrows, int mode,
             const uint8_t *__restrict__ btag, const TV *__restrict__ src, const TV *x, TV *y, TV w,
             TV c, int keep_boundary)
 {
@@ -214,18 +223,65 @@ rows_kernel(EllT<TW> A, const int *__restrict__ slices, int nslices, int nrows, 
     const int wave = threadIdx.x >> 6;
     const long k = xcd_block(blockIdx.x, gridDim.x) * WAVES + wave;
     if (k >= nslices) return;
-    const long slice = slices ? (long)slices[k] : k;
+    const long slice = slices ? (long)slices[k] : (long)first + k;
     slice_rows<S, TW, TV>(A, slice, nrows, mode, btag, src, x, y, w, c, keep_boundary,
                           prod + (size_t)wave * A.m * S);
 }
 
-// Whole relaxation of a small level in one workgroup, with the level's p held
-// in LDS: num_relax sweeps over the colour classes in order (workgroup barrier
-// between classes), each sweep followed, for the T and F variants, by
-// p -= p[node 0] and the residual form 1 (T) or 2 (F, with the lap operator
-// A2).  Weights and columns stream from L2; neighbour gathers and updates hit
-// LDS, so a colour step costs a barrier instead of a launch.  Identical
-// arithmetic to the per-colour launches.
+// Weights and columns of this lane's entries (j = t*LPR + jo) of one slice.
+template <int S, int NIT>
+__device__ inline void load_slice(const Ell &A, int slice, int ri, int jo, double *wq, int *cq)
+{
+    constexpr int LPR = 64 / S;
+    // entry j = t*LPR + jo of row ri sits at slice*S*m + j*S + ri = lane base + t*64
+    const size_t off = (size_t)slice * S * A.m + (size_t)jo * S + ri;
+    const double *wp = A.w + off;
+    const int *cp = A.col + off;
+#pragma unroll
+    for (int t = 0; t < NIT; t++)
+        if (t * LPR + jo < A.m) {
+            wq[t] = wp[t * 64];
+            cq[t] = cp[t * 64];
+        }
+}
+
+// One SOR update of a slice's rows, p, source and tags in LDS, the slice's
+// weights and columns already in registers (wq[0] of lane jo == 0 is the
+// diagonal).  Same arithmetic as slice_rows in mode M_SOR.
+template <int S, int NIT>
+__device__ inline void sor_slice_lds(const Ell &A, int slice, int nrows, const uint8_t *tg, const double *sr,
+                                     double *xs, double w, double c, const double *wq, const int *cq,
+                                     double *P)
+{
+    constexpr int LPR = 64 / S;
+    const int lane = threadIdx.x & 63;
+    const int ri = lane % S, jo = lane / S;
+    const int m = A.m;
+    const int r = slice * S + ri;
+    const bool inr = r < nrows;
+    double xv[NIT];
+#pragma unroll
+    for (int t = 0; t < NIT; t++)
+        if (inr && t * LPR + jo < m) xv[t] = xs[cq[t]];
+#pragma unroll
+    for (int t = 0; t < NIT; t++)
+        if (inr && t * LPR + jo < m) P[(t * LPR + jo) * S + ri] = wq[t] * xv[t];
+    wave_sync();
+    if (jo == 0 && inr && tg[r] != ROW_PAD) {
+        const double sum = ordered_sum<false>(P, S, ri, 1, m, 0.0);
+        xs[r] = w * ((sr[r] - sum) / wq[0]) + c * xs[r];
+    }
+    wave_sync();
+}
+
+// Whole relaxation of a small level in one workgroup, with the level's p,
+// source and tags held in LDS: num_relax sweeps over the colour classes in
+// order (workgroup barrier between classes), each sweep followed, for the T
+// and F variants, by p -= p[node 0] and the residual form 1 (T) or 2 (F, with
+// the lap operator A2).  Weights and columns stream from L2; each wave loads
+// its first slice of the next colour step before the barrier, so a step costs
+// the LDS gathers and the ordered sums.  Identical arithmetic to the
+// per-colour launches.
 template <int S>
 __global__ void __launch_bounds__(SMALL_BLK_MAX)
 relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int *__restrict__ col_soff,
@@ -233,22 +289,58 @@ relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int 
                    const double *__restrict__ src, double *p, double *res, double w, double c,
                    int num_relax, int variant)
 {
+    // register slots per lane: stencils up to 72 wide (36 at S = 16, the
+    // slice height used only for stencils of at most 36 entries)
+    constexpr int NIT = S == 16 ? 9 : (72 * S + 63) / 64;
     extern __shared__ unsigned char lds_raw[];
     const int NW = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int ri = lane % S, jo = lane / S;
     const int mw = A.m > A2.m ? A.m : A2.m;
-    double *xs = reinterpret_cast<double *>(lds_raw);                 // [nrows]
-    double *P = xs + nrows + (size_t)wave * mw * S;                   // [NW][mw][S]
+    const int nlist = col_soff[ncol];
+    double *xs = reinterpret_cast<double *>(lds_raw);                 // [nrows] p
+    double *sr = xs + nrows;                                          // [nrows] source
+    double *P = sr + nrows + (size_t)wave * mw * S;                   // [NW][mw][S]
+    int *soff = reinterpret_cast<int *>(sr + nrows + (size_t)NW * mw * S);  // [ncol+1]
+    int *slist = soff + ncol + 1;                                     // [nlist]
+    uint8_t *tg = reinterpret_cast<uint8_t *>(slist + nlist);         // [nrows]
     __shared__ double pref;
-    for (int r = threadIdx.x; r < nrows; r += blockDim.x) xs[r] = p[r];
+    for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
+        xs[r] = p[r];
+        sr[r] = src[r];
+        tg[r] = btag[r];
+    }
+    for (int k = threadIdx.x; k <= ncol; k += blockDim.x) soff[k] = col_soff[k];
+    for (int k = threadIdx.x; k < nlist; k += blockDim.x) slist[k] = col_slices[k];
     __syncthreads();
+    const bool regs = A.m <= NIT * (64 / S);
+    double wq[NIT];
+    int cq[NIT];
+    auto first_slice = [&](int cl) { const int k = soff[cl] + wave; return k < soff[cl + 1] ? slist[k] : -1; };
+    if (regs && num_relax > 0) {
+        const int f = first_slice(0);
+        if (f >= 0) load_slice<S, NIT>(A, f, ri, jo, wq, cq);
+    }
     const long nsl = (nrows + S - 1) / S;
     for (int it = 0; it < num_relax; it++) {
         for (int cl = 0; cl < ncol; cl++) {
-            const int s0 = col_soff[cl], s1 = col_soff[cl + 1];
-            for (int k = s0 + wave; k < s1; k += NW)
-                slice_rows<S, double, double, 72 * S / 64>(A, col_slices[k], nrows, M_SOR, btag, src, xs, xs,
-                                                            w, c, 0, P);
+            const int s0 = soff[cl], s1 = soff[cl + 1];
+            for (int k = s0 + wave; k < s1; k += NW) {
+                if (!regs) {
+                    slice_rows<S, double, double>(A, slist[k], nrows, M_SOR, tg, sr, xs, xs, w, c, 0, P);
+                    continue;
+                }
+                if (k != s0 + wave) load_slice<S, NIT>(A, slist[k], ri, jo, wq, cq);
+                sor_slice_lds<S, NIT>(A, slist[k], nrows, tg, sr, xs, w, c, wq, cq, P);
+            }
+            // this wave's first slice of the next colour step, before the barrier
+            // (across a sweep boundary only when no pin/residual pass intervenes)
+            const int ncl = cl + 1 < ncol ? cl + 1 : 0;
+            if (regs && (cl + 1 < ncol || (it + 1 < num_relax && variant == MGM_VAR_H))) {
+                const int f = first_slice(ncl);
+                if (f >= 0) load_slice<S, NIT>(A, f, ri, jo, wq, cq);
+            }
             __syncthreads();
         }
         if (variant == MGM_VAR_H) continue;
@@ -256,12 +348,16 @@ relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int 
         __syncthreads();
         const double pr = pref;
         for (int r = threadIdx.x; r < nrows; r += blockDim.x)
-            if (btag[r] != ROW_PAD) xs[r] = xs[r] - pr;
+            if (tg[r] != ROW_PAD) xs[r] = xs[r] - pr;
         __syncthreads();
         const Ell &R = (variant == MGM_VAR_T) ? A : A2;
         const int form = (variant == MGM_VAR_T) ? M_RES1 : M_RES2;
         for (long k = wave; k < nsl; k += NW)
-            slice_rows<S, double, double, 72 * S / 64>(R, k, nrows, form, btag, src, xs, res, 0.0, 0.0, 0, P);
+            slice_rows<S, double, double>(R, k, nrows, form, tg, sr, xs, res, 0.0, 0.0, 0, P);
+        if (regs && it + 1 < num_relax) {
+            const int f = first_slice(0);
+            if (f >= 0) load_slice<S, NIT>(A, f, ri, jo, wq, cq);
+        }
         __syncthreads();
     }
     for (int r = threadIdx.x; r < nrows; r += blockDim.x) p[r] = xs[r];
@@ -428,22 +524,23 @@ dirichlet_rows_kernel(int N, int m, const uint8_t *__restrict__ btag, double *__
 
 // slices == nullptr: all slices covering rows [0, nrows)
 template <typename TW, typename TV>
-int launch_rows(EllT<TW> A, const int *slices, int nslices, int nrows, int mode, const uint8_t *btag,
-                const TV *src, const TV *x, TV *y, TV w, TV c, int keep, hipStream_t s)
+int launch_rows(EllT<TW> A, const int *slices, int first, int nslices, int nrows, int mode,
+                const uint8_t *btag, const TV *src, const TV *x, TV *y, TV w, TV c, int keep,
+                hipStream_t s)
 {
-    if (!slices) nslices = (nrows + A.S - 1) / A.S;
+    if (nslices < 0) nslices = (nrows + A.S - 1) / A.S;  // full pass
     if (nslices <= 0 || nrows <= 0) return 0;
     int blocks = (nslices + WAVES - 1) / WAVES;
     size_t shm = (size_t)WAVES * A.m * A.S * sizeof(TV);
     if (A.S == 4)
-        hipLaunchKernelGGL((rows_kernel<4, TW, TV>), dim3(blocks), dim3(BLK), shm, s, A, slices, nslices,
-                           nrows, mode, btag, src, x, y, w, c, keep);
+        hipLaunchKernelGGL((rows_kernel<4, TW, TV>), dim3(blocks), dim3(BLK), shm, s, A, slices, first,
+                           nslices, nrows, mode, btag, src, x, y, w, c, keep);
     else if (A.S == 8)
-        hipLaunchKernelGGL((rows_kernel<8, TW, TV>), dim3(blocks), dim3(BLK), shm, s, A, slices, nslices,
-                           nrows, mode, btag, src, x, y, w, c, keep);
+        hipLaunchKernelGGL((rows_kernel<8, TW, TV>), dim3(blocks), dim3(BLK), shm, s, A, slices, first,
+                           nslices, nrows, mode, btag, src, x, y, w, c, keep);
     else if (A.S == 16)
-        hipLaunchKernelGGL((rows_kernel<16, TW, TV>), dim3(blocks), dim3(BLK), shm, s, A, slices, nslices,
-                           nrows, mode, btag, src, x, y, w, c, keep);
+        hipLaunchKernelGGL((rows_kernel<16, TW, TV>), dim3(blocks), dim3(BLK), shm, s, A, slices, first,
+                           nslices, nrows, mode, btag, src, x, y, w, c, keep);
     else
         MGM_FAIL(-1, "unsupported slice height");
     HIP_TRY(hipGetLastError());
@@ -474,19 +571,22 @@ int slice_height(int m)
 template <typename TW, typename TV>
 int launch_spmv(EllT<TW> A, int nrows, const uint8_t *btag, const TV *x, TV *y, hipStream_t s)
 {
-    return launch_rows<TW, TV>(A, nullptr, 0, nrows, M_SPMV, btag, nullptr, x, y, TV(0), TV(0), 0, s);
+    return launch_rows<TW, TV>(A, nullptr, 0, -1, nrows, M_SPMV, btag, nullptr, x, y, TV(0), TV(0), 0, s);
 }
 template <typename TW, typename TV>
-int launch_sor_slices(EllT<TW> A, const int *slices, int nslices, const uint8_t *btag, const TV *src,
-                      TV *p, TV w, TV c, hipStream_t s)
+int launch_sor_slices(EllT<TW> A, const int *slices, int first, int nslices, const uint8_t *btag,
+                      const TV *src, TV *p, TV w, TV c, hipStream_t s)
 {
     if (nslices <= 0) return 0;
-    return launch_rows<TW, TV>(A, slices, nslices, 0x7fffffff, M_SOR, btag, src, p, p, w, c, 0, s);
+    return launch_rows<TW, TV>(A, slices, first, nslices, 0x7fffffff, M_SOR, btag, src, p, p, w, c, 0, s);
 }
-// LDS bytes of relax_small_kernel with nw waves; 0 if it does not fit
+// LDS bytes of relax_small_kernel with nw waves: p, source, staging, the
+// colour tables (ncol + 1 + nrows/S ints, bounded by 2*nrows + 1) and the
+// tags; 0 if it does not fit
 static size_t relax_small_lds(int nrows, int mw, int S, int nw)
 {
-    size_t b = ((size_t)nrows + (size_t)nw * mw * S) * sizeof(double);
+    size_t b = (2 * (size_t)nrows + (size_t)nw * mw * S) * sizeof(double) +
+               (2 * (size_t)nrows + 1) * sizeof(int) + (size_t)nrows;
     return b <= kSmallLdsMax ? b : 0;
 }
 static int relax_small_waves(int nrows, int mw, int S)
@@ -548,19 +648,19 @@ int launch_residual(EllT<TW> A, int N, int form, const uint8_t *btag, const TV *
 {
     static const int modes[4] = {M_RES0, M_RES1, M_RES2, M_RES3};
     if (form < 0 || form > 3) MGM_FAIL(-1, "bad residual form");
-    return launch_rows<TW, TV>(A, nullptr, 0, N, modes[form], btag, src, p, res, TV(0), TV(0), 0, s);
+    return launch_rows<TW, TV>(A, nullptr, 0, -1, N, modes[form], btag, src, p, res, TV(0), TV(0), 0, s);
 }
 template <typename TW, typename TV>
 int launch_restrict(EllT<TW> R, int N, const uint8_t *btag, int keep_boundary, const TV *in, TV *out,
                     hipStream_t s)
 {
-    return launch_rows<TW, TV>(R, nullptr, 0, N, M_RESTRICT, btag, nullptr, in, out, TV(0), TV(0),
+    return launch_rows<TW, TV>(R, nullptr, 0, -1, N, M_RESTRICT, btag, nullptr, in, out, TV(0), TV(0),
                                keep_boundary, s);
 }
 template <typename TW, typename TV>
 int launch_prolong(EllT<TW> P, int N, const uint8_t *btag, const TV *pc, TV *p, hipStream_t s)
 {
-    return launch_rows<TW, TV>(P, nullptr, 0, N, M_PROLONG, btag, nullptr, pc, p, TV(0), TV(0), 0, s);
+    return launch_rows<TW, TV>(P, nullptr, 0, -1, N, M_PROLONG, btag, nullptr, pc, p, TV(0), TV(0), 0, s);
 }
 template <typename TV>
 int launch_zero(int N, const uint8_t *btag, int interior_only, TV *p, hipStream_t s)
@@ -571,8 +671,8 @@ int launch_zero(int N, const uint8_t *btag, int interior_only, TV *p, hipStream_
 
 #define MGM_INSTANTIATE(TW, TV)                                                                      \
     template int launch_spmv<TW, TV>(EllT<TW>, int, const uint8_t *, const TV *, TV *, hipStream_t); \
-    template int launch_sor_slices<TW, TV>(EllT<TW>, const int *, int, const uint8_t *, const TV *,  \
-                                           TV *, TV, TV, hipStream_t);                               \
+    template int launch_sor_slices<TW, TV>(EllT<TW>, const int *, int, int, const uint8_t *,         \
+                                           const TV *, TV *, TV, TV, hipStream_t);                   \
     template int launch_residual<TW, TV>(EllT<TW>, int, int, const uint8_t *, const TV *,            \
                                          const TV *, TV *, hipStream_t);                             \
     template int launch_restrict<TW, TV>(EllT<TW>, int, const uint8_t *, int, const TV *, TV *,      \
