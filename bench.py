@@ -48,6 +48,7 @@ def fourier_rhs(x, y, z, seed=2, modes=16):
 
 
 def build_problem(mgm, n, levels, seed0=1):
+    """Jittered n^3 cube and `levels` coarser clouds (halving), bench stencil settings."""
     sizes = []
     v = n
     for _ in range(levels):
@@ -117,6 +118,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-mixed", action="store_true", help="skip the reduced-precision solves")
+    ap.add_argument("--no-learned", action="store_true", help="skip training and the learned-smoother solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -171,6 +173,36 @@ def main():
             print(f"[bench] precision {prec} ({tag}): {r['cycles']} cycles {np.median(ts):.4f}s "
                   f"final {r['history'][-1]:.3e} field diff vs FP64 {err:.2e}", file=sys.stderr, flush=True)
 
+    # learned smoother: trained on a 40^3 cloud with the same stencil settings,
+    # weights moved to the 1M problem; FP64 solve, first sweep per level learned
+    learned = None
+    if not args.no_learned:
+        Pt, _, _ = build_problem(mgm, 40, 3, seed0=21)
+        t0 = time.perf_counter()
+        l0, l1 = Pt.train_smoother(snapshots=8, samples_per_snapshot=8192, epochs=30, batch=4096, lr=1e-3,
+                                   seed=7)
+        t_train = time.perf_counter() - t0
+        P.set_mlp_weights(Pt.mlp_weights())
+        del Pt
+        P.use_learned_smoother(True)
+        reset(P, f)
+        P.solve(args.tol, args.max_cycles)
+        ts = []
+        for _ in range(args.steps):
+            reset(P, f)
+            r = P.solve(args.tol, args.max_cycles)
+            ts.append(r["seconds"])
+        fb = P.learned_fell_back()
+        err = float(np.abs(P.field(0, "p") - p64).max() / np.abs(p64).max())
+        P.use_learned_smoother(False)
+        learned = {"s": round(float(np.median(ts)), 5), "cycles": r["cycles"], "final_ratio": float(r["history"][-1]),
+                   "fell_back_to_sor": fb, "train_s": round(t_train, 2), "train_loss": [round(l0, 4), round(l1, 4)],
+                   "max_rel_diff_vs_fp64_solution": err,
+                   "speedup_vs_fp64_sor": round(t_solve / float(np.median(ts)), 3)}
+        print(f"[bench] learned smoother: {r['cycles']} cycles {np.median(ts):.4f}s final {r['history'][-1]:.3e} "
+              f"fell back {fb} train {t_train:.1f}s loss {l0:.3f}->{l1:.3f} field diff {err:.2e}",
+              file=sys.stderr, flush=True)
+
     # stencil apply roofline: lap_Poison over level 0, FP64 weights + int32 columns
     t_apply = P.time_spmv(0, args.spmv_reps)
     bytes_apply = N * (m * (8 + 4) + 8 + 8)
@@ -204,7 +236,8 @@ def main():
                    "cycles": cycles[0], "converged": converged,
                    "final_ratio": finals[0], "setup_s": round(t_setup, 2),
                    "stencil_apply_GBps": round(achieved, 1),
-                   "reduced_precision": mixed},
+                   "reduced_precision": mixed,
+                   "learned_smoother": learned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,

@@ -160,6 +160,39 @@ int mgm_fs_step(mgm_problem *p, double *steady_error);
  * V-cycles on PPRIME, velocity/pressure update, Neumann boundary pressure. */
 int mgm_timple_step(mgm_problem *p, double *steady_error);
 
+/* ---- learned smoother (no reference counterpart) -------------------------------
+ * A per-node network replaces the first SOR sweep of every level's relaxation
+ * (H variant, multicolour order, FP64 solve).  Input of node i: the m stencil
+ * offsets (dim values each, divided by the level's mean spacing) and the m
+ * stencil residuals divided by their RMS sigma_i; layers in -> 64 -> 64 -> 1
+ * with ReLU; the correction is output * sigma_i / a_ii on interior nodes.
+ * Weights: FP32 masters, laid out W1[64][in], b1[64], W2[64][64], b2[64],
+ * w3[64], b3; inference uses BF16-rounded W1 (residual columns), W2, w3 with
+ * FP32 accumulation.  mgm_solve runs its first cycle plain and falls back to
+ * plain SOR for the rest of the solve as soon as a learned cycle reduces the
+ * residual less than that plain cycle did. */
+typedef struct {
+    int snapshots;              /* random sources; each gives 3 (error, residual) states */
+    int samples_per_snapshot;   /* interior nodes sampled per state */
+    int epochs, batch;
+    float lr;                   /* Adam step size */
+    uint64_t seed;
+} mgm_train_config;
+int mgm_mlp_inputs(const mgm_problem *p);             /* m*dim + m of level 0 */
+size_t mgm_mlp_weight_count(int inputs);
+int mgm_mlp_set_weights(mgm_problem *p, const float *w);
+int mgm_mlp_get_weights(const mgm_problem *p, float *w);
+/* Network on caller rows X (n x inputs, row-major): y = f(X); and the loss
+ * mean((f(X) - t)^2) with its gradient (layout of the weights), FP32. */
+int mgm_mlp_forward(mgm_problem *p, int n, const float *X, float *y);
+int mgm_mlp_gradient(mgm_problem *p, int n, const float *X, const float *t, double *loss, float *grad);
+/* Train on level 0 from (error, residual) pairs of the plain V-cycle (the
+ * fields of the problem are overwritten); loss over all samples before and
+ * after training. */
+int mgm_train_smoother(mgm_problem *p, const mgm_train_config *cfg, double *loss_first, double *loss_last);
+int mgm_use_learned_smoother(mgm_problem *p, int enable);
+int mgm_learned_fell_back(const mgm_problem *p);      /* 1 if the last solve fell back to SOR */
+
 /* ---- resident solve -------------------------------------------------------- */
 /* V-cycles on level-0 (field P, SOURCE) until l2(r)/l2(f) <= tol (l2_norm_gen
  * norms, r = source - lap_Poison*p on interior nodes) or max_cycles.  history

@@ -241,6 +241,50 @@ class Problem:
         check(lib().mgm_fs_step(self._h, C.byref(err)), "fs_step")
         return err.value
 
+    # ---- learned smoother ----
+    def mlp_inputs(self) -> int:
+        n = lib().mgm_mlp_inputs(self._h)
+        check(min(n, 0), "mlp_inputs")
+        return n
+
+    def mlp_weights(self) -> np.ndarray:
+        w = np.zeros(lib().mgm_mlp_weight_count(self.mlp_inputs()), np.float32)
+        check(lib().mgm_mlp_get_weights(self._h, w.ctypes.data), "mlp_get_weights")
+        return w
+
+    def set_mlp_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == lib().mgm_mlp_weight_count(self.mlp_inputs())
+        check(lib().mgm_mlp_set_weights(self._h, w.ctypes.data), "mlp_set_weights")
+
+    def mlp_forward(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.zeros(X.shape[0], np.float32)
+        check(lib().mgm_mlp_forward(self._h, X.shape[0], X.ctypes.data, y.ctypes.data), "mlp_forward")
+        return y
+
+    def mlp_gradient(self, X, t):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(lib().mgm_mlp_weight_count(self.mlp_inputs()), np.float32)
+        loss = C.c_double()
+        check(lib().mgm_mlp_gradient(self._h, X.shape[0], X.ctypes.data, t.ctypes.data, C.byref(loss),
+                                     g.ctypes.data), "mlp_gradient")
+        return loss.value, g
+
+    def train_smoother(self, snapshots=8, samples_per_snapshot=8192, epochs=30, batch=4096, lr=1e-3, seed=7):
+        from ._lib import TrainConfig
+        cfg = TrainConfig(snapshots, samples_per_snapshot, epochs, batch, lr, seed)
+        l0, l1 = C.c_double(), C.c_double()
+        check(lib().mgm_train_smoother(self._h, C.byref(cfg), C.byref(l0), C.byref(l1)), "train_smoother")
+        return l0.value, l1.value
+
+    def use_learned_smoother(self, enable=True):
+        check(lib().mgm_use_learned_smoother(self._h, int(bool(enable))), "use_learned_smoother")
+
+    def learned_fell_back(self) -> bool:
+        return bool(lib().mgm_learned_fell_back(self._h))
+
     def time_spmv(self, lev=0, reps=20) -> float:
         s = C.c_double()
         check(lib().mgm_time_spmv(self._h, lev, reps, C.byref(s)), "time_spmv")

@@ -34,6 +34,12 @@ class NsParams(C.Structure):
                 ("iter_momentum", C.c_int), ("iter_timple", C.c_int), ("num_vcycles", C.c_int)]
 
 
+class TrainConfig(C.Structure):
+    """mgm_train_config."""
+    _fields_ = [("snapshots", C.c_int), ("samples_per_snapshot", C.c_int), ("epochs", C.c_int),
+                ("batch", C.c_int), ("lr", C.c_float), ("seed", C.c_uint64)]
+
+
 # (name, restype, argtypes) for every symbol declared in include/mgm.h
 _vp, _i, _d, _f, _cp = C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_char_p
 SIGNATURES = [
@@ -78,6 +84,15 @@ SIGNATURES = [
     ("mgm_ns_set_params", _i, [_vp, _vp]),
     ("mgm_fs_step", _i, [_vp, C.POINTER(_d)]),
     ("mgm_timple_step", _i, [_vp, C.POINTER(_d)]),
+    ("mgm_mlp_inputs", _i, [_vp]),
+    ("mgm_mlp_weight_count", C.c_size_t, [_i]),
+    ("mgm_mlp_set_weights", _i, [_vp, _vp]),
+    ("mgm_mlp_get_weights", _i, [_vp, _vp]),
+    ("mgm_mlp_forward", _i, [_vp, _i, _vp, _vp]),
+    ("mgm_mlp_gradient", _i, [_vp, _i, _vp, _vp, C.POINTER(_d), _vp]),
+    ("mgm_train_smoother", _i, [_vp, C.POINTER(TrainConfig), C.POINTER(_d), C.POINTER(_d)]),
+    ("mgm_use_learned_smoother", _i, [_vp, _i]),
+    ("mgm_learned_fell_back", _i, [_vp]),
     ("mgm_solve", _i, [_vp, _i, _i, _d, _i, C.POINTER(_i), _vp, C.POINTER(_d)]),
     ("mgm_time_spmv", _i, [_vp, _i, _i, C.POINTER(_d)]),
 ]

@@ -1051,6 +1051,9 @@ static int small_level_rows()
     return e ? atoi(e) : 12000;
 }
 
+int op_learned_sweep(mgm_problem *pb, int l);
+bool learned_active(const mgm_problem *pb, int l, int variant);
+
 static int pfield(int variant) { return variant == MGM_VAR_T ? MGM_F_PPRIME : MGM_F_P; }
 
 int op_relax(mgm_problem *pb, int l, int variant)
@@ -1060,12 +1063,19 @@ int op_relax(mgm_problem *pb, int l, int variant)
     const double c = (float)(1 - pb->cfg.omega);  // (1-omega) evaluated in float, as the reference
     double *p = L.f[pfield(variant)].p, *src = L.f[MGM_F_SOURCE].p, *res = L.f[MGM_F_RES].p;
     Ell A = L.op(OP_LAPP);
+    int nrelax = pb->cfg.num_relax;
+    if (learned_active(pb, l, variant) && nrelax > 0) {
+        // the learned correction takes the place of the first sweep
+        int rc = op_learned_sweep(pb, l);
+        if (rc) return rc;
+        nrelax--;
+    }
     if (pb->cfg.sweep_order != MGM_ORDER_NATURAL && L.Nd <= small_level_rows() &&
         relax_small_fits(L.Nd, L.m, L.m, L.S))
-        return launch_relax_small(A, L.op(OP_LAP), L.d_col_slices.p, L.d_col_soff.p, L.ncol, L.Nd,
-                                  L.dbtag.p, L.ref0.p, src, p, res, w, c, pb->cfg.num_relax, variant,
-                                  g_stream);
-    for (int it = 0; it < pb->cfg.num_relax; it++) {
+        return nrelax ? launch_relax_small(A, L.op(OP_LAP), L.d_col_slices.p, L.d_col_soff.p, L.ncol, L.Nd,
+                                           L.dbtag.p, L.ref0.p, src, p, res, w, c, nrelax, variant, g_stream)
+                      : 0;
+    for (int it = 0; it < nrelax; it++) {
         int rc;
         if (pb->cfg.sweep_order == MGM_ORDER_NATURAL) {
             rc = launch_sor_natural(A, L.N, L.d_iperm.p, src, p, w, c, g_stream);

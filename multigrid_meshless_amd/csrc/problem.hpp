@@ -2,6 +2,7 @@
 #pragma once
 
 #include <array>
+#include <memory>
 #include <vector>
 
 #include "device.hpp"
@@ -78,6 +79,26 @@ struct Level {
 
 }  // namespace mgm
 
+namespace mgm {
+// Learned smoother state: FP32 master weights, BF16 inference copies, the
+// per-level geometric part of layer 1 and device-order coordinates.
+struct LearnedState {
+    int in = 0;                       // network inputs: m*dim + m of level 0
+    std::vector<float> w;             // FP32 masters (host)
+    DevBuf<float> W;                  // FP32 masters (device)
+    bool have = false;                // weights set or trained
+    bool enabled = false;             // relax uses the network as its first sweep
+    bool prepared = false;            // inference copies and G match W
+    DevBuf<bf16> W1r, W2, w3;            // W1r: residual columns of W1, [j][k]
+    DevBuf<float> b2;
+    float b3 = 0;
+    std::vector<DevBuf<float>> G;     // per level: Nd x MLP_H
+    std::vector<DevBuf<double>> dx, dy, dz;
+    std::vector<char> lev_ok;         // level's stencils fit the network
+    int fell_back = 0;                // the last learned solve switched to SOR
+};
+}  // namespace mgm
+
 struct mgm_problem {
     mgm_config cfg{};
     int L = 0;
@@ -89,5 +110,6 @@ struct mgm_problem {
     mgm_ns_params ns{};
     bool ns_set = false;
     int mp_built = 0;              // precision mode the FP32/BF16 copies hold (0: none / stale)
+    std::unique_ptr<mgm::LearnedState> ml;
     ~mgm_problem();
 };

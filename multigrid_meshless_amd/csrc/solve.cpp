@@ -79,35 +79,58 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
     if ((rc = read_scalar(pb, &rs))) return rc;
     if (history) history[0] = sqrt(rs / L0.N) / fnorm;
 
-    // capture one cycle (V-cycle + fine residual + norm) into a graph, unless
-    // MGM_GRAPH=0 asks for plain stream launches (profilers that do not follow
-    // graph launches)
+    // The learned smoother's cycles are checked against one plain cycle: if a
+    // learned cycle reduces the residual less than the plain one did, the rest
+    // of the solve falls back to plain SOR.
+    LearnedState *ml = pb->ml.get();
+    const bool learned = ml && ml->enabled && ml->prepared && variant == MGM_VAR_H && precision == 0;
+    if (ml) ml->fell_back = 0;
+    auto set_learned = [&](bool on) { if (ml) ml->enabled = on; };
+    const bool ml_enabled = ml && ml->enabled;
+
+    // capture one cycle (V-cycle + fine residual + norm) into a graph -- one
+    // plain and, with the learned smoother, one learned -- unless MGM_GRAPH=0
+    // asks for plain stream launches (profilers that do not follow graphs)
     const char *ge = getenv("MGM_GRAPH");
     const bool use_graph = !(ge && ge[0] == '0');
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    auto cleanup = [&]() {
+        for (int g = 0; g < 2; g++) {
+            if (exec[g]) (void)hipGraphExecDestroy(exec[g]);
+            if (graph[g]) (void)hipGraphDestroy(graph[g]);
+        }
+        set_learned(ml_enabled);
+    };
     if (use_graph) {
-        HIP_TRY(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-        rc = cycle();
-        if (!rc) rc = enqueue_fine_residual(pb, variant);
-        hipError_t ce = hipStreamEndCapture(stream(), &graph);
-        if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
-        if (ce != hipSuccess) MGM_FAIL(-10, std::string("graph capture failed: ") + hipGetErrorString(ce));
-        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (ie != hipSuccess) {
-            (void)hipGraphDestroy(graph);
-            MGM_FAIL(-10, std::string("graph instantiate failed: ") + hipGetErrorString(ie));
+        for (int g = 0; g < (learned ? 2 : 1); g++) {
+            set_learned(g == 1);
+            HIP_TRY(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+            rc = cycle();
+            if (!rc) rc = enqueue_fine_residual(pb, variant);
+            hipError_t ce = hipStreamEndCapture(stream(), &graph[g]);
+            if (rc) { cleanup(); return rc; }
+            if (ce != hipSuccess) { cleanup(); MGM_FAIL(-10, std::string("graph capture failed: ") + hipGetErrorString(ce)); }
+            hipError_t ie = hipGraphInstantiate(&exec[g], graph[g], nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                cleanup();
+                MGM_FAIL(-10, std::string("graph instantiate failed: ") + hipGetErrorString(ie));
+            }
         }
     }
     HIP_TRY(hipStreamSynchronize(stream()));
     double t0 = wall();
     int k = 0;
     double ratio = history ? history[0] : 1.0;
+    double prev = sqrt(rs / L0.N) / fnorm, rho_plain = 0;
+    bool use_learned = false;
     for (k = 1; k <= max_cycles; k++) {
+        const int g = use_learned ? 1 : 0;
         if (use_graph) {
-            hipError_t le = hipGraphLaunch(exec, stream());
+            hipError_t le = hipGraphLaunch(exec[g], stream());
             if (le != hipSuccess) { rc = -10; set_error(hipGetErrorString(le)); break; }
         } else {
+            set_learned(use_learned);
             if ((rc = cycle())) break;
             if ((rc = enqueue_fine_residual(pb, variant))) break;
         }
@@ -115,10 +138,20 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
         ratio = sqrt(rs / L0.N) / fnorm;
         if (history) history[k] = ratio;
         if (!(ratio > tol)) break;  // also stops on NaN
+        if (learned) {
+            const double rho = prev > 0 ? ratio / prev : 1.0;
+            if (k == 1) {
+                rho_plain = rho;
+                use_learned = true;
+            } else if (use_learned && rho > rho_plain) {
+                use_learned = false;
+                ml->fell_back = 1;
+            }
+        }
+        prev = ratio;
     }
     double t1 = wall();
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    cleanup();
     if (rc) return rc;
     if (cycles_done) *cycles_done = (k > max_cycles) ? max_cycles : k;
     if (seconds) *seconds = t1 - t0;

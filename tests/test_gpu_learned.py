@@ -1,0 +1,124 @@
+"""Learned smoother on the GPU against the network oracle (oracle/mlp.py).
+
+No reference counterpart (the reference smooths with SOR only).  Bars:
+  * forward outputs, loss and loss gradient of the FP32 training kernels equal
+    the float64 oracle to 1e-5 / 1e-4 relative;
+  * one learned sweep changes p on every interior node by the oracle's
+    correction (BF16 inference weights, FP32 accumulation) to 1e-3 of the
+    largest correction, and leaves boundary nodes alone;
+  * training on (error, residual) pairs of the plain V-cycle lowers the loss;
+    the learned solve reaches the tolerance, and what it gains over plain SOR
+    (cycles, seconds, or a fallback) is printed.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+
+def problem(mgm, dim, sizes, mult=None):
+    pts = [mgm.Points.generate(dim, n, 11 + k, 0.2) for k, n in enumerate(sizes)]
+    cfg = mgm.config(dimension=dim, poly_degree=3, coarse_poly_degree=3, phs_degree=3,
+                     cloud_size_multiplier=mult or (3.5 if dim == 3 else 2.0), omega=1.0, num_relax=3,
+                     transfer_rows=mgm.ROWS_BY_TAG, sweep_order=mgm.ORDER_COLOUR)
+    P = mgm.Problem(cfg, pts).setup()
+    P.dirichlet_rows()
+    return P
+
+
+def he_weights(n_in, seed):
+    rng = np.random.default_rng(seed)
+    w = np.zeros(mlp.n_weights(n_in), np.float32)
+    W1, b1, W2, b2, w3, b3 = mlp.unpack(np.zeros(mlp.n_weights(n_in)), n_in)
+    o = 0
+    w[o:o + W1.size] = rng.standard_normal(W1.size) * np.sqrt(2 / n_in); o += W1.size
+    w[o:o + 64] = 0.1 * rng.standard_normal(64); o += 64
+    w[o:o + W2.size] = rng.standard_normal(W2.size) * np.sqrt(2 / 64); o += W2.size
+    w[o:o + 64] = 0.1 * rng.standard_normal(64); o += 64
+    w[o:o + 64] = rng.standard_normal(64) / 8; o += 64
+    w[o] = 0.05
+    return w
+
+
+def test_forward_and_gradient_match_oracle(gpu):
+    P = problem(gpu, 2, (24, 12))
+    n_in = P.mlp_inputs()
+    assert n_in == 20 * 2 + 20
+    w = he_weights(n_in, 1)
+    P.set_mlp_weights(w)
+    np.testing.assert_array_equal(P.mlp_weights(), w)
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((300, n_in)).astype(np.float32)
+    t = rng.standard_normal(300).astype(np.float32)
+    y = P.mlp_forward(X)
+    y_ref, _ = mlp.forward(X, w)
+    assert np.abs(y - y_ref).max() <= 1e-5 * max(1.0, np.abs(y_ref).max())
+    loss, g = P.mlp_gradient(X, t)
+    loss_ref, g_ref = mlp.loss_grad(X, t.astype(np.float64), w.astype(np.float64))
+    assert abs(loss - loss_ref) <= 1e-5 * loss_ref
+    assert np.abs(g - g_ref).max() <= 1e-4 * np.abs(g_ref).max()
+
+
+@pytest.mark.parametrize("dim,sizes", [(2, (40, 20, 10)), (3, (16, 8))])
+def test_learned_sweep_matches_oracle(gpu, dim, sizes):
+    print(f"[gpu learned sweep] dim {dim} sizes {sizes}", flush=True)
+    P = problem(gpu, dim, sizes)
+    n_in = P.mlp_inputs()
+    w = he_weights(n_in, 3)
+    P.set_mlp_weights(w)
+    P.use_learned_smoother(True)
+    P.set_smoother(1.0, 1)                       # relax = the learned sweep alone
+    geo = P.geometry(0)
+    N = geo["x"].size
+    rng = np.random.default_rng(4)
+    interior = geo["btag"] == 0
+    p0 = rng.standard_normal(N)
+    s = np.where(interior, rng.standard_normal(N), 0.0)
+    P.set_field(0, "p", p0)
+    P.set_field(0, "source", s)
+    P.relax(0)
+    dp = P.field(0, "p") - p0
+    cl, Lp = P.cloud(0), P.matrix(0, "lap_Poison")
+    res = np.where(interior, s - (Lp * p0[cl]).sum(1), 0.0)
+    h = P.info(0)["d_avg"]
+    want = np.zeros(N)
+    for i in np.flatnonzero(interior):
+        want[i] = mlp.correction(geo["x"], geo["y"], geo["z"], dim, cl, h, Lp, res, w, i)
+    assert np.all(dp[~interior] == 0)
+    err = np.abs(dp - want).max() / np.abs(want).max()
+    print(f"  max |dp - oracle| / max |oracle| = {err:.2e}", flush=True)
+    assert err < 1e-3
+
+
+def test_training_and_learned_solve(gpu):
+    import bench
+    P = problem(gpu, 2, (64, 32, 16))
+    l0, l1 = P.train_smoother(snapshots=6, samples_per_snapshot=4096, epochs=20, batch=2048, lr=2e-3, seed=5)
+    print(f"[gpu learned training] loss {l0:.4f} -> {l1:.4f}", flush=True)
+    assert l1 < 0.5 * l0
+    g = P.geometry(0)
+    f = bench.fourier_rhs(g["x"], g["y"], 0 * g["x"])
+    f[g["btag"] != 0] = 0.0
+    out = {}
+    for learned in (False, True):
+        P.use_learned_smoother(learned)
+        P.zero_fields()
+        P.set_field(0, "source", f)
+        r = P.solve(tol=1e-8, max_cycles=300)
+        out[learned] = r
+        assert r["history"][-1] <= 1e-8
+    fb = P.learned_fell_back()
+    print(f"  plain: {out[False]['cycles']} cycles {out[False]['seconds']:.4f}s; learned: "
+          f"{out[True]['cycles']} cycles {out[True]['seconds']:.4f}s; fell back: {fb}", flush=True)
+    if not fb:
+        # every learned cycle reduced the residual at least as much as the plain one
+        h = out[True]["history"]
+        rho_plain = h[1] / h[0]
+        assert np.all(h[2:] / h[1:-1] <= rho_plain * (1 + 1e-12))
