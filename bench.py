@@ -27,6 +27,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 1 << 30
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stencil_apply_3d_1m_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per level-0 stencil apply from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on the streaming copy), or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":")[1]))
+    except OSError:
+        pass
+    return None
 
 
 def fourier_rhs(x, y, z, seed=2, modes=16):
@@ -119,6 +133,8 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-mixed", action="store_true", help="skip the reduced-precision solves")
     ap.add_argument("--no-learned", action="store_true", help="skip training and the learned-smoother solve")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="set up, time the level-0 stencil apply and the streaming copy, print them, exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -130,6 +146,16 @@ def main():
     P, sizes, t_setup = build_problem(mgm, args.n, args.levels)
     info0 = P.info(0)
     N, m = info0["N"], info0["m"]
+    bytes_apply = N * (m * (8 + 4) + 8 + 8)
+    if args.spmv_only:
+        # the roofline kernel and the copy of known size that calibrates HBM counters
+        t_apply = P.time_spmv(0, args.spmv_reps)
+        t_copy = mgm.stream_copy_seconds(COPY_BYTES, args.spmv_reps)
+        print(json.dumps({"stencil_apply_us": round(t_apply * 1e6, 2),
+                          "stencil_apply_GBps": round(bytes_apply / t_apply / 1e9, 1),
+                          "copy_bytes": COPY_BYTES, "copy_us": round(t_copy * 1e6, 2),
+                          "copy_GBps": round(2 * COPY_BYTES / t_copy / 1e9, 1)}), flush=True)
+        return
     f = set_rhs(P)
     print(f"[bench] setup {t_setup:.2f}s levels={sizes} N0={N} m={m} colours0={info0['num_colours']} "
           f"phases={ {k: round(v, 3) for k, v in P.timings().items()} }", file=sys.stderr, flush=True)
@@ -205,8 +231,9 @@ def main():
 
     # stencil apply roofline: lap_Poison over level 0, FP64 weights + int32 columns
     t_apply = P.time_spmv(0, args.spmv_reps)
-    bytes_apply = N * (m * (8 + 4) + 8 + 8)
     achieved = bytes_apply / t_apply / 1e9
+    t_copy = mgm.stream_copy_seconds(COPY_BYTES, args.spmv_reps)
+    copy_gbs = 2 * COPY_BYTES / t_copy / 1e9
 
     cpu = None
     if not args.no_cpu:
@@ -236,10 +263,13 @@ def main():
                    "cycles": cycles[0], "converged": converged,
                    "final_ratio": finals[0], "setup_s": round(t_setup, 2),
                    "stencil_apply_GBps": round(achieved, 1),
+                   "stream_copy_GBps_measured": round(copy_gbs, 1),
+                   "stencil_apply_frac_of_measured_copy": round(achieved / copy_gbs, 4),
                    "reduced_precision": mixed,
                    "learned_smoother": learned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

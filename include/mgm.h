@@ -204,6 +204,9 @@ int mgm_learned_fell_back(const mgm_problem *p);      /* 1 if the last solve fel
  *            1 and 2 support the H variant in the multicolour order. */
 int mgm_solve(mgm_problem *p, int variant, int precision, double tol, int max_cycles,
               int *cycles_done, double *history, double *seconds);
+/* Seconds per copy of `bytes` (rounded down to 16) between two device buffers
+ * (a streaming-copy kernel): the measured HBM reference of the roofline. */
+int mgm_stream_copy(size_t bytes, int reps, double *seconds);
 /* Time `reps` applications of lap_Poison on level lev; returns seconds per apply. */
 int mgm_time_spmv(mgm_problem *p, int lev, int reps, double *seconds);
 

@@ -23,7 +23,7 @@ MAT = dict(Dx=0, Dy=1, Dz=2, lap=3, lap_Poison=4, restr=5, prol=6)
 FIELD = dict(p=0, source=1, res=2, pprime=3, u=4, v=5, w=6, u_new=7, v_new=8, w_new=9,
              p_old=10, T=11, dpdn=12, dpdx=13, dpdy=14, dpdz=15, u_old=16, v_old=17, w_old=18)
 
-__all__ = ["Config", "Points", "Problem", "init", "device_count", "MgmError", "config",
+__all__ = ["Config", "Points", "Problem", "init", "device_count", "MgmError", "config", "stream_copy_seconds",
            "VAR_H", "VAR_T", "VAR_F", "MAT", "FIELD"]
 
 
@@ -33,6 +33,13 @@ def device_count() -> int:
 
 def init(device: int = 0):
     check(lib().mgm_init(device), "mgm_init")
+
+
+def stream_copy_seconds(nbytes: int, reps: int = 20) -> float:
+    """Seconds per device-to-device streaming copy of nbytes (measured HBM reference)."""
+    s = C.c_double()
+    check(lib().mgm_stream_copy(int(nbytes), int(reps), C.byref(s)), "stream_copy")
+    return s.value
 
 
 def config(**kw) -> Config:

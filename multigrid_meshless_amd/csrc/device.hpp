@@ -184,6 +184,8 @@ int launch_pin(int N, const uint8_t *btag, const int *ref0_dev, double *p, doubl
 // sum over interior rows of (a-b)^2 into *out (device scalar); b nullable
 int launch_sqdiff_sum(int N, const uint8_t *btag, const double *a, const double *b,
                       double *partial, double *out, hipStream_t s);
+// out[i] = in[i] for n16 16-byte elements (streaming copy, HBM reference).
+int launch_stream_copy(long n16, const void *in, void *out, hipStream_t s);
 // Mixed precision: element-wise narrowing (n values) and p += e on non-padding rows.
 int launch_narrow_f32(long n, const double *in, float *out, hipStream_t s);
 int launch_narrow_bf16(long n, const double *in, bf16 *out, hipStream_t s);

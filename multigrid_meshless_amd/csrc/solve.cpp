@@ -159,6 +159,32 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
     return 0;
 }
 
+int mgm_stream_copy(size_t bytes, int reps, double *seconds)
+{
+    if (bytes < 16 || reps < 1 || !seconds) MGM_FAIL(-1, "stream_copy: bad arguments");
+    const long n16 = (long)(bytes / 16);
+    DevBuf<uint8_t> a, b;
+    HIP_TRY(a.alloc((size_t)n16 * 16));
+    HIP_TRY(b.alloc((size_t)n16 * 16));
+    HIP_TRY(hipMemsetAsync(a.p, 1, (size_t)n16 * 16, stream()));
+    int rc = launch_stream_copy(n16, a.p, b.p, stream());
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, stream()));
+    for (int r = 0; r < reps; r++)
+        if ((rc = launch_stream_copy(n16, a.p, b.p, stream()))) return rc;
+    HIP_TRY(hipEventRecord(e1, stream()));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *seconds = ms * 1e-3 / reps;
+    return 0;
+}
+
 int mgm_time_spmv(mgm_problem *pb, int lev, int reps, double *seconds)
 {
     if (!pb || !pb->ready) MGM_FAIL(-1, "problem is not set up");
