@@ -211,23 +211,28 @@ def main():
         P.set_mlp_weights(Pt.mlp_weights())
         del Pt
         P.use_learned_smoother(True)
-        reset(P, f)
-        P.solve(args.tol, args.max_cycles)
-        ts = []
-        for _ in range(args.steps):
+        learned = {"train_s": round(t_train, 2), "train_loss": [round(l0, 4), round(l1, 4)]}
+        for prec, tag in ((0, "fp64"), (1, "fp32"), (2, "fp32_bf16_coarse")):
+            if prec and args.no_mixed:
+                continue
             reset(P, f)
-            r = P.solve(args.tol, args.max_cycles)
-            ts.append(r["seconds"])
-        fb = P.learned_fell_back()
-        err = float(np.abs(P.field(0, "p") - p64).max() / np.abs(p64).max())
+            P.solve(args.tol, args.max_cycles, precision=prec)
+            ts = []
+            for _ in range(args.steps):
+                reset(P, f)
+                r = P.solve(args.tol, args.max_cycles, precision=prec)
+                ts.append(r["seconds"])
+            fb = P.learned_fell_back()
+            err = float(np.abs(P.field(0, "p") - p64).max() / np.abs(p64).max())
+            learned[tag] = {"s": round(float(np.median(ts)), 5), "cycles": r["cycles"],
+                            "final_ratio": float(r["history"][-1]), "fell_back_to_sor": fb,
+                            "max_rel_diff_vs_fp64_solution": err,
+                            "speedup_vs_fp64_sor": round(t_solve / float(np.median(ts)), 3)}
+            print(f"[bench] learned smoother, precision {prec} ({tag}): {r['cycles']} cycles {np.median(ts):.4f}s "
+                  f"final {r['history'][-1]:.3e} fell back {fb} field diff {err:.2e}", file=sys.stderr, flush=True)
+        print(f"[bench] learned smoother trained in {t_train:.1f}s, loss {l0:.3f} -> {l1:.3f}", file=sys.stderr,
+              flush=True)
         P.use_learned_smoother(False)
-        learned = {"s": round(float(np.median(ts)), 5), "cycles": r["cycles"], "final_ratio": float(r["history"][-1]),
-                   "fell_back_to_sor": fb, "train_s": round(t_train, 2), "train_loss": [round(l0, 4), round(l1, 4)],
-                   "max_rel_diff_vs_fp64_solution": err,
-                   "speedup_vs_fp64_sor": round(t_solve / float(np.median(ts)), 3)}
-        print(f"[bench] learned smoother: {r['cycles']} cycles {np.median(ts):.4f}s final {r['history'][-1]:.3e} "
-              f"fell back {fb} train {t_train:.1f}s loss {l0:.3f}->{l1:.3f} field diff {err:.2e}",
-              file=sys.stderr, flush=True)
 
     # stencil apply roofline: lap_Poison over level 0, FP64 weights + int32 columns
     t_apply = P.time_spmv(0, args.spmv_reps)

@@ -140,6 +140,18 @@ int op_learned_sweep(mgm_problem *pb, int l)
                        L.f[MGM_F_RES].p, L.f[MGM_F_P].p, stream());
 }
 
+// The same sweep inside the reduced-precision V-cycle: FP32 residual (res32
+// = src32 - A p32 with the level's narrowed weights) and FP32 correction.
+int op_learned_sweep_mp(mgm_problem *pb, int l, int (*residual_mp)(Level &))
+{
+    LearnedState &S = *pb->ml;
+    Level &L = pb->lv[l];
+    int rc = residual_mp(L);
+    if (rc) return rc;
+    return mlp_correct(L.Nd, L.op(OP_LAPP), L.dbtag.p, S.G[l].p, S.W1r.p, S.W2.p, S.b2.p, S.w3.p, S.b3,
+                       (const float *)L.res32.p, L.p32.p, stream());
+}
+
 bool learned_active(const mgm_problem *pb, int l, int variant)
 {
     return pb->ml && pb->ml->enabled && pb->ml->prepared && variant == MGM_VAR_H &&

@@ -9,6 +9,9 @@
 
 namespace mgm {
 
+bool learned_active(const mgm_problem *pb, int l, int variant);
+int op_learned_sweep_mp(mgm_problem *pb, int l, int (*residual_mp)(Level &));
+
 namespace {
 
 template <typename F>
@@ -18,12 +21,23 @@ int with_op(const Level &L, F &&f)
     return f(EllT<float>{L.col.p, L.lapp32.p, L.m, L.S});
 }
 
-int relax_mp(mgm_problem *pb, Level &L)
+int residual_mp(Level &L);
+
+// With the learned smoother active, its FP32 correction takes the place of
+// the first sweep.
+int relax_mp(mgm_problem *pb, int l)
 {
+    Level &L = pb->lv[l];
     const float w = pb->cfg.omega;
     const float c = 1.0f - pb->cfg.omega;
+    int it0 = 0;
+    if (learned_active(pb, l, MGM_VAR_H) && pb->cfg.num_relax > 0) {
+        int rc = op_learned_sweep_mp(pb, l, residual_mp);
+        if (rc) return rc;
+        it0 = 1;
+    }
     return with_op(L, [&](auto A) {
-        for (int it = 0; it < pb->cfg.num_relax; it++)
+        for (int it = it0; it < pb->cfg.num_relax; it++)
             for (int k = 0; k < L.ncol; k++) {
                 const int *list;
                 int first, n;
@@ -118,13 +132,13 @@ int op_correction_mp(mgm_problem *pb)
         if ((rc = launch_zero(L.Nd, L.dbtag.p, 0, L.p32.p, stream()))) return rc;
     const int nl = pb->L;
     for (int l = 0; l < nl; l++) {
-        if ((rc = relax_mp(pb, pb->lv[l]))) return rc;
+        if ((rc = relax_mp(pb, l))) return rc;
         if ((rc = residual_mp(pb->lv[l]))) return rc;
         if (l != nl - 1 && (rc = restrict_mp(pb->lv[l], pb->lv[l + 1]))) return rc;
     }
     for (int l = nl - 1; l > 0; l--) {
         if ((rc = prolong_mp(pb->lv[l - 1], pb->lv[l]))) return rc;
-        if (l != 1 && (rc = relax_mp(pb, pb->lv[l - 1]))) return rc;
+        if (l != 1 && (rc = relax_mp(pb, l - 1))) return rc;
     }
     return launch_add_correction(L0.Nd, L0.dbtag.p, L0.p32.p, L0.f[MGM_F_P].p, stream());
 }

@@ -248,10 +248,11 @@ geom_kernel(int Nd, Ell A, int dim, double h, const double *__restrict__ dx, con
 // j's 64 weights are contiguous), W2[k][q], w3[k]; FP32 biases b2, b3 and
 // FP32 accumulation.  Two passes over the stencil residuals: their RMS sigma,
 // then layer 1 accumulated in registers, one stencil entry at a time.
+template <typename TV>
 __global__ void __launch_bounds__(BLK)
 correct_kernel(int Nd, Ell A, const uint8_t *__restrict__ btag, const float *__restrict__ G,
                const bf16 *__restrict__ W1rT, const bf16 *__restrict__ W2, const float *__restrict__ b2,
-               const bf16 *__restrict__ w3, float b3, const double *__restrict__ res, double *__restrict__ p)
+               const bf16 *__restrict__ w3, float b3, const TV *__restrict__ res, TV *__restrict__ p)
 {
     const int r = blockIdx.x * BLK + threadIdx.x;
     if (r >= Nd || btag[r] != ROW_INTERIOR) return;
@@ -284,7 +285,7 @@ correct_kernel(int Nd, Ell A, const uint8_t *__restrict__ btag, const float *__r
         y += bf(w3[k]) * relu(a);
     }
     const double diag = A.w[ell_at(m, A.S, r, 0)];
-    p[r] = p[r] + (double)y * sigma / diag;
+    p[r] = p[r] + (TV)((double)y * sigma / diag);
 }
 
 }  // namespace
@@ -366,12 +367,17 @@ int mlp_geometry(int Nd, Ell A, int dim, double h, const double *dx, const doubl
     return 0;
 }
 
+template <typename TV>
 int mlp_correct(int Nd, Ell A, const uint8_t *btag, const float *G, const bf16 *W1rT, const bf16 *W2,
-                const float *b2, const bf16 *w3, float b3, const double *res, double *p, hipStream_t s)
+                const float *b2, const bf16 *w3, float b3, const TV *res, TV *p, hipStream_t s)
 {
     if (A.m > MLP_MAX_M) MGM_FAIL(-1, "learned smoother: stencil wider than supported");
-    LAUNCH(correct_kernel, Nd, Nd, A, btag, G, W1rT, W2, b2, w3, b3, res, p);
+    LAUNCH(correct_kernel<TV>, Nd, Nd, A, btag, G, W1rT, W2, b2, w3, b3, res, p);
     return 0;
 }
+template int mlp_correct<double>(int, Ell, const uint8_t *, const float *, const bf16 *, const bf16 *,
+                                 const float *, const bf16 *, float, const double *, double *, hipStream_t);
+template int mlp_correct<float>(int, Ell, const uint8_t *, const float *, const bf16 *, const bf16 *,
+                                const float *, const bf16 *, float, const float *, float *, hipStream_t);
 
 }  // namespace mgm

@@ -83,7 +83,7 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
     // learned cycle reduces the residual less than the plain one did, the rest
     // of the solve falls back to plain SOR.
     LearnedState *ml = pb->ml.get();
-    const bool learned = ml && ml->enabled && ml->prepared && variant == MGM_VAR_H && precision == 0;
+    const bool learned = ml && ml->enabled && ml->prepared && variant == MGM_VAR_H;
     if (ml) ml->fell_back = 0;
     auto set_learned = [&](bool on) { if (ml) ml->enabled = on; };
     const bool ml_enabled = ml && ml->enabled;

@@ -122,3 +122,12 @@ def test_training_and_learned_solve(gpu):
         h = out[True]["history"]
         rho_plain = h[1] / h[0]
         assert np.all(h[2:] / h[1:-1] <= rho_plain * (1 + 1e-12))
+    # inside the reduced-precision V-cycles as well: the FP64 residual reaches the tolerance
+    P.use_learned_smoother(True)
+    for prec in (1, 2):
+        P.zero_fields()
+        P.set_field(0, "source", f)
+        r = P.solve(tol=1e-10, max_cycles=300, precision=prec)
+        print(f"  learned, precision {prec}: {r['cycles']} cycles {r['seconds']:.4f}s "
+              f"fell back {P.learned_fell_back()}", flush=True)
+        assert r["history"][-1] <= 1e-10
