@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_FLOPS = 78.6e12
 COPY_BYTES = 1 << 30
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stencil_apply_3d_1m_traffic.md")
 
@@ -240,6 +241,18 @@ def main():
     t_copy = mgm.stream_copy_seconds(COPY_BYTES, args.spmv_reps)
     copy_gbs = 2 * COPY_BYTES / t_copy / 1e9
 
+    # weight build of level 0 (kernel time only): systems/s and the algorithmic
+    # flop rate 2/3 mpn^3 + 2 mpn^2 nrhs per system (LU + operator right-hand sides)
+    t_w, n_sys = P.rebuild_weights(0)
+    P.dirichlet_rows()                  # the rebuild restored the Neumann boundary rows
+    mpn, nrhs = m + info0["n"], 4
+    flop_sys = 2.0 / 3.0 * mpn ** 3 + 2.0 * mpn ** 2 * nrhs
+    weights = {"systems": n_sys, "kernel_s": round(t_w, 4), "systems_per_s": round(n_sys / t_w),
+               "mflop_per_system": round(flop_sys / 1e6, 4), "gflops": round(n_sys * flop_sys / t_w / 1e9, 1),
+               "frac_fp64_peak": round(n_sys * flop_sys / t_w / FP64_PEAK_FLOPS, 4)}
+    print(f"[bench] weights level 0: {n_sys} systems in {t_w:.3f}s = {n_sys / t_w:.0f} systems/s, "
+          f"{weights['gflops']} GFLOP/s", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         t_cpu, t_cyc = cpu_baseline(P, cycles[0], args.cpu_cycles, f)
@@ -271,6 +284,7 @@ def main():
                    "stream_copy_GBps_measured": round(copy_gbs, 1),
                    "stencil_apply_frac_of_measured_copy": round(achieved / copy_gbs, 4),
                    "reduced_precision": mixed,
+                   "weight_build": weights,
                    "learned_smoother": learned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

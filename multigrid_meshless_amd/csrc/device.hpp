@@ -129,7 +129,11 @@ struct EllT {
     const int *col;
     const TW *w;
     int m, S;
+    bool stream = false;   // slab larger than the last-level cache: load it non-temporally
 };
+// Slabs above this size cannot stay in the 256 MB last-level cache between passes.
+constexpr size_t kStreamSlabBytes = 256ull << 20;
+inline bool stream_slab(size_t rows, int m, size_t wbytes) { return rows * m * (wbytes + 4) > kStreamSlabBytes; }
 using Ell = EllT<double>;
 __host__ __device__ inline size_t ell_at(int m, int S, int r, int j)
 {

@@ -17,8 +17,8 @@ namespace {
 template <typename F>
 int with_op(const Level &L, F &&f)
 {
-    if (L.w16) return f(EllT<bf16>{L.col.p, L.lapp16.p, L.m, L.S});
-    return f(EllT<float>{L.col.p, L.lapp32.p, L.m, L.S});
+    if (L.w16) return f(EllT<bf16>{L.col.p, L.lapp16.p, L.m, L.S, stream_slab(L.Nd, L.m, 2)});
+    return f(EllT<float>{L.col.p, L.lapp32.p, L.m, L.S, stream_slab(L.Nd, L.m, 4)});
 }
 
 int residual_mp(Level &L);
@@ -61,9 +61,9 @@ int residual_mp(Level &L)
 int restrict_mp(Level &F, Level &C)
 {
     if (C.w16)
-        return launch_restrict(EllT<bf16>{C.rcol.p, C.rw16.p, C.mr, C.Sr}, C.Nd, C.dbtag.p, 0,
+        return launch_restrict(EllT<bf16>{C.rcol.p, C.rw16.p, C.mr, C.Sr, stream_slab(C.Nd, C.mr, 2)}, C.Nd, C.dbtag.p, 0,
                                (const float *)F.res32.p, C.src32.p, stream());
-    return launch_restrict(EllT<float>{C.rcol.p, C.rw32.p, C.mr, C.Sr}, C.Nd, C.dbtag.p, 0,
+    return launch_restrict(EllT<float>{C.rcol.p, C.rw32.p, C.mr, C.Sr, stream_slab(C.Nd, C.mr, 4)}, C.Nd, C.dbtag.p, 0,
                            (const float *)F.res32.p, C.src32.p, stream());
 }
 
@@ -71,10 +71,10 @@ int prolong_mp(Level &F, Level &C)
 {
     int rc;
     if (F.w16)
-        rc = launch_prolong(EllT<bf16>{F.pcol.p, F.pw16.p, F.mp, F.Sp}, F.Nd, F.dbtag.p,
+        rc = launch_prolong(EllT<bf16>{F.pcol.p, F.pw16.p, F.mp, F.Sp, stream_slab(F.Nd, F.mp, 2)}, F.Nd, F.dbtag.p,
                             (const float *)C.p32.p, F.p32.p, stream());
     else
-        rc = launch_prolong(EllT<float>{F.pcol.p, F.pw32.p, F.mp, F.Sp}, F.Nd, F.dbtag.p,
+        rc = launch_prolong(EllT<float>{F.pcol.p, F.pw32.p, F.mp, F.Sp, stream_slab(F.Nd, F.mp, 4)}, F.Nd, F.dbtag.p,
                             (const float *)C.p32.p, F.p32.p, stream());
     if (rc) return rc;
     return launch_zero(C.Nd, C.dbtag.p, 1, C.p32.p, stream());

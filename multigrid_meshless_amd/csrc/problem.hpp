@@ -65,7 +65,7 @@ struct Level {
     DevBuf<bf16> lapp16, rw16, pw16;
     DevBuf<float> p32, src32, res32;
 
-    Ell op(int which) const { return Ell{col.p, slab[which].p, m, S}; }
+    Ell op(int which) const { return Ell{col.p, slab[which].p, m, S, stream_slab(Nd, m, 8)}; }
     // slices of colour k for launch_sor_slices: a device list, or (null, first)
     void colour_slices(int k, const int **list, int *first, int *n) const
     {
@@ -73,8 +73,8 @@ struct Level {
         *list = col_contig ? nullptr : d_col_slices.p + col_soff[k];
         *first = (col_contig && *n) ? col_slices[col_soff[k]] : 0;
     }
-    Ell restr() const { return Ell{rcol.p, rw.p, mr, Sr}; }
-    Ell prol() const { return Ell{pcol.p, pw.p, mp, Sp}; }
+    Ell restr() const { return Ell{rcol.p, rw.p, mr, Sr, stream_slab(Nd, mr, 8)}; }
+    Ell prol() const { return Ell{pcol.p, pw.p, mp, Sp, stream_slab(Nd, mp, 8)}; }
 };
 
 }  // namespace mgm
