@@ -9,6 +9,8 @@
 //   [ (x_cj - x_i)/h for j < m, per axis d < dim ]  (m*dim values, h = level spacing)
 //   [ r_cj / sigma_i for j < m ]                     (m values, sigma_i = RMS of r over the stencil)
 // Output y; the correction is delta_i = y * sigma_i / a_ii.
+#include <cstdlib>
+
 #include "mlp.hpp"
 
 namespace mgm {
@@ -244,48 +246,110 @@ geom_kernel(int Nd, Ell A, int dim, double h, const double *__restrict__ dx, con
 
 // p[r] += y * sigma / a_rr on interior rows, y = network(offsets, scaled
 // stencil residuals) with the geometric part of layer 1 taken from G.
-// BF16 weights: W1rT[j][k] (residual columns of W1, transposed so that entry
-// j's 64 weights are contiguous), W2[k][q], w3[k]; FP32 biases b2, b3 and
-// FP32 accumulation.  Two passes over the stencil residuals: their RMS sigma,
-// then layer 1 accumulated in registers, one stencil entry at a time.
-template <typename TV>
+// BF16 weights W1rT[j][k] (residual columns of W1, transposed), W2[k][q],
+// w3[k] are widened into LDS once per block (rows padded to 65 floats so the
+// threads of one node read different banks); FP32 biases and accumulation.
+// T threads share a node: T = 1 for large levels (the node's work in one
+// thread's registers), T = 16 for small ones (more threads in flight): the
+// node's stencil residuals go to LDS, each thread computes 64/T hidden units
+// of each layer, and the output is summed across the T lanes.
+constexpr int MLP_LD = MLP_H + 1;
+template <typename TV, int T>
 __global__ void __launch_bounds__(BLK)
 correct_kernel(int Nd, Ell A, const uint8_t *__restrict__ btag, const float *__restrict__ G,
                const bf16 *__restrict__ W1rT, const bf16 *__restrict__ W2, const float *__restrict__ b2,
                const bf16 *__restrict__ w3, float b3, const TV *__restrict__ res, TV *__restrict__ p)
 {
-    const int r = blockIdx.x * BLK + threadIdx.x;
-    if (r >= Nd || btag[r] != ROW_INTERIOR) return;
+    constexpr int NPB = BLK / T;            // nodes per block
+    constexpr int KPT = MLP_H / T;          // hidden units per thread
+    __shared__ float sW1[MLP_MAX_M * MLP_LD], sW2[MLP_H * MLP_LD], sw3[MLP_H], sb2[MLP_H];
+    __shared__ double sR[T > 1 ? NPB : 1][MLP_MAX_M + 1];
+    __shared__ float sH[T > 1 ? NPB : 1][MLP_LD];
     const int m = A.m;
-    double s2 = 0;
-    for (int j = 0; j < m; j++) {
-        const double v = res[A.col[ell_at(m, A.S, r, j)]];
-        s2 += v * v;
+    for (int i = threadIdx.x; i < m * MLP_H; i += BLK) sW1[(i / MLP_H) * MLP_LD + i % MLP_H] = bf(W1rT[i]);
+    for (int i = threadIdx.x; i < MLP_H * MLP_H; i += BLK) sW2[(i / MLP_H) * MLP_LD + i % MLP_H] = bf(W2[i]);
+    if (threadIdx.x < MLP_H) {
+        sw3[threadIdx.x] = bf(w3[threadIdx.x]);
+        sb2[threadIdx.x] = b2[threadIdx.x];
     }
-    const double sigma = sqrt(s2 / m);
-    if (!(sigma > 0)) return;
-    const double inv = 1.0 / sigma;
-    float h1[MLP_H];
+    __syncthreads();
+    const int nl = threadIdx.x / T, sub = threadIdx.x % T;
+    const int r = blockIdx.x * NPB + nl;
+    const bool valid = r < Nd && btag[r] == ROW_INTERIOR;
+    if constexpr (T == 1) {
+        if (!valid) return;
+        double s2 = 0;
+        for (int j = 0; j < m; j++) {
+            const double v = res[A.col[ell_at(m, A.S, r, j)]];
+            s2 += v * v;
+        }
+        const double sigma = sqrt(s2 / m);
+        if (!(sigma > 0)) return;
+        const double inv = 1.0 / sigma;
+        float h1[MLP_H];
 #pragma unroll
-    for (int k = 0; k < MLP_H; k++) h1[k] = G[(size_t)r * MLP_H + k];
-    for (int j = 0; j < m; j++) {
-        const float u = (float)(res[A.col[ell_at(m, A.S, r, j)]] * inv);
-        const bf16 *wj = W1rT + (size_t)j * MLP_H;
+        for (int k = 0; k < MLP_H; k++) h1[k] = G[(size_t)r * MLP_H + k];
+        for (int j = 0; j < m; j++) {
+            const float u = (float)(res[A.col[ell_at(m, A.S, r, j)]] * inv);
 #pragma unroll
-        for (int k = 0; k < MLP_H; k++) h1[k] += bf(wj[k]) * u;
+            for (int k = 0; k < MLP_H; k++) h1[k] += sW1[j * MLP_LD + k] * u;
+        }
+#pragma unroll
+        for (int k = 0; k < MLP_H; k++) h1[k] = relu(h1[k]);
+        float y = b3;
+        for (int k = 0; k < MLP_H; k++) {
+            float a = sb2[k];
+#pragma unroll
+            for (int q = 0; q < MLP_H; q++) a += sW2[k * MLP_LD + q] * h1[q];
+            y += sw3[k] * relu(a);
+        }
+        const double diag = A.w[ell_at(m, A.S, r, 0)];
+        p[r] = p[r] + (TV)((double)y * sigma / diag);
+    } else {
+        // stencil residuals of the node, split over its T lanes
+        double s2 = 0;
+        if (valid)
+            for (int j = sub; j < m; j += T) {
+                const double v = res[A.col[ell_at(m, A.S, r, j)]];
+                sR[nl][j] = v;
+                s2 += v * v;
+            }
+#pragma unroll
+        for (int o = T / 2; o > 0; o /= 2) s2 += __shfl_xor(s2, o, T);
+        const double sigma = sqrt(s2 / m);
+        const bool go = valid && sigma > 0;
+        const double inv = go ? 1.0 / sigma : 0.0;
+        __syncthreads();
+        if (go) {
+            float h[KPT];
+#pragma unroll
+            for (int i = 0; i < KPT; i++) h[i] = G[(size_t)r * MLP_H + sub * KPT + i];
+            for (int j = 0; j < m; j++) {
+                const float u = (float)(sR[nl][j] * inv);
+#pragma unroll
+                for (int i = 0; i < KPT; i++) h[i] += sW1[j * MLP_LD + sub * KPT + i] * u;
+            }
+#pragma unroll
+            for (int i = 0; i < KPT; i++) sH[nl][sub * KPT + i] = relu(h[i]);
+        }
+        __syncthreads();
+        float y = 0.f;
+        if (go) {
+#pragma unroll
+            for (int i = 0; i < KPT; i++) {
+                const int k = sub * KPT + i;
+                float a = sb2[k];
+                for (int q = 0; q < MLP_H; q++) a += sW2[k * MLP_LD + q] * sH[nl][q];
+                y += sw3[k] * relu(a);
+            }
+        }
+#pragma unroll
+        for (int o = T / 2; o > 0; o /= 2) y += __shfl_xor(y, o, T);
+        if (go && sub == 0) {
+            const double diag = A.w[ell_at(m, A.S, r, 0)];
+            p[r] = p[r] + (TV)((double)(y + b3) * sigma / diag);
+        }
     }
-#pragma unroll
-    for (int k = 0; k < MLP_H; k++) h1[k] = relu(h1[k]);
-    float y = b3;
-    for (int k = 0; k < MLP_H; k++) {
-        const bf16 *wk = W2 + (size_t)k * MLP_H;
-        float a = b2[k];
-#pragma unroll
-        for (int q = 0; q < MLP_H; q++) a += bf(wk[q]) * h1[q];
-        y += bf(w3[k]) * relu(a);
-    }
-    const double diag = A.w[ell_at(m, A.S, r, 0)];
-    p[r] = p[r] + (TV)((double)y * sigma / diag);
 }
 
 }  // namespace
@@ -372,7 +436,13 @@ int mlp_correct(int Nd, Ell A, const uint8_t *btag, const float *G, const bf16 *
                 const float *b2, const bf16 *w3, float b3, const TV *res, TV *p, hipStream_t s)
 {
     if (A.m > MLP_MAX_M) MGM_FAIL(-1, "learned smoother: stencil wider than supported");
-    LAUNCH(correct_kernel<TV>, Nd, Nd, A, btag, G, W1rT, W2, b2, w3, b3, res, p);
+    const char *e = getenv("MGM_MLP_WIDE_ROWS");
+    const int wide = e ? atoi(e) : 200000;   // levels with more rows: one thread per node
+    if (Nd > wide) {
+        LAUNCH((correct_kernel<TV, 1>), Nd, Nd, A, btag, G, W1rT, W2, b2, w3, b3, res, p);
+    } else {
+        LAUNCH((correct_kernel<TV, 16>), (long)Nd * 16, Nd, A, btag, G, W1rT, W2, b2, w3, b3, res, p);
+    }
     return 0;
 }
 template int mlp_correct<double>(int, Ell, const uint8_t *, const float *, const bf16 *, const bf16 *,

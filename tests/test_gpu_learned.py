@@ -66,9 +66,12 @@ def test_forward_and_gradient_match_oracle(gpu):
     assert np.abs(g - g_ref).max() <= 1e-4 * np.abs(g_ref).max()
 
 
+@pytest.mark.parametrize("wide_rows", ["0", "200000"], ids=["thread_per_node", "16_threads_per_node"])
 @pytest.mark.parametrize("dim,sizes", [(2, (40, 20, 10)), (3, (16, 8))])
-def test_learned_sweep_matches_oracle(gpu, dim, sizes):
-    print(f"[gpu learned sweep] dim {dim} sizes {sizes}", flush=True)
+def test_learned_sweep_matches_oracle(gpu, dim, sizes, wide_rows, monkeypatch):
+    """MGM_MLP_WIDE_ROWS picks the kernel form: levels above it use one thread per node."""
+    monkeypatch.setenv("MGM_MLP_WIDE_ROWS", wide_rows)
+    print(f"[gpu learned sweep] dim {dim} sizes {sizes} wide_rows {wide_rows}", flush=True)
     P = problem(gpu, dim, sizes)
     n_in = P.mlp_inputs()
     w = he_weights(n_in, 3)
