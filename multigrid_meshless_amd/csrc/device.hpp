@@ -111,6 +111,9 @@ struct WeightsArgs {
     int batch;
     int *status;        // nonzero if a zero pivot was met
 };
+// Systems with mpn + 4 <= 128 run one workgroup per system in registers
+// (weights_tile(mpn) > 0) and need no workspace; wider ones use ws/rhs/piv.
+int weights_tile(int mpn);
 int launch_weights(const WeightsArgs &a, hipStream_t s);
 
 // ---- stencil kernels (stencil.hip) ----------------------------------------

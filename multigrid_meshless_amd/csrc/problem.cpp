@@ -540,13 +540,15 @@ static int level_weights(mgm_problem *pb, int l, double *kernel_seconds)
     if (!prow.empty()) { HIP_TRY(d_prow.upload(prow, g_stream)); HIP_TRY(d_pxyz.upload(pxyz, g_stream)); }
     HIP_TRY(d_status.alloc(1));
     HIP_TRY(hipMemsetAsync(d_status.p, 0, sizeof(int), g_stream));
-    // batch sized for ~4 GiB of factor workspace
+    // batch sized for ~4 GiB of factor workspace (only the wide-system kernel uses one)
     long batch = (long)((4.0 * 1024 * 1024 * 1024) / (8.0 * mpn * mpn));
     batch = std::max(64L, std::min(batch, (long)sys.size()));
     batch = (batch + 63) / 64 * 64;
-    HIP_TRY(ws.alloc((size_t)mpn * mpn * batch));
-    HIP_TRY(rhs.alloc((size_t)4 * mpn * batch));
-    HIP_TRY(piv.alloc((size_t)mpn * batch));
+    if (!weights_tile(mpn)) {
+        HIP_TRY(ws.alloc((size_t)mpn * mpn * batch));
+        HIP_TRY(rhs.alloc((size_t)4 * mpn * batch));
+        HIP_TRY(piv.alloc((size_t)mpn * batch));
+    }
     WeightsArgs a{};
     a.nsys = (int)sys.size(); a.sys = d_sys.p; a.m = m; a.n = L.n; a.dim = dim; a.phs = pb->cfg.phs_degree;
     a.px = L.d_px.p; a.py = L.d_py.p; a.pz = L.d_pz.p;

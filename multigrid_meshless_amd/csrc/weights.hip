@@ -228,10 +228,269 @@ weights_kernel(WeightsArgs a, int base, int count)
 #undef RHS
 }
 
+// ---- register-tiled LU: one workgroup per system ---------------------------
+//
+// The augmented matrix [A | b_1 .. b_R] (NC = 16*TILE rows and columns, the
+// first mpn of each used, then up to NC - mpn right-hand sides) is spread
+// cyclically over a 16 x 16 grid of the 256 threads: element (i, j) lives in
+// thread (i % 16, j % 16), register slot (i / 16, j / 16).  Gaussian
+// elimination with partial pivoting (largest |a_ik| among the rows not yet
+// used as pivot, first such row on ties) permutes rows logically: each step
+// broadcasts the pivot column and the pivot row through LDS (double-buffered
+// by step parity) and updates every remaining row in registers, right-hand
+// sides included.  Back substitution then walks the pivot rows in reverse
+// pivot order.  Systems with more right-hand sides than free columns run in
+// several passes.
+constexpr int LU_T = 256;
+
+// Entry (i, j) of the augmented matrix: A (PHS block, polynomial blocks,
+// zero block), right-hand side j - mpn for j >= mpn, 0 in the padding.  Kept
+// out of line: the kernel calls it TILE^2 times per pass.
+__device__ __attribute__((noinline)) double lu_entry(const WeightsArgs &a, int i, int j, int m, int n, int nrhs,
+                                                      int q0, int nops, const int *ops, const double *cx,
+                                                      const double *cy, const double *cz, const double (*tgt)[3],
+                                                      double sx, double sy, double sz)
+{
+    const int mpn = m + n, dim = a.dim, phs = a.phs;
+    if (i >= mpn) return 0;
+    if (j < mpn) {
+        if (i < m && j < m)
+            return i == j ? 0.0 : rbf_phs(dist_sum(cx[i], cy[i], cz[i], cx[j], cy[j], cz[j], dim), phs);
+        if (i >= m && j >= m) return 0;
+        const int pt = i < m ? i : j, k = (i < m ? j : i) - m;
+        const double dx = cx[pt] - sx, dy = cy[pt] - sy, dz = cz[pt] - sz;
+        double w = 1;
+        if (a.px[k]) w *= ipow(dx, a.px[k]);
+        if (a.py[k]) w *= ipow(dy, a.py[k]);
+        if (a.pz[k]) w *= ipow(dz, a.pz[k]);
+        return w;
+    }
+    if (j >= mpn + nrhs) return 0;
+    const int r = j - mpn, q = q0 + r;
+    if (q < nops) {
+        // row 0 of the operator's right-hand-side matrix (x = seed)
+        const int op = ops[q];
+        if (i < m) {
+            const double s2 = dist_sum(sx, sy, sz, cx[i], cy[i], cz[i], dim);
+            const double rb = rbf_phs(s2, phs - 2);
+            if (op < 3) {
+                const double d = (op == 0) ? sx - cx[i] : (op == 1 ? sy - cy[i] : sz - cz[i]);
+                return phs * d * rb;
+            }
+            return (phs * phs) * rb;
+        }
+        const int k = i - m, ex = a.px[k], ey = a.py[k], ez = a.pz[k];
+        if (op == 0 && ex == 1 && ey == 0 && ez == 0) return 1;
+        if (op == 1 && ex == 0 && ey == 1 && ez == 0) return 1;
+        if (op == 2 && ex == 0 && ey == 0 && ez == 1) return 1;
+        if (op == 3 && ((ex == 2 && ey == 0 && ez == 0) || (ex == 0 && ey == 2 && ez == 0) ||
+                        (ex == 0 && ey == 0 && ez == 2)))
+            return 2;
+        return 0;
+    }
+    if (i < m) return rbf_phs(dist_sum(tgt[r][0], tgt[r][1], tgt[r][2], cx[i], cy[i], cz[i], dim), phs);
+    const int k = i - m;
+    return ipow(tgt[r][0] - sx, a.px[k]) * ipow(tgt[r][1] - sy, a.py[k]) * ipow(tgt[r][2] - sz, a.pz[k]);
+}
+
+template <int TILE>
+__global__ void __launch_bounds__(LU_T)
+lu_kernel(WeightsArgs a, int base)
+{
+    constexpr int NC = 16 * TILE;
+    __shared__ double cx[NC], cy[NC], cz[NC];
+    __shared__ double colk[2][NC], rowp[2][NC], xk[2][NC];
+    __shared__ int sp[2], porder[NC], pos[NC];
+    __shared__ double *outp[NC];
+    __shared__ double tgt[NC][3];
+    __shared__ int s_bad;
+    const int s = a.sys[base + blockIdx.x];
+    const int m = a.m, n = a.n, mpn = m + n, dim = a.dim, phs = a.phs;
+    const int tid = threadIdx.x, ti = tid % 16, tj = tid / 16;
+    const int *cl = a.cloud + (size_t)s * m;
+    for (int i = tid; i < m; i += LU_T) {
+        cx[i] = a.x[cl[i]];
+        cy[i] = a.y[cl[i]];
+        cz[i] = a.z[cl[i]];
+    }
+    // right-hand sides of this system: operators, then interpolation targets
+    __shared__ int ops[4];
+    const int nops = (a.Dx ? 1 : 0) + (a.Dy ? 1 : 0) + ((a.Dz && dim == 3) ? 1 : 0) + (a.lap ? 1 : 0);
+    if (tid == 0) {
+        int q = 0;
+        if (a.Dx) ops[q++] = 0;
+        if (a.Dy) ops[q++] = 1;
+        if (a.Dz && dim == 3) ops[q++] = 2;
+        if (a.lap) ops[q++] = 3;
+    }
+    const int nr = a.rt_off ? a.rt_off[s + 1] - a.rt_off[s] : 0;
+    const int np = a.pt_off ? a.pt_off[s + 1] - a.pt_off[s] : 0;
+    const int total = nops + nr + np;
+    const int cap = NC - mpn;
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    const double sx = cx[0], sy = cy[0], sz = cz[0];
+    for (int q0 = 0; q0 < total; q0 += cap) {
+        const int nrhs = (total - q0 < cap) ? total - q0 : cap;
+        // pivot positions: NC = row not yet used as pivot
+        for (int i = tid; i < NC; i += LU_T) pos[i] = NC;
+        // output rows and target points of this pass
+        for (int r = tid; r < nrhs; r += LU_T) {
+            const int q = q0 + r;
+            if (q < nops) {
+                const int op = ops[q];
+                outp[r] = (op == 0 ? a.Dx : op == 1 ? a.Dy : op == 2 ? a.Dz : a.lap) + (size_t)s * m;
+            } else {
+                const double *t;
+                if (q < nops + nr) {
+                    outp[r] = a.R_out + (size_t)a.rt_row[a.rt_off[s] + q - nops] * m;
+                    t = a.rt_xyz + 3 * (size_t)(a.rt_off[s] + q - nops);
+                } else {
+                    outp[r] = a.P_out + (size_t)a.pt_row[a.pt_off[s] + q - nops - nr] * m;
+                    t = a.pt_xyz + 3 * (size_t)(a.pt_off[s] + q - nops - nr);
+                }
+                tgt[r][0] = t[0]; tgt[r][1] = t[1]; tgt[r][2] = t[2];
+            }
+        }
+        __syncthreads();
+        // ---- assemble the tile (hf/rbf.h:42-77 and the right-hand sides) ----
+        double A[TILE][TILE];
+#pragma unroll
+        for (int ai = 0; ai < TILE; ai++)
+#pragma unroll
+            for (int cj = 0; cj < TILE; cj++)
+                A[ai][cj] = lu_entry(a, ti + 16 * ai, tj + 16 * cj, m, n, nrhs, q0, nops, ops, cx, cy, cz,
+                                     tgt, sx, sy, sz);
+        const int ncols = mpn + nrhs;
+        // ---- elimination ----
+        for (int k = 0; k < mpn; k++) {
+            const int bf = k & 1;
+            if (tj == k % 16)
+#pragma unroll
+                for (int ai = 0; ai < TILE; ai++)
+#pragma unroll
+                    for (int cj = 0; cj < TILE; cj++)
+                        if (cj == k / 16 && ti + 16 * ai < mpn) colk[bf][ti + 16 * ai] = A[ai][cj];
+            __syncthreads();
+            if (tid < 64) {
+                double best = -1;
+                int bi = NC;
+                for (int i = tid; i < mpn; i += 64) {
+                    if (pos[i] < k) continue;   // already a pivot row
+                    const double v = fabs(colk[bf][i]);
+                    if (v > best) { best = v; bi = i; }
+                }
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    const double ob = __shfl_xor(best, o);
+                    const int oi = __shfl_xor(bi, o);
+                    if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+                }
+                if (tid == 0) {
+                    if (!(best > 0)) s_bad = 1;
+                    sp[bf] = bi;
+                    porder[k] = bi;
+                    pos[bi] = k;
+                }
+            }
+            __syncthreads();
+            if (s_bad) break;
+            const int p = sp[bf];
+            if (ti == p % 16)
+#pragma unroll
+                for (int ai = 0; ai < TILE; ai++)
+                    if (ai == p / 16)
+#pragma unroll
+                        for (int cj = 0; cj < TILE; cj++) rowp[bf][tj + 16 * cj] = A[ai][cj];
+            __syncthreads();
+            const double rinv = 1.0 / colk[bf][p];
+#pragma unroll
+            for (int ai = 0; ai < TILE; ai++) {
+                const int i = ti + 16 * ai;
+                if (i >= mpn || pos[i] <= k) continue;   // used as pivot (at this step or before)
+                const double l = colk[bf][i] * rinv;
+#pragma unroll
+                for (int cj = 0; cj < TILE; cj++) {
+                    const int j = tj + 16 * cj;
+                    if (j > k && j < ncols) A[ai][cj] -= l * rowp[bf][j];
+                }
+            }
+        }
+        if (s_bad) {
+            if (tid == 0) atomicOr(a.status, 1);
+            return;
+        }
+        // ---- back substitution in reverse pivot order ----
+        for (int k = mpn - 1; k >= 0; k--) {
+            const int bf = k & 1;
+            const int pk = porder[k];
+            if (tj == k % 16)
+#pragma unroll
+                for (int ai = 0; ai < TILE; ai++)
+#pragma unroll
+                    for (int cj = 0; cj < TILE; cj++)
+                        if (cj == k / 16 && ti + 16 * ai < mpn) colk[bf][ti + 16 * ai] = A[ai][cj];
+            __syncthreads();
+            if (ti == pk % 16)
+#pragma unroll
+                for (int ai = 0; ai < TILE; ai++)
+                    if (ai == pk / 16)
+#pragma unroll
+                        for (int cj = 0; cj < TILE; cj++) {
+                            const int j = tj + 16 * cj;
+                            if (j >= mpn && j < ncols) {
+                                const double x = A[ai][cj] / colk[bf][pk];
+                                xk[bf][j - mpn] = x;
+                                if (k < m) outp[j - mpn][k] = x;
+                            }
+                        }
+            __syncthreads();
+#pragma unroll
+            for (int ai = 0; ai < TILE; ai++) {
+                const int i = ti + 16 * ai;
+                if (i >= mpn || pos[i] >= k) continue;   // rows still to be solved
+                const double u = colk[bf][i];
+#pragma unroll
+                for (int cj = 0; cj < TILE; cj++) {
+                    const int j = tj + 16 * cj;
+                    if (j >= mpn && j < ncols) A[ai][cj] -= u * xk[bf][j - mpn];
+                }
+            }
+        }
+        __syncthreads();   // LDS tables are rewritten by the next pass
+    }
+}
+
 }  // namespace
+
+int weights_tile(int mpn)
+{
+    for (int t = 2; t <= 8; t++)
+        if (16 * t >= mpn + 4) return t;
+    return 0;
+}
 
 int launch_weights(const WeightsArgs &a, hipStream_t s)
 {
+    const int tile = weights_tile(a.m + a.n);
+    if (tile) {
+        // one workgroup per system; launches of at most 2^20 systems
+        for (int base = 0; base < a.nsys; base += 1 << 20) {
+            const int count = (a.nsys - base < (1 << 20)) ? a.nsys - base : (1 << 20);
+            switch (tile) {
+            case 2: hipLaunchKernelGGL(lu_kernel<2>, dim3(count), dim3(LU_T), 0, s, a, base); break;
+            case 3: hipLaunchKernelGGL(lu_kernel<3>, dim3(count), dim3(LU_T), 0, s, a, base); break;
+            case 4: hipLaunchKernelGGL(lu_kernel<4>, dim3(count), dim3(LU_T), 0, s, a, base); break;
+            case 5: hipLaunchKernelGGL(lu_kernel<5>, dim3(count), dim3(LU_T), 0, s, a, base); break;
+            case 6: hipLaunchKernelGGL(lu_kernel<6>, dim3(count), dim3(LU_T), 0, s, a, base); break;
+            case 7: hipLaunchKernelGGL(lu_kernel<7>, dim3(count), dim3(LU_T), 0, s, a, base); break;
+            default: hipLaunchKernelGGL(lu_kernel<8>, dim3(count), dim3(LU_T), 0, s, a, base); break;
+            }
+            HIP_TRY(hipGetLastError());
+        }
+        return 0;
+    }
+    if (!a.ws) MGM_FAIL(-1, "weights: workspace missing for wide systems");
     for (int base = 0; base < a.nsys; base += a.batch) {
         int count = (a.nsys - base < a.batch) ? a.nsys - base : a.batch;
         hipLaunchKernelGGL(weights_kernel, dim3((count + 63) / 64), dim3(64), 0, s, a, base, count);
