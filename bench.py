@@ -93,6 +93,55 @@ def reset(P, f):
     P.set_field(0, "source", f)
 
 
+def build_cavity(mgm, n, levels, Re, courant, num_relax, num_vcycles, seed0=31):
+    """Lid-driven cavity (init/init_LDC.c): jittered n^2 square and coarser clouds
+    (halving), PHS r^5 + degree-4 polynomials on level 0 and degree 3 below,
+    multiplier 2.0, dt from the CFL number (hf/general_functions.h:34-41)."""
+    from multigrid_meshless_amd import cases
+    sizes = [((n - 1) >> k) + 1 for k in range(levels)]
+    pts = [mgm.Points.generate(2, s, seed0 + k, 0.2) for k, s in enumerate(sizes)]
+    cfg = mgm.config(dimension=2, poly_degree=4, coarse_poly_degree=3, phs_degree=5,
+                     cloud_size_multiplier=2.0, omega=1.0, num_relax=num_relax,
+                     transfer_rows=mgm.ROWS_BY_TAG, sweep_order=mgm.ORDER_COLOUR)
+    P = mgm.Problem(cfg, pts).setup()
+    cases.lid_driven_cavity(P)
+    prm = cases.ns_params(Re, P.info(0)["d_avg"], courant, num_vcycles=num_vcycles)
+    P.ns_set_params(**prm)
+    return P, sizes, prm
+
+
+def cavity_march(mgm, args):
+    """Explicit fractional-step march of the lid-driven cavity, one whole step
+    (its V-cycles included) captured as a hipGraph and replayed; FP64 and the
+    FP32 pressure cycle in FP64 defect correction."""
+    out = {}
+    fields = {}
+    for prec, tag in ((0, "fp64"), (1, "fp32_pressure_cycle")):
+        P, sizes, prm = build_cavity(mgm, args.ldc_n, args.ldc_levels, args.ldc_re, 0.5, 5, 10)
+        P.ns_march(mgm.VAR_F, steps=2, precision=prec)          # warm-up: graph build, FP32 copies
+        P, sizes, prm = build_cavity(mgm, args.ldc_n, args.ldc_levels, args.ldc_re, 0.5, 5, 10)
+        r = P.ns_march(mgm.VAR_F, steps=args.ldc_steps, precision=prec)
+        fields[tag] = {k: P.field(0, k) for k in ("u", "v", "p")}
+        out[tag] = {"steps": r["steps"], "s": round(r["seconds"], 4),
+                    "ms_per_step": round(1e3 * r["seconds"] / max(r["steps"], 1), 4),
+                    "vcycles_per_s": round(r["steps"] * prm["num_vcycles"] / r["seconds"], 1),
+                    "steady_error_first_last": [float(r["errors"][0]), float(r["errors"][-1])]}
+        print(f"[bench] cavity {sizes} Re {args.ldc_re} {tag}: {r['steps']} steps in {r['seconds']:.3f}s "
+              f"= {out[tag]['ms_per_step']:.3f} ms/step, steady error {r['errors'][0]:.3e} -> "
+              f"{r['errors'][-1]:.3e}", file=sys.stderr, flush=True)
+        info = P.info(0)
+        del P
+    out["fp32_pressure_cycle"]["max_rel_diff_vs_fp64"] = {
+        k: float(np.abs(fields["fp32_pressure_cycle"][k] - fields["fp64"][k]).max() / np.abs(fields["fp64"][k]).max())
+        for k in ("u", "v", "p")}
+    out["workload"] = (f"2D lid-driven cavity, fractional step, Re {args.ldc_re}, CFL 0.5 (dt {prm['dt']:.3e}), "
+                       f"levels {sizes} ({info['N']} nodes, m={info['m']}), PHS r^5 + degree-4 polynomials "
+                       f"(degree 3 coarse), SOR omega 1 x5 sweeps, 10 V-cycles per step")
+    out["why_not_512"] = ("the reference's pressure multigrid (SOR on all-Neumann rows) diverges on this "
+                          "cavity from 257^2 up, in FP64 too, so the 512^2 case cannot march")
+    return out
+
+
 def cpu_baseline(P, cycles_needed, sample_cycles, f):
     """Oracle V-cycles (sequential sweep in the GPU's colour order) on the same operators."""
     import oracle
@@ -134,6 +183,11 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-mixed", action="store_true", help="skip the reduced-precision solves")
     ap.add_argument("--no-learned", action="store_true", help="skip training and the learned-smoother solve")
+    ap.add_argument("--no-ldc", action="store_true", help="skip the lid-driven-cavity time march")
+    ap.add_argument("--ldc-n", type=int, default=129, help="cavity points per side on the finest level")
+    ap.add_argument("--ldc-levels", type=int, default=3)
+    ap.add_argument("--ldc-re", type=float, default=100.0)
+    ap.add_argument("--ldc-steps", type=int, default=2000)
     ap.add_argument("--spmv-only", action="store_true",
                     help="set up, time the level-0 stencil apply and the streaming copy, print them, exit")
     args = ap.parse_args()
@@ -253,6 +307,8 @@ def main():
     print(f"[bench] weights level 0: {n_sys} systems in {t_w:.3f}s = {n_sys / t_w:.0f} systems/s, "
           f"{weights['gflops']} GFLOP/s", file=sys.stderr, flush=True)
 
+    cavity = None if args.no_ldc else cavity_march(mgm, args)
+
     cpu = None
     if not args.no_cpu:
         t_cpu, t_cyc = cpu_baseline(P, cycles[0], args.cpu_cycles, f)
@@ -285,7 +341,8 @@ def main():
                    "stencil_apply_frac_of_measured_copy": round(achieved / copy_gbs, 4),
                    "reduced_precision": mixed,
                    "weight_build": weights,
-                   "learned_smoother": learned},
+                   "learned_smoother": learned,
+                   "cavity_time_march": cavity},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic()},

@@ -159,6 +159,18 @@ int mgm_fs_step(mgm_problem *p, double *steady_error);
  * Gauss-Seidel (sweep order as configured), divergence source, T-variant
  * V-cycles on PPRIME, velocity/pressure update, Neumann boundary pressure. */
 int mgm_timple_step(mgm_problem *p, double *steady_error);
+/* The time loop of mg_NS_solver.c:68-83: up to max_steps steps of scheme
+ * MGM_VAR_F (fractional_step_explicit) or MGM_VAR_T (time_implicit_solver),
+ * stopping after the first step whose steady-state error is below steady_tol.
+ * One whole step, V-cycles included, is captured into a hipGraph once and
+ * replayed (the environment variable MGM_GRAPH=0 launches on the stream
+ * instead); errors[k] receives step k's error, *seconds the loop's wall time.
+ * precision 0: FP64 V-cycles, bitwise those of mgm_fs_step / mgm_timple_step.
+ * precision 1 (no reference counterpart; multicolour order): each pressure
+ * V-cycle becomes an FP64 defect correction by one FP32 V-cycle of the same
+ * variant; divergence, projection and the outer residual stay FP64. */
+int mgm_ns_march(mgm_problem *p, int scheme, int precision, int max_steps, double steady_tol,
+                 double *errors, int *steps_done, double *seconds);
 
 /* ---- learned smoother (no reference counterpart) -------------------------------
  * A per-node network replaces the first SOR sweep of every level's relaxation

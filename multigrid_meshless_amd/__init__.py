@@ -248,6 +248,15 @@ class Problem:
         check(lib().mgm_fs_step(self._h, C.byref(err)), "fs_step")
         return err.value
 
+    def ns_march(self, scheme=VAR_F, steps=1, steady_tol=0.0, precision=0) -> dict:
+        """The mg_NS_solver.c time loop: up to `steps` steps of the fractional step
+        (VAR_F) or TIMPLE (VAR_T), one step captured as a graph and replayed."""
+        errs = np.zeros(max(steps, 1))
+        done, sec = C.c_int(), C.c_double()
+        check(lib().mgm_ns_march(self._h, scheme, precision, steps, steady_tol, errs.ctypes.data,
+                                 C.byref(done), C.byref(sec)), "ns_march")
+        return {"steps": done.value, "errors": errs[:done.value], "seconds": sec.value}
+
     # ---- learned smoother ----
     def mlp_inputs(self) -> int:
         n = lib().mgm_mlp_inputs(self._h)

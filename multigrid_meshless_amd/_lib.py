@@ -84,6 +84,7 @@ SIGNATURES = [
     ("mgm_ns_set_params", _i, [_vp, _vp]),
     ("mgm_fs_step", _i, [_vp, C.POINTER(_d)]),
     ("mgm_timple_step", _i, [_vp, C.POINTER(_d)]),
+    ("mgm_ns_march", _i, [_vp, _i, _i, _i, _d, _vp, C.POINTER(_i), C.POINTER(_d)]),
     ("mgm_mlp_inputs", _i, [_vp]),
     ("mgm_mlp_weight_count", C.c_size_t, [_i]),
     ("mgm_mlp_set_weights", _i, [_vp, _vp]),

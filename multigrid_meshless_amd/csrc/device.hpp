@@ -186,8 +186,9 @@ template <typename TW, typename TV>
 int launch_prolong(EllT<TW> P, int N, const uint8_t *btag, const TV *pc, TV *p, hipStream_t s);
 template <typename TV>
 int launch_zero(int N, const uint8_t *btag, int interior_only, TV *p, hipStream_t s);
-int launch_pin(int N, const uint8_t *btag, const int *ref0_dev, double *p, double *scratch,
-               hipStream_t s);
+// p -= p[ref0] on real rows; scratch: one device double
+template <typename TV>
+int launch_pin(int N, const uint8_t *btag, const int *ref0_dev, TV *p, double *scratch, hipStream_t s);
 // sum over interior rows of (a-b)^2 into *out (device scalar); b nullable
 int launch_sqdiff_sum(int N, const uint8_t *btag, const double *a, const double *b,
                       double *partial, double *out, hipStream_t s);

@@ -5,6 +5,11 @@
 // the FP64 one (hf/heat_conduction.h:25-112 order and boundary handling) in
 // multicolour order, started from a zero correction.  It has no reference
 // counterpart: the reference solves in FP64 only.
+//
+// The time steps' pressure solve has an FP32 form too (op_correction_ns_mp):
+// each of its V-cycles becomes one FP64 defect correction by an FP32 cycle
+// of the step's own variant (T: hf/timple_solver.h:235-328, F:
+// hf/fractional_step_solver.h:168-246), applied to the correction e.
 #include "problem.hpp"
 
 namespace mgm {
@@ -80,6 +85,55 @@ int prolong_mp(Level &F, Level &C)
     return launch_zero(C.Nd, C.dbtag.p, 1, C.p32.p, stream());
 }
 
+EllT<float> op32(const Level &L, const DevBuf<float> &w)
+{
+    return EllT<float>{L.col.p, w.p, L.m, L.S, stream_slab(L.Nd, L.m, 4)};
+}
+
+// Source of the lap residual forms of variant F: level 0 has its own
+// (r_lap = s - lap p), coarse levels share the SOR source.
+const float *srcl(const Level &L, int l) { return l == 0 ? L.srcl32.p : L.src32.p; }
+
+// relaxation of variant T or F on the FP32 correction: SOR sweeps with
+// lap_Poison, each followed by the pin and the variant's residual
+int relax_ns_mp(mgm_problem *pb, int l, int variant)
+{
+    Level &L = pb->lv[l];
+    const float w = pb->cfg.omega;
+    const float c = 1.0f - pb->cfg.omega;
+    const EllT<float> A = op32(L, L.lapp32);
+    for (int it = 0; it < pb->cfg.num_relax; it++) {
+        int rc;
+        for (int k = 0; k < L.ncol; k++) {
+            const int *list;
+            int first, n;
+            L.colour_slices(k, &list, &first, &n);
+            if ((rc = launch_sor_slices(A, list, first, n, L.dbtag.p, (const float *)L.src32.p, L.p32.p, w, c,
+                                        stream())))
+                return rc;
+        }
+        if ((rc = launch_pin(L.Nd, L.dbtag.p, L.ref0.p, L.p32.p, pb->scal.p + 8, stream()))) return rc;
+        if (variant == MGM_VAR_T)
+            rc = launch_residual(A, L.Nd, 1, L.dbtag.p, (const float *)L.src32.p, (const float *)L.p32.p,
+                                 L.res32.p, stream());
+        else
+            rc = launch_residual(op32(L, L.lap32), L.Nd, 2, L.dbtag.p, srcl(L, l), (const float *)L.p32.p,
+                                 L.res32.p, stream());
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int residual_ns_mp(mgm_problem *pb, int l, int variant)
+{
+    Level &L = pb->lv[l];
+    if (variant == MGM_VAR_F)
+        return launch_residual(op32(L, L.lap32), L.Nd, 3, L.dbtag.p, srcl(L, l), (const float *)L.p32.p,
+                               L.res32.p, stream());
+    return launch_residual(op32(L, L.lapp32), L.Nd, 0, L.dbtag.p, (const float *)L.src32.p,
+                           (const float *)L.p32.p, L.res32.p, stream());
+}
+
 template <typename T>
 int narrow(const DevBuf<double> &src, DevBuf<T> &dst)
 {
@@ -141,6 +195,76 @@ int op_correction_mp(mgm_problem *pb)
         if (l != 1 && (rc = relax_mp(pb, l - 1))) return rc;
     }
     return launch_add_correction(L0.Nd, L0.dbtag.p, L0.p32.p, L0.f[MGM_F_P].p, stream());
+}
+
+// FP32 copies for the time steps' pressure cycle: those of precision 1 plus
+// lap on every level and level 0's second source.
+int mp_prepare_ns(mgm_problem *pb)
+{
+    int rc;
+    if ((rc = mp_prepare(pb, 1))) return rc;
+    if (pb->mp_ns_built) return 0;
+    for (auto &L : pb->lv)
+        if ((rc = narrow(L.slab[OP_LAP], L.lap32))) return rc;
+    Level &L0 = pb->lv[0];
+    if (L0.srcl32.n != (size_t)L0.Nd) HIP_TRY(L0.srcl32.alloc(L0.Nd));
+    HIP_TRY(hipStreamSynchronize(stream()));
+    pb->mp_ns_built = true;
+    return 0;
+}
+
+// One pressure V-cycle of a time step (variant T on p', F on p) as an FP64
+// defect correction: p is pinned in FP64, the residuals of p that the cycle's
+// level-0 forms need are formed in FP64 on every row (lap_Poison for the SOR
+// source; for F also lap), narrowed to FP32, and the FP32 cycle of the same
+// variant runs on the correction e from zero; then p += e on every row.
+// Relaxation, residual, restriction and prolongation are linear in (e, source)
+// and the pin commutes with a constant shift (rows of lap_Poison and lap sum
+// to zero), so in exact arithmetic p + e is what the FP64 cycle produces.
+int op_correction_ns_mp(mgm_problem *pb, int variant)
+{
+    Level &L0 = pb->lv[0];
+    double *p = L0.f[variant == MGM_VAR_T ? MGM_F_PPRIME : MGM_F_P].p;
+    const double *s = L0.f[MGM_F_SOURCE].p;
+    double *res = L0.f[MGM_F_RES].p;
+    int rc;
+    if ((rc = launch_pin(L0.Nd, L0.dbtag.p, L0.ref0.p, p, pb->scal.p + 8, stream()))) return rc;
+    if ((rc = launch_residual(L0.op(OP_LAPP), L0.Nd, 2, L0.dbtag.p, s, (const double *)p, res, stream()))) return rc;
+    if ((rc = launch_narrow_f32(L0.Nd, res, L0.src32.p, stream()))) return rc;
+    if (variant == MGM_VAR_F) {
+        if ((rc = launch_residual(L0.op(OP_LAP), L0.Nd, 2, L0.dbtag.p, s, (const double *)p, res, stream())))
+            return rc;
+        if ((rc = launch_narrow_f32(L0.Nd, res, L0.srcl32.p, stream()))) return rc;
+    }
+    // e = 0 on level 0.  Coarse levels start, as in the FP64 cycle, from what
+    // the last prolongation left: zero for T; for F zero interior rows and the
+    // boundary rows' values of the previous cycle, since F's prolongation zeroes
+    // only interior coarse rows (hf/fractional_step_solver.h:229-233).
+    const int keep = variant == MGM_VAR_F;
+    for (int l = 0; l < pb->L; l++) {
+        Level &L = pb->lv[l];
+        if ((rc = launch_zero(L.Nd, L.dbtag.p, l > 0 && keep, L.p32.p, stream()))) return rc;
+    }
+    const int nl = pb->L;
+    for (int l = 0; l < nl; l++) {
+        if ((rc = relax_ns_mp(pb, l, variant))) return rc;
+        if ((rc = residual_ns_mp(pb, l, variant))) return rc;
+        if (l != nl - 1) {
+            Level &F = pb->lv[l], &C = pb->lv[l + 1];
+            if ((rc = launch_restrict(EllT<float>{C.rcol.p, C.rw32.p, C.mr, C.Sr, stream_slab(C.Nd, C.mr, 4)}, C.Nd,
+                                      C.dbtag.p, keep, (const float *)F.res32.p, C.src32.p, stream())))
+                return rc;
+        }
+    }
+    for (int l = nl - 1; l > 0; l--) {
+        Level &F = pb->lv[l - 1], &C = pb->lv[l];
+        if ((rc = launch_prolong(EllT<float>{F.pcol.p, F.pw32.p, F.mp, F.Sp, stream_slab(F.Nd, F.mp, 4)}, F.Nd,
+                                 F.dbtag.p, (const float *)C.p32.p, F.p32.p, stream())))
+            return rc;
+        if ((rc = launch_zero(C.Nd, C.dbtag.p, variant != MGM_VAR_T, C.p32.p, stream()))) return rc;
+        if (l != 1 && (rc = relax_ns_mp(pb, l - 1, variant))) return rc;
+    }
+    return launch_add_correction(L0.Nd, L0.dbtag.p, L0.p32.p, p, stream());
 }
 
 }  // namespace mgm

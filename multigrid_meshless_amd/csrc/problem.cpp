@@ -588,6 +588,7 @@ static int level_slab(mgm_problem *pb, int l, int which)
 {
     Level &L = pb->lv[l];
     pb->mp_built = 0;  // narrowed copies no longer match
+    pb->mp_ns_built = false;
     if (!L.rows[which].p) return 0;
     size_t sz = ell_size(L.Nd, L.m, L.S);
     if (L.slab[which].n != sz) {
@@ -605,6 +606,7 @@ static int level_slabs(mgm_problem *pb, int l)
 {
     Level &L = pb->lv[l];
     pb->mp_built = 0;
+    pb->mp_ns_built = false;
     size_t sz = ell_size(L.Nd, L.m, L.S);
     if (L.col.n != sz) {
         HIP_TRY(L.col.alloc(sz));

@@ -64,6 +64,7 @@ struct Level {
     DevBuf<float> lapp32, rw32, pw32;
     DevBuf<bf16> lapp16, rw16, pw16;
     DevBuf<float> p32, src32, res32;
+    DevBuf<float> lap32, srcl32;               // time steps: FP32 lap; level 0's source for lap residuals
 
     Ell op(int which) const { return Ell{col.p, slab[which].p, m, S, stream_slab(Nd, m, 8)}; }
     // slices of colour k for launch_sor_slices: a device list, or (null, first)
@@ -110,6 +111,7 @@ struct mgm_problem {
     mgm_ns_params ns{};
     bool ns_set = false;
     int mp_built = 0;              // precision mode the FP32/BF16 copies hold (0: none / stale)
+    bool mp_ns_built = false;      // FP32 lap copies of the time steps' pressure cycle are current
     std::unique_ptr<mgm::LearnedState> ml;
     ~mgm_problem();
 };

@@ -399,18 +399,20 @@ zero_kernel(int N, const uint8_t *__restrict__ btag, int interior_only, TV *p)
     if (!interior_only || !btag[r]) p[r] = 0;
 }
 
-__global__ void copy_scalar_kernel(const int *ref0, const double *p, double *dst)
+template <typename TV>
+__global__ void copy_scalar_kernel(const int *ref0, const TV *p, double *dst)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = p[*ref0];
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = (double)p[*ref0];
 }
 
 // p -= p[node 0] (hf/timple_solver.h:248-252, hf/fractional_step_solver.h:180-183)
+template <typename TV>
 __global__ void __launch_bounds__(BLK)
-pin_kernel(int N, const uint8_t *__restrict__ btag, const double *__restrict__ pref, double *p)
+pin_kernel(int N, const uint8_t *__restrict__ btag, const double *__restrict__ pref, TV *p)
 {
     int r = blockIdx.x * BLK + threadIdx.x;
     if (r >= N || btag[r] == ROW_PAD) return;
-    p[r] = p[r] - *pref;
+    p[r] = p[r] - (TV)*pref;
 }
 
 // Sum over interior rows of (a-b)^2: block partial sums, then one block.
@@ -704,14 +706,16 @@ MGM_INSTANTIATE(bf16, float)
 #undef MGM_INSTANTIATE
 template int launch_zero<double>(int, const uint8_t *, int, double *, hipStream_t);
 template int launch_zero<float>(int, const uint8_t *, int, float *, hipStream_t);
-int launch_pin(int N, const uint8_t *btag, const int *ref0_dev, double *p, double *scratch,
-               hipStream_t s)
+template <typename TV>
+int launch_pin(int N, const uint8_t *btag, const int *ref0_dev, TV *p, double *scratch, hipStream_t s)
 {
-    hipLaunchKernelGGL(copy_scalar_kernel, dim3(1), dim3(64), 0, s, ref0_dev, p, scratch);
+    hipLaunchKernelGGL(copy_scalar_kernel<TV>, dim3(1), dim3(64), 0, s, ref0_dev, (const TV *)p, scratch);
     HIP_TRY(hipGetLastError());
-    LAUNCH(pin_kernel, N, N, btag, scratch, p);
+    LAUNCH(pin_kernel<TV>, N, N, btag, scratch, p);
     return 0;
 }
+template int launch_pin<double>(int, const uint8_t *, const int *, double *, double *, hipStream_t);
+template int launch_pin<float>(int, const uint8_t *, const int *, float *, double *, hipStream_t);
 int launch_sqdiff_sum(int N, const uint8_t *btag, const double *a, const double *b,
                       double *partial, double *out, hipStream_t s)
 {

@@ -1,0 +1,90 @@
+"""Lid-driven cavity time march on the GPU (init/init_LDC.c:16-44, the time loop
+of mg_NS_solver.c:68-83), multicolour order.
+
+Bars:
+  * mgm_ns_march (one whole step captured as a hipGraph and replayed) leaves
+    every level-0 field and every step's steady-state error bitwise equal to
+    as many eager mgm_fs_step / mgm_timple_step calls (the eager steps are
+    pinned to the reference harness in test_gpu_parity.py);
+  * the march stops after the first step whose error is below steady_tol;
+  * with the FP32 pressure cycle (precision 1, no reference counterpart) the
+    fields after one step and after several steps (NS_CASES) stay within 1e-4
+    of the FP64 march, relative to each field's largest magnitude, and the
+    error history within 1e-4 relative.  The FP32 cycle works on the correction; a wrong
+    source or pin in that form moves the one-step pressure by O(1), which the
+    one-step check catches.
+"""
+import numpy as np
+import pytest
+
+from multigrid_meshless_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("u", "v", "p", "u_new", "v_new", "p_old", "pprime")
+
+
+def ldc(mgm, scheme, sizes=(33, 17, 9), num_vcycles=3, omega=1.0, Re=100.0, num_relax=3, seed0=40):
+    pts = [mgm.Points.generate(2, n, seed0 + k, 0.2) for k, n in enumerate(sizes)]
+    cfg = mgm.config(dimension=2, poly_degree=4, coarse_poly_degree=3, phs_degree=5,
+                     cloud_size_multiplier=2.0, omega=omega, num_relax=num_relax,
+                     transfer_rows=mgm.ROWS_BY_TAG, sweep_order=mgm.ORDER_COLOUR)
+    P = mgm.Problem(cfg, pts).setup()
+    cases.lid_driven_cavity(P, timple=scheme == "timple")
+    P.ns_set_params(**cases.ns_params(Re, P.info(0)["d_avg"], 0.5, num_vcycles=num_vcycles,
+                                      iter_momentum=3, iter_timple=2))
+    return P
+
+
+def fields(P):
+    return {k: P.field(0, k) for k in FIELDS}
+
+
+@pytest.mark.parametrize("scheme", ["fs", "timple"])
+def test_graph_march_equals_eager_steps(gpu, scheme):
+    print(f"[gpu ns march] ldc 33/17/9 {scheme}: graph vs eager", flush=True)
+    var = gpu.VAR_F if scheme == "fs" else gpu.VAR_T
+    A, B = ldc(gpu, scheme), ldc(gpu, scheme)
+    eager = [A.fs_step() if scheme == "fs" else A.timple_step() for _ in range(4)]
+    assert min(eager) > 0
+    r = B.ns_march(var, steps=4)
+    assert r["steps"] == 4
+    np.testing.assert_array_equal(r["errors"], eager)
+    fa, fb = fields(A), fields(B)
+    for k in FIELDS:
+        np.testing.assert_array_equal(fa[k], fb[k], err_msg=k)
+    # the loop stops after the first step below the tolerance
+    C = ldc(gpu, scheme)
+    tol = 0.5 * (eager[1] + eager[2]) if eager[2] < eager[1] else eager[2] * 1.0000001
+    r = C.ns_march(var, steps=4, steady_tol=tol)
+    print(f"  errors {eager}; tol {tol:.6e} -> stopped after {r['steps']} steps", flush=True)
+    want = next(i + 1 for i, e in enumerate(eager) if e < tol)
+    assert r["steps"] == want
+
+
+# The reference's pressure multigrid on the all-Neumann cavity stagnates at these
+# sizes and diverges on finer clouds; whether a march stays finite depends on the
+# cloud.  The fractional-step case is the cloud the benchmark marches stably for
+# 2000 steps.
+NS_CASES = {"fs": dict(sizes=(129, 65, 33), num_vcycles=10, num_relax=5, seed0=31, steps=(1, 20)),
+            "timple": dict(sizes=(33, 17, 9), num_vcycles=3, num_relax=3, steps=(1, 3))}
+
+
+@pytest.mark.parametrize("scheme", ["fs", "timple"])
+def test_fp32_pressure_cycle_tracks_fp64(gpu, scheme):
+    var = gpu.VAR_F if scheme == "fs" else gpu.VAR_T
+    case = dict(NS_CASES[scheme])
+    for steps in case.pop("steps"):
+        print(f"[gpu ns march] ldc {case['sizes']} {scheme}: FP32 pressure cycle, {steps} steps", flush=True)
+        A, B = ldc(gpu, scheme, **case), ldc(gpu, scheme, **case)
+        ra = A.ns_march(var, steps=steps, precision=0)
+        rb = B.ns_march(var, steps=steps, precision=1)
+        fa, fb = fields(A), fields(B)
+        worst = {}
+        for k in (("u", "v", "p") if scheme == "fs" else ("u_new", "v_new", "p_old")):
+            scale = np.abs(fa[k]).max()
+            worst[k] = float(np.abs(fa[k] - fb[k]).max() / scale) if scale > 0 else 0.0
+        rel_err = np.abs(ra["errors"] - rb["errors"]) / np.abs(ra["errors"])
+        print(f"  field diffs {worst}; error history diff {rel_err.max():.2e}", flush=True)
+        assert all(v < 1e-4 for v in worst.values()), worst
+        assert rel_err.max() < 1e-4

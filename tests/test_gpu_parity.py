@@ -227,14 +227,16 @@ STEP_OUTPUTS = {
 }
 
 
+@pytest.mark.parametrize("march", [False, True], ids=["eager_steps", "graph_march"])
 @pytest.mark.parametrize("name,step", [("timple2d", "timple"), ("fs2d", "fs")])
-def test_time_steps_match_reference(name, step, gpu):
+def test_time_steps_match_reference(name, step, march, gpu):
     """Two whole time steps (time_implicit_solver, hf/timple_solver.h:40; fractional_step_explicit,
     hf/fractional_step_solver.h:47) from the reference's own state, with its operators and its
     natural sweep order: the step's fields equal the reference's bit for bit, its steady-state
-    error to 1e-14 relative."""
+    error to 1e-14 relative.  graph_march takes both steps through mgm_ns_march, which captures
+    one step as a hipGraph and replays it."""
     g = load_golden(name)
-    print(f"[gpu time step] {name}", flush=True)
+    print(f"[gpu time step] {name} march={march}", flush=True)
     P = gpu_problem(gpu, g)
     _load_reference_operators(P, g)
     dim = case_meta(g)[0]
@@ -246,16 +248,26 @@ def test_time_steps_match_reference(name, step, gpu):
     d, iv = g["st_params_d"], g["st_params_i"]
     P.ns_set_params(dt=d[0], rho=d[1], mu=d[2], nu=d[3], facRe=d[4], facdt=d[5], field_rho=d[6],
                     iter_momentum=int(iv[0]), iter_timple=int(iv[1]), num_vcycles=int(iv[2]))
+    if march:
+        r = P.ns_march(gpu.VAR_T if step == "timple" else gpu.VAR_F, steps=2)
+        assert r["steps"] == 2
+        errs = list(r["errors"])
+        checks = [1]          # fields after the last step
+    else:
+        errs, checks = [], [0, 1]
     for s in range(2):
-        err = P.timple_step() if step == "timple" else P.fs_step()
-        bad = []
-        for fid in range(19):
-            a, b = P.field(0, FIELD_NAMES[fid]), g[f"st{s}_L0_{fid}"]
-            if not np.array_equal(a, b):
-                bad.append((FIELD_NAMES[fid], float(np.abs(a - b).max()), float(np.abs(b).max())))
-        print(f"  step {s}: err {err!r} ref {float(g[f'st{s}_err'])!r} mismatched {bad}", flush=True)
-        assert not [b for b in bad if b[0] in STEP_OUTPUTS[step]], (s, bad)
+        if not march:
+            errs.append(P.timple_step() if step == "timple" else P.fs_step())
+        if s in checks:
+            bad = []
+            for fid in range(19):
+                a, b = P.field(0, FIELD_NAMES[fid]), g[f"st{s}_L0_{fid}"]
+                if not np.array_equal(a, b):
+                    bad.append((FIELD_NAMES[fid], float(np.abs(a - b).max()), float(np.abs(b).max())))
+            print(f"  step {s}: err {errs[s]!r} ref {float(g[f'st{s}_err'])!r} mismatched {bad}", flush=True)
+            assert not [b for b in bad if b[0] in STEP_OUTPUTS[step]], (s, bad)
+    for s in range(2):
         # the steady-state error is a sum of squares over all nodes: the GPU adds
         # it as a tree, the reference left to right, so it may differ in the last bit
         ref_err = float(g[f"st{s}_err"])
-        assert abs(err - ref_err) <= 1e-14 * abs(ref_err), (s, err, ref_err)
+        assert abs(errs[s] - ref_err) <= 1e-14 * abs(ref_err), (s, errs[s], ref_err)
