@@ -299,8 +299,8 @@ lu_kernel(WeightsArgs a, int base)
 {
     constexpr int NC = 16 * TILE;
     __shared__ double cx[NC], cy[NC], cz[NC];
-    __shared__ double colk[2][NC], rowp[2][NC], xk[2][NC];
-    __shared__ int sp[2], porder[NC], pos[NC];
+    __shared__ double colk[2][NC], rowp[2][NC], diag[NC];
+    __shared__ int sp[2], pos[NC];
     __shared__ double *outp[NC];
     __shared__ double tgt[NC][3];
     __shared__ int s_bad;
@@ -362,98 +362,84 @@ lu_kernel(WeightsArgs a, int base)
                 A[ai][cj] = lu_entry(a, ti + 16 * ai, tj + 16 * cj, m, n, nrhs, q0, nops, ops, cx, cy, cz,
                                      tgt, sx, sy, sz);
         const int ncols = mpn + nrhs;
-        // ---- elimination ----
+        // ---- Gauss-Jordan elimination with partial pivoting ----
+        // The 16 lanes holding column k (one lane group of one wave) extract it
+        // and find its pivot among the rows not yet used as pivots; the pivot
+        // row's owners publish it; then every other row takes -= l * (pivot
+        // row), branch-free, which clears column k above and below the pivot.
+        // Padding rows get l = 0, and columns past the right-hand sides hold
+        // zeros in every row.  colk / rowp / sp alternate between two buffers,
+        // so two barriers per step suffice.
         for (int k = 0; k < mpn; k++) {
             const int bf = k & 1;
-            if (tj == k % 16)
-#pragma unroll
-                for (int ai = 0; ai < TILE; ai++)
-#pragma unroll
-                    for (int cj = 0; cj < TILE; cj++)
-                        if (cj == k / 16 && ti + 16 * ai < mpn) colk[bf][ti + 16 * ai] = A[ai][cj];
-            __syncthreads();
-            if (tid < 64) {
+            const int kc = k / 16;
+            if (tj == k % 16) {
                 double best = -1;
                 int bi = NC;
-                for (int i = tid; i < mpn; i += 64) {
-                    if (pos[i] < k) continue;   // already a pivot row
-                    const double v = fabs(colk[bf][i]);
-                    if (v > best) { best = v; bi = i; }
+#pragma unroll
+                for (int ai = 0; ai < TILE; ai++) {
+                    double v = A[ai][0];
+#pragma unroll
+                    for (int cj = 1; cj < TILE; cj++) v = (cj == kc) ? A[ai][cj] : v;
+                    const int i = ti + 16 * ai;
+                    if (i < mpn) {
+                        colk[bf][i] = v;
+                        const double av = pos[i] < k ? -1.0 : fabs(v);   // pivot rows are out
+                        if (av > best) { best = av; bi = i; }
+                    }
                 }
 #pragma unroll
-                for (int o = 32; o > 0; o >>= 1) {
+                for (int o = 8; o > 0; o >>= 1) {   // within the 16-lane group
                     const double ob = __shfl_xor(best, o);
                     const int oi = __shfl_xor(bi, o);
                     if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
                 }
-                if (tid == 0) {
+                if (ti == 0) {
                     if (!(best > 0)) s_bad = 1;
                     sp[bf] = bi;
-                    porder[k] = bi;
                     pos[bi] = k;
                 }
             }
             __syncthreads();
             if (s_bad) break;
-            const int p = sp[bf];
-            if (ti == p % 16)
+            const int p = sp[bf], pr = p / 16;
+            if (ti == p % 16) {
 #pragma unroll
-                for (int ai = 0; ai < TILE; ai++)
-                    if (ai == p / 16)
+                for (int cj = 0; cj < TILE; cj++) {
+                    double v = A[0][cj];
 #pragma unroll
-                        for (int cj = 0; cj < TILE; cj++) rowp[bf][tj + 16 * cj] = A[ai][cj];
+                    for (int ai = 1; ai < TILE; ai++) v = (ai == pr) ? A[ai][cj] : v;
+                    rowp[bf][tj + 16 * cj] = v;
+                }
+            }
+            if (tid == 0) diag[k] = colk[bf][p];
             __syncthreads();
             const double rinv = 1.0 / colk[bf][p];
 #pragma unroll
             for (int ai = 0; ai < TILE; ai++) {
                 const int i = ti + 16 * ai;
-                if (i >= mpn || pos[i] <= k) continue;   // used as pivot (at this step or before)
-                const double l = colk[bf][i] * rinv;
+                const double l = (i < mpn && i != p) ? colk[bf][i] * rinv : 0.0;
 #pragma unroll
-                for (int cj = 0; cj < TILE; cj++) {
-                    const int j = tj + 16 * cj;
-                    if (j > k && j < ncols) A[ai][cj] -= l * rowp[bf][j];
-                }
+                for (int cj = 0; cj < TILE; cj++) A[ai][cj] -= l * rowp[bf][tj + 16 * cj];
             }
         }
         if (s_bad) {
             if (tid == 0) atomicOr(a.status, 1);
             return;
         }
-        // ---- back substitution in reverse pivot order ----
-        for (int k = mpn - 1; k >= 0; k--) {
-            const int bf = k & 1;
-            const int pk = porder[k];
-            if (tj == k % 16)
+        // ---- solution: unknown k is the right-hand-side entries of its pivot
+        // row over the pivot; only the first m unknowns (the stencil weights)
+        // are written.  Every row is done by its own lanes, in parallel.
 #pragma unroll
-                for (int ai = 0; ai < TILE; ai++)
-#pragma unroll
-                    for (int cj = 0; cj < TILE; cj++)
-                        if (cj == k / 16 && ti + 16 * ai < mpn) colk[bf][ti + 16 * ai] = A[ai][cj];
-            __syncthreads();
-            if (ti == pk % 16)
-#pragma unroll
-                for (int ai = 0; ai < TILE; ai++)
-                    if (ai == pk / 16)
-#pragma unroll
-                        for (int cj = 0; cj < TILE; cj++) {
-                            const int j = tj + 16 * cj;
-                            if (j >= mpn && j < ncols) {
-                                const double x = A[ai][cj] / colk[bf][pk];
-                                xk[bf][j - mpn] = x;
-                                if (k < m) outp[j - mpn][k] = x;
-                            }
-                        }
-            __syncthreads();
-#pragma unroll
-            for (int ai = 0; ai < TILE; ai++) {
-                const int i = ti + 16 * ai;
-                if (i >= mpn || pos[i] >= k) continue;   // rows still to be solved
-                const double u = colk[bf][i];
+        for (int ai = 0; ai < TILE; ai++) {
+            const int i = ti + 16 * ai;
+            const int k = i < mpn ? pos[i] : NC;
+            if (k < m) {
+                const double d = diag[k];
 #pragma unroll
                 for (int cj = 0; cj < TILE; cj++) {
                     const int j = tj + 16 * cj;
-                    if (j >= mpn && j < ncols) A[ai][cj] -= u * xk[bf][j - mpn];
+                    if (j >= mpn && j < ncols) outp[j - mpn][k] = A[ai][cj] / d;
                 }
             }
         }
