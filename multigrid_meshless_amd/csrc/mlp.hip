@@ -215,7 +215,9 @@ features_kernel(int n, Ell A, int dim, double h, const double *__restrict__ dx, 
     t[b] = sigma > 0.f ? (float)(err_snaps[(size_t)snap[b] * Nd + r] * diag / (double)sigma) : 0.f;
 }
 
-// Per-row constant part of layer 1: G[r][k] = b1[k] + sum_{j,d} W1[k][j*dim+d] off_jd.
+// Per-row constant part of layer 1: b1[k] + sum_{j,d} W1[k][j*dim+d] off_jd,
+// stored hidden-unit-major at G[k*Nd + r] (the correction reads four
+// consecutive rows of one hidden unit per lane).
 __global__ void __launch_bounds__(BLK)
 geom_kernel(int Nd, Ell A, int dim, double h, const double *__restrict__ dx, const double *__restrict__ dy,
             const double *__restrict__ dz, const uint8_t *__restrict__ btag, const float *__restrict__ W,
@@ -239,116 +241,159 @@ geom_kernel(int Nd, Ell A, int dim, double h, const double *__restrict__ dx, con
         }
     }
 #pragma unroll
-    for (int k = 0; k < MLP_H; k++) G[(size_t)r * MLP_H + k] = acc[k];
+    for (int k = 0; k < MLP_H; k++) G[(size_t)k * Nd + r] = acc[k];
 }
 
 // ---- inference ----------------------------------------------------------------
 
 // p[r] += y * sigma / a_rr on interior rows, y = network(offsets, scaled
-// stencil residuals) with the geometric part of layer 1 taken from G.
-// BF16 weights W1rT[j][k] (residual columns of W1, transposed), W2[k][q],
-// w3[k] are widened into LDS once per block (rows padded to 65 floats so the
-// threads of one node read different banks); FP32 biases and accumulation.
-// T threads share a node: T = 1 for large levels (the node's work in one
-// thread's registers), T = 16 for small ones (more threads in flight): the
-// node's stencil residuals go to LDS, each thread computes 64/T hidden units
-// of each layer, and the output is summed across the T lanes.
-constexpr int MLP_LD = MLP_H + 1;
-template <typename TV, int T>
-__global__ void __launch_bounds__(BLK)
+// stencil residuals) with the geometric part of layer 1 taken from G.  Both
+// layers run on BF16 matrix cores (v_mfma_f32_16x16x32_bf16): BF16 weights
+// and activations, FP32 accumulation.  A wave takes tiles of 16 nodes:
+//   layer 1  C[16 x 64] = G tile + U[16 x K] W1r[K x 64], K = m residual
+//            columns padded to 32-multiples, U = BF16(r_cj / sigma);
+//   layer 2  C2 = b2 + BF16(relu(C)) W2^T (C moved through LDS into the
+//            operand layout); y = b3 + w3 . relu(C2), summed across lanes.
+// Operand layouts (lane l): A[i][k] at i = l%16, k = (l/16)*8 + e; B[k][n] at
+// k = (l/16)*8 + e, n = l%16; C[i][n] at n = l%16, i = (l/16)*4 + v.  The
+// weight fragments are laid out in LDS once per block in exactly the B
+// layout, so each is one 16-byte read.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int MF_KS1 = (MLP_MAX_M + 31) / 32;   // K steps of layer 1
+constexpr int MF_KS2 = MLP_H / 32;              // K steps of layer 2
+constexpr int MF_NT = MLP_H / 16;               // 16-wide column tiles of a hidden layer
+constexpr int MF_WAVES = BLK / 64;
+constexpr int MF_TPW = 4;                       // node tiles per wave
+
+__device__ inline __bf16 bits_bf(bf16 v) { return __builtin_bit_cast(__bf16, v.bits); }
+
+// LDS written by some lanes of a wave is visible to the other lanes of that
+// wave afterwards (each wave uses its own LDS rows, so no workgroup barrier).
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Three waves per SIMD: the 24 residual gathers a lane keeps in flight need
+// the registers (a cap of four spills).
+template <typename TV>
+__global__ void __launch_bounds__(BLK, 3)
 correct_kernel(int Nd, Ell A, const uint8_t *__restrict__ btag, const float *__restrict__ G,
                const bf16 *__restrict__ W1rT, const bf16 *__restrict__ W2, const float *__restrict__ b2,
                const bf16 *__restrict__ w3, float b3, const TV *__restrict__ res, TV *__restrict__ p)
 {
-    constexpr int NPB = BLK / T;            // nodes per block
-    constexpr int KPT = MLP_H / T;          // hidden units per thread
-    __shared__ float sW1[MLP_MAX_M * MLP_LD], sW2[MLP_H * MLP_LD], sw3[MLP_H], sb2[MLP_H];
-    __shared__ double sR[T > 1 ? NPB : 1][MLP_MAX_M + 1];
-    __shared__ float sH[T > 1 ? NPB : 1][MLP_LD];
+    __shared__ bf16x8 f1[MF_KS1][MF_NT][64];
+    __shared__ bf16x8 f2[MF_KS2][MF_NT][64];
+    __shared__ float sb2[MLP_H], sw3[MLP_H];
+    __shared__ __attribute__((aligned(16))) __bf16 hb[MF_WAVES][16][MLP_H];
+    __shared__ float yb[MF_WAVES][16];
     const int m = A.m;
-    for (int i = threadIdx.x; i < m * MLP_H; i += BLK) sW1[(i / MLP_H) * MLP_LD + i % MLP_H] = bf(W1rT[i]);
-    for (int i = threadIdx.x; i < MLP_H * MLP_H; i += BLK) sW2[(i / MLP_H) * MLP_LD + i % MLP_H] = bf(W2[i]);
+    for (int q = threadIdx.x; q < MF_KS1 * MF_NT * 64; q += BLK) {
+        const int l = q % 64, t = (q / 64) % MF_NT, ks = q / (64 * MF_NT);
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int k = ks * 32 + (l / 16) * 8 + e, n = t * 16 + l % 16;   // residual column k, hidden unit n
+            v[e] = k < m ? bits_bf(W1rT[k * MLP_H + n]) : (__bf16)0.f;
+        }
+        f1[ks][t][l] = v;
+    }
+    for (int q = threadIdx.x; q < MF_KS2 * MF_NT * 64; q += BLK) {
+        const int l = q % 64, t = (q / 64) % MF_NT, ks = q / (64 * MF_NT);
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int k = ks * 32 + (l / 16) * 8 + e, n = t * 16 + l % 16;   // layer-2 input k, output n
+            v[e] = bits_bf(W2[n * MLP_H + k]);
+        }
+        f2[ks][t][l] = v;
+    }
     if (threadIdx.x < MLP_H) {
-        sw3[threadIdx.x] = bf(w3[threadIdx.x]);
         sb2[threadIdx.x] = b2[threadIdx.x];
+        sw3[threadIdx.x] = bf(w3[threadIdx.x]);
     }
     __syncthreads();
-    const int nl = threadIdx.x / T, sub = threadIdx.x % T;
-    const int r = blockIdx.x * NPB + nl;
-    const bool valid = r < Nd && btag[r] == ROW_INTERIOR;
-    if constexpr (T == 1) {
-        if (!valid) return;
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, li = l % 16, g = l / 16;
+    const long ntiles = ((long)Nd + 15) / 16;
+    for (int it = 0; it < MF_TPW; it++) {
+        const long tile = ((long)blockIdx.x * MF_WAVES + wave) * MF_TPW + it;
+        if (tile >= ntiles) break;
+        const long nb = tile * 16, r = nb + li;
+        const bool valid = r < Nd && btag[r] == ROW_INTERIOR;
+        // this lane's residual columns of node li: j = ks*32 + g*8 + e
+        double vq[MF_KS1 * 8];
         double s2 = 0;
-        for (int j = 0; j < m; j++) {
-            const double v = res[A.col[ell_at(m, A.S, r, j)]];
-            s2 += v * v;
-        }
-        const double sigma = sqrt(s2 / m);
-        if (!(sigma > 0)) return;
-        const double inv = 1.0 / sigma;
-        float h1[MLP_H];
 #pragma unroll
-        for (int k = 0; k < MLP_H; k++) h1[k] = G[(size_t)r * MLP_H + k];
-        for (int j = 0; j < m; j++) {
-            const float u = (float)(res[A.col[ell_at(m, A.S, r, j)]] * inv);
+        for (int ks = 0; ks < MF_KS1; ks++)
 #pragma unroll
-            for (int k = 0; k < MLP_H; k++) h1[k] += sW1[j * MLP_LD + k] * u;
-        }
-#pragma unroll
-        for (int k = 0; k < MLP_H; k++) h1[k] = relu(h1[k]);
-        float y = b3;
-        for (int k = 0; k < MLP_H; k++) {
-            float a = sb2[k];
-#pragma unroll
-            for (int q = 0; q < MLP_H; q++) a += sW2[k * MLP_LD + q] * h1[q];
-            y += sw3[k] * relu(a);
-        }
-        const double diag = A.w[ell_at(m, A.S, r, 0)];
-        p[r] = p[r] + (TV)((double)y * sigma / diag);
-    } else {
-        // stencil residuals of the node, split over its T lanes
-        double s2 = 0;
-        if (valid)
-            for (int j = sub; j < m; j += T) {
-                const double v = res[A.col[ell_at(m, A.S, r, j)]];
-                sR[nl][j] = v;
+            for (int e = 0; e < 8; e++) {
+                const int j = ks * 32 + g * 8 + e;
+                double v = 0;
+                if (valid && j < m) v = (double)res[A.col[ell_at(m, A.S, r, j)]];
+                vq[ks * 8 + e] = v;
                 s2 += v * v;
             }
-#pragma unroll
-        for (int o = T / 2; o > 0; o /= 2) s2 += __shfl_xor(s2, o, T);
+        s2 += __shfl_xor(s2, 16);
+        s2 += __shfl_xor(s2, 32);
         const double sigma = sqrt(s2 / m);
-        const bool go = valid && sigma > 0;
-        const double inv = go ? 1.0 / sigma : 0.0;
-        __syncthreads();
-        if (go) {
-            float h[KPT];
+        const double inv = sigma > 0 ? 1.0 / sigma : 0.0;
+        f32x4 c[MF_NT];
 #pragma unroll
-            for (int i = 0; i < KPT; i++) h[i] = G[(size_t)r * MLP_H + sub * KPT + i];
-            for (int j = 0; j < m; j++) {
-                const float u = (float)(sR[nl][j] * inv);
+        for (int t = 0; t < MF_NT; t++)
 #pragma unroll
-                for (int i = 0; i < KPT; i++) h[i] += sW1[j * MLP_LD + sub * KPT + i] * u;
+            for (int v = 0; v < 4; v++) {
+                const long node = nb + g * 4 + v;
+                c[t][v] = node < Nd ? G[(size_t)(t * 16 + li) * Nd + node] : 0.f;
             }
 #pragma unroll
-            for (int i = 0; i < KPT; i++) sH[nl][sub * KPT + i] = relu(h[i]);
-        }
-        __syncthreads();
-        float y = 0.f;
-        if (go) {
+        for (int ks = 0; ks < MF_KS1; ks++) {
+            bf16x8 a;
 #pragma unroll
-            for (int i = 0; i < KPT; i++) {
-                const int k = sub * KPT + i;
-                float a = sb2[k];
-                for (int q = 0; q < MLP_H; q++) a += sW2[k * MLP_LD + q] * sH[nl][q];
-                y += sw3[k] * relu(a);
-            }
+            for (int e = 0; e < 8; e++) a[e] = (__bf16)(float)(vq[ks * 8 + e] * inv);
+#pragma unroll
+            for (int t = 0; t < MF_NT; t++)
+                c[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, f1[ks][t][l], c[t], 0, 0, 0);
         }
 #pragma unroll
-        for (int o = T / 2; o > 0; o /= 2) y += __shfl_xor(y, o, T);
-        if (go && sub == 0) {
+        for (int t = 0; t < MF_NT; t++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) hb[wave][g * 4 + v][t * 16 + li] = (__bf16)relu(c[t][v]);
+        wave_sync();
+        f32x4 c2[MF_NT];
+#pragma unroll
+        for (int t = 0; t < MF_NT; t++) {
+            const float bb = sb2[t * 16 + li];
+            c2[t] = (f32x4){bb, bb, bb, bb};
+        }
+#pragma unroll
+        for (int ks = 0; ks < MF_KS2; ks++) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&hb[wave][li][ks * 32 + g * 8]);
+#pragma unroll
+            for (int t = 0; t < MF_NT; t++)
+                c2[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, f2[ks][t][l], c2[t], 0, 0, 0);
+        }
+        float sv[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            float acc = 0.f;
+#pragma unroll
+            for (int t = 0; t < MF_NT; t++) acc += sw3[t * 16 + li] * relu(c2[t][v]);
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) acc += __shfl_xor(acc, o);   // the 16 lanes of group g
+            sv[v] = acc;
+        }
+        if (li == 0)
+#pragma unroll
+            for (int v = 0; v < 4; v++) yb[wave][g * 4 + v] = sv[v];
+        wave_sync();
+        if (g == 0 && valid && sigma > 0) {
             const double diag = A.w[ell_at(m, A.S, r, 0)];
-            p[r] = p[r] + (TV)((double)(y + b3) * sigma / diag);
+            p[r] = p[r] + (TV)((double)(yb[wave][li] + b3) * sigma / diag);
         }
+        wave_sync();   // hb and yb are rewritten by the next tile
     }
 }
 
@@ -436,12 +481,12 @@ int mlp_correct(int Nd, Ell A, const uint8_t *btag, const float *G, const bf16 *
                 const float *b2, const bf16 *w3, float b3, const TV *res, TV *p, hipStream_t s)
 {
     if (A.m > MLP_MAX_M) MGM_FAIL(-1, "learned smoother: stencil wider than supported");
-    const char *e = getenv("MGM_MLP_WIDE_ROWS");
-    const int wide = e ? atoi(e) : 200000;   // levels with more rows: one thread per node
-    if (Nd > wide) {
-        LAUNCH((correct_kernel<TV, 1>), Nd, Nd, A, btag, G, W1rT, W2, b2, w3, b3, res, p);
-    } else {
-        LAUNCH((correct_kernel<TV, 16>), (long)Nd * 16, Nd, A, btag, G, W1rT, W2, b2, w3, b3, res, p);
+    const long tiles = ((long)Nd + 15) / 16, waves = (tiles + MF_TPW - 1) / MF_TPW;
+    const int blocks = (int)((waves + MF_WAVES - 1) / MF_WAVES);
+    if (blocks > 0) {
+        hipLaunchKernelGGL(correct_kernel<TV>, dim3(blocks), dim3(BLK), 0, s, Nd, A, btag, G, W1rT, W2, b2, w3, b3,
+                           res, p);
+        HIP_TRY(hipGetLastError());
     }
     return 0;
 }

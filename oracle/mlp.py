@@ -89,22 +89,31 @@ def node_features(x, y, z, dim, cloud, h, res, i):
     return np.concatenate([off, r / sigma if sigma > 0 else 0 * r]), sigma
 
 
-def correction(x, y, z, dim, cloud, h, Lp, res, w, i, bf16_weights=True):
+def correction(x, y, z, dim, cloud, h, Lp, res, w, i, bf16_weights=True, bf16_activations=True):
     """The learned correction of interior node i: f(features) * sigma / a_ii,
-    with the weights the inference kernel uses (BF16-rounded W2, w3 and residual
-    columns of W1 when bf16_weights)."""
+    as the inference kernel evaluates it: BF16-rounded W2, w3 and residual
+    columns of W1 (bf16_weights) and BF16-rounded matrix-core inputs -- the
+    scaled residuals and the first hidden layer -- (bf16_activations), with
+    sums in float64 here (FP32 on the GPU)."""
     feats, sigma = node_features(x, y, z, dim, cloud, h, res, i)
     if sigma == 0:
         return 0.0
     w = np.asarray(w, dtype=np.float32).copy()
     n_in = feats.size
+    m = cloud.shape[1]
     if bf16_weights:
-        m = cloud.shape[1]
         W1 = w[:H * n_in].reshape(H, n_in)
         W1[:, m * dim:] = bf16_round(W1[:, m * dim:])
         o = H * n_in + H
         w[o:o + H * H] = bf16_round(w[o:o + H * H])
         o += H * H + H
         w[o:o + H] = bf16_round(w[o:o + H])
-    out, _ = forward(feats[None, :], w.astype(np.float64))
-    return float(out[0]) * sigma / Lp[i, 0]
+    if not bf16_activations:
+        out, _ = forward(feats[None, :], w.astype(np.float64))
+        return float(out[0]) * sigma / Lp[i, 0]
+    W1, b1, W2, b2, w3, b3 = unpack(w.astype(np.float64), n_in)
+    u = bf16_round(feats[m * dim:].astype(np.float32)).astype(np.float64)
+    z1 = W1[:, :m * dim] @ feats[:m * dim] + b1 + W1[:, m * dim:] @ u
+    a1 = bf16_round(np.maximum(z1, 0).astype(np.float32)).astype(np.float64)
+    a2 = np.maximum(W2 @ a1 + b2, 0)
+    return float(a2 @ w3 + b3) * sigma / Lp[i, 0]

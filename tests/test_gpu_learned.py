@@ -4,8 +4,9 @@ No reference counterpart (the reference smooths with SOR only).  Bars:
   * forward outputs, loss and loss gradient of the FP32 training kernels equal
     the float64 oracle to 1e-5 / 1e-4 relative;
   * one learned sweep changes p on every interior node by the oracle's
-    correction (BF16 inference weights, FP32 accumulation) to 1e-3 of the
-    largest correction, and leaves boundary nodes alone;
+    correction (BF16 matrix cores: BF16 weights and activations, FP32
+    accumulation) to 1e-3 of the largest correction, and leaves boundary
+    nodes alone;
   * training on (error, residual) pairs of the plain V-cycle lowers the loss;
     the learned solve reaches the tolerance, and what it gains over plain SOR
     (cycles, seconds, or a fallback) is printed.
@@ -66,12 +67,9 @@ def test_forward_and_gradient_match_oracle(gpu):
     assert np.abs(g - g_ref).max() <= 1e-4 * np.abs(g_ref).max()
 
 
-@pytest.mark.parametrize("wide_rows", ["0", "200000"], ids=["thread_per_node", "16_threads_per_node"])
 @pytest.mark.parametrize("dim,sizes", [(2, (40, 20, 10)), (3, (16, 8))])
-def test_learned_sweep_matches_oracle(gpu, dim, sizes, wide_rows, monkeypatch):
-    """MGM_MLP_WIDE_ROWS picks the kernel form: levels above it use one thread per node."""
-    monkeypatch.setenv("MGM_MLP_WIDE_ROWS", wide_rows)
-    print(f"[gpu learned sweep] dim {dim} sizes {sizes} wide_rows {wide_rows}", flush=True)
+def test_learned_sweep_matches_oracle(gpu, dim, sizes):
+    print(f"[gpu learned sweep] dim {dim} sizes {sizes}", flush=True)
     P = problem(gpu, dim, sizes)
     n_in = P.mlp_inputs()
     w = he_weights(n_in, 3)
