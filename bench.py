@@ -33,8 +33,9 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stencil_apply_3d_1m_traffic.md")
 
 
 def measured_traffic():
-    """HBM bytes per level-0 stencil apply from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated on the streaming copy), or None."""
+    """HBM bytes per level-0 stencil apply of the 100^3 workload from the
+    committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on
+    the streaming copy), or None."""
     try:
         for line in open(TRAFFIC_FILE):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -363,7 +364,7 @@ def main():
                    "cavity_time_march": cavity},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic()},
+                     "traffic": measured_traffic() if (args.n, args.levels) == (100, 5) else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
