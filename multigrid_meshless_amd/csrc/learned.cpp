@@ -210,13 +210,15 @@ int mgm_mlp_gradient(mgm_problem *pb, int n, const float *X, const float *t, dou
     if (!S.have) MGM_FAIL(-1, "learned smoother: no weights");
     if (n <= 0 || !X || !t) MGM_FAIL(-1, "mlp_gradient: bad arguments");
     const size_t nw = mlp_weight_count(S.in);
-    DevBuf<float> dX, dt, dH1, dH2, dy, ddy, dZ1, dZ2, dG;
+    DevBuf<float> dX, dt, dH1, dH2, dy, ddy, dZ1, dZ2, dG, dpart;
     HIP_TRY(dX.upload(X, (size_t)n * S.in, stream()));
     HIP_TRY(dt.upload(t, n, stream()));
     for (DevBuf<float> *b : {&dH1, &dH2, &dZ1, &dZ2}) HIP_TRY(b->alloc((size_t)n * MLP_H));
     HIP_TRY(dy.alloc(n)); HIP_TRY(ddy.alloc(n)); HIP_TRY(dG.alloc(nw));
+    HIP_TRY(dpart.alloc(mlp_gradient_workspace(S.in)));
     int rc = mlp_forward(n, S.in, dX.p, S.W.p, dH1.p, dH2.p, dy.p, stream());
-    if (!rc) rc = mlp_gradient(n, S.in, dX.p, dt.p, S.W.p, dH1.p, dH2.p, dy.p, ddy.p, dZ2.p, dZ1.p, dG.p, stream());
+    if (!rc) rc = mlp_gradient(n, S.in, dX.p, dt.p, S.W.p, dH1.p, dH2.p, dy.p, ddy.p, dZ2.p, dZ1.p, dpart.p, dG.p,
+                               stream());
     if (!rc) rc = mlp_sqerr(n, dy.p, dt.p, pb->partial.p, pb->scal.p, stream());
     if (rc) return rc;
     double s = 0;
@@ -319,8 +321,9 @@ int mgm_train_smoother(mgm_problem *pb, const mgm_train_config *cfg, double *los
         for (int i = 0; i < MLP_H; i++) w3[i] = (float)(normal(seed) * std::sqrt(1.0 / MLP_H));
     }
     if ((rc = set_weights(pb, S, w.data()))) return rc;
-    DevBuf<float> M, V, G, H1, H2, y, dy, dZ1, dZ2, Xb, tb;
+    DevBuf<float> M, V, G, H1, H2, y, dy, dZ1, dZ2, Xb, tb, part;
     HIP_TRY(M.alloc(nw)); HIP_TRY(V.alloc(nw)); HIP_TRY(G.alloc(nw));
+    HIP_TRY(part.alloc(mlp_gradient_workspace(in)));
     HIP_TRY(hipMemsetAsync(M.p, 0, sizeof(float) * nw, stream()));
     HIP_TRY(hipMemsetAsync(V.p, 0, sizeof(float) * nw, stream()));
     const int B = std::min(cfg->batch, n);
@@ -352,7 +355,8 @@ int mgm_train_smoother(mgm_problem *pb, const mgm_train_config *cfg, double *los
             const float lr_t = cfg->lr * (float)(std::sqrt(1.0 - std::pow(b2, step)) / (1.0 - std::pow(b1, step)));
             rc = mlp_gather_batch(B, in, didx.p + b0, X.p, t.p, Xb.p, tb.p, stream());
             if (!rc) rc = mlp_forward(B, in, Xb.p, S.W.p, H1.p, H2.p, y.p, stream());
-            if (!rc) rc = mlp_gradient(B, in, Xb.p, tb.p, S.W.p, H1.p, H2.p, y.p, dy.p, dZ2.p, dZ1.p, G.p, stream());
+            if (!rc) rc = mlp_gradient(B, in, Xb.p, tb.p, S.W.p, H1.p, H2.p, y.p, dy.p, dZ2.p, dZ1.p, part.p, G.p,
+                                       stream());
             if (!rc) rc = mlp_adam((int)nw, S.W.p, G.p, M.p, V.p, lr_t, b1, b2, eps, stream());
             if (rc) return rc;
         }

@@ -25,32 +25,66 @@ __device__ inline float bf(bf16 v) { return __uint_as_float((unsigned)v.bits << 
 
 // ---- training ---------------------------------------------------------------
 
-// One sample per thread: H1, H2 (stored for the backward pass) and y.
+// Forward pass storing H1, H2 (for the backward pass) and y.  A block takes
+// FWD_BS samples, their inputs staged in LDS transposed (xs[f][sample]); the
+// thread for hidden unit k and sample group g computes k for four samples,
+// streaming row k of W1 / W2 through L1.  Each unit's sum runs in input order
+// from its bias.
+constexpr int FWD_BS = 16;
+constexpr int FWD_MAX_IN = MLP_MAX_M * 3 + MLP_MAX_M;
 __global__ void __launch_bounds__(BLK)
 fwd_kernel(int n, int in, const float *__restrict__ X, const float *__restrict__ W, float *__restrict__ H1,
            float *__restrict__ H2, float *__restrict__ y)
 {
-    const int b = blockIdx.x * BLK + threadIdx.x;
-    if (b >= n) return;
+    __shared__ __attribute__((aligned(16))) float xs[FWD_MAX_IN][FWD_BS];
+    __shared__ __attribute__((aligned(16))) float hs[FWD_BS][MLP_H];
     const float *W1 = W, *b1 = W1 + MLP_H * in, *W2 = b1 + MLP_H, *b2 = W2 + MLP_H * MLP_H;
     const float *w3 = b2 + MLP_H, *b3 = w3 + MLP_H;
-    const float *x = X + (size_t)b * in;
-    float h1[MLP_H];
-    for (int k = 0; k < MLP_H; k++) {
-        float a = b1[k];
-        for (int f = 0; f < in; f++) a += W1[k * in + f] * x[f];
-        h1[k] = relu(a);
-        H1[(size_t)b * MLP_H + k] = h1[k];
+    const long b0 = (long)blockIdx.x * FWD_BS;
+    const int nb = (int)((n - b0 < FWD_BS) ? n - b0 : FWD_BS);
+    for (int q = threadIdx.x; q < FWD_BS * in; q += BLK) {
+        const int sm = q / in, f = q % in;
+        xs[f][sm] = sm < nb ? X[(b0 + sm) * in + f] : 0.f;
     }
-    float out = b3[0];
-    for (int k = 0; k < MLP_H; k++) {
-        float a = b2[k];
-        for (int q = 0; q < MLP_H; q++) a += W2[k * MLP_H + q] * h1[q];
-        const float h = relu(a);
-        H2[(size_t)b * MLP_H + k] = h;
-        out += w3[k] * h;
+    __syncthreads();
+    const int k = threadIdx.x & 63, g = threadIdx.x >> 6;   // wave g: samples 4g .. 4g+3
+    float a[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) a[v] = b1[k];
+    const float *w1 = W1 + (size_t)k * in;
+    for (int f = 0; f < in; f++) {
+        const float w = w1[f];
+        const float4 xv = *reinterpret_cast<const float4 *>(&xs[f][4 * g]);
+        a[0] += w * xv.x;
+        a[1] += w * xv.y;
+        a[2] += w * xv.z;
+        a[3] += w * xv.w;
     }
-    y[b] = out;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int sm = 4 * g + v;
+        const float h = relu(a[v]);
+        hs[sm][k] = h;
+        if (sm < nb) H1[(b0 + sm) * MLP_H + k] = h;
+        a[v] = b2[k];
+    }
+    __syncthreads();
+    const float *w2 = W2 + (size_t)k * MLP_H;
+    for (int q = 0; q < MLP_H; q++) {
+        const float w = w2[q];
+#pragma unroll
+        for (int v = 0; v < 4; v++) a[v] += w * hs[4 * g + v][q];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int sm = 4 * g + v;
+        const float h = relu(a[v]);
+        if (sm < nb) H2[(b0 + sm) * MLP_H + k] = h;
+        float o = w3[k] * h;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) o += __shfl_xor(o, off);
+        if (k == 0 && sm < nb) y[b0 + sm] = b3[0] + o;
+    }
 }
 
 // Per-sample deltas of the loss L = mean_b (y_b - t_b)^2:
@@ -77,38 +111,56 @@ delta_kernel(int n, const float *__restrict__ W, int in, const float *__restrict
     }
 }
 
-// Weight gradients, one weight per thread, summed over the samples:
+// Weight gradients summed over the samples, in WG_SPLIT chunks of samples
+// (partial sums part[c][g], one weight g per thread) and then over the chunks
+// in order:
 //   dW1[k][f] = sum_b dZ1[b][k] X[b][f]   db1[k] = sum_b dZ1[b][k]
 //   dW2[k][q] = sum_b dZ2[b][k] H1[b][q]  db2[k] = sum_b dZ2[b][k]
 //   dw3[k]    = sum_b dy[b] H2[b][k]      db3    = sum_b dy[b]
+// Consecutive threads take consecutive inputs f (or q) of one unit k, so the
+// sample rows are read contiguously and the delta is one broadcast value.
+constexpr int WG_SPLIT = 32;
 __global__ void __launch_bounds__(BLK)
-wgrad_kernel(int n, int in, const float *__restrict__ X, const float *__restrict__ H1,
-             const float *__restrict__ H2, const float *__restrict__ dy, const float *__restrict__ dZ2,
-             const float *__restrict__ dZ1, float *__restrict__ G)
+wgrad_part_kernel(int n, int in, const float *__restrict__ X, const float *__restrict__ H1,
+                  const float *__restrict__ H2, const float *__restrict__ dy, const float *__restrict__ dZ2,
+                  const float *__restrict__ dZ1, float *__restrict__ part)
 {
     const int nw1 = MLP_H * in, o_b1 = nw1, o_w2 = o_b1 + MLP_H, o_b2 = o_w2 + MLP_H * MLP_H;
     const int o_w3 = o_b2 + MLP_H, o_b3 = o_w3 + MLP_H, total = o_b3 + 1;
     const int g = blockIdx.x * BLK + threadIdx.x;
+    const int c = blockIdx.y;
     if (g >= total) return;
+    const int chunk = (n + WG_SPLIT - 1) / WG_SPLIT;
+    const int bs = c * chunk, be = (bs + chunk < n) ? bs + chunk : n;
     float a = 0.f;
     if (g < o_b1) {
         const int k = g / in, f = g % in;
-        for (int b = 0; b < n; b++) a += dZ1[(size_t)b * MLP_H + k] * X[(size_t)b * in + f];
+        for (int b = bs; b < be; b++) a += dZ1[(size_t)b * MLP_H + k] * X[(size_t)b * in + f];
     } else if (g < o_w2) {
         const int k = g - o_b1;
-        for (int b = 0; b < n; b++) a += dZ1[(size_t)b * MLP_H + k];
+        for (int b = bs; b < be; b++) a += dZ1[(size_t)b * MLP_H + k];
     } else if (g < o_b2) {
         const int k = (g - o_w2) / MLP_H, q = (g - o_w2) % MLP_H;
-        for (int b = 0; b < n; b++) a += dZ2[(size_t)b * MLP_H + k] * H1[(size_t)b * MLP_H + q];
+        for (int b = bs; b < be; b++) a += dZ2[(size_t)b * MLP_H + k] * H1[(size_t)b * MLP_H + q];
     } else if (g < o_w3) {
         const int k = g - o_b2;
-        for (int b = 0; b < n; b++) a += dZ2[(size_t)b * MLP_H + k];
+        for (int b = bs; b < be; b++) a += dZ2[(size_t)b * MLP_H + k];
     } else if (g < o_b3) {
         const int k = g - o_w3;
-        for (int b = 0; b < n; b++) a += dy[b] * H2[(size_t)b * MLP_H + k];
+        for (int b = bs; b < be; b++) a += dy[b] * H2[(size_t)b * MLP_H + k];
     } else {
-        for (int b = 0; b < n; b++) a += dy[b];
+        for (int b = bs; b < be; b++) a += dy[b];
     }
+    part[(size_t)c * total + g] = a;
+}
+
+__global__ void __launch_bounds__(BLK)
+wgrad_sum_kernel(int total, const float *__restrict__ part, float *__restrict__ G)
+{
+    const int g = blockIdx.x * BLK + threadIdx.x;
+    if (g >= total) return;
+    float a = 0.f;
+    for (int c = 0; c < WG_SPLIT; c++) a += part[(size_t)c * total + g];
     G[g] = a;
 }
 
@@ -411,18 +463,28 @@ size_t mlp_weight_count(int in) { return (size_t)MLP_H * in + MLP_H + MLP_H * ML
 
 int mlp_forward(int n, int in, const float *X, const float *W, float *H1, float *H2, float *y, hipStream_t s)
 {
-    LAUNCH(fwd_kernel, n, n, in, X, W, H1, H2, y);
+    if (in > FWD_MAX_IN) MGM_FAIL(-1, "mlp_forward: too many inputs");
+    const int blocks = (n + FWD_BS - 1) / FWD_BS;
+    if (blocks > 0) {
+        hipLaunchKernelGGL(fwd_kernel, dim3(blocks), dim3(BLK), 0, s, n, in, X, W, H1, H2, y);
+        HIP_TRY(hipGetLastError());
+    }
     return 0;
 }
-
+size_t mlp_gradient_workspace(int in) { return (size_t)WG_SPLIT * mlp_weight_count(in); }
 int mlp_gradient(int n, int in, const float *X, const float *t, const float *W, const float *H1, const float *H2,
-                 const float *y, float *dy, float *dZ2, float *dZ1, float *G, hipStream_t s)
+                 const float *y, float *dy, float *dZ2, float *dZ1, float *part, float *G, hipStream_t s)
 {
     LAUNCH(delta_kernel, n, n, W, in, H1, H2, y, t, dy, dZ2, dZ1);
-    LAUNCH(wgrad_kernel, (long)mlp_weight_count(in), n, in, X, H1, H2, dy, dZ2, dZ1, G);
+    const int total = (int)mlp_weight_count(in);
+    if (n > 0) {
+        hipLaunchKernelGGL(wgrad_part_kernel, dim3(nblk(total), WG_SPLIT), dim3(BLK), 0, s, n, in, X, H1, H2, dy,
+                           dZ2, dZ1, part);
+        HIP_TRY(hipGetLastError());
+    }
+    LAUNCH(wgrad_sum_kernel, total, total, part, G);
     return 0;
 }
-
 int mlp_sqerr(int n, const float *y, const float *t, double *partial, double *out, hipStream_t s)
 {
     int nb = nblk(n);

@@ -14,8 +14,10 @@ size_t mlp_weight_count(int in);
 // Training (FP32): forward storing hidden activations; loss gradient for
 // L = mean (y - t)^2 into G (layout of the weight vector); sum of squared errors.
 int mlp_forward(int n, int in, const float *X, const float *W, float *H1, float *H2, float *y, hipStream_t s);
+// part: workspace of mlp_gradient_workspace(in) floats (partial sums over sample chunks).
+size_t mlp_gradient_workspace(int in);
 int mlp_gradient(int n, int in, const float *X, const float *t, const float *W, const float *H1, const float *H2,
-                 const float *y, float *dy, float *dZ2, float *dZ1, float *G, hipStream_t s);
+                 const float *y, float *dy, float *dZ2, float *dZ1, float *part, float *G, hipStream_t s);
 int mlp_sqerr(int n, const float *y, const float *t, double *partial, double *out, hipStream_t s);
 int mlp_adam(int nw, float *W, const float *G, float *M, float *V, float lr_t, float b1, float b2, float eps,
              hipStream_t s);

@@ -86,7 +86,8 @@ def node_features(x, y, z, dim, cloud, h, res, i):
     off = np.stack([(a[c] - a[i]) / h for a in crd], 1).ravel()
     r = res[c]
     sigma = np.sqrt(np.mean(r * r))
-    return np.concatenate([off, r / sigma if sigma > 0 else 0 * r]), sigma
+    # scaled by the reciprocal, as the kernels do (rounding points of BF16 inputs agree)
+    return np.concatenate([off, r * (1.0 / sigma) if sigma > 0 else 0 * r]), sigma
 
 
 def correction(x, y, z, dim, cloud, h, Lp, res, w, i, bf16_weights=True, bf16_activations=True):
