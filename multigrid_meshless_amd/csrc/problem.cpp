@@ -405,7 +405,12 @@ static void level_colour_order(Level &L, int dim)
     std::vector<int> byk(N);
     std::iota(byk.begin(), byk.end(), 0);
     std::sort(byk.begin(), byk.end(), [&](int a, int b) { return key[a] != key[b] ? key[a] < key[b] : a < b; });
-    const int kBlock = N > kBlockedRows ? 4096 : N;
+    static const int block_rows = [] {
+        const char *e = getenv("MGM_BLOCK_ROWS");
+        const int v = e ? atoi(e) : 4096;
+        return v > 0 ? v : 4096;
+    }();
+    const int kBlock = N > kBlockedRows ? block_rows : N;
     std::vector<int> block(N);
     for (int r = 0; r < N; r++) block[byk[r]] = r / kBlock;
     std::vector<int> ord(byk);
