@@ -321,7 +321,8 @@ def main():
     flop_sys = 2.0 / 3.0 * mpn ** 3 + 2.0 * mpn ** 2 * nrhs
     weights = {"systems": n_sys, "kernel_s": round(t_w, 4), "systems_per_s": round(n_sys / t_w),
                "mflop_per_system": round(flop_sys / 1e6, 4), "gflops": round(n_sys * flop_sys / t_w / 1e9, 1),
-               "frac_fp64_peak": round(n_sys * flop_sys / t_w / FP64_PEAK_FLOPS, 4)}
+               "frac_fp64_peak": round(n_sys * flop_sys / t_w / FP64_PEAK_FLOPS, 4),
+               "autotuned_kernel": dict(zip(("tile", "waves_per_simd"), P.weight_kernel_choice(0) or (None, None)))}
     print(f"[bench] weights level 0: {n_sys} systems in {t_w:.3f}s = {n_sys / t_w:.0f} systems/s, "
           f"{weights['gflops']} GFLOP/s", file=sys.stderr, flush=True)
 

@@ -138,6 +138,14 @@ class Problem:
         check(lib().mgm_rebuild_weights(self._h, lev, C.byref(s), C.byref(n)), "rebuild_weights")
         return s.value, n.value
 
+    def weight_kernel_choice(self, lev: int = 0):
+        """(tile, waves_per_simd) the autotuner chose for this level's system size, or None."""
+        info = self.info(lev)
+        t, w = C.c_int(), C.c_int()
+        if lib().mgm_weight_kernel_choice(info["m"] + info["n"], C.byref(t), C.byref(w)) != 0:
+            return None
+        return t.value, w.value
+
     def geometry(self, lev: int) -> dict:
         N = self.info(lev)["N"]
         out = {k: np.zeros(N) for k in ("x", "y", "z", "nx", "ny", "nz")}

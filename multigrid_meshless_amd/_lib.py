@@ -62,6 +62,7 @@ SIGNATURES = [
     ("mgm_level_info_get", _i, [_vp, _i, C.POINTER(LevelInfo)]),
     ("mgm_setup_timings", _i, [_vp, _vp]),
     ("mgm_rebuild_weights", _i, [_vp, _i, C.POINTER(_d), C.POINTER(C.c_long)]),
+    ("mgm_weight_kernel_choice", _i, [_i, C.POINTER(_i), C.POINTER(_i)]),
     ("mgm_get_geometry", _i, [_vp, _i] + [_vp] * 7),
     ("mgm_get_cloud", _i, [_vp, _i, _vp]),
     ("mgm_get_points", _i, [_vp, _i, _i, _vp]),

@@ -115,6 +115,9 @@ struct WeightsArgs {
 // (weights_tile(mpn) > 0) and need no workspace; wider ones use ws/rhs/piv.
 int weights_tile(int mpn);
 int launch_weights(const WeightsArgs &a, hipStream_t s);
+// Tile grid and register budget the autotuner chose for systems of size mpn
+// (-1 before the first build of that size).
+int weights_kernel_choice(int mpn, int *tile, int *waves_per_simd);
 
 // ---- stencil kernels (stencil.hip) ----------------------------------------
 // Fixed-width rows in sliced-ELL layout: rows are grouped in slices of S

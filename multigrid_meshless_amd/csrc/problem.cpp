@@ -903,6 +903,12 @@ int mgm_rebuild_weights(mgm_problem *pb, int lev, double *seconds, long *systems
     return 0;
 }
 
+int mgm_weight_kernel_choice(int mpn, int *tile, int *waves_per_simd)
+{
+    if (!tile || !waves_per_simd) MGM_FAIL(-1, "null argument");
+    return weights_kernel_choice(mpn, tile, waves_per_simd);
+}
+
 int mgm_get_geometry(const mgm_problem *pb, int lev, double *x, double *y, double *z, double *nx,
                      double *ny, double *nz, uint8_t *btag)
 {

@@ -17,6 +17,12 @@
 // wave touch 64 consecutive doubles on every access.
 #pragma clang fp contract(off)
 
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <utility>
+#include <vector>
+
 #include "device.hpp"
 
 namespace mgm {
@@ -299,13 +305,13 @@ __device__ __attribute__((noinline)) double lu_entry(PolyTerms a, int dim, int p
     return ipow(tgt[r][0] - sx, a.px[k]) * ipow(tgt[r][1] - sy, a.py[k]) * ipow(tgt[r][2] - sz, a.pz[k]);
 }
 
-// Waves per SIMD the register budget aims at: four systems per CU while the
-// tile grid fits 128 registers (measured 20% faster than the unconstrained
-// three at TILE 6; five spills inside the elimination loop).
-#define LU_WAVES_PER_SIMD(T) ((T) <= 6 ? 4 : 1)
-
-template <int TILE>
-__global__ void __launch_bounds__(LU_T, LU_WAVES_PER_SIMD(TILE))
+// WPS: waves per SIMD the register budget aims at (1: no constraint).  Four
+// systems per CU fit while the tile grid fits 128 registers (measured 20%
+// faster than the unconstrained three at TILE 6; five spills inside the
+// elimination loop).  The launcher picks TILE and WPS per system size by
+// timing (lu_choose).
+template <int TILE, int WPS>
+__global__ void __launch_bounds__(LU_T, WPS)
 lu_kernel(WeightsArgs a, int base)
 {
     constexpr int NC = 16 * TILE;
@@ -467,25 +473,106 @@ int weights_tile(int mpn)
     return 0;
 }
 
+using LuFn = void (*)(WeightsArgs, int);
+
+template <int W>
+static LuFn lu_fn_w(int tile)
+{
+    switch (tile) {
+    case 2: return lu_kernel<2, W>;
+    case 3: return lu_kernel<3, W>;
+    case 4: return lu_kernel<4, W>;
+    case 5: return lu_kernel<5, W>;
+    case 6: return lu_kernel<6, W>;
+    case 7: return lu_kernel<7, W>;
+    default: return lu_kernel<8, W>;
+    }
+}
+static LuFn lu_fn(int tile, int wps) { return wps == 4 ? lu_fn_w<4>(tile) : lu_fn_w<1>(tile); }
+
+static int lu_run(LuFn f, const WeightsArgs &a, int base, int count, hipStream_t s)
+{
+    // one workgroup per system; launches of at most 2^20 systems
+    for (int b = base; b < base + count; b += 1 << 20) {
+        const int c = (base + count - b < (1 << 20)) ? base + count - b : (1 << 20);
+        hipLaunchKernelGGL(f, dim3(c), dim3(LU_T), 0, s, a, b);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+// Autotuner of the weight kernel: per system size mpn, the tile grid (the
+// smallest that holds [A | 4 RHS] and the next) and the register budget (four
+// waves per SIMD or unconstrained) are timed on the first 8192 systems of the
+// first build -- every variant computes identical results, so those systems
+// are simply recomputed by the winner -- and the fastest is kept for the
+// process.  MGM_LU_TUNE=0 takes the smallest grid and the four-wave budget.
+static std::map<int, std::pair<int, int>> g_lu_choice;
+
+static int lu_choose(const WeightsArgs &a, hipStream_t s, int *tile, int *wps)
+{
+    const int mpn = a.m + a.n, t0 = weights_tile(mpn);
+    // MGM_LU_VARIANT=tile,wps forces a variant (tests compare them), uncached
+    if (const char *v = getenv("MGM_LU_VARIANT")) {
+        int t = 0, w = 0;
+        if (sscanf(v, "%d,%d", &t, &w) == 2 && t >= t0 && t <= 8 && (w == 1 || (w == 4 && t <= 6))) {
+            *tile = t;
+            *wps = w;
+            return 0;
+        }
+        MGM_FAIL(-1, "MGM_LU_VARIANT: expected tile,wps with tile >= the smallest grid and wps 1 or 4");
+    }
+    auto it = g_lu_choice.find(mpn);
+    if (it != g_lu_choice.end()) { *tile = it->second.first; *wps = it->second.second; return 0; }
+    std::pair<int, int> best{t0, t0 <= 6 ? 4 : 1};
+    const char *e = getenv("MGM_LU_TUNE");
+    const int sample = a.nsys < 8192 ? a.nsys : 8192;
+    if (!(e && e[0] == '0') && sample >= 1024) {
+        std::vector<std::pair<int, int>> cand;
+        for (int t = t0; t <= t0 + 1 && t <= 8; t++) {
+            if (t <= 6) cand.push_back({t, 4});
+            cand.push_back({t, 1});
+        }
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        float best_ms = 1e30f;
+        for (auto c : cand) {
+            LuFn f = lu_fn(c.first, c.second);
+            int rc = lu_run(f, a, 0, sample, s);   // first launch loads the code object
+            if (!rc) HIP_TRY(hipEventRecord(e0, s));
+            if (!rc) rc = lu_run(f, a, 0, sample, s);
+            if (!rc) HIP_TRY(hipEventRecord(e1, s));
+            if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) { best_ms = ms; best = c; }
+        }
+        HIP_TRY(hipEventDestroy(e0));
+        HIP_TRY(hipEventDestroy(e1));
+    }
+    g_lu_choice[mpn] = best;
+    *tile = best.first;
+    *wps = best.second;
+    return 0;
+}
+
+int weights_kernel_choice(int mpn, int *tile, int *wps)
+{
+    auto it = g_lu_choice.find(mpn);
+    if (it == g_lu_choice.end()) return -1;
+    *tile = it->second.first;
+    *wps = it->second.second;
+    return 0;
+}
+
 int launch_weights(const WeightsArgs &a, hipStream_t s)
 {
-    const int tile = weights_tile(a.m + a.n);
-    if (tile) {
-        // one workgroup per system; launches of at most 2^20 systems
-        for (int base = 0; base < a.nsys; base += 1 << 20) {
-            const int count = (a.nsys - base < (1 << 20)) ? a.nsys - base : (1 << 20);
-            switch (tile) {
-            case 2: hipLaunchKernelGGL(lu_kernel<2>, dim3(count), dim3(LU_T), 0, s, a, base); break;
-            case 3: hipLaunchKernelGGL(lu_kernel<3>, dim3(count), dim3(LU_T), 0, s, a, base); break;
-            case 4: hipLaunchKernelGGL(lu_kernel<4>, dim3(count), dim3(LU_T), 0, s, a, base); break;
-            case 5: hipLaunchKernelGGL(lu_kernel<5>, dim3(count), dim3(LU_T), 0, s, a, base); break;
-            case 6: hipLaunchKernelGGL(lu_kernel<6>, dim3(count), dim3(LU_T), 0, s, a, base); break;
-            case 7: hipLaunchKernelGGL(lu_kernel<7>, dim3(count), dim3(LU_T), 0, s, a, base); break;
-            default: hipLaunchKernelGGL(lu_kernel<8>, dim3(count), dim3(LU_T), 0, s, a, base); break;
-            }
-            HIP_TRY(hipGetLastError());
-        }
-        return 0;
+    if (weights_tile(a.m + a.n)) {
+        int tile = 0, wps = 0, rc;
+        if ((rc = lu_choose(a, s, &tile, &wps))) return rc;
+        return lu_run(lu_fn(tile, wps), a, 0, a.nsys, s);
     }
     if (!a.ws) MGM_FAIL(-1, "weights: workspace missing for wide systems");
     for (int base = 0; base < a.nsys; base += a.batch) {

@@ -271,3 +271,25 @@ def test_time_steps_match_reference(name, step, march, gpu):
         # it as a tree, the reference left to right, so it may differ in the last bit
         ref_err = float(g[f"st{s}_err"])
         assert abs(errs[s] - ref_err) <= 1e-14 * abs(ref_err), (s, errs[s], ref_err)
+
+
+def test_weight_kernel_variants_agree(gpu, monkeypatch):
+    """Every variant the weight-kernel autotuner may pick (tile grid, register
+    budget) produces the same weights bit for bit: the pivots and each entry's
+    update sequence do not depend on which lane holds the entry."""
+    pts = [gpu.Points.generate(2, n, 5 + k, 0.2) for k, n in enumerate((48, 24))]
+    cfg = gpu.config(dimension=2, poly_degree=4, coarse_poly_degree=3, phs_degree=5,
+                     cloud_size_multiplier=2.0, transfer_rows=gpu.ROWS_BY_TAG)
+    P = gpu.Problem(cfg, pts).setup()
+    info = P.info(0)
+    mpn = info["m"] + info["n"]
+    t0 = next(t for t in range(2, 9) if 16 * t >= mpn + 4)
+    print(f"[gpu weight variants] 2D 48^2, mpn {mpn}, autotuned choice {P.weight_kernel_choice(0)}", flush=True)
+    ref = {k: P.matrix(0, k) for k in ("Dx", "Dy", "lap")}
+    for tile, wps in ((t0, 4), (t0, 1), (t0 + 1, 4), (t0 + 1, 1)):
+        if wps == 4 and tile > 6:
+            continue
+        monkeypatch.setenv("MGM_LU_VARIANT", f"{tile},{wps}")
+        P.rebuild_weights(0)
+        for k, a in ref.items():
+            np.testing.assert_array_equal(P.matrix(0, k), a, err_msg=f"{k} tile {tile} wps {wps}")
