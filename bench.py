@@ -9,8 +9,10 @@ of multicolour SOR (omega 1, 3 sweeps) until l2_norm_gen(r)/l2_norm_gen(f) <= 1e
 
 Prints one JSON line (rank 0, one GPU).  `roofline` is the FP64 stencil apply
 (lap_Poison over level 0) timed with HIP events on the library stream;
-`cpu_baseline` is the oracle's sequential V-cycle in the same colour order on
-the same operators, timed for a bounded number of cycles on the host.
+`cpu_baseline` is the oracle's V-cycle in the same colour order on the same
+operators, the rows of each colour class swept in parallel on every available
+host thread (the 1-thread time is in its sample), timed for a bounded number of
+cycles.
 """
 from __future__ import annotations
 
@@ -143,31 +145,40 @@ def cavity_march(mgm, args):
     return out
 
 
-def cpu_baseline(P, cycles_needed, sample_cycles, f):
-    """Oracle V-cycles (sequential sweep in the GPU's colour order) on the same operators."""
+def cpu_baseline(P, sample_cycles, f):
+    """Seconds per V-cycle (+ fine residual norm) of the oracle's colour-ordered
+    sweep on the same operators, at 1 thread and at every available thread (the
+    rows of a colour class in parallel; same results): {threads: s/cycle}."""
     import oracle
     L = P.L
     lv = []
     for k in range(L):
         col = P.colours(k)
+        order = np.lexsort((np.arange(len(col)), col)).astype(np.int32)
+        classes = np.searchsorted(col[order], np.arange(col.max() + 2)).astype(np.int32)
         d = dict(cloud=P.cloud(k), Lp=P.matrix(k, "lap_Poison"), btag=P.geometry(k)["btag"],
-                 rp=P.points(k, "restriction"), pp=P.points(k, "prolongation"),
-                 order=np.lexsort((np.arange(len(col)), col)).astype(np.int32))
+                 rp=P.points(k, "restriction"), pp=P.points(k, "prolongation"), order=order, classes=classes)
         if k > 0:
             d["R"] = P.matrix(k, "restr")
         if k < L - 1:
             d["P"] = P.matrix(k, "prol")
         lv.append(d)
     H = oracle.Hierarchy(lv)
-    H.lv[0]["s"][:] = f
     fnorm = oracle.l2_norm_gen(lv[0]["btag"], f)
-    t0 = time.perf_counter()
-    for _ in range(sample_cycles):
-        H.vcycle(oracle.VAR_H, 1.0, 3)
-        H.residual(oracle.VAR_H, 0)
-        oracle.l2_norm_gen(lv[0]["btag"], H.lv[0]["res"]) / fnorm
-    dt = (time.perf_counter() - t0) / sample_cycles
-    return dt * cycles_needed, dt
+    out = {}
+    for nt in sorted({1, oracle.num_threads()}):
+        oracle.set_num_threads(nt)
+        for d in H.lv:
+            d["p"][:] = 0
+            d["s"][:] = 0
+        H.lv[0]["s"][:] = f
+        t0 = time.perf_counter()
+        for _ in range(sample_cycles):
+            H.vcycle(oracle.VAR_H, 1.0, 3)
+            H.residual(oracle.VAR_H, 0)
+            oracle.l2_norm_gen(lv[0]["btag"], H.lv[0]["res"]) / fnorm
+        out[nt] = (time.perf_counter() - t0) / sample_cycles
+    return out
 
 
 def main():
@@ -330,12 +341,13 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        t_cpu, t_cyc = cpu_baseline(P, cycles[0], args.cpu_cycles, f)
-        import oracle
-        cpu = {"value": round(t_cpu, 3), "unit": "s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_cycles} V-cycles (+ fine residual norm) of the oracle's sequential "
-                         f"colour-ordered sweep on the same 1M-point operators, {t_cyc:.3f} s/cycle, "
-                         f"x {cycles[0]} cycles the GPU needed; threads={oracle.num_threads()} available, 1 used"}
+        per = cpu_baseline(P, args.cpu_cycles, f)
+        nt = max(per)
+        cpu = {"value": round(per[nt] * cycles[0], 3), "unit": "s", "cores": nt, "kind": "port",
+               "sample": f"{args.cpu_cycles} V-cycles (+ fine residual norm) of the oracle's colour-ordered sweep "
+                         f"on the same 1M-point operators (rows of a colour class in parallel), "
+                         f"{per[nt]:.3f} s/cycle on {nt} threads, x {cycles[0]} cycles the GPU needed; "
+                         f"1 thread: {per[1]:.3f} s/cycle = {per[1] * cycles[0]:.1f} s"}
 
     out = {
         "metric": "time-to-1e-8 residual, 3D 1M-pt MG Poisson (s); stencil-apply HBM GB/s",

@@ -50,7 +50,8 @@ class Level(C.Structure):
                 ("cloud", C.c_void_p), ("Lp", C.c_void_p), ("lap", C.c_void_p),
                 ("btag", C.c_void_p), ("order", C.c_void_p),
                 ("p", C.c_void_p), ("s", C.c_void_p), ("res", C.c_void_p),
-                ("rp", C.c_void_p), ("R", C.c_void_p), ("pp", C.c_void_p), ("P", C.c_void_p)]
+                ("rp", C.c_void_p), ("R", C.c_void_p), ("pp", C.c_void_p), ("P", C.c_void_p),
+                ("cls", C.c_void_p), ("ncls", C.c_int)]
 
 
 def lib():
@@ -256,12 +257,18 @@ def num_threads() -> int:
     return lib().orc_num_threads()
 
 
+def set_num_threads(n: int) -> None:
+    lib().orc_set_num_threads(n)
+
+
 class Hierarchy:
     """Holds numpy arrays for each level and exposes the V-cycle primitives.
 
     levels: list of dicts with keys cloud (N,m) int32, Lp (N,m), lap (N,m) optional,
-    btag (N,) uint8, and optionally order (N,) int32 sweep order, rp/R (coarse
-    side), pp/P (fine side).  p/s/res are created zero if absent.
+    btag (N,) uint8, and optionally order (N,) int32 sweep order, classes
+    (ncls+1,) int32 offsets of the colour classes in order (their rows are then
+    swept in parallel), rp/R (coarse side), pp/P (fine side).  p/s/res are
+    created zero if absent.
     """
 
     def __init__(self, levels):
@@ -296,6 +303,8 @@ class Hierarchy:
             L.R = ptr("R", np.float64)
             L.pp = ptr("pp", np.int32)
             L.P = ptr("P", np.float64)
+            L.cls = ptr("classes", np.int32)
+            L.ncls = len(d["classes"]) - 1 if d.get("classes") is not None else 0
         self.arr = arr
 
     def relax(self, var, k, omega, num_relax):

@@ -737,25 +737,43 @@ int orc_prolongation(int Nf, int nbf, const double *xf, const double *yf, const 
 /* V-cycle primitives.  `order` gives the sweep order; consecutive entries of
  * one colour class are independent, so rows of one class may be processed in
  * parallel with identical results. */
-static void sweep(orc_level *L, float omega, int sweep_all)
+static inline void sor_row(orc_level *L, int i, double w, double c)
 {
-    const int N = L->N, m = L->m;
+    const int m = L->m;
+    const double *Wr = L->Lp + (size_t)i * m;
+    const int *cr = L->cloud + (size_t)i * m;
+    double sum = 0.0;
+    for (int j = 1; j < m; j++) sum += Wr[j] * L->p[cr[j]];
+    L->p[i] = w * ((L->s[i] - sum) / Wr[0]) + c * L->p[i];
+}
+
+/* One sweep in `order`; with colour classes given, the rows of a class (no two
+ * of them neighbours) are updated in parallel -- the same values as in order. */
+static void sweep(orc_level *L, float omega)
+{
+    const int N = L->N;
     const double w = omega, c = (float)(1 - omega);      /* omega is float; (1-omega) is float */
-    (void)sweep_all;
-    for (int q = 0; q < N; q++) {
-        int i = L->order ? L->order[q] : q;
-        const double *Wr = L->Lp + (size_t)i * m;
-        const int *cr = L->cloud + (size_t)i * m;
-        double sum = 0.0;
-        for (int j = 1; j < m; j++) sum += Wr[j] * L->p[cr[j]];
-        L->p[i] = w * ((L->s[i] - sum) / Wr[0]) + c * L->p[i];
+    if (L->order && L->cls) {
+        for (int k = 0; k < L->ncls; k++) {
+            const int q0 = L->cls[k], q1 = L->cls[k + 1];
+#pragma omp parallel for schedule(static)
+            for (int q = q0; q < q1; q++) sor_row(L, L->order[q], w, c);
+        }
+        return;
     }
+    for (int q = 0; q < N; q++) sor_row(L, L->order ? L->order[q] : q, w, c);
 }
 
 static void pin(orc_level *L)
 {
     double pref = L->p[0];
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < L->N; i++) L->p[i] = L->p[i] - pref;
+}
+
+void orc_set_num_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
 }
 
 /* relaxation: H hf/heat_conduction.h:25-56; T hf/timple_solver.h:235-272;
@@ -764,10 +782,11 @@ void orc_relax(int variant, orc_level *L, float omega, int num_relax)
 {
     const int N = L->N, m = L->m;
     for (int it = 0; it < num_relax; it++) {
-        sweep(L, omega, 1);
+        sweep(L, omega);
         if (variant == ORC_VAR_H) continue;
         pin(L);
         const double *W = (variant == ORC_VAR_T) ? L->Lp : L->lap;
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < N; i++) {
             if (variant == ORC_VAR_T && L->btag[i]) { L->res[i] = 0; continue; }
             double r = L->s[i];
@@ -783,6 +802,7 @@ void orc_residual(int variant, orc_level *L)
 {
     const int N = L->N, m = L->m;
     const double *W = (variant == ORC_VAR_F) ? L->lap : L->Lp;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < N; i++) {
         if (variant != ORC_VAR_F) L->res[i] = 0.0;
         if (!L->btag[i]) {
@@ -798,6 +818,7 @@ void orc_residual(int variant, orc_level *L)
 void orc_restrict(int variant, orc_level *f, orc_level *c)
 {
     const int mf = f->m;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < c->N; i++) {
         double r = 0.0;
         if (!c->btag[i]) {
@@ -815,6 +836,7 @@ void orc_restrict(int variant, orc_level *f, orc_level *c)
 void orc_prolong(int variant, orc_level *f, orc_level *c)
 {
     const int mc = c->m;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < f->N; i++) {
         if (f->btag[i]) continue;
         int ip = f->pp[i];

@@ -105,7 +105,12 @@ typedef struct {
     const double *R;         /* restr_mat N*m_fine */
     const int *pp;           /* prolongation_points (this level as fine) */
     const double *P;         /* prol_mat N*m_coarse */
+    const int *cls;          /* optional: colour class c is order[cls[c] .. cls[c+1]-1] */
+    int ncls;
 } orc_level;
+
+/* Threads the OpenMP loops use (results do not depend on it). */
+void orc_set_num_threads(int n);
 
 enum { ORC_VAR_H = 0, ORC_VAR_T = 1, ORC_VAR_F = 2 };
 

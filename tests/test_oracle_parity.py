@@ -227,3 +227,29 @@ def test_splitmix64_known_answer(oracle):
     """SplitMix64 with seed 0: published first outputs of the generator."""
     v = oracle.splitmix64(0, 3)
     assert [int(t) for t in v] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_colour_class_parallel_sweep_is_the_sequential_sweep():
+    """The oracle sweeps the rows of one colour class in parallel when given the
+    class offsets (the CPU baseline's all-core timing): same bits as the
+    sequential sweep in the same order, since no two rows of a class are
+    neighbours."""
+    import oracle
+    rng = np.random.default_rng(11)
+    N, m = 600, 12
+    cloud = np.stack([np.concatenate([[i], rng.choice(np.delete(np.arange(N), i), m - 1, replace=False)])
+                      for i in range(N)]).astype(np.int32)
+    Lp = rng.standard_normal((N, m))
+    Lp[:, 0] = 20 + rng.random(N)
+    _, col = oracle.greedy_colour(cloud)
+    order = np.lexsort((np.arange(N), col)).astype(np.int32)
+    classes = np.searchsorted(col[order], np.arange(col.max() + 2)).astype(np.int32)
+    s = rng.standard_normal(N)
+    out = []
+    for cls in (None, classes):
+        lv = [dict(cloud=cloud, Lp=Lp, btag=np.zeros(N, np.uint8), order=order, s=s.copy(), classes=cls)]
+        H = oracle.Hierarchy(lv)
+        oracle.set_num_threads(4)
+        H.relax(oracle.VAR_H, 0, 1.0, 3)
+        out.append(H.lv[0]["p"].copy())
+    np.testing.assert_array_equal(out[0], out[1])
