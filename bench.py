@@ -366,6 +366,7 @@ def main():
                                f"{len(sizes)} levels {sizes}, m={m}, FP64 multicolour SOR V-cycle, "
                                f"tol {args.tol}",
                    "cycles": cycles[0], "converged": converged,
+                   "vcycles_per_s": round(cycles[0] / t_solve, 1),
                    "final_ratio": finals[0], "setup_s": round(t_setup, 2),
                    "max_rel_err_vs_converged": plain_err,
                    "stencil_apply_GBps": round(achieved, 1),
