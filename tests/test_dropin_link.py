@@ -4,8 +4,9 @@ Runs where the reference tree exists (its drivers are not shipped with this
 repository): each driver is copied into a scratch directory next to a
 `header_files` link to this repository's drop-in headers and the reference's
 own init/ plug-ins, then compiled with gcc and linked against libmgm.so.
-Running the binaries needs a GPU (covered by the project's own drivers in
-tests/dropin/ on the GPU side).
+The reference tree is absent where the GPU is, so the drop-in surface is run
+on the GPU through tests/dropin/heat_driver.c, a driver of this repository
+that follows mg_heat_conduction.c's call sequence (tests/test_gpu_dropin.py).
 """
 import os
 import shutil
