@@ -136,7 +136,20 @@ struct EllT {
     const TW *w;
     int m, S;
     bool stream = false;   // slab larger than the last-level cache: load it non-temporally
+    // Gather-sorted form: inside each slice the S*m entries are stored in
+    // ascending column order, so the 64 lanes of one gather read nearby x
+    // values.  col then packs (slot << kSlotShift) | column, slot = the
+    // entry's offset j*S + ri in the plain slice, and w holds the weights in
+    // the same order; w0 is the plain slab (entry 0 of row ri at the plain
+    // offset, the SOR diagonal).  Products land in the same LDS slots as in
+    // the plain form, so the row sums are unchanged.
+    const TW *w0 = nullptr;
+    bool sorted = false;
 };
+constexpr int kSlotShift = 22;                 // columns below 2^22, slots below 2^10
+constexpr unsigned kColMask = (1u << kSlotShift) - 1;
+// A slab of N rows, width m and slice height S can be stored gather-sorted.
+inline bool slab_sortable(int S, int m, int ncols) { return S * m <= 1024 && ncols < (int)kColMask; }
 // Slabs above this size cannot stay in the 256 MB last-level cache between passes.
 constexpr size_t kStreamSlabBytes = 256ull << 20;
 inline bool stream_slab(size_t rows, int m, size_t wbytes) { return rows * m * (wbytes + 4) > kStreamSlabBytes; }
@@ -205,6 +218,11 @@ int launch_add_correction(int N, const uint8_t *btag, const float *e, double *p,
 // reference -> device row); columns mapped through cloud and iperm.
 int launch_gather_rows(int N, int m, int S, const double *rows, const int *iperm,
                        const int *cloud, double *w_out, int *col_out, hipStream_t s);
+// Gather-sorted columns of a slab (nslices slices of S*m entries): per slice,
+// the packed (slot, column) pairs in ascending (column, slot) order.
+int launch_gather_sort(int nslices, int m, int S, const int *col, int *gcol, hipStream_t s);
+// wg[e] = w[slice base of e + slot of gcol[e]]: weights in gather-sorted order.
+int launch_permute_slab(long n, int m, int S, const int *gcol, const double *w, double *wg, hipStream_t s);
 // Transfer rows of the target level (Nt x m, reference order) -> slab over the
 // target's device order; column j = iperm_src[cloud_src[pts[i]*m + j]].
 int launch_transfer_slab(int Nt, int m, int S, const double *rows, const int *iperm_t,

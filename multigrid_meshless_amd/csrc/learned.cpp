@@ -101,7 +101,7 @@ int prepare(mgm_problem *pb, LearnedState &S)
         Level &L = pb->lv[l];
         if (L.m != m0 || L.m > MLP_MAX_M) continue;   // network input width differs: SOR only
         HIP_TRY(S.G[l].alloc((size_t)L.Nd * MLP_H));
-        if ((rc = mlp_geometry(L.Nd, L.op(OP_LAPP), dim, L.d_avg, S.dx[l].p, S.dy[l].p, S.dz[l].p, L.dbtag.p,
+        if ((rc = mlp_geometry(L.Nd, L.op_plain(OP_LAPP), dim, L.d_avg, S.dx[l].p, S.dy[l].p, S.dz[l].p, L.dbtag.p,
                                S.W.p, S.G[l].p, stream())))
             return rc;
         S.lev_ok[l] = 1;
@@ -136,7 +136,7 @@ int op_learned_sweep(mgm_problem *pb, int l)
     int rc = launch_residual(L.op(OP_LAPP), L.Nd, 0, L.dbtag.p, (const double *)L.f[MGM_F_SOURCE].p,
                              (const double *)L.f[MGM_F_P].p, L.f[MGM_F_RES].p, stream());
     if (rc) return rc;
-    return mlp_correct(L.Nd, L.op(OP_LAPP), L.dbtag.p, S.G[l].p, S.W1r.p, S.W2.p, S.b2.p, S.w3.p, S.b3,
+    return mlp_correct(L.Nd, L.op_plain(OP_LAPP), L.dbtag.p, S.G[l].p, S.W1r.p, S.W2.p, S.b2.p, S.w3.p, S.b3,
                        L.f[MGM_F_RES].p, L.f[MGM_F_P].p, stream());
 }
 
@@ -148,7 +148,7 @@ int op_learned_sweep_mp(mgm_problem *pb, int l, int (*residual_mp)(Level &))
     Level &L = pb->lv[l];
     int rc = residual_mp(L);
     if (rc) return rc;
-    return mlp_correct(L.Nd, L.op(OP_LAPP), L.dbtag.p, S.G[l].p, S.W1r.p, S.W2.p, S.b2.p, S.w3.p, S.b3,
+    return mlp_correct(L.Nd, L.op_plain(OP_LAPP), L.dbtag.p, S.G[l].p, S.W1r.p, S.W2.p, S.b2.p, S.w3.p, S.b3,
                        (const float *)L.res32.p, L.p32.p, stream());
 }
 
@@ -301,7 +301,7 @@ int mgm_train_smoother(mgm_problem *pb, const mgm_train_config *cfg, double *los
     HIP_TRY(dsnap.upload(snaps, stream()));
     HIP_TRY(X.alloc((size_t)n * in));
     HIP_TRY(t.alloc(n));
-    if ((rc = mlp_features(n, L0.op(OP_LAPP), dim, L0.d_avg, S.dx[0].p, S.dy[0].p, S.dz[0].p, drows.p, dsnap.p,
+    if ((rc = mlp_features(n, L0.op_plain(OP_LAPP), dim, L0.d_avg, S.dx[0].p, S.dy[0].p, S.dz[0].p, drows.p, dsnap.p,
                            Nd, res_s.p, err_s.p, X.p, t.p, stream())))
         return rc;
     std::vector<float> th(n);

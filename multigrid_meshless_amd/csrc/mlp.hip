@@ -526,6 +526,7 @@ int mlp_features(int n, Ell A, int dim, double h, const double *dx, const double
                  float *X, float *t, hipStream_t s)
 {
     if (A.m > MLP_MAX_M) MGM_FAIL(-1, "learned smoother: stencil wider than supported");
+    if (A.sorted) MGM_FAIL(-1, "learned smoother: needs the plain slab");
     LAUNCH(features_kernel, n, n, A, dim, h, dx, dy, dz, rows, snap, Nd, res_snaps, err_snaps, X, t);
     return 0;
 }
@@ -534,6 +535,7 @@ int mlp_geometry(int Nd, Ell A, int dim, double h, const double *dx, const doubl
                  const uint8_t *btag, const float *W, float *G, hipStream_t s)
 {
     if (A.m > MLP_MAX_M) MGM_FAIL(-1, "learned smoother: stencil wider than supported");
+    if (A.sorted) MGM_FAIL(-1, "learned smoother: needs the plain slab");
     LAUNCH(geom_kernel, Nd, Nd, A, dim, h, dx, dy, dz, btag, W, G);
     return 0;
 }
@@ -543,6 +545,7 @@ int mlp_correct(int Nd, Ell A, const uint8_t *btag, const float *G, const bf16 *
                 const float *b2, const bf16 *w3, float b3, const TV *res, TV *p, hipStream_t s)
 {
     if (A.m > MLP_MAX_M) MGM_FAIL(-1, "learned smoother: stencil wider than supported");
+    if (A.sorted) MGM_FAIL(-1, "learned smoother: needs the plain slab");
     const long tiles = ((long)Nd + 15) / 16, waves = (tiles + MF_TPW - 1) / MF_TPW;
     const int blocks = (int)((waves + MF_WAVES - 1) / MF_WAVES);
     if (blocks > 0) {

@@ -589,6 +589,21 @@ static int level_lap_poisson(mgm_problem *pb, int l)
                               L.rny.p, L.rnz.p, L.rbtag.p, L.rows[OP_LAPP].p, g_stream);
 }
 
+// Streaming passes over slabs of more than MGM_GATHER_SORT_ROWS rows
+// (default kBlockedRows) read them gather-sorted (EllT::sorted): such a pass
+// streams its slab from HBM, and the sorted order cuts the cache lines one
+// gather instruction touches.  MGM_GATHER_SORT=0 keeps every slab plain.
+// Results are identical either way.  Both are read whenever a level's slabs
+// are built.
+static bool gather_sort_on(int rows, int S, int m, int ncols)
+{
+    const char *e = getenv("MGM_GATHER_SORT");
+    if (e && atoi(e) == 0) return false;
+    const char *t = getenv("MGM_GATHER_SORT_ROWS");
+    const long min_rows = t ? atol(t) : kBlockedRows;
+    return rows > min_rows && slab_sortable(S, m, ncols);
+}
+
 static int level_slab(mgm_problem *pb, int l, int which)
 {
     Level &L = pb->lv[l];
@@ -600,8 +615,11 @@ static int level_slab(mgm_problem *pb, int l, int which)
         HIP_TRY(L.slab[which].alloc(sz));
         HIP_TRY(hipMemsetAsync(L.slab[which].p, 0, sizeof(double) * sz, g_stream));
     }
-    return launch_gather_rows(L.N, L.m, L.S, L.rows[which].p, L.d_iperm.p, L.rcloud.p,
-                              L.slab[which].p, nullptr, g_stream);
+    int rc = launch_gather_rows(L.N, L.m, L.S, L.rows[which].p, L.d_iperm.p, L.rcloud.p,
+                                L.slab[which].p, nullptr, g_stream);
+    if (rc || !L.gs || (which != OP_LAP && which != OP_LAPP)) return rc;
+    if (L.gslab[which].n != sz) HIP_TRY(L.gslab[which].alloc(sz));
+    return launch_permute_slab((long)sz, L.m, L.S, L.gcol.p, L.slab[which].p, L.gslab[which].p, g_stream);
 }
 
 // (Re)build every slab of level l: stencil columns, the five operators and
@@ -620,6 +638,12 @@ static int level_slabs(mgm_problem *pb, int l)
     int rc = launch_gather_rows(L.N, L.m, L.S, L.rows[OP_LAP].p, L.d_iperm.p, L.rcloud.p,
                                 nullptr, L.col.p, g_stream);
     if (rc) return rc;
+    L.gs = gather_sort_on(L.Nd, L.S, L.m, L.Nd);
+    if (L.gs) {
+        if (L.gcol.n != sz) HIP_TRY(L.gcol.alloc(sz));
+        if ((rc = launch_gather_sort((int)(sz / ((size_t)L.S * L.m)), L.m, L.S, L.col.p, L.gcol.p, g_stream)))
+            return rc;
+    }
     for (int o = 0; o < NUM_OPS; o++)
         if ((rc = level_slab(pb, l, o))) return rc;
     if (l > 0) {
@@ -649,6 +673,14 @@ static int level_slabs(mgm_problem *pb, int l)
         rc = launch_transfer_slab(L.N, L.mp, L.Sp, L.Prows.p, L.d_iperm.p, L.d_pp.p, C.rcloud.p, C.d_iperm.p,
                                   L.pw.p, L.pcol.p, g_stream);
         if (rc) return rc;
+        L.gsp = gather_sort_on(L.Nd, L.Sp, L.mp, C.Nd);
+        if (L.gsp) {
+            if (L.gpcol.n != n) { HIP_TRY(L.gpcol.alloc(n)); HIP_TRY(L.gpw.alloc(n)); }
+            if ((rc = launch_gather_sort((int)(n / ((size_t)L.Sp * L.mp)), L.mp, L.Sp, L.pcol.p, L.gpcol.p,
+                                         g_stream)))
+                return rc;
+            if ((rc = launch_permute_slab((long)n, L.mp, L.Sp, L.gpcol.p, L.pw.p, L.gpw.p, g_stream))) return rc;
+        }
     }
     return 0;
 }
@@ -1087,13 +1119,14 @@ int op_relax(mgm_problem *pb, int l, int variant)
     }
     if (pb->cfg.sweep_order != MGM_ORDER_NATURAL && L.Nd <= small_level_rows() &&
         relax_small_fits(L.Nd, L.m, L.m, L.S))
-        return nrelax ? launch_relax_small(A, L.op(OP_LAP), L.d_col_slices.p, L.d_col_soff.p, L.ncol, L.Nd,
+        return nrelax ? launch_relax_small(L.op_plain(OP_LAPP), L.op_plain(OP_LAP), L.d_col_slices.p,
+                                           L.d_col_soff.p, L.ncol, L.Nd,
                                            L.dbtag.p, L.ref0.p, src, p, res, w, c, nrelax, variant, g_stream)
                       : 0;
     for (int it = 0; it < nrelax; it++) {
         int rc;
         if (pb->cfg.sweep_order == MGM_ORDER_NATURAL) {
-            rc = launch_sor_natural(A, L.N, L.d_iperm.p, src, p, w, c, g_stream);
+            rc = launch_sor_natural(L.op_plain(OP_LAPP), L.N, L.d_iperm.p, src, p, w, c, g_stream);
             if (rc) return rc;
         } else {
             for (int k = 0; k < L.ncol; k++) {

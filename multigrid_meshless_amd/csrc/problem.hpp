@@ -66,7 +66,21 @@ struct Level {
     DevBuf<float> p32, src32, res32;
     DevBuf<float> lap32, srcl32;               // time steps: FP32 lap; level 0's source for lap residuals
 
-    Ell op(int which) const { return Ell{col.p, slab[which].p, m, S, stream_slab(Nd, m, 8)}; }
+    // ---- gather-sorted copies for the streaming passes (large levels) ----
+    bool gs = false, gsp = false;              // operator / prolongation slabs have them
+    DevBuf<int> gcol, gpcol;
+    std::array<DevBuf<double>, NUM_OPS> gslab; // OP_LAP and OP_LAPP only
+    DevBuf<double> gpw;
+
+    // The operator as the stencil passes read it: gather-sorted where built.
+    Ell op(int which) const
+    {
+        if (gs && gslab[which].p)
+            return Ell{gcol.p, gslab[which].p, m, S, stream_slab(Nd, m, 8), slab[which].p, true};
+        return op_plain(which);
+    }
+    // The operator in plain slice order (entry j of row r at ell_at(m, S, r, j)).
+    Ell op_plain(int which) const { return Ell{col.p, slab[which].p, m, S, stream_slab(Nd, m, 8)}; }
     // slices of colour k for launch_sor_slices: a device list, or (null, first)
     void colour_slices(int k, const int **list, int *first, int *n) const
     {
@@ -75,7 +89,11 @@ struct Level {
         *first = (col_contig && *n) ? col_slices[col_soff[k]] : 0;
     }
     Ell restr() const { return Ell{rcol.p, rw.p, mr, Sr, stream_slab(Nd, mr, 8)}; }
-    Ell prol() const { return Ell{pcol.p, pw.p, mp, Sp, stream_slab(Nd, mp, 8)}; }
+    Ell prol() const
+    {
+        if (gsp) return Ell{gpcol.p, gpw.p, mp, Sp, stream_slab(Nd, mp, 8), pw.p, true};
+        return Ell{pcol.p, pw.p, mp, Sp, stream_slab(Nd, mp, 8)};
+    }
 };
 
 }  // namespace mgm

@@ -30,6 +30,7 @@ namespace {
 constexpr int BLK = 256;
 constexpr int WAVES = BLK / 64;
 constexpr int SMALL_BLK_MAX = 1024;
+constexpr int GS_BLK = 256;   // gather_sort_kernel workgroup
 // dynamic LDS of the single-workgroup relax: gfx950 gives a workgroup 160 KB
 constexpr size_t kSmallLdsMax = 160 * 1024 - 1024;
 
@@ -92,6 +93,29 @@ __device__ inline void stage_products(const EllT<TW> &A, size_t base, int ri, in
 {
     constexpr int LPR = 64 / S;
     const int m = A.m;
+    if (A.sorted) {
+        // Gather-sorted slab: lane l takes the slice's entries l, l+64, ...
+        // in column order and writes each product to its plain slot.
+        const int n = S * m;
+        const size_t sb = base - ri;
+        const int lane = jo * S + ri;
+        const TW *wp = A.w + sb;
+        const unsigned *cp = reinterpret_cast<const unsigned *>(A.col) + sb;
+        if (A.stream) {
+#pragma unroll 4
+            for (int q = lane; q < n; q += 64) {
+                const unsigned v = ldnt(cp + q);
+                P[v >> kSlotShift] = wv(ldnt(wp + q)) * x[v & kColMask];
+            }
+        } else {
+#pragma unroll 4
+            for (int q = lane; q < n; q += 64) {
+                const unsigned v = cp[q];
+                P[v >> kSlotShift] = wv(wp[q]) * x[v & kColMask];
+            }
+        }
+        return;
+    }
     // entry j = t*LPR + jo sits at base + j*S = lane base + t*64
     const TW *wp = A.w + base + (size_t)jo * S;
     const int *cp = A.col + base + (size_t)jo * S;
@@ -174,8 +198,9 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
     if (comb && mode != M_SPMV && mode != M_RESTRICT && mode != M_PROLONG && !(bnd && skips_boundary(mode)))
         sv = src[r];
     if (comb && (mode == M_SOR || (mode == M_PROLONG && !bnd))) yv = y[r];
-    if (comb && mode == M_SOR) dg = wv(A.w[base]);
-    if (need) stage_products<S, NIT, TW, TV>(A, base, ri, jo, x, P);
+    if (comb && mode == M_SOR) dg = wv((A.sorted ? A.w0 : A.w)[base]);
+    // a gather-sorted slice mixes its rows over the lanes: every lane loads
+    if (need || A.sorted) stage_products<S, NIT, TW, TV>(A, base, ri, jo, x, P);
     wave_sync();
     if (jo == 0 && act) {
         switch (mode) {
@@ -222,7 +247,7 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
 // Slice k of the launch is slices[k] when a slice list is given (one colour
 // class), else first + k (a contiguous colour class, or a full pass).  Rows at or beyond nrows and padding
 // rows (tag ROW_PAD) are neither read nor written.
-template <int S, typename TW, typename TV>
+template <int S, typename TW, typename TV, int NIT>
 __global__ void __launch_bounds__(BLK)
 rows_kernel(EllT<TW> A, const int *__restrict__ slices, int first, int nslices, int nrows, int mode,
             const uint8_t *__restrict__ btag, const TV *__restrict__ src, const TV *x, TV *y, TV w,
@@ -240,8 +265,8 @@ rows_kernel(EllT<TW> A, const int *__restrict__ slices, int first, int nslices, 
     const long k = xcd_block(blockIdx.x, nblocks) * WAVES + wave;   // nblocks = gridDim.x
     if (k >= nslices) return;
     const long slice = slices ? (long)slices[k] : (long)first + k;
-    slice_rows<S, TW, TV>(A, slice, nrows, mode, btag, src, x, y, w, c, keep_boundary,
-                          prod + (size_t)wave * A.m * S);
+    slice_rows<S, TW, TV, NIT>(A, slice, nrows, mode, btag, src, x, y, w, c, keep_boundary,
+                               prod + (size_t)wave * A.m * S);
 }
 
 // Weights and columns of this lane's entries (j = t*LPR + jo) of one slice.
@@ -495,6 +520,49 @@ __global__ void __launch_bounds__(BLK) stream_copy_kernel(long n16, const uint4 
 }
 
 // Row-major reference-order rows -> sliced-ELL device-order slab.
+// One workgroup per slice: bitonic sort of the slice's (column << 10 | slot)
+// keys in LDS (padded to 1024 with keys above every real one), written back
+// as (slot << kSlotShift) | column.  Requires S*m <= 1024 and columns below
+// kColMask (slab_sortable).
+__global__ void __launch_bounds__(GS_BLK)
+gather_sort_kernel(int m, int S, const int *__restrict__ col, int *__restrict__ gcol)
+{
+    __shared__ unsigned key[1024];
+    const int n = S * m;
+    const size_t sb = (size_t)blockIdx.x * n;
+    for (int i = threadIdx.x; i < 1024; i += GS_BLK)
+        key[i] = i < n ? ((unsigned)col[sb + i] << 10) | (unsigned)i : 0xFFFFFFFFu;
+    __syncthreads();
+    for (int k = 2; k <= 1024; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < 1024; i += GS_BLK) {
+                const int o = i ^ j;
+                if (o > i) {
+                    const unsigned a = key[i], b = key[o];
+                    if ((a > b) == ((i & k) == 0)) {
+                        key[i] = b;
+                        key[o] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int q = threadIdx.x; q < n; q += GS_BLK) {
+        const unsigned v = key[q];
+        gcol[sb + q] = (int)(((v & 1023u) << kSlotShift) | (v >> 10));
+    }
+}
+
+__global__ void __launch_bounds__(BLK)
+permute_slab_kernel(long n, int per, const int *__restrict__ gcol, const double *__restrict__ w,
+                    double *__restrict__ wg)
+{
+    const long e = (long)blockIdx.x * BLK + threadIdx.x;
+    if (e >= n) return;
+    const long sb = e - e % per;
+    wg[e] = w[sb + ((unsigned)gcol[e] >> kSlotShift)];
+}
+
 __global__ void __launch_bounds__(BLK)
 gather_rows_kernel(int N, int m, int S, const double *__restrict__ rows, const int *__restrict__ iperm,
                    const int *__restrict__ cloud, double *__restrict__ w_out,
@@ -550,6 +618,21 @@ dirichlet_rows_kernel(int N, int m, const uint8_t *__restrict__ btag, double *__
     if (btag[i]) rows[t] = (j == 0) ? 1.0 : 0.0;
 }
 
+// Loading scheme of rows_kernel (stage_products): 0 interleaves loads and
+// gathers, 4 or 9 issues a chunk of that many entries' loads first.
+// MGM_ROWS_NIT_SOR selects it for colour steps, MGM_ROWS_NIT for full passes.
+int rows_nit(bool sor)
+{
+    static const int v[2] = {[] {
+        const char *e = getenv("MGM_ROWS_NIT");
+        return e ? atoi(e) : 0;
+    }(), [] {
+        const char *e = getenv("MGM_ROWS_NIT_SOR");
+        return e ? atoi(e) : 0;
+    }()};
+    return v[sor ? 1 : 0];
+}
+
 // slices == nullptr: all slices covering rows [0, nrows)
 template <typename TW, typename TV>
 int launch_rows(EllT<TW> A, const int *slices, int first, int nslices, int nrows, int mode,
@@ -561,17 +644,26 @@ int launch_rows(EllT<TW> A, const int *slices, int first, int nslices, int nrows
     int blocks = (nslices + WAVES - 1) / WAVES;
     size_t shm = (size_t)WAVES * A.m * A.S * sizeof(TV);
     const dim3 grid(blocks);
+    const int nit = rows_nit(mode == M_SOR);
+#define MGM_ROWS(SS, NN)                                                                             \
+    hipLaunchKernelGGL((rows_kernel<SS, TW, TV, NN>), grid, dim3(BLK), shm, s, A, slices, first,     \
+                       nslices, nrows, mode, btag, src, x, y, w, c, keep, blocks)
+#define MGM_ROWS_S(SS)                                                                               \
+    do {                                                                                             \
+        if (nit == 4) MGM_ROWS(SS, 4);                                                               \
+        else if (nit == 9) MGM_ROWS(SS, 9);                                                          \
+        else MGM_ROWS(SS, 0);                                                                        \
+    } while (0)
     if (A.S == 4)
-        hipLaunchKernelGGL((rows_kernel<4, TW, TV>), grid, dim3(BLK), shm, s, A, slices, first,
-                           nslices, nrows, mode, btag, src, x, y, w, c, keep, blocks);
+        MGM_ROWS_S(4);
     else if (A.S == 8)
-        hipLaunchKernelGGL((rows_kernel<8, TW, TV>), grid, dim3(BLK), shm, s, A, slices, first,
-                           nslices, nrows, mode, btag, src, x, y, w, c, keep, blocks);
+        MGM_ROWS_S(8);
     else if (A.S == 16)
-        hipLaunchKernelGGL((rows_kernel<16, TW, TV>), grid, dim3(BLK), shm, s, A, slices, first,
-                           nslices, nrows, mode, btag, src, x, y, w, c, keep, blocks);
+        MGM_ROWS_S(16);
     else
         MGM_FAIL(-1, "unsupported slice height");
+#undef MGM_ROWS_S
+#undef MGM_ROWS
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -585,6 +677,20 @@ int launch_rows(EllT<TW> A, const int *slices, int first, int nslices, int nrows
             HIP_TRY(hipGetLastError());                                              \
         }                                                                            \
     } while (0)
+
+int launch_gather_sort(int nslices, int m, int S, const int *col, int *gcol, hipStream_t s)
+{
+    if (!slab_sortable(S, m, 0)) MGM_FAIL(-1, "gather_sort: slice of more than 1024 entries");
+    if (nslices <= 0) return 0;
+    hipLaunchKernelGGL(gather_sort_kernel, dim3(nslices), dim3(GS_BLK), 0, s, m, S, col, gcol);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+int launch_permute_slab(long n, int m, int S, const int *gcol, const double *w, double *wg, hipStream_t s)
+{
+    LAUNCH(permute_slab_kernel, n, n, S * m, gcol, w, wg);
+    return 0;
+}
 
 int slice_height(int m)
 {
@@ -650,6 +756,7 @@ int launch_relax_small(Ell A, Ell A2, const int *col_slices, const int *col_soff
                        double w, double c, int num_relax, int variant, hipStream_t s)
 {
     if (A.S != A2.S) MGM_FAIL(-1, "relax_small: operators differ in slice height");
+    if (A.sorted || A2.sorted) MGM_FAIL(-1, "relax_small: needs plain slabs");
     const int mw = A.m > A2.m ? A.m : A2.m;
     const int nw = relax_small_waves(nrows, mw, A.S);
     if (!nw) MGM_FAIL(-1, "relax_small: level does not fit in LDS");
@@ -667,6 +774,7 @@ int launch_relax_small(Ell A, Ell A2, const int *col_slices, const int *col_soff
 int launch_sor_natural(Ell A, int N, const int *iperm, const double *src, double *p, double w,
                        double c, hipStream_t s)
 {
+    if (A.sorted) MGM_FAIL(-1, "sor_natural: needs the plain slab");
     hipLaunchKernelGGL(sor_natural_kernel, dim3(1), dim3(64), 0, s, A, N, iperm, src, p, w, c);
     HIP_TRY(hipGetLastError());
     return 0;
