@@ -19,11 +19,28 @@ int op_learned_sweep_mp(mgm_problem *pb, int l, int (*residual_mp)(Level &));
 
 namespace {
 
+// A level's narrowed operator: gather-sorted where the level has the FP64
+// slab sorted (EllT::sorted; plain is the diagonal source), else plain.
+template <typename T>
+EllT<T> narrowed(const Level &L, const DevBuf<T> &plain, const DevBuf<T> &sorted)
+{
+    const bool st = stream_slab(L.Nd, L.m, sizeof(T));
+    if (L.gs && sorted.p) return EllT<T>{L.gcol.p, sorted.p, L.m, L.S, st, plain.p, true};
+    return EllT<T>{L.col.p, plain.p, L.m, L.S, st};
+}
+template <typename T>
+EllT<T> narrowed_prol(const Level &F, const DevBuf<T> &plain, const DevBuf<T> &sorted)
+{
+    const bool st = stream_slab(F.Nd, F.mp, sizeof(T));
+    if (F.gsp && sorted.p) return EllT<T>{F.gpcol.p, sorted.p, F.mp, F.Sp, st, plain.p, true};
+    return EllT<T>{F.pcol.p, plain.p, F.mp, F.Sp, st};
+}
+
 template <typename F>
 int with_op(const Level &L, F &&f)
 {
-    if (L.w16) return f(EllT<bf16>{L.col.p, L.lapp16.p, L.m, L.S, stream_slab(L.Nd, L.m, 2)});
-    return f(EllT<float>{L.col.p, L.lapp32.p, L.m, L.S, stream_slab(L.Nd, L.m, 4)});
+    if (L.w16) return f(narrowed(L, L.lapp16, L.glapp16));
+    return f(narrowed(L, L.lapp32, L.glapp32));
 }
 
 int residual_mp(Level &L);
@@ -76,18 +93,18 @@ int prolong_mp(Level &F, Level &C)
 {
     int rc;
     if (F.w16)
-        rc = launch_prolong(EllT<bf16>{F.pcol.p, F.pw16.p, F.mp, F.Sp, stream_slab(F.Nd, F.mp, 2)}, F.Nd, F.dbtag.p,
-                            (const float *)C.p32.p, F.p32.p, stream());
+        rc = launch_prolong(narrowed_prol(F, F.pw16, F.gpw16), F.Nd, F.dbtag.p, (const float *)C.p32.p, F.p32.p,
+                            stream());
     else
-        rc = launch_prolong(EllT<float>{F.pcol.p, F.pw32.p, F.mp, F.Sp, stream_slab(F.Nd, F.mp, 4)}, F.Nd, F.dbtag.p,
-                            (const float *)C.p32.p, F.p32.p, stream());
+        rc = launch_prolong(narrowed_prol(F, F.pw32, F.gpw32), F.Nd, F.dbtag.p, (const float *)C.p32.p, F.p32.p,
+                            stream());
     if (rc) return rc;
     return launch_zero(C.Nd, C.dbtag.p, 1, C.p32.p, stream());
 }
 
-EllT<float> op32(const Level &L, const DevBuf<float> &w)
+EllT<float> op32(const Level &L, const DevBuf<float> &w, const DevBuf<float> &gw)
 {
-    return EllT<float>{L.col.p, w.p, L.m, L.S, stream_slab(L.Nd, L.m, 4)};
+    return narrowed(L, w, gw);
 }
 
 // Source of the lap residual forms of variant F: level 0 has its own
@@ -101,7 +118,7 @@ int relax_ns_mp(mgm_problem *pb, int l, int variant)
     Level &L = pb->lv[l];
     const float w = pb->cfg.omega;
     const float c = 1.0f - pb->cfg.omega;
-    const EllT<float> A = op32(L, L.lapp32);
+    const EllT<float> A = op32(L, L.lapp32, L.glapp32);
     for (int it = 0; it < pb->cfg.num_relax; it++) {
         int rc;
         for (int k = 0; k < L.ncol; k++) {
@@ -117,7 +134,7 @@ int relax_ns_mp(mgm_problem *pb, int l, int variant)
             rc = launch_residual(A, L.Nd, 1, L.dbtag.p, (const float *)L.src32.p, (const float *)L.p32.p,
                                  L.res32.p, stream());
         else
-            rc = launch_residual(op32(L, L.lap32), L.Nd, 2, L.dbtag.p, srcl(L, l), (const float *)L.p32.p,
+            rc = launch_residual(op32(L, L.lap32, L.glap32), L.Nd, 2, L.dbtag.p, srcl(L, l), (const float *)L.p32.p,
                                  L.res32.p, stream());
         if (rc) return rc;
     }
@@ -128,9 +145,9 @@ int residual_ns_mp(mgm_problem *pb, int l, int variant)
 {
     Level &L = pb->lv[l];
     if (variant == MGM_VAR_F)
-        return launch_residual(op32(L, L.lap32), L.Nd, 3, L.dbtag.p, srcl(L, l), (const float *)L.p32.p,
+        return launch_residual(op32(L, L.lap32, L.glap32), L.Nd, 3, L.dbtag.p, srcl(L, l), (const float *)L.p32.p,
                                L.res32.p, stream());
-    return launch_residual(op32(L, L.lapp32), L.Nd, 0, L.dbtag.p, (const float *)L.src32.p,
+    return launch_residual(op32(L, L.lapp32, L.glapp32), L.Nd, 0, L.dbtag.p, (const float *)L.src32.p,
                            (const float *)L.p32.p, L.res32.p, stream());
 }
 
@@ -153,15 +170,19 @@ int mp_prepare(mgm_problem *pb, int precision)
         L.w16 = (precision == 2 && l > 0);
         int rc;
         if (L.w16) {
-            L.lapp32.release(); L.rw32.release(); L.pw32.release();
+            L.lapp32.release(); L.rw32.release(); L.pw32.release(); L.glapp32.release(); L.gpw32.release();
             if ((rc = narrow(L.slab[OP_LAPP], L.lapp16))) return rc;
             if (l > 0 && (rc = narrow(L.rw, L.rw16))) return rc;
             if (l + 1 < pb->L && (rc = narrow(L.pw, L.pw16))) return rc;
+            if (L.gs && (rc = narrow(L.gslab[OP_LAPP], L.glapp16))) return rc;
+            if (L.gsp && (rc = narrow(L.gpw, L.gpw16))) return rc;
         } else {
-            L.lapp16.release(); L.rw16.release(); L.pw16.release();
+            L.lapp16.release(); L.rw16.release(); L.pw16.release(); L.glapp16.release(); L.gpw16.release();
             if ((rc = narrow(L.slab[OP_LAPP], L.lapp32))) return rc;
             if (l > 0 && (rc = narrow(L.rw, L.rw32))) return rc;
             if (l + 1 < pb->L && (rc = narrow(L.pw, L.pw32))) return rc;
+            if (L.gs && (rc = narrow(L.gslab[OP_LAPP], L.glapp32))) return rc;
+            if (L.gsp && (rc = narrow(L.gpw, L.gpw32))) return rc;
         }
         for (DevBuf<float> *f : {&L.p32, &L.src32, &L.res32}) {
             if (f->n != (size_t)L.Nd) {
@@ -204,8 +225,10 @@ int mp_prepare_ns(mgm_problem *pb)
     int rc;
     if ((rc = mp_prepare(pb, 1))) return rc;
     if (pb->mp_ns_built) return 0;
-    for (auto &L : pb->lv)
+    for (auto &L : pb->lv) {
         if ((rc = narrow(L.slab[OP_LAP], L.lap32))) return rc;
+        if (L.gs && L.gslab[OP_LAP].p && (rc = narrow(L.gslab[OP_LAP], L.glap32))) return rc;
+    }
     Level &L0 = pb->lv[0];
     if (L0.srcl32.n != (size_t)L0.Nd) HIP_TRY(L0.srcl32.alloc(L0.Nd));
     HIP_TRY(hipStreamSynchronize(stream()));
@@ -258,8 +281,8 @@ int op_correction_ns_mp(mgm_problem *pb, int variant)
     }
     for (int l = nl - 1; l > 0; l--) {
         Level &F = pb->lv[l - 1], &C = pb->lv[l];
-        if ((rc = launch_prolong(EllT<float>{F.pcol.p, F.pw32.p, F.mp, F.Sp, stream_slab(F.Nd, F.mp, 4)}, F.Nd,
-                                 F.dbtag.p, (const float *)C.p32.p, F.p32.p, stream())))
+        if ((rc = launch_prolong(narrowed_prol(F, F.pw32, F.gpw32), F.Nd, F.dbtag.p, (const float *)C.p32.p,
+                                 F.p32.p, stream())))
             return rc;
         if ((rc = launch_zero(C.Nd, C.dbtag.p, variant != MGM_VAR_T, C.p32.p, stream()))) return rc;
         if (l != 1 && (rc = relax_ns_mp(pb, l - 1, variant))) return rc;

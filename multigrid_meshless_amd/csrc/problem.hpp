@@ -65,6 +65,8 @@ struct Level {
     DevBuf<bf16> lapp16, rw16, pw16;
     DevBuf<float> p32, src32, res32;
     DevBuf<float> lap32, srcl32;               // time steps: FP32 lap; level 0's source for lap residuals
+    DevBuf<float> glapp32, glap32, gpw32;      // gather-sorted narrowed copies (levels with gs / gsp)
+    DevBuf<bf16> glapp16, gpw16;
 
     // ---- gather-sorted copies for the streaming passes (large levels) ----
     bool gs = false, gsp = false;              // operator / prolongation slabs have them

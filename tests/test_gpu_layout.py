@@ -101,3 +101,13 @@ def test_time_steps_bitwise(pair):
             errs.append([P.fs_step() if scheme == "fs" else P.timple_step() for _ in range(2)])
         np.testing.assert_array_equal(errs[0], errs[1])
         same_fields(A, B, ("p", "pprime", "u", "v", "u_new", "v_new", "p_old"))
+    # the FP32 pressure cycle of a captured march
+    for var in (mgm.VAR_F, mgm.VAR_T):
+        runs = []
+        for P in (A, B):
+            P.zero_fields()
+            cases.lid_driven_cavity(P, timple=var == mgm.VAR_T)
+            P.ns_set_params(**cases.ns_params(100.0, P.info(0)["d_avg"], 0.5, num_vcycles=3))
+            runs.append(P.ns_march(var, steps=2, precision=1))
+        np.testing.assert_array_equal(runs[0]["errors"], runs[1]["errors"])
+        same_fields(A, B, ("p", "pprime", "u", "v", "u_new", "v_new", "p_old"))
