@@ -247,7 +247,7 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
 // Slice k of the launch is slices[k] when a slice list is given (one colour
 // class), else first + k (a contiguous colour class, or a full pass).  Rows at or beyond nrows and padding
 // rows (tag ROW_PAD) are neither read nor written.
-template <int S, typename TW, typename TV, int NIT>
+template <int S, typename TW, typename TV>
 __global__ void __launch_bounds__(BLK)
 rows_kernel(EllT<TW> A, const int *__restrict__ slices, int first, int nslices, int nrows, int mode,
             const uint8_t *__restrict__ btag, const TV *__restrict__ src, const TV *x, TV *y, TV w,
@@ -265,8 +265,8 @@ rows_kernel(EllT<TW> A, const int *__restrict__ slices, int first, int nslices, 
     const long k = xcd_block(blockIdx.x, nblocks) * WAVES + wave;   // nblocks = gridDim.x
     if (k >= nslices) return;
     const long slice = slices ? (long)slices[k] : (long)first + k;
-    slice_rows<S, TW, TV, NIT>(A, slice, nrows, mode, btag, src, x, y, w, c, keep_boundary,
-                               prod + (size_t)wave * A.m * S);
+    slice_rows<S, TW, TV>(A, slice, nrows, mode, btag, src, x, y, w, c, keep_boundary,
+                          prod + (size_t)wave * A.m * S);
 }
 
 // Weights and columns of this lane's entries (j = t*LPR + jo) of one slice.
@@ -618,21 +618,6 @@ dirichlet_rows_kernel(int N, int m, const uint8_t *__restrict__ btag, double *__
     if (btag[i]) rows[t] = (j == 0) ? 1.0 : 0.0;
 }
 
-// Loading scheme of rows_kernel (stage_products): 0 interleaves loads and
-// gathers, 4 or 9 issues a chunk of that many entries' loads first.
-// MGM_ROWS_NIT_SOR selects it for colour steps, MGM_ROWS_NIT for full passes.
-int rows_nit(bool sor)
-{
-    static const int v[2] = {[] {
-        const char *e = getenv("MGM_ROWS_NIT");
-        return e ? atoi(e) : 0;
-    }(), [] {
-        const char *e = getenv("MGM_ROWS_NIT_SOR");
-        return e ? atoi(e) : 0;
-    }()};
-    return v[sor ? 1 : 0];
-}
-
 // slices == nullptr: all slices covering rows [0, nrows)
 template <typename TW, typename TV>
 int launch_rows(EllT<TW> A, const int *slices, int first, int nslices, int nrows, int mode,
@@ -644,26 +629,17 @@ int launch_rows(EllT<TW> A, const int *slices, int first, int nslices, int nrows
     int blocks = (nslices + WAVES - 1) / WAVES;
     size_t shm = (size_t)WAVES * A.m * A.S * sizeof(TV);
     const dim3 grid(blocks);
-    const int nit = rows_nit(mode == M_SOR);
-#define MGM_ROWS(SS, NN)                                                                             \
-    hipLaunchKernelGGL((rows_kernel<SS, TW, TV, NN>), grid, dim3(BLK), shm, s, A, slices, first,     \
-                       nslices, nrows, mode, btag, src, x, y, w, c, keep, blocks)
-#define MGM_ROWS_S(SS)                                                                               \
-    do {                                                                                             \
-        if (nit == 4) MGM_ROWS(SS, 4);                                                               \
-        else if (nit == 9) MGM_ROWS(SS, 9);                                                          \
-        else MGM_ROWS(SS, 0);                                                                        \
-    } while (0)
     if (A.S == 4)
-        MGM_ROWS_S(4);
+        hipLaunchKernelGGL((rows_kernel<4, TW, TV>), grid, dim3(BLK), shm, s, A, slices, first,
+                           nslices, nrows, mode, btag, src, x, y, w, c, keep, blocks);
     else if (A.S == 8)
-        MGM_ROWS_S(8);
+        hipLaunchKernelGGL((rows_kernel<8, TW, TV>), grid, dim3(BLK), shm, s, A, slices, first,
+                           nslices, nrows, mode, btag, src, x, y, w, c, keep, blocks);
     else if (A.S == 16)
-        MGM_ROWS_S(16);
+        hipLaunchKernelGGL((rows_kernel<16, TW, TV>), grid, dim3(BLK), shm, s, A, slices, first,
+                           nslices, nrows, mode, btag, src, x, y, w, c, keep, blocks);
     else
         MGM_FAIL(-1, "unsupported slice height");
-#undef MGM_ROWS_S
-#undef MGM_ROWS
     HIP_TRY(hipGetLastError());
     return 0;
 }
