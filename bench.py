@@ -113,6 +113,64 @@ def build_cavity(mgm, n, levels, Re, courant, num_relax, num_vcycles, seed0=31):
     return P, sizes, prm
 
 
+def build_annulus(mgm, n, levels, seed0):
+    """2D annulus r in [0.5, 1] (cases.annulus): n points per side of the
+    bounding grid on level 0, `levels` clouds halving in spacing; PHS r^3 +
+    cubic polynomials, multiplier 2 (m = 20); Dirichlet rows on both circles."""
+    from multigrid_meshless_amd import cases
+    sizes = [((n - 1) >> k) + 1 for k in range(levels)]
+    pts = [cases.annulus(s, seed0 + k) for k, s in enumerate(sizes)]
+    cfg = mgm.config(dimension=2, poly_degree=3, coarse_poly_degree=3, phs_degree=3, cloud_size_multiplier=2.0,
+                     omega=1.0, num_relax=3, transfer_rows=mgm.ROWS_BY_TAG, sweep_order=mgm.ORDER_COLOUR)
+    P = mgm.Problem(cfg, pts).setup()
+    P.dirichlet_rows()
+    return P, sizes
+
+
+def annulus_learned(mgm, args):
+    """Learned smoother on the annulus: plain FP64 SOR V-cycles against the
+    learned first sweep (FP64 and FP32 inner cycles), network trained on a
+    coarser annulus hierarchy (181 per side, 3 levels) and moved over."""
+    P, sizes = build_annulus(mgm, args.ann_n, args.ann_levels, 41)
+    g = P.geometry(0)
+    f = fourier_rhs(g["x"], g["y"], np.zeros_like(g["x"]))
+    f[g["btag"] != 0] = 0.0
+    N = P.info(0)["N"]
+
+    def run(prec):
+        for _ in range(2):              # first: graph capture and copies; second: timed
+            reset(P, f)
+            r = P.solve(args.tol, args.ann_max_cycles, precision=prec)
+        return r
+
+    r = run(0)
+    p0 = P.field(0, "p")
+    out = {"workload": f"2D annulus r in [0.5, 1], {N} jittered points, {len(sizes)} levels {sizes} "
+                       f"(grid points per side), m={P.info(0)['m']}, Dirichlet zero on both circles, "
+                       f"16-mode Fourier source, tol {args.tol}",
+           "plain_fp64": {"s": round(r["seconds"], 5), "cycles": r["cycles"],
+                          "final_ratio": float(r["history"][-1])}}
+    t_plain = r["seconds"]
+    T, _ = build_annulus(mgm, 181, 3, 51)
+    t0 = time.perf_counter()
+    l0, l1 = T.train_smoother(snapshots=8, samples_per_snapshot=8192, epochs=30, batch=4096, lr=1e-3, seed=7)
+    out["train_s"] = round(time.perf_counter() - t0, 2)
+    out["train_loss"] = [round(l0, 4), round(l1, 4)]
+    P.set_mlp_weights(T.mlp_weights())
+    del T
+    P.use_learned_smoother(True)
+    for prec, tag in ((0, "learned_fp64"), (1, "learned_fp32")):
+        r = run(prec)
+        out[tag] = {"s": round(r["seconds"], 5), "cycles": r["cycles"], "final_ratio": float(r["history"][-1]),
+                    "fell_back_to_sor": P.learned_fell_back(),
+                    "max_rel_diff_vs_plain_solution": float(np.abs(P.field(0, "p") - p0).max() / np.abs(p0).max()),
+                    "speedup_vs_plain": round(t_plain / r["seconds"], 3)}
+        print(f"[bench] annulus {N} points: plain {out['plain_fp64']['cycles']} cycles {t_plain:.4f}s; {tag} "
+              f"{r['cycles']} cycles {r['seconds']:.4f}s", file=sys.stderr, flush=True)
+    P.use_learned_smoother(False)
+    return out
+
+
 def cavity_march(mgm, args):
     """Explicit fractional-step march of the lid-driven cavity, one whole step
     (its V-cycles included) captured as a hipGraph and replayed; FP64 and the
@@ -201,6 +259,10 @@ def main():
     ap.add_argument("--ldc-levels", type=int, default=3)
     ap.add_argument("--ldc-re", type=float, default=100.0)
     ap.add_argument("--ldc-steps", type=int, default=2000)
+    ap.add_argument("--no-annulus", action="store_true", help="skip the learned smoother on the 2D annulus")
+    ap.add_argument("--ann-n", type=int, default=714, help="annulus grid points per side (714: 301k points)")
+    ap.add_argument("--ann-levels", type=int, default=4)
+    ap.add_argument("--ann-max-cycles", type=int, default=3000)
     ap.add_argument("--spmv-only", action="store_true",
                     help="set up, time the level-0 stencil apply and the streaming copy, print them, exit")
     args = ap.parse_args()
@@ -338,6 +400,7 @@ def main():
           f"{weights['gflops']} GFLOP/s", file=sys.stderr, flush=True)
 
     cavity = None if args.no_ldc else cavity_march(mgm, args)
+    annulus = None if (args.no_annulus or args.no_learned) else annulus_learned(mgm, args)
 
     cpu = None
     if not args.no_cpu:
@@ -375,6 +438,7 @@ def main():
                    "reduced_precision": mixed,
                    "weight_build": weights,
                    "learned_smoother": learned,
+                   "learned_smoother_annulus": annulus,
                    "cavity_time_march": cavity},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

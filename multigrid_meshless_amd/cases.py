@@ -2,7 +2,9 @@
 of a set-up Problem.
 
 lid_driven_cavity follows init/init_LDC.c:16-44; calculate_dt follows
-hf/general_functions.h:34-41.
+hf/general_functions.h:34-41.  annulus builds the point cloud of the
+annular Poisson case the learned smoother is also measured on (the reference
+reads such clouds from Gmsh files, e.g. its mesh/2d/TC annuli).
 """
 from __future__ import annotations
 
@@ -43,3 +45,41 @@ def ns_params(Re: float, d_avg: float, courant: float, rho: float = 1.0, facRe: 
     nu = mu / rho
     return dict(dt=calculate_dt(d_avg, nu, courant), rho=rho, mu=mu, nu=nu, facRe=facRe, facdt=facdt,
                 field_rho=rho, iter_momentum=iter_momentum, iter_timple=iter_timple, num_vcycles=num_vcycles)
+
+
+def annulus(n: int, seed: int, jitter: float = 0.2, r_in: float = 0.5, r_out: float = 1.0):
+    """Jittered point cloud on the annulus r_in <= r <= r_out, laid out as the
+    Gmsh reader leaves a level (boundary nodes first, no corners).
+
+    Interior: the nodes of an n x n Cartesian grid of spacing h = 2 r_out / (n - 1)
+    over [-r_out, r_out]^2, each moved by U(-jitter h, jitter h) per axis, kept
+    where r_in + h/2 < r < r_out - h/2.  Boundary: round(2 pi r / h) nodes on
+    each circle, equally spaced in angle with each angle moved by
+    U(-jitter, jitter) of a spacing; unit normals point out of the domain (+r
+    on the outer circle, -r on the inner one).  numpy's default generator,
+    seeded with `seed`."""
+    from . import Points
+    rng = np.random.default_rng(seed)
+    h = 2.0 * r_out / (n - 1)
+    bx, by, bnx, bny = [], [], [], []
+    for r, sgn in ((r_out, 1.0), (r_in, -1.0)):
+        k = max(8, int(round(2 * np.pi * r / h)))
+        t = (np.arange(k) + jitter * rng.uniform(-1.0, 1.0, k)) * (2 * np.pi / k)
+        bx.append(r * np.cos(t)); by.append(r * np.sin(t))
+        bnx.append(sgn * np.cos(t)); bny.append(sgn * np.sin(t))
+    g = -r_out + h * np.arange(n)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    X = X.ravel() + jitter * h * rng.uniform(-1.0, 1.0, X.size)
+    Y = Y.ravel() + jitter * h * rng.uniform(-1.0, 1.0, Y.size)
+    R = np.hypot(X, Y)
+    keep = (R > r_in + 0.5 * h) & (R < r_out - 0.5 * h)
+    x = np.concatenate(bx + [X[keep]])
+    y = np.concatenate(by + [Y[keep]])
+    nb = sum(len(b) for b in bx)
+    N = x.size
+    nx = np.concatenate(bnx + [np.zeros(N - nb)])
+    ny = np.concatenate(bny + [np.zeros(N - nb)])
+    btag = np.zeros(N, np.uint8)
+    btag[:nb] = 1
+    zero = np.zeros(N)
+    return Points.from_arrays(N, nb, 2 * N, h, x, y, zero, nx, ny, zero, btag, np.zeros(N, np.uint8))

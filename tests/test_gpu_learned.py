@@ -132,3 +132,57 @@ def test_training_and_learned_solve(gpu):
         print(f"  learned, precision {prec}: {r['cycles']} cycles {r['seconds']:.4f}s "
               f"fell back {P.learned_fell_back()}", flush=True)
         assert r["history"][-1] <= 1e-10
+
+
+def annulus_problem(mgm, n, levels, seed0):
+    """The 2D annulus r in [0.5, 1] (cases.annulus), `levels` clouds halving in
+    spacing; PHS r^3 + cubic polynomials, multiplier 2 (m = 20); Dirichlet rows
+    on both circles."""
+    from multigrid_meshless_amd import cases
+    sizes = [((n - 1) >> k) + 1 for k in range(levels)]
+    pts = [cases.annulus(s, seed0 + k) for k, s in enumerate(sizes)]
+    cfg = mgm.config(dimension=2, poly_degree=3, coarse_poly_degree=3, phs_degree=3, cloud_size_multiplier=2.0,
+                     omega=1.0, num_relax=3, transfer_rows=mgm.ROWS_BY_TAG, sweep_order=mgm.ORDER_COLOUR)
+    P = mgm.Problem(cfg, pts).setup()
+    P.dirichlet_rows()
+    return P
+
+
+def annulus_rhs(g, seed=2, modes=16):
+    rng = np.random.default_rng(seed)
+    f = np.zeros_like(g["x"])
+    for _ in range(modes):
+        k = rng.integers(1, 5, 2)
+        f += rng.standard_normal() * np.sin(np.pi * (k[0] * g["x"] + k[1] * g["y"]))
+    f[g["btag"] != 0] = 0.0
+    return f
+
+
+def test_learned_smoother_annulus(gpu):
+    """Trained on a coarser annulus hierarchy, used on a finer one: the learned
+    solve reaches the tolerance, its field agrees with the plain solve's, and
+    it needs fewer V-cycles (or reports that it fell back to SOR)."""
+    print("[gpu learned] annulus 181/91/46/23, network trained on 91/46/23", flush=True)
+    P = annulus_problem(gpu, 181, 4, 41)
+    f = annulus_rhs(P.geometry(0))
+    P.zero_fields()
+    P.set_field(0, "source", f)
+    plain = P.solve(tol=1e-8, max_cycles=300)
+    p_plain = P.field(0, "p")
+    assert plain["history"][-1] <= 1e-8
+    T = annulus_problem(gpu, 91, 3, 51)
+    assert T.mlp_inputs() == P.mlp_inputs() == 20 * 2 + 20
+    l0, l1 = T.train_smoother(snapshots=8, samples_per_snapshot=4096, epochs=30, batch=2048, lr=1e-3, seed=7)
+    assert l1 < l0
+    P.set_mlp_weights(T.mlp_weights())
+    P.use_learned_smoother(True)
+    P.zero_fields()
+    P.set_field(0, "source", f)
+    r = P.solve(tol=1e-8, max_cycles=300)
+    fb = P.learned_fell_back()
+    diff = np.abs(P.field(0, "p") - p_plain).max() / np.abs(p_plain).max()
+    print(f"  plain {plain['cycles']} cycles {plain['seconds']:.4f}s; learned {r['cycles']} cycles "
+          f"{r['seconds']:.4f}s, fell back {fb}, field diff {diff:.2e}, loss {l0:.3f} -> {l1:.3f}", flush=True)
+    assert r["history"][-1] <= 1e-8
+    assert diff <= 1e-6
+    assert fb or r["cycles"] < plain["cycles"]
