@@ -113,3 +113,28 @@ def test_setup_without_gpu_fails_loudly():
     P = mgm.Problem(mgm.config(), [pts])
     with pytest.raises(mgm.MgmError):
         P.setup()
+
+
+def test_annulus_cloud_geometry():
+    """cases.annulus: boundary nodes first, on the two circles, with unit normals
+    pointing out of the domain; interior nodes strictly between the circles."""
+    from multigrid_meshless_amd import cases
+    P = cases.annulus(91, 5)
+    info, a = P.info(), P.arrays()
+    nb, N = info["nb"], info["N"]
+    assert np.all(a["btag"][:nb] == 1) and np.all(a["btag"][nb:] == 0) and not a["ctag"].any()
+    r = np.hypot(a["x"], a["y"])
+    outer = np.abs(r[:nb] - 1.0) < 1e-12
+    inner = np.abs(r[:nb] - 0.5) < 1e-12
+    assert np.all(outer | inner) and outer.sum() > inner.sum() > 0
+    n = np.stack([a["nx"][:nb], a["ny"][:nb]], 1)
+    radial = np.stack([a["x"][:nb], a["y"][:nb]], 1) / r[:nb, None]
+    np.testing.assert_allclose(np.hypot(n[:, 0], n[:, 1]), 1.0, atol=1e-14)
+    np.testing.assert_allclose(n[outer], radial[outer], atol=1e-14)
+    np.testing.assert_allclose(n[inner], -radial[inner], atol=1e-14)
+    h = 2.0 / 90
+    assert np.all(r[nb:] > 0.5 + 0.5 * h) and np.all(r[nb:] < 1.0 - 0.5 * h)
+    assert info["d_avg"] == h and N - nb > 10 * nb
+    # the same seed gives the same cloud, another seed another one
+    np.testing.assert_array_equal(cases.annulus(91, 5).arrays()["x"], a["x"])
+    assert not np.array_equal(cases.annulus(91, 6).arrays()["x"], a["x"])
