@@ -323,7 +323,7 @@ __device__ inline void sor_slice_lds(const Ell &A, int slice, int nrows, const u
 // its first slice of the next colour step before the barrier, so a step costs
 // the LDS gathers and the ordered sums.  Identical arithmetic to the
 // per-colour launches.
-template <int S>
+template <int S, bool PIN>
 __global__ void __launch_bounds__(SMALL_BLK_MAX)
 relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int *__restrict__ col_soff,
                    int ncol, int nrows, const uint8_t *__restrict__ btag, const int *__restrict__ ref0,
@@ -378,13 +378,13 @@ relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int 
             // this wave's first slice of the next colour step, before the barrier
             // (across a sweep boundary only when no pin/residual pass intervenes)
             const int ncl = cl + 1 < ncol ? cl + 1 : 0;
-            if (regs && (cl + 1 < ncol || (it + 1 < num_relax && variant == MGM_VAR_H))) {
+            if (regs && (cl + 1 < ncol || (it + 1 < num_relax && !PIN))) {
                 const int f = first_slice(ncl);
                 if (f >= 0) load_slice<S, NIT>(A, f, ri, jo, wq, cq);
             }
             __syncthreads();
         }
-        if (variant == MGM_VAR_H) continue;
+        if constexpr (!PIN) continue;   // variant H: no pin, no residual pass
         if (threadIdx.x == 0) pref = xs[*ref0];
         __syncthreads();
         const double pr = pref;
@@ -710,22 +710,36 @@ bool relax_small_fits(int nrows, int m, int m2, int S)
 {
     return relax_small_waves(nrows, m > m2 ? m : m2, S) > 0;
 }
-template <int S>
-static int launch_relax_small_t(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol,
+template <int S, bool PIN>
+static int launch_relax_small_v(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol,
                                 int nrows, const uint8_t *btag, const int *ref0, const double *src,
                                 double *p, double *res, double w, double c, int num_relax, int variant,
                                 int nw, size_t shm, hipStream_t s)
 {
     static bool attr = false;
     if (!attr) {
-        HIP_TRY(hipFuncSetAttribute((const void *)relax_small_kernel<S>,
+        HIP_TRY(hipFuncSetAttribute((const void *)relax_small_kernel<S, PIN>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSmallLdsMax));
         attr = true;
     }
-    hipLaunchKernelGGL(relax_small_kernel<S>, dim3(1), dim3(64 * nw), shm, s, A, A2, col_slices, col_soff,
-                       ncol, nrows, btag, ref0, src, p, res, w, c, num_relax, variant);
+    hipLaunchKernelGGL((relax_small_kernel<S, PIN>), dim3(1), dim3(64 * nw), shm, s, A, A2, col_slices,
+                       col_soff, ncol, nrows, btag, ref0, src, p, res, w, c, num_relax, variant);
     HIP_TRY(hipGetLastError());
     return 0;
+}
+// The H variant has no pin or residual pass: its own instance, without their
+// registers.
+template <int S>
+static int launch_relax_small_t(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol,
+                                int nrows, const uint8_t *btag, const int *ref0, const double *src,
+                                double *p, double *res, double w, double c, int num_relax, int variant,
+                                int nw, size_t shm, hipStream_t s)
+{
+    if (variant == MGM_VAR_H)
+        return launch_relax_small_v<S, false>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p, res,
+                                              w, c, num_relax, variant, nw, shm, s);
+    return launch_relax_small_v<S, true>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p, res, w,
+                                         c, num_relax, variant, nw, shm, s);
 }
 int launch_relax_small(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol, int nrows,
                        const uint8_t *btag, const int *ref0, const double *src, double *p, double *res,
