@@ -131,6 +131,18 @@ int mgm_get_field(const mgm_problem *p, int lev, int which, double *out);
 int mgm_set_field(mgm_problem *p, int lev, int which, const double *in);
 int mgm_zero_fields(mgm_problem *p);
 
+/* ---- output (reference node order, the drivers' formats) ------------------ */
+/* Solution file of level lev: one line per node, "x, y[, z], f1, f2, ...",
+ * every value printed with "%lf" and ", " between them, as the drivers write
+ * Solution.csv: with_z = 0 and fields {P} is mg_heat_conduction.c:89-93,
+ * with_z = 1 and fields {U, V, W, POLD} is mg_NS_solver.c:84-90.  fields are
+ * MGM_F_* (at most 16). */
+int mgm_write_solution(const mgm_problem *p, int lev, const char *path, int with_z, int nfields,
+                       const int *fields);
+/* Convergence file: line i is "i, value" with the value printed "%e"
+ * (scientific != 0, mg_NS_solver.c:77-78) or "%lf" (mg_heat_conduction.c:77). */
+int mgm_write_history(const char *path, int n, const double *values, int scientific);
+
 /* Smoother parameters used by subsequent relaxations (parameters.omega and
  * parameters.num_relax in the reference, header_files/structures.h:26,31). */
 int mgm_set_smoother(mgm_problem *p, float omega, int num_relax);

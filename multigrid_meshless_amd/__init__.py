@@ -24,6 +24,7 @@ FIELD = dict(p=0, source=1, res=2, pprime=3, u=4, v=5, w=6, u_new=7, v_new=8, w_
              p_old=10, T=11, dpdn=12, dpdx=13, dpdy=14, dpdz=15, u_old=16, v_old=17, w_old=18)
 
 __all__ = ["Config", "Points", "Problem", "init", "device_count", "MgmError", "config", "stream_copy_seconds",
+           "write_history",
            "VAR_H", "VAR_T", "VAR_F", "MAT", "FIELD"]
 
 
@@ -40,6 +41,13 @@ def stream_copy_seconds(nbytes: int, reps: int = 20) -> float:
     s = C.c_double()
     check(lib().mgm_stream_copy(int(nbytes), int(reps), C.byref(s)), "stream_copy")
     return s.value
+
+
+def write_history(path: str, values, scientific: bool = True) -> None:
+    """Convergence file: line i is "i, value", printed "%e" (mg_NS_solver.c:77-78)
+    or "%lf" (mg_heat_conduction.c:77)."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    check(lib().mgm_write_history(path.encode(), int(v.size), v.ctypes.data, int(scientific)), "write_history")
 
 
 def config(**kw) -> Config:
@@ -201,6 +209,15 @@ class Problem:
         a = np.ascontiguousarray(a, dtype=np.float64)
         assert a.shape == (self.info(lev)["N"],)
         check(lib().mgm_set_field(self._h, lev, FIELD[name], a.ctypes.data), f"set_field {name}")
+
+    def write_solution(self, path: str, fields=("p",), with_z: bool = False, lev: int = 0) -> None:
+        """One line per node in reference order, "x, y[, z], fields..." printed
+        "%lf": ("p",) without z is the heat driver's Solution.csv
+        (mg_heat_conduction.c:89-93), ("u", "v", "w", "p_old") with z the NS
+        driver's (mg_NS_solver.c:84-90)."""
+        ids = (C.c_int * len(fields))(*[FIELD[k] for k in fields])
+        check(lib().mgm_write_solution(self._h, lev, path.encode(), int(with_z), len(fields), ids),
+              "write_solution")
 
     def zero_fields(self):
         check(lib().mgm_zero_fields(self._h))

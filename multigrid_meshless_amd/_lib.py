@@ -71,6 +71,8 @@ SIGNATURES = [
     ("mgm_set_matrix", _i, [_vp, _i, _i, _vp]),
     ("mgm_dirichlet_rows", _i, [_vp]),
     ("mgm_get_field", _i, [_vp, _i, _i, _vp]),
+    ("mgm_write_solution", _i, [_vp, _i, C.c_char_p, _i, _i, _vp]),
+    ("mgm_write_history", _i, [C.c_char_p, _i, _vp, _i]),
     ("mgm_set_field", _i, [_vp, _i, _i, _vp]),
     ("mgm_zero_fields", _i, [_vp]),
     ("mgm_set_smoother", _i, [_vp, _f, _i]),

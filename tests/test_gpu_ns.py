@@ -88,3 +88,22 @@ def test_fp32_pressure_cycle_tracks_fp64(gpu, scheme):
         print(f"  field diffs {worst}; error history diff {rel_err.max():.2e}", flush=True)
         assert all(v < 1e-4 for v in worst.values()), worst
         assert rel_err.max() < 1e-4
+
+
+def test_solution_files_in_the_drivers_formats(gpu, tmp_path):
+    """mgm_write_solution writes level 0 in reference node order with the
+    drivers' formats: x, y, z, u, v, w, p_old (mg_NS_solver.c:84-90) and
+    x, y, p (mg_heat_conduction.c:89-93), every value "%lf"."""
+    print("[gpu ns] ldc 33/17/9 fs: solution files", flush=True)
+    P = ldc(gpu, "fs")
+    for _ in range(2):
+        P.fs_step()
+    g = P.geometry(0)
+    cases_ = {"ns": (("u", "v", "w", "p_old"), True), "heat": (("p",), False)}
+    for tag, (names, with_z) in cases_.items():
+        path = tmp_path / f"Solution_{tag}.csv"
+        P.write_solution(str(path), names, with_z=with_z)
+        cols = [g["x"], g["y"]] + ([g["z"]] if with_z else []) + [P.field(0, k) for k in names]
+        want = "".join(", ".join("%f" % c[i] for c in cols) + "\n" for i in range(len(g["x"])))
+        assert path.read_text() == want, tag
+    assert np.abs(P.field(0, "u")).max() > 0 and np.abs(P.field(0, "p")).max() > 0

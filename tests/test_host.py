@@ -138,3 +138,15 @@ def test_annulus_cloud_geometry():
     # the same seed gives the same cloud, another seed another one
     np.testing.assert_array_equal(cases.annulus(91, 5).arrays()["x"], a["x"])
     assert not np.array_equal(cases.annulus(91, 6).arrays()["x"], a["x"])
+
+
+def test_write_history_formats(tmp_path):
+    """The convergence file in the drivers' two formats ("%d, %e" and "%d, %lf")."""
+    import multigrid_meshless_amd as mgm
+    v = [6.398105703570e-03, 3.2277e-05, 0.0, 1.5]
+    for sci, fmt in ((True, "%d, %e\n"), (False, "%d, %f\n")):
+        p = tmp_path / f"conv{int(sci)}.csv"
+        mgm.write_history(str(p), v, scientific=sci)
+        assert p.read_text() == "".join(fmt % (i, x) for i, x in enumerate(v))
+    with pytest.raises(mgm.MgmError):
+        mgm.write_history(str(tmp_path / "missing" / "conv.csv"), v)
