@@ -138,18 +138,30 @@ struct EllT {
     bool stream = false;   // slab larger than the last-level cache: load it non-temporally
     // Gather-sorted form: inside each slice the S*m entries are stored in
     // ascending column order, so the 64 lanes of one gather read nearby x
-    // values.  col then packs (slot << kSlotShift) | column, slot = the
-    // entry's offset j*S + ri in the plain slice, and w holds the weights in
-    // the same order; w0 is the plain slab (entry 0 of row ri at the plain
-    // offset, the SOR diagonal).  Products land in the same LDS slots as in
-    // the plain form, so the row sums are unchanged.
+    // values.  col then packs (slot << shift) | column, slot = the entry's
+    // offset j*S + ri in the plain slice (slot_bits(S, m) bits at the top,
+    // shift = 32 - those), and w holds the weights in the same order; w0 is
+    // the plain slab (entry 0 of row ri at the plain offset, the SOR
+    // diagonal).  Products land in the same LDS slots as in the plain form, so
+    // the row sums are unchanged.
     const TW *w0 = nullptr;
     bool sorted = false;
+    int shift = 0;
 };
-constexpr int kSlotShift = 22;                 // columns below 2^22, slots below 2^10
-constexpr unsigned kColMask = (1u << kSlotShift) - 1;
-// A slab of N rows, width m and slice height S can be stored gather-sorted.
-inline bool slab_sortable(int S, int m, int ncols) { return S * m <= 1024 && ncols < (int)kColMask; }
+// Bits of a slot index of a slice of S*m entries (S*m <= 1024).
+inline int slot_bits(int S, int m)
+{
+    int b = 0;
+    while ((1 << b) < S * m) b++;
+    return b;
+}
+// A slab of slice height S and width m whose columns index a vector of ncols
+// entries can be stored gather-sorted: its slots and columns fit 32 bits
+// (the largest column stays below the all-ones key that pads the sort).
+inline bool slab_sortable(int S, int m, long ncols)
+{
+    return S * m <= 1024 && ncols < (1L << (32 - slot_bits(S, m))) - 1;
+}
 // Slabs above this size cannot stay in the 256 MB last-level cache between passes.
 constexpr size_t kStreamSlabBytes = 256ull << 20;
 inline bool stream_slab(size_t rows, int m, size_t wbytes) { return rows * m * (wbytes + 4) > kStreamSlabBytes; }

@@ -25,14 +25,16 @@ template <typename T>
 EllT<T> narrowed(const Level &L, const DevBuf<T> &plain, const DevBuf<T> &sorted)
 {
     const bool st = stream_slab(L.Nd, L.m, sizeof(T));
-    if (L.gs && sorted.p) return EllT<T>{L.gcol.p, sorted.p, L.m, L.S, st, plain.p, true};
+    if (L.gs && sorted.p)
+        return EllT<T>{L.gcol.p, sorted.p, L.m, L.S, st, plain.p, true, 32 - slot_bits(L.S, L.m)};
     return EllT<T>{L.col.p, plain.p, L.m, L.S, st};
 }
 template <typename T>
 EllT<T> narrowed_prol(const Level &F, const DevBuf<T> &plain, const DevBuf<T> &sorted)
 {
     const bool st = stream_slab(F.Nd, F.mp, sizeof(T));
-    if (F.gsp && sorted.p) return EllT<T>{F.gpcol.p, sorted.p, F.mp, F.Sp, st, plain.p, true};
+    if (F.gsp && sorted.p) return EllT<T>{F.gpcol.p, sorted.p, F.mp, F.Sp, st, plain.p, true,
+                                                    32 - slot_bits(F.Sp, F.mp)};
     return EllT<T>{F.pcol.p, plain.p, F.mp, F.Sp, st};
 }
 

@@ -347,6 +347,27 @@ static uint64_t spread_bits(uint64_t v, int stride)
 // Levels above this many rows are ordered in Morton blocks of 4096 rows.
 constexpr int kBlockedRows = 262144;
 
+static int small_level_rows();
+
+// Slice height of a level's operator slabs.  Stencils of at most 36 entries:
+// 16.  Wider ones: 8 on levels small enough for the single-workgroup relax
+// and on large levels (bandwidth-bound passes over gather-sorted slabs); 4 on
+// the levels between, whose colour steps are latency-bound -- half as many
+// entries per wave shortens each wave's load chain (level-2 relaxation of the
+// 1M cube 441 instead of 520 us).  A large level whose padded rows exceed the
+// gather-sorted column range at 8 (above ~4.2M rows) takes 4, which keeps it
+// sortable.  MGM_SLICE_HEIGHT overrides all of this.
+static int level_slice_height(int m, int N, int ncol, int nblocks)
+{
+    const int base = slice_height(m);
+    if (getenv("MGM_SLICE_HEIGHT") || base != 8) return base;
+    if (N <= small_level_rows()) return 8;
+    if (N <= kBlockedRows) return 4;
+    const auto padded = [&](int S) { return (long)N + (long)S * ncol * nblocks; };
+    if (!slab_sortable(8, m, padded(8)) && slab_sortable(4, m, padded(4))) return 4;
+    return 8;
+}
+
 // Greedy colouring on the symmetrised stencil graph, in reference order, then
 // the device order: colour classes contiguous, Morton order inside a class.
 static void level_colour_order(Level &L, int dim)
@@ -418,7 +439,7 @@ static void level_colour_order(Level &L, int dim)
         if (block[a] != block[b]) return block[a] < block[b];
         return L.colour[a] < L.colour[b];
     });
-    L.S = slice_height(L.m);
+    L.S = level_slice_height(L.m, N, L.ncol, (N + kBlock - 1) / kBlock);
     const int S = L.S;
     std::vector<std::vector<int>> slices(L.ncol);
     L.perm.clear();

@@ -78,7 +78,8 @@ struct Level {
     Ell op(int which) const
     {
         if (gs && gslab[which].p)
-            return Ell{gcol.p, gslab[which].p, m, S, stream_slab(Nd, m, 8), slab[which].p, true};
+            return Ell{gcol.p, gslab[which].p, m, S, stream_slab(Nd, m, 8), slab[which].p, true,
+                       32 - slot_bits(S, m)};
         return op_plain(which);
     }
     // The operator in plain slice order (entry j of row r at ell_at(m, S, r, j)).
@@ -93,7 +94,8 @@ struct Level {
     Ell restr() const { return Ell{rcol.p, rw.p, mr, Sr, stream_slab(Nd, mr, 8)}; }
     Ell prol() const
     {
-        if (gsp) return Ell{gpcol.p, gpw.p, mp, Sp, stream_slab(Nd, mp, 8), pw.p, true};
+        if (gsp)
+            return Ell{gpcol.p, gpw.p, mp, Sp, stream_slab(Nd, mp, 8), pw.p, true, 32 - slot_bits(Sp, mp)};
         return Ell{pcol.p, pw.p, mp, Sp, stream_slab(Nd, mp, 8)};
     }
 };

@@ -101,17 +101,19 @@ __device__ inline void stage_products(const EllT<TW> &A, size_t base, int ri, in
         const int lane = jo * S + ri;
         const TW *wp = A.w + sb;
         const unsigned *cp = reinterpret_cast<const unsigned *>(A.col) + sb;
+        const int sh = A.shift;
+        const unsigned mask = (1u << sh) - 1;
         if (A.stream) {
 #pragma unroll 4
             for (int q = lane; q < n; q += 64) {
                 const unsigned v = ldnt(cp + q);
-                P[v >> kSlotShift] = wv(ldnt(wp + q)) * x[v & kColMask];
+                P[v >> sh] = wv(ldnt(wp + q)) * x[v & mask];
             }
         } else {
 #pragma unroll 4
             for (int q = lane; q < n; q += 64) {
                 const unsigned v = cp[q];
-                P[v >> kSlotShift] = wv(wp[q]) * x[v & kColMask];
+                P[v >> sh] = wv(wp[q]) * x[v & mask];
             }
         }
         return;
@@ -520,47 +522,47 @@ __global__ void __launch_bounds__(BLK) stream_copy_kernel(long n16, const uint4 
 }
 
 // Row-major reference-order rows -> sliced-ELL device-order slab.
-// One workgroup per slice: bitonic sort of the slice's (column << 10 | slot)
-// keys in LDS (padded to 1024 with keys above every real one), written back
-// as (slot << kSlotShift) | column.  Requires S*m <= 1024 and columns below
-// kColMask (slab_sortable).
+// One workgroup per slice: bitonic sort of the slice's (column << b | slot)
+// keys in LDS (b = slot_bits; padded to 1024 with keys above every real one),
+// written back as (slot << (32 - b)) | column.  Requires slab_sortable.
 __global__ void __launch_bounds__(GS_BLK)
-gather_sort_kernel(int m, int S, const int *__restrict__ col, int *__restrict__ gcol)
+gather_sort_kernel(int m, int S, int b, const int *__restrict__ col, int *__restrict__ gcol)
 {
     __shared__ unsigned key[1024];
     const int n = S * m;
     const size_t sb = (size_t)blockIdx.x * n;
     for (int i = threadIdx.x; i < 1024; i += GS_BLK)
-        key[i] = i < n ? ((unsigned)col[sb + i] << 10) | (unsigned)i : 0xFFFFFFFFu;
+        key[i] = i < n ? ((unsigned)col[sb + i] << b) | (unsigned)i : 0xFFFFFFFFu;
     __syncthreads();
     for (int k = 2; k <= 1024; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int i = threadIdx.x; i < 1024; i += GS_BLK) {
                 const int o = i ^ j;
                 if (o > i) {
-                    const unsigned a = key[i], b = key[o];
-                    if ((a > b) == ((i & k) == 0)) {
-                        key[i] = b;
+                    const unsigned a = key[i], c = key[o];
+                    if ((a > c) == ((i & k) == 0)) {
+                        key[i] = c;
                         key[o] = a;
                     }
                 }
             }
             __syncthreads();
         }
+    const unsigned smask = (1u << b) - 1;
     for (int q = threadIdx.x; q < n; q += GS_BLK) {
         const unsigned v = key[q];
-        gcol[sb + q] = (int)(((v & 1023u) << kSlotShift) | (v >> 10));
+        gcol[sb + q] = (int)(((v & smask) << (32 - b)) | (v >> b));
     }
 }
 
 __global__ void __launch_bounds__(BLK)
-permute_slab_kernel(long n, int per, const int *__restrict__ gcol, const double *__restrict__ w,
+permute_slab_kernel(long n, int per, int shift, const int *__restrict__ gcol, const double *__restrict__ w,
                     double *__restrict__ wg)
 {
     const long e = (long)blockIdx.x * BLK + threadIdx.x;
     if (e >= n) return;
     const long sb = e - e % per;
-    wg[e] = w[sb + ((unsigned)gcol[e] >> kSlotShift)];
+    wg[e] = w[sb + ((unsigned)gcol[e] >> shift)];
 }
 
 __global__ void __launch_bounds__(BLK)
@@ -658,13 +660,13 @@ int launch_gather_sort(int nslices, int m, int S, const int *col, int *gcol, hip
 {
     if (!slab_sortable(S, m, 0)) MGM_FAIL(-1, "gather_sort: slice of more than 1024 entries");
     if (nslices <= 0) return 0;
-    hipLaunchKernelGGL(gather_sort_kernel, dim3(nslices), dim3(GS_BLK), 0, s, m, S, col, gcol);
+    hipLaunchKernelGGL(gather_sort_kernel, dim3(nslices), dim3(GS_BLK), 0, s, m, S, slot_bits(S, m), col, gcol);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 int launch_permute_slab(long n, int m, int S, const int *gcol, const double *w, double *wg, hipStream_t s)
 {
-    LAUNCH(permute_slab_kernel, n, n, S * m, gcol, w, wg);
+    LAUNCH(permute_slab_kernel, n, n, S * m, 32 - slot_bits(S, m), gcol, w, wg);
     return 0;
 }
 
