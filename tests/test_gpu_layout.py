@@ -111,3 +111,28 @@ def test_time_steps_bitwise(pair):
             runs.append(P.ns_march(var, steps=2, precision=1))
         np.testing.assert_array_equal(runs[0]["errors"], runs[1]["errors"])
         same_fields(A, B, ("p", "pprime", "u", "v", "u_new", "v_new", "p_old"))
+
+
+def test_default_thresholds_bitwise(gpu, monkeypatch):
+    """The layout the 1M benchmark runs: level 0 above 262,144 rows gather-sorted
+    at slice height 8, the level below it at slice height 4, plain slabs below;
+    the same FP64 solve as with every slab plain."""
+    print("[gpu layout] 3D 66^3/33^3/17^3/9^3 at the default thresholds", flush=True)
+    monkeypatch.delenv("MGM_GATHER_SORT_ROWS", raising=False)
+    sizes = (66, 33, 17, 9)
+    runs = []
+    for sort in (False, True):
+        monkeypatch.setenv("MGM_GATHER_SORT", "1" if sort else "0")
+        cfg = gpu.config(dimension=3, poly_degree=3, coarse_poly_degree=3, phs_degree=3,
+                         cloud_size_multiplier=3.5, omega=1.0, num_relax=3,
+                         transfer_rows=gpu.ROWS_BY_TAG, sweep_order=gpu.ORDER_COLOUR)
+        P = gpu.Problem(cfg, [gpu.Points.generate(3, n, 1 + k, 0.2) for k, n in enumerate(sizes)]).setup()
+        assert P.info(0)["N"] > 262144
+        P.dirichlet_rows()
+        f = np.random.default_rng(4).standard_normal(P.info(0)["N"])
+        f[P.geometry(0)["btag"] != 0] = 0.0
+        P.set_field(0, "source", f)
+        r = P.solve(1e-8, 12)
+        runs.append((r, P.field(0, "p")))
+    np.testing.assert_array_equal(runs[0][0]["history"], runs[1][0]["history"])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
