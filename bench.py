@@ -267,8 +267,7 @@ def main():
                     help="set up, time the level-0 stencil apply and the streaming copy, print them, exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
-              "(the sharding design is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU; runs on more than one GPU are not supported.", file=sys.stderr)
         sys.exit(2)
 
     import multigrid_meshless_amd as mgm

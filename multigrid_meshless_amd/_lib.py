@@ -108,8 +108,7 @@ def lib():
     global _LIB
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
-                               "or python multigrid_meshless_amd/_build.py")
+            raise RuntimeError(f"{LIB_PATH} is not built; run python multigrid_meshless_amd/_build.py")
         L = C.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:
             fn = getattr(L, name)
