@@ -128,28 +128,39 @@ int set_weights(mgm_problem *pb, LearnedState &S, const float *w)
 namespace mgm {
 
 // The learned first sweep of level l: RES = s - A p (residual form 0), then
-// p += network correction on interior rows.
+// p += network correction on interior rows.  Inside mgm_solve level 0's RES
+// already holds that residual when a cycle starts (the loop's own fine
+// residual, same form, p unchanged since), so it is not formed again.
 int op_learned_sweep(mgm_problem *pb, int l)
 {
     LearnedState &S = *pb->ml;
     Level &L = pb->lv[l];
-    int rc = launch_residual(L.op(OP_LAPP), L.Nd, 0, L.dbtag.p, (const double *)L.f[MGM_F_SOURCE].p,
-                             (const double *)L.f[MGM_F_P].p, L.f[MGM_F_RES].p, stream());
-    if (rc) return rc;
+    if (!(l == 0 && pb->res0_current)) {
+        int rc = launch_residual(L.op(OP_LAPP), L.Nd, 0, L.dbtag.p, (const double *)L.f[MGM_F_SOURCE].p,
+                                 (const double *)L.f[MGM_F_P].p, L.f[MGM_F_RES].p, stream());
+        if (rc) return rc;
+    }
     return mlp_correct(L.Nd, L.op_plain(OP_LAPP), L.dbtag.p, S.G[l].p, S.W1r.p, S.W2.p, S.b2.p, S.w3.p, S.b3,
                        L.f[MGM_F_RES].p, L.f[MGM_F_P].p, stream());
 }
 
 // The same sweep inside the reduced-precision V-cycle: FP32 residual (res32
 // = src32 - A p32 with the level's narrowed weights) and FP32 correction.
+// Level 0 is relaxed once per correction, first, with p32 = 0: its residual
+// is src32 itself (the row sums of zero products are +0, and src32 is 0 on
+// boundary rows as residual form 0 leaves them), so src32 is used directly.
 int op_learned_sweep_mp(mgm_problem *pb, int l, int (*residual_mp)(Level &))
 {
     LearnedState &S = *pb->ml;
     Level &L = pb->lv[l];
-    int rc = residual_mp(L);
-    if (rc) return rc;
+    const float *r = L.src32.p;
+    if (l != 0) {
+        int rc = residual_mp(L);
+        if (rc) return rc;
+        r = L.res32.p;
+    }
     return mlp_correct(L.Nd, L.op_plain(OP_LAPP), L.dbtag.p, S.G[l].p, S.W1r.p, S.W2.p, S.b2.p, S.w3.p, S.b3,
-                       (const float *)L.res32.p, L.p32.p, stream());
+                       r, L.p32.p, stream());
 }
 
 bool learned_active(const mgm_problem *pb, int l, int variant)

@@ -134,6 +134,7 @@ struct mgm_problem {
     bool ns_set = false;
     int mp_built = 0;              // precision mode the FP32/BF16 copies hold (0: none / stale)
     bool mp_ns_built = false;      // FP32 lap copies of the time steps' pressure cycle are current
+    bool res0_current = false;     // level 0's RES holds s - lap_Poison p (form 0) whenever a cycle starts
     std::unique_ptr<mgm::LearnedState> ml;
     ~mgm_problem();
 };

@@ -102,6 +102,13 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
         }
         set_learned(ml_enabled);
     };
+    // Every cycle below starts right after a fine residual of the FP64 iterate;
+    // the flag holds until this call returns, on every path.
+    struct Res0Flag {
+        mgm_problem *pb;
+        ~Res0Flag() { pb->res0_current = false; }
+    } res0_flag{pb};
+    pb->res0_current = precision == 0 && variant == MGM_VAR_H;
     if (use_graph) {
         for (int g = 0; g < (learned ? 2 : 1); g++) {
             set_learned(g == 1);
