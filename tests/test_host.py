@@ -150,3 +150,38 @@ def test_write_history_formats(tmp_path):
         assert p.read_text() == "".join(fmt % (i, x) for i, x in enumerate(v))
     with pytest.raises(mgm.MgmError):
         mgm.write_history(str(tmp_path / "missing" / "conv.csv"), v)
+
+
+def test_c_abi_rejects_bad_arguments(tmp_path):
+    """Every entry point checks its arguments before any device work: bad
+    configurations, calls on a hierarchy that is not set up, out-of-range
+    levels and fields come back as MgmError with the library's message."""
+    import ctypes as C
+
+    import multigrid_meshless_amd as mgm
+    from multigrid_meshless_amd._lib import lib
+    pts = mgm.Points.generate(2, 12, 0)
+    for bad in (dict(dimension=4), dict(phs_degree=0), dict(poly_degree=0), dict(coarse_poly_degree=0)):
+        with pytest.raises(mgm.MgmError):
+            mgm.Problem(mgm.config(**bad), [pts])
+    with pytest.raises(mgm.MgmError):
+        mgm.Problem(mgm.config(), [])
+    with pytest.raises(KeyError):
+        mgm.config(no_such_field=1)
+    P = mgm.Problem(mgm.config(), [pts])          # created, never set up
+    calls = [lambda: P.relax(0), lambda: P.residual(0), lambda: P.vcycle(), lambda: P.solve(1e-8, 5),
+             lambda: P.field(0, "p"), lambda: P.set_field(0, "p", np.zeros(pts.info()["N"])),
+             lambda: P.info(0), lambda: P.matrix(0, "lap"), lambda: P.fs_step(),
+             lambda: P.write_solution(str(tmp_path / "s.csv"))]
+    for call in calls:
+        with pytest.raises(mgm.MgmError, match="not set up|level out of range"):
+            call()
+    assert not (tmp_path / "s.csv").exists()
+    h = C.c_void_p()
+    z = np.zeros(4)
+    t = np.zeros(4, np.uint8)
+    assert lib().mgm_points_from_arrays(4, 5, 0, 0.1, *[z.ctypes.data] * 6, t.ctypes.data, t.ctypes.data,
+                                        C.byref(h)) != 0
+    assert b"bad sizes" in lib().mgm_last_error()
+    assert lib().mgm_set_smoother(P._h, 2.5, 3) != 0 and b"omega" in lib().mgm_last_error()
+    assert lib().mgm_set_smoother(P._h, 1.0, -1) != 0
