@@ -392,14 +392,15 @@ correct_kernel(int Nd, Ell A, const uint8_t *__restrict__ btag, const float *__r
         s2 += __shfl_xor(s2, 32);
         const double sigma = sqrt(s2 / m);
         const double inv = sigma > 0 ? 1.0 / sigma : 0.0;
+        // the geometric part of this lane's four nodes nb + 4g .. nb + 4g + 3:
+        // one 16-byte load per column tile (Nd and nb are multiples of 4, so
+        // the four are all rows or all beyond Nd, and the load is aligned)
         f32x4 c[MF_NT];
+        const long n4 = nb + g * 4;
 #pragma unroll
         for (int t = 0; t < MF_NT; t++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const long node = nb + g * 4 + v;
-                c[t][v] = node < Nd ? G[(size_t)(t * 16 + li) * Nd + node] : 0.f;
-            }
+            c[t] = n4 < Nd ? *reinterpret_cast<const f32x4 *>(G + (size_t)(t * 16 + li) * Nd + n4)
+                           : (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int ks = 0; ks < MF_KS1; ks++) {
             bf16x8 a;
@@ -546,6 +547,7 @@ int mlp_correct(int Nd, Ell A, const uint8_t *btag, const float *G, const bf16 *
 {
     if (A.m > MLP_MAX_M) MGM_FAIL(-1, "learned smoother: stencil wider than supported");
     if (A.sorted) MGM_FAIL(-1, "learned smoother: needs the plain slab");
+    if (Nd % 4) MGM_FAIL(-1, "learned smoother: device rows must be a multiple of 4");
     const long tiles = ((long)Nd + 15) / 16, waves = (tiles + MF_TPW - 1) / MF_TPW;
     const int blocks = (int)((waves + MF_WAVES - 1) / MF_WAVES);
     if (blocks > 0) {
