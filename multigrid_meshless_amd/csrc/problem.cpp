@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 #include "problem.hpp"
 
@@ -239,6 +240,22 @@ static int level_knn(Level &L, int dim)
 }
 
 // rcm_reordering_with_boundarynodes (hf/mesh_functions.h:774-872).
+// f(i0, i1) over [0, N) in contiguous chunks on up to 16 host threads
+// (one thread below 65536 rows).
+template <typename F>
+static void parallel_rows(int N, F &&f)
+{
+    const int nt = N < 65536 ? 1 : (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (nt == 1) { f(0, N); return; }
+    std::vector<std::thread> th;
+    const int chunk = (N + nt - 1) / nt;
+    for (int t = 0; t < nt; t++) {
+        const int a = std::min(N, t * chunk), b = std::min(N, a + chunk);
+        th.emplace_back([&f, a, b] { f(a, b); });
+    }
+    for (auto &t : th) t.join();
+}
+
 static int level_bfs(Level &L)
 {
     const int N = L.N, m = L.m;
@@ -267,11 +284,13 @@ static int level_bfs(Level &L)
     }
     L.ctag = P.ctag;
     std::vector<int> nc((size_t)N * m);
-    for (int i = 0; i < N; i++) {
-        const int *src = &L.cloud[(size_t)q1[i] * m];
-        int *dst = &nc[(size_t)i * m];
-        for (int j = 0; j < m; j++) dst[j] = q2[src[j]];
-    }
+    parallel_rows(N, [&](int i0, int i1) {
+        for (int i = i0; i < i1; i++) {
+            const int *src = &L.cloud[(size_t)q1[i] * m];
+            int *dst = &nc[(size_t)i * m];
+            for (int j = 0; j < m; j++) dst[j] = q2[src[j]];
+        }
+    });
     L.cloud.swap(nc);
     return 0;
 }
@@ -368,11 +387,42 @@ static int level_slice_height(int m, int N, int ncol, int nblocks)
     return 8;
 }
 
+static void order_by_colour(Level &L, int dim);
+
 // Greedy colouring on the symmetrised stencil graph, in reference order, then
 // the device order: colour classes contiguous, Morton order inside a class.
 static void level_colour_order(Level &L, int dim)
 {
     const int N = L.N, m = L.m;
+    // Nodes are coloured in index order, each with the smallest colour that no
+    // earlier node adjacent to it in the symmetrised graph has: neither an
+    // earlier node of its own stencil nor an earlier node whose stencil holds
+    // it.  The second set is pushed forward as a bit mask when that node is
+    // coloured, which needs no transposed graph; with more than 64 colours the
+    // transposed-graph form below takes over (same colours).
+    L.colour.assign(N, -1);
+    {
+        std::vector<uint64_t> pushed(N, 0);
+        int maxc = 0, i = 0;
+        for (; i < N; i++) {
+            const int *cr = &L.cloud[(size_t)i * m];
+            uint64_t forb = pushed[i];
+            for (int j = 0; j < m; j++)
+                if (cr[j] < i) forb |= 1ull << L.colour[cr[j]];
+            if (~forb == 0) break;
+            const int col = __builtin_ctzll(~forb);
+            L.colour[i] = col;
+            maxc = std::max(maxc, col + 1);
+            const uint64_t bit = 1ull << col;
+            for (int j = 0; j < m; j++)
+                if (cr[j] > i) pushed[cr[j]] |= bit;
+        }
+        if (i == N) {
+            L.ncol = maxc;
+            order_by_colour(L, dim);
+            return;
+        }
+    }
     std::vector<int> deg(N + 1, 0);
     for (int i = 0; i < N; i++)
         for (int j = 0; j < m; j++) {
@@ -405,6 +455,14 @@ static void level_colour_order(Level &L, int dim)
         maxc = std::max(maxc, col + 1);
     }
     L.ncol = maxc;
+    order_by_colour(L, dim);
+}
+
+// Device order of a coloured level: Morton-ranked blocks, colour classes
+// inside a block, slices padded per (block, colour) chunk.
+static void order_by_colour(Level &L, int dim)
+{
+    const int N = L.N;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     const std::vector<double> *P[3] = {&L.x, &L.y, &L.z};
     for (int d = 0; d < 3; d++)
