@@ -52,8 +52,13 @@ def compile_driver(d, build="heat"):
     return exe
 
 
-def run_driver(exe, d):
-    r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=120)
+def run_driver(exe, d, order="natural"):
+    env = dict(os.environ)
+    if order == "colour":
+        env["MGM_SWEEP_ORDER"] = "colour"
+    else:
+        env.pop("MGM_SWEEP_ORDER", None)
+    r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=120, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     return r.stdout
 
@@ -82,18 +87,19 @@ def test_dropin_driver_compiles_and_links(build):
         assert os.path.isfile(compile_driver(d, build))
 
 
-def resident_problem(mgm, paths, omega, num_relax):
+def resident_problem(mgm, paths, omega, num_relax, order="natural"):
     """The hierarchy read_complete_mesh_data builds from flow_parameters.csv
     (degree 3 polynomials and PHS, multiplier 2; coarse levels degree 3; the
-    reference's transfer rows and natural sweep order)."""
+    reference's transfer rows; the natural sweep order, or the multicolour one
+    under MGM_SWEEP_ORDER=colour)."""
     cfg = mgm.config(dimension=2, poly_degree=3, coarse_poly_degree=3, phs_degree=3, cloud_size_multiplier=2.0,
                      omega=omega, num_relax=num_relax, transfer_rows=mgm.ROWS_REFERENCE,
-                     sweep_order=mgm.ORDER_NATURAL)
+                     sweep_order=mgm.ORDER_COLOUR if order == "colour" else mgm.ORDER_NATURAL)
     return mgm.Problem(cfg, [mgm.Points.read_gmsh(p, 2) for p in paths]).setup()
 
 
-def resident_cycles(mgm, paths, omega, num_vcycles, num_relax, source):
-    P = resident_problem(mgm, paths, omega, num_relax)
+def resident_cycles(mgm, paths, omega, num_vcycles, num_relax, source, order="natural"):
+    P = resident_problem(mgm, paths, omega, num_relax, order)
     geo = P.geometry(0)
     A = P.matrix(0, "lap_Poison")
     wall = (geo["btag"] != 0) & (geo["x"] == 0)
@@ -117,20 +123,21 @@ def resident_cycles(mgm, paths, omega, num_vcycles, num_relax, source):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("order", ["natural", "colour"])
 @pytest.mark.parametrize("sizes,seed,num_vcycles,num_relax", [((33, 17, 9), 5, 6, 3),
                                                               ((65, 33, 17, 9), 11, 4, 2)])
-def test_dropin_heat_driver_matches_resident_api(gpu, sizes, seed, num_vcycles, num_relax):
+def test_dropin_heat_driver_matches_resident_api(gpu, sizes, seed, num_vcycles, num_relax, order):
     mgm = gpu
     omega = 1.6
     with tempfile.TemporaryDirectory() as d:
         exe = compile_driver(d)
         paths = write_case(mgm, d, sizes, seed, omega, num_vcycles, num_relax)
-        print(f"[dropin] heat driver, levels {sizes}, seed {seed}", flush=True)
-        stdout = run_driver(exe, d)
+        print(f"[dropin] heat driver, levels {sizes}, seed {seed}, {order} order", flush=True)
+        stdout = run_driver(exe, d, order)
         assert stdout.count("Pressure Residual") == num_vcycles * (2 * len(sizes) - 2)
         out = np.fromfile(os.path.join(d, "out.bin"), dtype=np.float64).reshape(4, -1)
         x, y, source, p = out
-        P, geo, wall = resident_cycles(mgm, paths, omega, num_vcycles, num_relax, source)
+        P, geo, wall = resident_cycles(mgm, paths, omega, num_vcycles, num_relax, source, order)
 
     np.testing.assert_array_equal(x, geo["x"])
     np.testing.assert_array_equal(y, geo["y"])
@@ -143,20 +150,21 @@ NS_FIELDS = ("u", "v", "p", "u_new", "v_new", "p_old", "pprime")
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("order", ["natural", "colour"])
 @pytest.mark.parametrize("scheme", ["timple", "fs"])
-def test_dropin_ns_driver_matches_resident_api(gpu, scheme):
+def test_dropin_ns_driver_matches_resident_api(gpu, scheme, order):
     mgm = gpu
     sizes, seed, omega, num_relax, num_vcycles, steps = (33, 17, 9), 23, 1.2, 3, 4, 3
     ns = dict(Re=100, courant_number=0.5, iter_momentum=3, iter_timple=2)
     with tempfile.TemporaryDirectory() as d:
         exe = compile_driver(d, scheme)
         paths = write_case(mgm, d, sizes, seed, omega, num_vcycles, num_relax, num_time_steps=steps, **ns)
-        print(f"[dropin] ns driver {scheme}, levels {sizes}, seed {seed}, {steps} steps", flush=True)
-        stdout = run_driver(exe, d)
+        print(f"[dropin] ns driver {scheme}, levels {sizes}, seed {seed}, {steps} steps, {order} order", flush=True)
+        stdout = run_driver(exe, d, order)
         assert stdout.count("Steady state error") == steps
         rec = np.fromfile(os.path.join(d, "steps.bin"), dtype=np.float64)
         out = np.fromfile(os.path.join(d, "out.bin"), dtype=np.float64).reshape(9, -1)
-        P = resident_problem(mgm, paths, omega, num_relax)
+        P = resident_problem(mgm, paths, omega, num_relax, order)
 
     dt, errors = rec[0], rec[1:]
     geo = P.geometry(0)
