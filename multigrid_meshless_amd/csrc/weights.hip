@@ -387,58 +387,69 @@ lu_kernel(WeightsArgs a, int base)
         // Padding rows get l = 0, and columns past the right-hand sides hold
         // zeros in every row.  colk / rowp / sp alternate between two buffers,
         // so two barriers per step suffice.
-        for (int k = 0; k < mpn; k++) {
-            const int bf = k & 1;
-            const int kc = k / 16;
-            if (tj == k % 16) {
-                double best = -1;
-                int bi = NC;
+        // The step loop runs tile column by tile column (kc unrolled), so the
+        // pivot column is read from registers directly, the update is one
+        // fused multiply-add per element, and tile columns left of kc are
+        // skipped: their columns are eliminated and never read again (pivots
+        // come from columns >= k, the solution from the right-hand sides).
 #pragma unroll
-                for (int ai = 0; ai < TILE; ai++) {
-                    double v = A[ai][0];
+        for (int kc = 0; kc < TILE; kc++) {
+            for (int k16 = 0; k16 < 16; k16++) {
+                const int k = kc * 16 + k16;
+                if (k >= mpn) break;
+                const int bf = k & 1;
+                if (tj == k16) {
+                    double best = -1;
+                    int bi = NC;
 #pragma unroll
-                    for (int cj = 1; cj < TILE; cj++) v = (cj == kc) ? A[ai][cj] : v;
-                    const int i = ti + 16 * ai;
-                    if (i < mpn) {
-                        colk[bf][i] = v;
-                        const double av = pos[i] < k ? -1.0 : fabs(v);   // pivot rows are out
-                        if (av > best) { best = av; bi = i; }
+                    for (int ai = 0; ai < TILE; ai++) {
+                        const double v = A[ai][kc];
+                        const int i = ti + 16 * ai;
+                        if (i < mpn) {
+                            colk[bf][i] = v;
+                            const double av = pos[i] < k ? -1.0 : fabs(v);   // pivot rows are out
+                            if (av > best) { best = av; bi = i; }
+                        }
+                    }
+#pragma unroll
+                    for (int o = 8; o > 0; o >>= 1) {   // within the 16-lane group
+                        const double ob = __shfl_xor(best, o);
+                        const int oi = __shfl_xor(bi, o);
+                        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+                    }
+                    if (ti == 0) {
+                        if (!(best > 0)) s_bad = 1;
+                        sp[bf] = bi;
+                        pos[bi] = k;
                     }
                 }
+                __syncthreads();
+                if (s_bad) break;
+                const int p = sp[bf], pr = p / 16;
+                if (ti == p % 16) {
 #pragma unroll
-                for (int o = 8; o > 0; o >>= 1) {   // within the 16-lane group
-                    const double ob = __shfl_xor(best, o);
-                    const int oi = __shfl_xor(bi, o);
-                    if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+                    for (int cj = kc; cj < TILE; cj++) {
+                        double v = A[0][cj];
+#pragma unroll
+                        for (int ai = 1; ai < TILE; ai++) v = (ai == pr) ? A[ai][cj] : v;
+                        rowp[bf][tj + 16 * cj] = v;
+                    }
                 }
-                if (ti == 0) {
-                    if (!(best > 0)) s_bad = 1;
-                    sp[bf] = bi;
-                    pos[bi] = k;
+                if (tid == 0) diag[k] = colk[bf][p];
+                __syncthreads();
+                const double rinv = 1.0 / colk[bf][p];
+#pragma unroll
+                for (int ai = 0; ai < TILE; ai++) {
+                    const int i = ti + 16 * ai;
+                    const double l = (i < mpn && i != p) ? colk[bf][i] * rinv : 0.0;
+#pragma unroll
+                    for (int cj = kc; cj < TILE; cj++)
+                        A[ai][cj] = __builtin_fma(-l, rowp[bf][tj + 16 * cj], A[ai][cj]);
                 }
             }
-            __syncthreads();
+            // s_bad is written only before a step's first barrier and read after
+            // it, so every thread leaves both loops at the same step
             if (s_bad) break;
-            const int p = sp[bf], pr = p / 16;
-            if (ti == p % 16) {
-#pragma unroll
-                for (int cj = 0; cj < TILE; cj++) {
-                    double v = A[0][cj];
-#pragma unroll
-                    for (int ai = 1; ai < TILE; ai++) v = (ai == pr) ? A[ai][cj] : v;
-                    rowp[bf][tj + 16 * cj] = v;
-                }
-            }
-            if (tid == 0) diag[k] = colk[bf][p];
-            __syncthreads();
-            const double rinv = 1.0 / colk[bf][p];
-#pragma unroll
-            for (int ai = 0; ai < TILE; ai++) {
-                const int i = ti + 16 * ai;
-                const double l = (i < mpn && i != p) ? colk[bf][i] * rinv : 0.0;
-#pragma unroll
-                for (int cj = 0; cj < TILE; cj++) A[ai][cj] -= l * rowp[bf][tj + 16 * cj];
-            }
         }
         if (s_bad) {
             if (tid == 0) atomicOr(a.status, 1);
