@@ -425,15 +425,17 @@ lu_kernel(WeightsArgs a, int base)
                 }
                 __syncthreads();
                 if (s_bad) break;
-                const int p = sp[bf], pr = p / 16;
+                const int p = sp[bf];
+                // the pivot's tile row is the same for every thread: a scalar,
+                // so the row is read straight from its registers
+                const int pr = __builtin_amdgcn_readfirstlane(p / 16);
                 if (ti == p % 16) {
 #pragma unroll
-                    for (int cj = kc; cj < TILE; cj++) {
-                        double v = A[0][cj];
+                    for (int ai = 0; ai < TILE; ai++)
+                        if (ai == pr) {
 #pragma unroll
-                        for (int ai = 1; ai < TILE; ai++) v = (ai == pr) ? A[ai][cj] : v;
-                        rowp[bf][tj + 16 * cj] = v;
-                    }
+                            for (int cj = kc; cj < TILE; cj++) rowp[bf][tj + 16 * cj] = A[ai][cj];
+                        }
                 }
                 if (tid == 0) diag[k] = colk[bf][p];
                 __syncthreads();
