@@ -349,8 +349,10 @@ lu_kernel(WeightsArgs a, int base)
     const double sx = cx[0], sy = cy[0], sz = cz[0];
     for (int q0 = 0; q0 < total; q0 += cap) {
         const int nrhs = (total - q0 < cap) ? total - q0 : cap;
-        // pivot positions: NC = row not yet used as pivot
+        // pivot positions: NC = row not yet used as pivot; the pivot-column
+        // buffers hold 0 on padding rows (never written by the search)
         for (int i = tid; i < NC; i += LU_T) pos[i] = NC;
+        for (int i = tid; i < 2 * NC; i += LU_T) colk[i / NC][i % NC] = 0.0;
         // output rows and target points of this pass
         for (int r = tid; r < nrhs; r += LU_T) {
             const int q = q0 + r;
@@ -437,13 +439,15 @@ lu_kernel(WeightsArgs a, int base)
                             for (int cj = kc; cj < TILE; cj++) rowp[bf][tj + 16 * cj] = A[ai][cj];
                         }
                 }
-                if (tid == 0) diag[k] = colk[bf][p];
+                if (tid == 0) {   // the pivot's own multiplier is 0: its row stays
+                    diag[k] = colk[bf][p];
+                    colk[bf][p] = 0.0;
+                }
                 __syncthreads();
-                const double rinv = 1.0 / colk[bf][p];
+                const double rinv = 1.0 / diag[k];
 #pragma unroll
                 for (int ai = 0; ai < TILE; ai++) {
-                    const int i = ti + 16 * ai;
-                    const double l = (i < mpn && i != p) ? colk[bf][i] * rinv : 0.0;
+                    const double l = colk[bf][ti + 16 * ai] * rinv;
 #pragma unroll
                     for (int cj = kc; cj < TILE; cj++)
                         A[ai][cj] = __builtin_fma(-l, rowp[bf][tj + 16 * cj], A[ai][cj]);
