@@ -33,6 +33,9 @@ constexpr int SMALL_BLK_MAX = 1024;
 constexpr int GS_BLK = 256;   // gather_sort_kernel workgroup
 // dynamic LDS of the single-workgroup relax: gfx950 gives a workgroup 160 KB
 constexpr size_t kSmallLdsMax = 160 * 1024 - 1024;
+// Staging slots per row in relax_small_kernel: the wider of the two
+// operators, and at least the MT slots its sums read.
+__host__ __device__ inline int small_slots(int mw, int mt) { return mw > mt ? mw : mt; }
 
 inline int nblk(long n) { return (int)((n + BLK - 1) / BLK); }
 
@@ -290,29 +293,49 @@ __device__ inline void load_slice(const Ell &A, int slice, int ri, int jo, doubl
 
 // One SOR update of a slice's rows, p, source and tags in LDS, the slice's
 // weights and columns already in registers (wq[0] of lane jo == 0 is the
-// diagonal).  Same arithmetic as slice_rows in mode M_SOR.
-template <int S, int NIT>
-__device__ inline void sor_slice_lds(const Ell &A, int slice, int nrows, const uint8_t *tg, const double *sr,
-                                     double *xs, double w, double c, const double *wq, const int *cq,
-                                     double *P)
+// diagonal).  Same arithmetic as slice_rows in mode M_SOR, in two halves so
+// the caller can issue the loads of its next slice in between.
+//
+// sor_stage writes the products of entries j < m, and +0.0 into the slots
+// m <= j < MT, to the wave's LDS region (MT slots of S rows); it returns the
+// diagonal.
+template <int S, int NIT, int MT>
+__device__ inline double sor_stage(int slice, int nrows, int m, const double *xs, const double *wq,
+                                   const int *cq, double *P)
 {
     constexpr int LPR = 64 / S;
     const int lane = threadIdx.x & 63;
     const int ri = lane % S, jo = lane / S;
-    const int m = A.m;
-    const int r = slice * S + ri;
-    const bool inr = r < nrows;
+    const bool inr = slice * S + ri < nrows;
     double xv[NIT];
 #pragma unroll
     for (int t = 0; t < NIT; t++)
         if (inr && t * LPR + jo < m) xv[t] = xs[cq[t]];
 #pragma unroll
     for (int t = 0; t < NIT; t++)
-        if (inr && t * LPR + jo < m) P[(t * LPR + jo) * S + ri] = wq[t] * xv[t];
+        if (inr && t * LPR + jo < MT) P[(t * LPR + jo) * S + ri] = t * LPR + jo < m ? wq[t] * xv[t] : 0.0;
+    return wq[0];
+}
+
+// sor_finish sums slots 1..MT-1 of each row in order and updates the row.  The
+// sum has a fixed trip count, so every LDS read can be issued ahead of the
+// adds.  It starts at +0.0, and a round-to-nearest sum that starts at +0.0 is
+// never -0.0, so adding the +0.0 slots past m leaves it unchanged: the result
+// is bitwise the sum over 1 <= j < m.
+template <int S, int MT>
+__device__ inline void sor_finish(int slice, int nrows, const uint8_t *tg, const double *sr, double *xs,
+                                  double w, double c, double dg, const double *P)
+{
+    const int lane = threadIdx.x & 63;
+    const int ri = lane % S, jo = lane / S;
+    const int r = slice * S + ri;
     wave_sync();
-    if (jo == 0 && inr && tg[r] != ROW_PAD) {
-        const double sum = ordered_sum<false>(P, S, ri, 1, m, 0.0);
-        xs[r] = w * ((sr[r] - sum) / wq[0]) + c * xs[r];
+    if (jo == 0 && r < nrows && tg[r] != ROW_PAD) {
+        const double *q = P + ri;
+        double sum = 0.0;
+#pragma unroll
+        for (int j = 1; j < MT; j++) sum += q[j * S];
+        xs[r] = w * ((sr[r] - sum) / dg) + c * xs[r];
     }
     wave_sync();
 }
@@ -325,7 +348,7 @@ __device__ inline void sor_slice_lds(const Ell &A, int slice, int nrows, const u
 // its first slice of the next colour step before the barrier, so a step costs
 // the LDS gathers and the ordered sums.  Identical arithmetic to the
 // per-colour launches.
-template <int S, bool PIN>
+template <int S, bool PIN, int MT>
 __global__ void __launch_bounds__(SMALL_BLK_MAX)
 relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int *__restrict__ col_soff,
                    int ncol, int nrows, const uint8_t *__restrict__ btag, const int *__restrict__ ref0,
@@ -340,7 +363,7 @@ relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int 
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int ri = lane % S, jo = lane / S;
-    const int mw = A.m > A2.m ? A.m : A2.m;
+    const int mw = small_slots(A.m > A2.m ? A.m : A2.m, MT);
     const int nlist = col_soff[ncol];
     double *xs = reinterpret_cast<double *>(lds_raw);                 // [nrows] p
     double *sr = xs + nrows;                                          // [nrows] source
@@ -357,11 +380,11 @@ relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int 
     for (int k = threadIdx.x; k <= ncol; k += blockDim.x) soff[k] = col_soff[k];
     for (int k = threadIdx.x; k < nlist; k += blockDim.x) slist[k] = col_slices[k];
     __syncthreads();
-    const bool regs = A.m <= NIT * (64 / S);
+    static_assert(MT <= NIT * (64 / S), "a lane's registers hold the MT slots of its rows");
     double wq[NIT];
     int cq[NIT];
     auto first_slice = [&](int cl) { const int k = soff[cl] + wave; return k < soff[cl + 1] ? slist[k] : -1; };
-    if (regs && num_relax > 0) {
+    if (num_relax > 0) {
         const int f = first_slice(0);
         if (f >= 0) load_slice<S, NIT>(A, f, ri, jo, wq, cq);
     }
@@ -369,21 +392,19 @@ relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int 
     for (int it = 0; it < num_relax; it++) {
         for (int cl = 0; cl < ncol; cl++) {
             const int s0 = soff[cl], s1 = soff[cl + 1];
-            for (int k = s0 + wave; k < s1; k += NW) {
-                if (!regs) {
-                    slice_rows<S, double, double>(A, slist[k], nrows, M_SOR, tg, sr, xs, xs, w, c, 0, P);
-                    continue;
-                }
-                if (k != s0 + wave) load_slice<S, NIT>(A, slist[k], ri, jo, wq, cq);
-                sor_slice_lds<S, NIT>(A, slist[k], nrows, tg, sr, xs, w, c, wq, cq, P);
-            }
-            // this wave's first slice of the next colour step, before the barrier
+            // the first slice of the next colour step is loaded ahead too
             // (across a sweep boundary only when no pin/residual pass intervenes)
-            const int ncl = cl + 1 < ncol ? cl + 1 : 0;
-            if (regs && (cl + 1 < ncol || (it + 1 < num_relax && !PIN))) {
-                const int f = first_slice(ncl);
-                if (f >= 0) load_slice<S, NIT>(A, f, ri, jo, wq, cq);
+            const bool ahead = cl + 1 < ncol || (it + 1 < num_relax && !PIN);
+            const int fnext = ahead ? first_slice(cl + 1 < ncol ? cl + 1 : 0) : -1;
+            for (int k = s0 + wave; k < s1; k += NW) {
+                const int sl = slist[k];
+                const double dg = sor_stage<S, NIT, MT>(sl, nrows, A.m, xs, wq, cq, P);
+                // this wave's next slice: the registers are free once staged
+                const int nx = k + NW < s1 ? slist[k + NW] : fnext;
+                if (nx >= 0) load_slice<S, NIT>(A, nx, ri, jo, wq, cq);
+                sor_finish<S, MT>(sl, nrows, tg, sr, xs, w, c, dg, P);
             }
+            if (s0 + wave >= s1 && fnext >= 0) load_slice<S, NIT>(A, fnext, ri, jo, wq, cq);
             __syncthreads();
         }
         if constexpr (!PIN) continue;   // variant H: no pin, no residual pass
@@ -397,7 +418,7 @@ relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int 
         const int form = (variant == MGM_VAR_T) ? M_RES1 : M_RES2;
         for (long k = wave; k < nsl; k += NW)
             slice_rows<S, double, double>(R, k, nrows, form, tg, sr, xs, res, 0.0, 0.0, 0, P);
-        if (regs && it + 1 < num_relax) {
+        if (it + 1 < num_relax) {
             const int f = first_slice(0);
             if (f >= 0) load_slice<S, NIT>(A, f, ri, jo, wq, cq);
         }
@@ -696,6 +717,13 @@ int launch_sor_slices(EllT<TW> A, const int *slices, int first, int nslices, con
 // LDS bytes of relax_small_kernel with nw waves: p, source, staging, the
 // colour tables (ncol + 1 + nrows/S ints, bounded by 2*nrows + 1) and the
 // tags; 0 if it does not fit
+// Stencil width the sums of relax_small_kernel are unrolled for: 24, 40 or 72
+// (at most 36 at S = 16, where a lane holds 9 entries of 4 per row).
+static int small_mt(int m, int S)
+{
+    const int mt = m <= 24 ? 24 : m <= 40 ? 40 : 72;
+    return S == 16 && mt > 36 ? 36 : mt;
+}
 static size_t relax_small_lds(int nrows, int mw, int S, int nw)
 {
     size_t b = (2 * (size_t)nrows + (size_t)nw * mw * S) * sizeof(double) +
@@ -710,9 +738,10 @@ static int relax_small_waves(int nrows, int mw, int S)
 }
 bool relax_small_fits(int nrows, int m, int m2, int S)
 {
-    return relax_small_waves(nrows, m > m2 ? m : m2, S) > 0;
+    if (m > small_mt(m, S)) return false;
+    return relax_small_waves(nrows, small_slots(m > m2 ? m : m2, small_mt(m, S)), S) > 0;
 }
-template <int S, bool PIN>
+template <int S, bool PIN, int MT>
 static int launch_relax_small_v(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol,
                                 int nrows, const uint8_t *btag, const int *ref0, const double *src,
                                 double *p, double *res, double w, double c, int num_relax, int variant,
@@ -720,28 +749,44 @@ static int launch_relax_small_v(Ell A, Ell A2, const int *col_slices, const int 
 {
     static bool attr = false;
     if (!attr) {
-        HIP_TRY(hipFuncSetAttribute((const void *)relax_small_kernel<S, PIN>,
+        HIP_TRY(hipFuncSetAttribute((const void *)relax_small_kernel<S, PIN, MT>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSmallLdsMax));
         attr = true;
     }
-    hipLaunchKernelGGL((relax_small_kernel<S, PIN>), dim3(1), dim3(64 * nw), shm, s, A, A2, col_slices,
+    hipLaunchKernelGGL((relax_small_kernel<S, PIN, MT>), dim3(1), dim3(64 * nw), shm, s, A, A2, col_slices,
                        col_soff, ncol, nrows, btag, ref0, src, p, res, w, c, num_relax, variant);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 // The H variant has no pin or residual pass: its own instance, without their
 // registers.
+template <int S, int MT>
+static int launch_relax_small_m(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol,
+                                int nrows, const uint8_t *btag, const int *ref0, const double *src,
+                                double *p, double *res, double w, double c, int num_relax, int variant,
+                                int nw, size_t shm, hipStream_t s)
+{
+    if (variant == MGM_VAR_H)
+        return launch_relax_small_v<S, false, MT>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p,
+                                                  res, w, c, num_relax, variant, nw, shm, s);
+    return launch_relax_small_v<S, true, MT>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p, res,
+                                             w, c, num_relax, variant, nw, shm, s);
+}
 template <int S>
 static int launch_relax_small_t(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol,
                                 int nrows, const uint8_t *btag, const int *ref0, const double *src,
                                 double *p, double *res, double w, double c, int num_relax, int variant,
                                 int nw, size_t shm, hipStream_t s)
 {
-    if (variant == MGM_VAR_H)
-        return launch_relax_small_v<S, false>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p, res,
-                                              w, c, num_relax, variant, nw, shm, s);
-    return launch_relax_small_v<S, true>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p, res, w,
-                                         c, num_relax, variant, nw, shm, s);
+    switch (small_mt(A.m, S)) {
+    case 24: return launch_relax_small_m<S, 24>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p,
+                                                res, w, c, num_relax, variant, nw, shm, s);
+    case 40: return launch_relax_small_m<S, S == 16 ? 36 : 40>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p,
+                                                res, w, c, num_relax, variant, nw, shm, s);
+    }
+    constexpr int MTMAX = S == 16 ? 36 : 72;
+    return launch_relax_small_m<S, MTMAX>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p, res,
+                                          w, c, num_relax, variant, nw, shm, s);
 }
 int launch_relax_small(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol, int nrows,
                        const uint8_t *btag, const int *ref0, const double *src, double *p, double *res,
@@ -749,7 +794,8 @@ int launch_relax_small(Ell A, Ell A2, const int *col_slices, const int *col_soff
 {
     if (A.S != A2.S) MGM_FAIL(-1, "relax_small: operators differ in slice height");
     if (A.sorted || A2.sorted) MGM_FAIL(-1, "relax_small: needs plain slabs");
-    const int mw = A.m > A2.m ? A.m : A2.m;
+    if (A.m > small_mt(A.m, A.S)) MGM_FAIL(-1, "relax_small: stencil too wide");
+    const int mw = small_slots(A.m > A2.m ? A.m : A2.m, small_mt(A.m, A.S));
     const int nw = relax_small_waves(nrows, mw, A.S);
     if (!nw) MGM_FAIL(-1, "relax_small: level does not fit in LDS");
     const size_t shm = relax_small_lds(nrows, mw, A.S, nw);
