@@ -189,13 +189,16 @@ int launch_sor_slices(EllT<TW> A, const int *slices, int first, int nslices, con
                       const TV *src, TV *p, TV w, TV c, hipStream_t s);
 // The whole relaxation of a small level (all sweeps, all colour classes, and
 // for the T / F variants the pin and residual after each sweep, A2 being the
-// operator of the F residual) in one single-workgroup launch.
-int launch_relax_small(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol, int nrows,
-                       const uint8_t *btag, const int *ref0, const double *src, double *p, double *res,
-                       double w, double c, int num_relax, int variant, hipStream_t s);
+// operator of the F residual) in one single-workgroup launch.  Weights TW
+// (double, float, or bf16 for the H variant), vectors and sums TV.
+template <typename TW, typename TV>
+int launch_relax_small(EllT<TW> A, EllT<TW> A2, const int *col_slices, const int *col_soff, int ncol, int nrows,
+                       const uint8_t *btag, const int *ref0, const TV *src, TV *p, TV *res, TV w, TV c,
+                       int num_relax, int variant, hipStream_t s);
 // Whether a level of nrows device rows and stencil widths m, m2 fits the
-// single-workgroup relax (its p vector and staging in one workgroup's LDS).
-bool relax_small_fits(int nrows, int m, int m2, int S);
+// single-workgroup relax (its vectors, of vbytes per value, and staging in
+// one workgroup's LDS).
+bool relax_small_fits(int nrows, int m, int m2, int S, int vbytes = 8);
 int launch_sor_natural(Ell A, int N, const int *iperm, const double *src, double *p, double w,
                        double c, hipStream_t s);
 // residual forms: 0 = interior, res = s - sum (boundary res = 0)            (H/T)

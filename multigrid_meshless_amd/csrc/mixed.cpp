@@ -15,6 +15,7 @@
 namespace mgm {
 
 bool learned_active(const mgm_problem *pb, int l, int variant);
+int small_level_rows();
 int op_learned_sweep_mp(mgm_problem *pb, int l, int (*residual_mp)(Level &));
 
 namespace {
@@ -38,6 +39,20 @@ EllT<T> narrowed_prol(const Level &F, const DevBuf<T> &plain, const DevBuf<T> &s
     return EllT<T>{F.pcol.p, plain.p, F.mp, F.Sp, st};
 }
 
+// A level's narrowed operator in the plain layout (the single-workgroup relax
+// reads it so).
+template <typename T>
+EllT<T> plain_op(const Level &L, const DevBuf<T> &w)
+{
+    return EllT<T>{L.col.p, w.p, L.m, L.S, stream_slab(L.Nd, L.m, sizeof(T))};
+}
+
+// Small levels relax in one single-workgroup launch, as in the FP64 cycle.
+bool small_relax(const Level &L)
+{
+    return L.Nd <= small_level_rows() && relax_small_fits(L.Nd, L.m, L.m, L.S, sizeof(float));
+}
+
 template <typename F>
 int with_op(const Level &L, F &&f)
 {
@@ -59,6 +74,17 @@ int relax_mp(mgm_problem *pb, int l)
         int rc = op_learned_sweep_mp(pb, l, residual_mp);
         if (rc) return rc;
         it0 = 1;
+    }
+    if (small_relax(L) && pb->cfg.num_relax > it0) {
+        const int n = pb->cfg.num_relax - it0;
+        if (L.w16) {
+            const EllT<bf16> A = plain_op(L, L.lapp16);
+            return launch_relax_small(A, A, L.d_col_slices.p, L.d_col_soff.p, L.ncol, L.Nd, L.dbtag.p, L.ref0.p,
+                                      (const float *)L.src32.p, L.p32.p, L.res32.p, w, c, n, MGM_VAR_H, stream());
+        }
+        const EllT<float> A = plain_op(L, L.lapp32);
+        return launch_relax_small(A, A, L.d_col_slices.p, L.d_col_soff.p, L.ncol, L.Nd, L.dbtag.p, L.ref0.p,
+                                  (const float *)L.src32.p, L.p32.p, L.res32.p, w, c, n, MGM_VAR_H, stream());
     }
     return with_op(L, [&](auto A) {
         for (int it = it0; it < pb->cfg.num_relax; it++)
@@ -120,6 +146,11 @@ int relax_ns_mp(mgm_problem *pb, int l, int variant)
     Level &L = pb->lv[l];
     const float w = pb->cfg.omega;
     const float c = 1.0f - pb->cfg.omega;
+    // (level 0 of variant F has its own residual source: per-colour launches)
+    if (small_relax(L) && !(variant == MGM_VAR_F && l == 0) && pb->cfg.num_relax > 0)
+        return launch_relax_small(plain_op(L, L.lapp32), plain_op(L, L.lap32), L.d_col_slices.p, L.d_col_soff.p,
+                                  L.ncol, L.Nd, L.dbtag.p, L.ref0.p, (const float *)L.src32.p, L.p32.p, L.res32.p,
+                                  w, c, pb->cfg.num_relax, variant, stream());
     const EllT<float> A = op32(L, L.lapp32, L.glapp32);
     for (int it = 0; it < pb->cfg.num_relax; it++) {
         int rc;

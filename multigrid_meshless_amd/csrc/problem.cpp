@@ -366,7 +366,7 @@ static uint64_t spread_bits(uint64_t v, int stride)
 // Levels above this many rows are ordered in Morton blocks of 4096 rows.
 constexpr int kBlockedRows = 262144;
 
-static int small_level_rows();
+int small_level_rows();
 
 // Slice height of a level's operator slabs.  Stencils of at most 36 entries:
 // 16.  Wider ones: 8 on levels small enough for the single-workgroup relax
@@ -1171,7 +1171,7 @@ namespace mgm {
 
 // Levels with at most this many device rows (and whose p vector fits in LDS)
 // relax in one single-workgroup launch; MGM_SMALL_ROWS overrides, 0 disables.
-static int small_level_rows()
+int small_level_rows()
 {
     const char *e = getenv("MGM_SMALL_ROWS");
     return e ? atoi(e) : 12000;

@@ -275,18 +275,18 @@ rows_kernel(EllT<TW> A, const int *__restrict__ slices, int first, int nslices, 
 }
 
 // Weights and columns of this lane's entries (j = t*LPR + jo) of one slice.
-template <int S, int NIT>
-__device__ inline void load_slice(const Ell &A, int slice, int ri, int jo, double *wq, int *cq)
+template <int S, int NIT, typename TW, typename TV>
+__device__ inline void load_slice(const EllT<TW> &A, int slice, int ri, int jo, TV *wq, int *cq)
 {
     constexpr int LPR = 64 / S;
     // entry j = t*LPR + jo of row ri sits at slice*S*m + j*S + ri = lane base + t*64
     const size_t off = (size_t)slice * S * A.m + (size_t)jo * S + ri;
-    const double *wp = A.w + off;
+    const TW *wp = A.w + off;
     const int *cp = A.col + off;
 #pragma unroll
     for (int t = 0; t < NIT; t++)
         if (t * LPR + jo < A.m) {
-            wq[t] = wp[t * 64];
+            wq[t] = wv(wp[t * 64]);
             cq[t] = cp[t * 64];
         }
 }
@@ -299,21 +299,20 @@ __device__ inline void load_slice(const Ell &A, int slice, int ri, int jo, doubl
 // sor_stage writes the products of entries j < m, and +0.0 into the slots
 // m <= j < MT, to the wave's LDS region (MT slots of S rows); it returns the
 // diagonal.
-template <int S, int NIT, int MT>
-__device__ inline double sor_stage(int slice, int nrows, int m, const double *xs, const double *wq,
-                                   const int *cq, double *P)
+template <int S, int NIT, int MT, typename TV>
+__device__ inline TV sor_stage(int slice, int nrows, int m, const TV *xs, const TV *wq, const int *cq, TV *P)
 {
     constexpr int LPR = 64 / S;
     const int lane = threadIdx.x & 63;
     const int ri = lane % S, jo = lane / S;
     const bool inr = slice * S + ri < nrows;
-    double xv[NIT];
+    TV xv[NIT];
 #pragma unroll
     for (int t = 0; t < NIT; t++)
         if (inr && t * LPR + jo < m) xv[t] = xs[cq[t]];
 #pragma unroll
     for (int t = 0; t < NIT; t++)
-        if (inr && t * LPR + jo < MT) P[(t * LPR + jo) * S + ri] = t * LPR + jo < m ? wq[t] * xv[t] : 0.0;
+        if (inr && t * LPR + jo < MT) P[(t * LPR + jo) * S + ri] = t * LPR + jo < m ? wq[t] * xv[t] : TV(0);
     return wq[0];
 }
 
@@ -322,17 +321,17 @@ __device__ inline double sor_stage(int slice, int nrows, int m, const double *xs
 // adds.  It starts at +0.0, and a round-to-nearest sum that starts at +0.0 is
 // never -0.0, so adding the +0.0 slots past m leaves it unchanged: the result
 // is bitwise the sum over 1 <= j < m.
-template <int S, int MT>
-__device__ inline void sor_finish(int slice, int nrows, const uint8_t *tg, const double *sr, double *xs,
-                                  double w, double c, double dg, const double *P)
+template <int S, int MT, typename TV>
+__device__ inline void sor_finish(int slice, int nrows, const uint8_t *tg, const TV *sr, TV *xs, TV w, TV c,
+                                  TV dg, const TV *P)
 {
     const int lane = threadIdx.x & 63;
     const int ri = lane % S, jo = lane / S;
     const int r = slice * S + ri;
     wave_sync();
     if (jo == 0 && r < nrows && tg[r] != ROW_PAD) {
-        const double *q = P + ri;
-        double sum = 0.0;
+        const TV *q = P + ri;
+        TV sum = TV(0);
 #pragma unroll
         for (int j = 1; j < MT; j++) sum += q[j * S];
         xs[r] = w * ((sr[r] - sum) / dg) + c * xs[r];
@@ -348,12 +347,11 @@ __device__ inline void sor_finish(int slice, int nrows, const uint8_t *tg, const
 // its first slice of the next colour step before the barrier, so a step costs
 // the LDS gathers and the ordered sums.  Identical arithmetic to the
 // per-colour launches.
-template <int S, bool PIN, int MT>
+template <int S, bool PIN, int MT, typename TW, typename TV>
 __global__ void __launch_bounds__(SMALL_BLK_MAX)
-relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int *__restrict__ col_soff,
+relax_small_kernel(EllT<TW> A, EllT<TW> A2, const int *__restrict__ col_slices, const int *__restrict__ col_soff,
                    int ncol, int nrows, const uint8_t *__restrict__ btag, const int *__restrict__ ref0,
-                   const double *__restrict__ src, double *p, double *res, double w, double c,
-                   int num_relax, int variant)
+                   const TV *__restrict__ src, TV *p, TV *res, TV w, TV c, int num_relax, int variant)
 {
     // register slots per lane: stencils up to 72 wide (36 at S = 16, the
     // slice height used only for stencils of at most 36 entries)
@@ -365,13 +363,13 @@ relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int 
     const int ri = lane % S, jo = lane / S;
     const int mw = small_slots(A.m > A2.m ? A.m : A2.m, MT);
     const int nlist = col_soff[ncol];
-    double *xs = reinterpret_cast<double *>(lds_raw);                 // [nrows] p
-    double *sr = xs + nrows;                                          // [nrows] source
-    double *P = sr + nrows + (size_t)wave * mw * S;                   // [NW][mw][S]
+    TV *xs = reinterpret_cast<TV *>(lds_raw);                         // [nrows] p
+    TV *sr = xs + nrows;                                              // [nrows] source
+    TV *P = sr + nrows + (size_t)wave * mw * S;                       // [NW][mw][S]
     int *soff = reinterpret_cast<int *>(sr + nrows + (size_t)NW * mw * S);  // [ncol+1]
     int *slist = soff + ncol + 1;                                     // [nlist]
     uint8_t *tg = reinterpret_cast<uint8_t *>(slist + nlist);         // [nrows]
-    __shared__ double pref;
+    __shared__ TV pref;
     for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
         xs[r] = p[r];
         sr[r] = src[r];
@@ -381,7 +379,7 @@ relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int 
     for (int k = threadIdx.x; k < nlist; k += blockDim.x) slist[k] = col_slices[k];
     __syncthreads();
     static_assert(MT <= NIT * (64 / S), "a lane's registers hold the MT slots of its rows");
-    double wq[NIT];
+    TV wq[NIT];
     int cq[NIT];
     auto first_slice = [&](int cl) { const int k = soff[cl] + wave; return k < soff[cl + 1] ? slist[k] : -1; };
     if (num_relax > 0) {
@@ -398,7 +396,7 @@ relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int 
             const int fnext = ahead ? first_slice(cl + 1 < ncol ? cl + 1 : 0) : -1;
             for (int k = s0 + wave; k < s1; k += NW) {
                 const int sl = slist[k];
-                const double dg = sor_stage<S, NIT, MT>(sl, nrows, A.m, xs, wq, cq, P);
+                const TV dg = sor_stage<S, NIT, MT>(sl, nrows, A.m, xs, wq, cq, P);
                 // this wave's next slice: the registers are free once staged
                 const int nx = k + NW < s1 ? slist[k + NW] : fnext;
                 if (nx >= 0) load_slice<S, NIT>(A, nx, ri, jo, wq, cq);
@@ -410,14 +408,14 @@ relax_small_kernel(Ell A, Ell A2, const int *__restrict__ col_slices, const int 
         if constexpr (!PIN) continue;   // variant H: no pin, no residual pass
         if (threadIdx.x == 0) pref = xs[*ref0];
         __syncthreads();
-        const double pr = pref;
+        const TV pr = pref;
         for (int r = threadIdx.x; r < nrows; r += blockDim.x)
             if (tg[r] != ROW_PAD) xs[r] = xs[r] - pr;
         __syncthreads();
-        const Ell &R = (variant == MGM_VAR_T) ? A : A2;
+        const EllT<TW> &R = (variant == MGM_VAR_T) ? A : A2;
         const int form = (variant == MGM_VAR_T) ? M_RES1 : M_RES2;
         for (long k = wave; k < nsl; k += NW)
-            slice_rows<S, double, double>(R, k, nrows, form, tg, sr, xs, res, 0.0, 0.0, 0, P);
+            slice_rows<S, TW, TV>(R, k, nrows, form, tg, sr, xs, res, TV(0), TV(0), 0, P);
         if (it + 1 < num_relax) {
             const int f = first_slice(0);
             if (f >= 0) load_slice<S, NIT>(A, f, ri, jo, wq, cq);
@@ -724,59 +722,61 @@ static int small_mt(int m, int S)
     const int mt = m <= 24 ? 24 : m <= 40 ? 40 : 72;
     return S == 16 && mt > 36 ? 36 : mt;
 }
-static size_t relax_small_lds(int nrows, int mw, int S, int nw)
+static size_t relax_small_lds(int nrows, int mw, int S, int nw, int vbytes)
 {
-    size_t b = (2 * (size_t)nrows + (size_t)nw * mw * S) * sizeof(double) +
+    size_t b = (2 * (size_t)nrows + (size_t)nw * mw * S) * (size_t)vbytes +
                (2 * (size_t)nrows + 1) * sizeof(int) + (size_t)nrows;
     return b <= kSmallLdsMax ? b : 0;
 }
-static int relax_small_waves(int nrows, int mw, int S)
+static int relax_small_waves(int nrows, int mw, int S, int vbytes)
 {
     for (int nw = SMALL_BLK_MAX / 64; nw >= 4; nw /= 2)
-        if (relax_small_lds(nrows, mw, S, nw)) return nw;
+        if (relax_small_lds(nrows, mw, S, nw, vbytes)) return nw;
     return 0;
 }
-bool relax_small_fits(int nrows, int m, int m2, int S)
+bool relax_small_fits(int nrows, int m, int m2, int S, int vbytes)
 {
     if (m > small_mt(m, S)) return false;
-    return relax_small_waves(nrows, small_slots(m > m2 ? m : m2, small_mt(m, S)), S) > 0;
+    return relax_small_waves(nrows, small_slots(m > m2 ? m : m2, small_mt(m, S)), S, vbytes) > 0;
 }
-template <int S, bool PIN, int MT>
-static int launch_relax_small_v(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol,
-                                int nrows, const uint8_t *btag, const int *ref0, const double *src,
-                                double *p, double *res, double w, double c, int num_relax, int variant,
-                                int nw, size_t shm, hipStream_t s)
+template <int S, bool PIN, int MT, typename TW, typename TV>
+static int launch_relax_small_v(EllT<TW> A, EllT<TW> A2, const int *col_slices, const int *col_soff, int ncol,
+                                int nrows, const uint8_t *btag, const int *ref0, const TV *src, TV *p, TV *res,
+                                TV w, TV c, int num_relax, int variant, int nw, size_t shm, hipStream_t s)
 {
     static bool attr = false;
     if (!attr) {
-        HIP_TRY(hipFuncSetAttribute((const void *)relax_small_kernel<S, PIN, MT>,
+        HIP_TRY(hipFuncSetAttribute((const void *)relax_small_kernel<S, PIN, MT, TW, TV>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSmallLdsMax));
         attr = true;
     }
-    hipLaunchKernelGGL((relax_small_kernel<S, PIN, MT>), dim3(1), dim3(64 * nw), shm, s, A, A2, col_slices,
+    hipLaunchKernelGGL((relax_small_kernel<S, PIN, MT, TW, TV>), dim3(1), dim3(64 * nw), shm, s, A, A2, col_slices,
                        col_soff, ncol, nrows, btag, ref0, src, p, res, w, c, num_relax, variant);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 // The H variant has no pin or residual pass: its own instance, without their
 // registers.
-template <int S, int MT>
-static int launch_relax_small_m(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol,
-                                int nrows, const uint8_t *btag, const int *ref0, const double *src,
-                                double *p, double *res, double w, double c, int num_relax, int variant,
-                                int nw, size_t shm, hipStream_t s)
+// (BF16 weights serve the H variant only.)
+template <int S, int MT, typename TW, typename TV>
+static int launch_relax_small_m(EllT<TW> A, EllT<TW> A2, const int *col_slices, const int *col_soff, int ncol,
+                                int nrows, const uint8_t *btag, const int *ref0, const TV *src, TV *p, TV *res,
+                                TV w, TV c, int num_relax, int variant, int nw, size_t shm, hipStream_t s)
 {
     if (variant == MGM_VAR_H)
         return launch_relax_small_v<S, false, MT>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p,
                                                   res, w, c, num_relax, variant, nw, shm, s);
-    return launch_relax_small_v<S, true, MT>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p, res,
-                                             w, c, num_relax, variant, nw, shm, s);
+    if constexpr (std::is_same<TW, bf16>::value) {
+        MGM_FAIL(-1, "relax_small: BF16 weights only for the H variant");
+    } else {
+        return launch_relax_small_v<S, true, MT>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p,
+                                                 res, w, c, num_relax, variant, nw, shm, s);
+    }
 }
-template <int S>
-static int launch_relax_small_t(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol,
-                                int nrows, const uint8_t *btag, const int *ref0, const double *src,
-                                double *p, double *res, double w, double c, int num_relax, int variant,
-                                int nw, size_t shm, hipStream_t s)
+template <int S, typename TW, typename TV>
+static int launch_relax_small_t(EllT<TW> A, EllT<TW> A2, const int *col_slices, const int *col_soff, int ncol,
+                                int nrows, const uint8_t *btag, const int *ref0, const TV *src, TV *p, TV *res,
+                                TV w, TV c, int num_relax, int variant, int nw, size_t shm, hipStream_t s)
 {
     switch (small_mt(A.m, S)) {
     case 24: return launch_relax_small_m<S, 24>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p,
@@ -788,17 +788,18 @@ static int launch_relax_small_t(Ell A, Ell A2, const int *col_slices, const int 
     return launch_relax_small_m<S, MTMAX>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p, res,
                                           w, c, num_relax, variant, nw, shm, s);
 }
-int launch_relax_small(Ell A, Ell A2, const int *col_slices, const int *col_soff, int ncol, int nrows,
-                       const uint8_t *btag, const int *ref0, const double *src, double *p, double *res,
-                       double w, double c, int num_relax, int variant, hipStream_t s)
+template <typename TW, typename TV>
+int launch_relax_small(EllT<TW> A, EllT<TW> A2, const int *col_slices, const int *col_soff, int ncol, int nrows,
+                       const uint8_t *btag, const int *ref0, const TV *src, TV *p, TV *res, TV w, TV c,
+                       int num_relax, int variant, hipStream_t s)
 {
     if (A.S != A2.S) MGM_FAIL(-1, "relax_small: operators differ in slice height");
     if (A.sorted || A2.sorted) MGM_FAIL(-1, "relax_small: needs plain slabs");
     if (A.m > small_mt(A.m, A.S)) MGM_FAIL(-1, "relax_small: stencil too wide");
     const int mw = small_slots(A.m > A2.m ? A.m : A2.m, small_mt(A.m, A.S));
-    const int nw = relax_small_waves(nrows, mw, A.S);
+    const int nw = relax_small_waves(nrows, mw, A.S, sizeof(TV));
     if (!nw) MGM_FAIL(-1, "relax_small: level does not fit in LDS");
-    const size_t shm = relax_small_lds(nrows, mw, A.S, nw);
+    const size_t shm = relax_small_lds(nrows, mw, A.S, nw, sizeof(TV));
     switch (A.S) {
     case 4: return launch_relax_small_t<4>(A, A2, col_slices, col_soff, ncol, nrows, btag, ref0, src, p, res,
                                            w, c, num_relax, variant, nw, shm, s);
@@ -809,6 +810,14 @@ int launch_relax_small(Ell A, Ell A2, const int *col_slices, const int *col_soff
     }
     MGM_FAIL(-1, "unsupported slice height");
 }
+#define RELAX_SMALL_INST(TW, TV)                                                                          \
+    template int launch_relax_small<TW, TV>(EllT<TW>, EllT<TW>, const int *, const int *, int, int,           \
+                                            const uint8_t *, const int *, const TV *, TV *, TV *, TV, TV, int, \
+                                            int, hipStream_t);
+RELAX_SMALL_INST(double, double)
+RELAX_SMALL_INST(float, float)
+RELAX_SMALL_INST(bf16, float)
+#undef RELAX_SMALL_INST
 int launch_sor_natural(Ell A, int N, const int *iperm, const double *src, double *p, double w,
                        double c, hipStream_t s)
 {
