@@ -274,20 +274,23 @@ rows_kernel(EllT<TW> A, const int *__restrict__ slices, int first, int nslices, 
                           prod + (size_t)wave * A.m * S);
 }
 
-// Weights and columns of this lane's entries (j = t*LPR + jo) of one slice.
-template <int S, int NIT, typename TW, typename TV>
+// Weights and columns of this lane's entries j = t*LPR + jo < MT of one
+// slice (entry j of row ri sits at slice*S*m + j*S + ri).  Slots j >= m load
+// entry m-1 again: every load is in the row's range and none is guarded, so
+// they issue back to back; sor_stage stores +0.0 for those slots.
+template <int S, int NIT, int MT, typename TW, typename TV>
 __device__ inline void load_slice(const EllT<TW> &A, int slice, int ri, int jo, TV *wq, int *cq)
 {
     constexpr int LPR = 64 / S;
-    // entry j = t*LPR + jo of row ri sits at slice*S*m + j*S + ri = lane base + t*64
-    const size_t off = (size_t)slice * S * A.m + (size_t)jo * S + ri;
+    const size_t off = (size_t)slice * S * A.m + ri;
     const TW *wp = A.w + off;
     const int *cp = A.col + off;
 #pragma unroll
     for (int t = 0; t < NIT; t++)
-        if (t * LPR + jo < A.m) {
-            wq[t] = wv(wp[t * 64]);
-            cq[t] = cp[t * 64];
+        if (t * LPR < MT) {
+            const int j = t * LPR + jo < A.m ? t * LPR + jo : A.m - 1;
+            wq[t] = wv(wp[(size_t)j * S]);
+            cq[t] = cp[(size_t)j * S];
         }
 }
 
@@ -309,10 +312,13 @@ __device__ inline TV sor_stage(int slice, int nrows, int m, const TV *xs, const 
     TV xv[NIT];
 #pragma unroll
     for (int t = 0; t < NIT; t++)
-        if (inr && t * LPR + jo < m) xv[t] = xs[cq[t]];
+        if (t * LPR < MT) xv[t] = xs[cq[t]];
 #pragma unroll
-    for (int t = 0; t < NIT; t++)
-        if (inr && t * LPR + jo < MT) P[(t * LPR + jo) * S + ri] = t * LPR + jo < m ? wq[t] * xv[t] : TV(0);
+    for (int t = 0; t < NIT; t++) {
+        const int j = t * LPR + jo;
+        // (the slot test is needed only where a slot group straddles MT)
+        if (t * LPR < MT && (t * LPR + LPR <= MT || j < MT) && inr) P[j * S + ri] = j < m ? wq[t] * xv[t] : TV(0);
+    }
     return wq[0];
 }
 
@@ -321,7 +327,7 @@ __device__ inline TV sor_stage(int slice, int nrows, int m, const TV *xs, const 
 // adds.  It starts at +0.0, and a round-to-nearest sum that starts at +0.0 is
 // never -0.0, so adding the +0.0 slots past m leaves it unchanged: the result
 // is bitwise the sum over 1 <= j < m.
-template <int S, int MT, typename TV>
+template <int S, int MT, int B, typename TV>
 __device__ inline void sor_finish(int slice, int nrows, const uint8_t *tg, const TV *sr, TV *xs, TV w, TV c,
                                   TV dg, const TV *P)
 {
@@ -332,8 +338,27 @@ __device__ inline void sor_finish(int slice, int nrows, const uint8_t *tg, const
     if (jo == 0 && r < nrows && tg[r] != ROW_PAD) {
         const TV *q = P + ri;
         TV sum = TV(0);
+        // slots in batches of B, each batch's reads issued before the previous
+        // batch's adds (the scheduling barriers keep that order), so the LDS
+        // latency overlaps the dependent adds
+        constexpr int NB = (MT - 1 + B - 1) / B;
+        TV v[2][B];
 #pragma unroll
-        for (int j = 1; j < MT; j++) sum += q[j * S];
+        for (int t = 0; t < B; t++)
+            if (1 + t < MT) v[0][t] = q[(1 + t) * S];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            if (b + 1 < NB) {
+#pragma unroll
+                for (int t = 0; t < B; t++)
+                    if (1 + (b + 1) * B + t < MT) v[(b + 1) & 1][t] = q[(1 + (b + 1) * B + t) * S];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int t = 0; t < B; t++)
+                if (1 + b * B + t < MT) sum += v[b & 1][t];
+            __builtin_amdgcn_sched_barrier(0);
+        }
         xs[r] = w * ((sr[r] - sum) / dg) + c * xs[r];
     }
     wave_sync();
@@ -384,7 +409,7 @@ relax_small_kernel(EllT<TW> A, EllT<TW> A2, const int *__restrict__ col_slices, 
     auto first_slice = [&](int cl) { const int k = soff[cl] + wave; return k < soff[cl + 1] ? slist[k] : -1; };
     if (num_relax > 0) {
         const int f = first_slice(0);
-        if (f >= 0) load_slice<S, NIT>(A, f, ri, jo, wq, cq);
+        if (f >= 0) load_slice<S, NIT, MT>(A, f, ri, jo, wq, cq);
     }
     const long nsl = (nrows + S - 1) / S;
     for (int it = 0; it < num_relax; it++) {
@@ -399,10 +424,11 @@ relax_small_kernel(EllT<TW> A, EllT<TW> A2, const int *__restrict__ col_slices, 
                 const TV dg = sor_stage<S, NIT, MT>(sl, nrows, A.m, xs, wq, cq, P);
                 // this wave's next slice: the registers are free once staged
                 const int nx = k + NW < s1 ? slist[k + NW] : fnext;
-                if (nx >= 0) load_slice<S, NIT>(A, nx, ri, jo, wq, cq);
-                sor_finish<S, MT>(sl, nrows, tg, sr, xs, w, c, dg, P);
+                if (nx >= 0) load_slice<S, NIT, MT>(A, nx, ri, jo, wq, cq);
+                // (batches of 8 in FP64, which would spill with 16)
+                sor_finish<S, MT, sizeof(TV) == 8 ? 8 : 16>(sl, nrows, tg, sr, xs, w, c, dg, P);
             }
-            if (s0 + wave >= s1 && fnext >= 0) load_slice<S, NIT>(A, fnext, ri, jo, wq, cq);
+            if (s0 + wave >= s1 && fnext >= 0) load_slice<S, NIT, MT>(A, fnext, ri, jo, wq, cq);
             __syncthreads();
         }
         if constexpr (!PIN) continue;   // variant H: no pin, no residual pass
@@ -418,7 +444,7 @@ relax_small_kernel(EllT<TW> A, EllT<TW> A2, const int *__restrict__ col_slices, 
             slice_rows<S, TW, TV>(R, k, nrows, form, tg, sr, xs, res, TV(0), TV(0), 0, P);
         if (it + 1 < num_relax) {
             const int f = first_slice(0);
-            if (f >= 0) load_slice<S, NIT>(A, f, ri, jo, wq, cq);
+            if (f >= 0) load_slice<S, NIT, MT>(A, f, ri, jo, wq, cq);
         }
         __syncthreads();
     }
