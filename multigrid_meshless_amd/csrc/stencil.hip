@@ -364,6 +364,41 @@ __device__ inline void sor_finish(int slice, int nrows, const uint8_t *tg, const
     wave_sync();
 }
 
+// Residual form 1 (s - A p, zero on boundary rows) or 2 (s - A p on every
+// row) of a small level, p and s in LDS, one row per thread: slice_rows'
+// arithmetic (products in TV, subtracted from s left to right), with a row's
+// loads and gathers issued eight entries at a time ahead of its subtractions.
+template <int S, typename TW, typename TV>
+__device__ inline void residual_rows_lds(const EllT<TW> &R, int nrows, int form, const uint8_t *tg,
+                                         const TV *sr, const TV *xs, TV *res)
+{
+    const int m = R.m;
+    for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
+        const uint8_t tag = tg[r];
+        if (tag == ROW_PAD) continue;
+        if (form == M_RES1 && tag == ROW_BOUNDARY) {
+            res[r] = TV(0);
+            continue;
+        }
+        const size_t base = (size_t)(r / S) * S * m + (r % S);
+        const TW *wp = R.w + base;
+        const int *cp = R.col + base;
+        TV acc = sr[r];
+        int j = 0;
+        for (; j + 8 <= m; j += 8) {
+            TV wq[8], xq[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) wq[t] = wv(wp[(size_t)(j + t) * S]);
+#pragma unroll
+            for (int t = 0; t < 8; t++) xq[t] = xs[cp[(size_t)(j + t) * S]];
+#pragma unroll
+            for (int t = 0; t < 8; t++) acc = acc - wq[t] * xq[t];
+        }
+        for (; j < m; j++) acc = acc - wv(wp[(size_t)j * S]) * xs[cp[(size_t)j * S]];
+        res[r] = acc;
+    }
+}
+
 // Whole relaxation of a small level in one workgroup, with the level's p,
 // source and tags held in LDS: num_relax sweeps over the colour classes in
 // order (workgroup barrier between classes), each sweep followed, for the T
@@ -411,7 +446,6 @@ relax_small_kernel(EllT<TW> A, EllT<TW> A2, const int *__restrict__ col_slices, 
         const int f = first_slice(0);
         if (f >= 0) load_slice<S, NIT, MT>(A, f, ri, jo, wq, cq);
     }
-    const long nsl = (nrows + S - 1) / S;
     for (int it = 0; it < num_relax; it++) {
         for (int cl = 0; cl < ncol; cl++) {
             const int s0 = soff[cl], s1 = soff[cl + 1];
@@ -438,10 +472,10 @@ relax_small_kernel(EllT<TW> A, EllT<TW> A2, const int *__restrict__ col_slices, 
         for (int r = threadIdx.x; r < nrows; r += blockDim.x)
             if (tg[r] != ROW_PAD) xs[r] = xs[r] - pr;
         __syncthreads();
-        const EllT<TW> &R = (variant == MGM_VAR_T) ? A : A2;
-        const int form = (variant == MGM_VAR_T) ? M_RES1 : M_RES2;
-        for (long k = wave; k < nsl; k += NW)
-            slice_rows<S, TW, TV>(R, k, nrows, form, tg, sr, xs, res, TV(0), TV(0), 0, P);
+        if (variant == MGM_VAR_T)
+            residual_rows_lds<S>(A, nrows, M_RES1, tg, sr, xs, res);
+        else
+            residual_rows_lds<S>(A2, nrows, M_RES2, tg, sr, xs, res);
         if (it + 1 < num_relax) {
             const int f = first_slice(0);
             if (f >= 0) load_slice<S, NIT, MT>(A, f, ri, jo, wq, cq);
