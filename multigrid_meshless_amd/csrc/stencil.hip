@@ -723,9 +723,10 @@ int launch_rows_w(EllT<TW> A, const int *slices, int first, int nslices, int nro
 }
 
 // slices == nullptr: all slices covering rows [0, nrows).  SOR colour steps
-// run two waves per workgroup, the full passes four: the colour steps are
-// latency-bound and gain from the finer spread (1M-point solve 2.4% faster
-// with 128-thread workgroups everywhere, while the full-pass apply lost 2.5%).
+// run one wave per workgroup, the full passes four: the colour steps are
+// latency-bound and gain from the finer spread (1M-point solve 0.313 s at
+// four waves, 0.304 s at two, 0.299 s at one), while the full-pass apply
+// loses 2.5% with smaller workgroups.
 template <typename TW, typename TV>
 int launch_rows(EllT<TW> A, const int *slices, int first, int nslices, int nrows, int mode,
                 const uint8_t *btag, const TV *src, const TV *x, TV *y, TV w, TV c, int keep,
@@ -734,7 +735,7 @@ int launch_rows(EllT<TW> A, const int *slices, int first, int nslices, int nrows
     if (nslices < 0) nslices = (nrows + A.S - 1) / A.S;  // full pass
     if (nslices <= 0 || nrows <= 0) return 0;
     if (mode == M_SOR)
-        return launch_rows_w<2>(A, slices, first, nslices, nrows, mode, btag, src, x, y, w, c, keep, s);
+        return launch_rows_w<1>(A, slices, first, nslices, nrows, mode, btag, src, x, y, w, c, keep, s);
     return launch_rows_w<WAVES>(A, slices, first, nslices, nrows, mode, btag, src, x, y, w, c, keep, s);
 }
 
