@@ -18,7 +18,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -47,48 +46,15 @@ def measured_traffic():
     return None
 
 
-def fourier_rhs(x, y, z, seed=2, modes=16):
-    """f = sum_q a_q sin(pi (k_q . x)), k_q in {1..4}^3, a_q ~ N(0,1) (Box-Muller on SplitMix64)."""
-    import ctypes as C
-    from multigrid_meshless_amd._lib import lib
-    s = C.c_uint64(seed)
-
-    def u01():
-        return (lib().mgm_splitmix64(C.byref(s)) >> 11) * 2.0 ** -53
-
-    f = np.zeros_like(x)
-    for _ in range(modes):
-        k = [1 + (lib().mgm_splitmix64(C.byref(s)) % 4) for _ in range(3)]
-        u1, u2 = u01(), u01()
-        a = math.sqrt(-2.0 * math.log(max(u1, 1e-300))) * math.cos(2 * math.pi * u2)
-        f += a * np.sin(math.pi * (k[0] * x + k[1] * y + k[2] * z))
-    return f
-
-
 def build_problem(mgm, n, levels, seed0=1):
-    """Jittered n^3 cube and `levels` coarser clouds (halving), bench stencil settings."""
-    sizes = []
-    v = n
-    for _ in range(levels):
-        sizes.append(v)
-        v = (v + 1) // 2
-    pts = [mgm.Points.generate(3, s, seed0 + k, 0.2) for k, s in enumerate(sizes)]
-    cfg = mgm.config(dimension=3, poly_degree=3, coarse_poly_degree=3, phs_degree=3,
-                     cloud_size_multiplier=3.5, omega=1.0, num_relax=3,
-                     transfer_rows=mgm.ROWS_BY_TAG, sweep_order=mgm.ORDER_COLOUR)
-    P = mgm.Problem(cfg, pts)
-    t0 = time.perf_counter()
-    P.setup()
-    t_setup = time.perf_counter() - t0
-    P.dirichlet_rows()
-    return P, sizes, t_setup
+    """The 3D cube Poisson hierarchy (cases.cube_poisson)."""
+    from multigrid_meshless_amd import cases
+    return cases.cube_poisson(n, levels, seed0)
 
 
 def set_rhs(P):
-    g = P.geometry(0)
-    f = fourier_rhs(g["x"], g["y"], g["z"])
-    f[g["btag"] != 0] = 0.0
-    return f
+    from multigrid_meshless_amd import cases
+    return cases.cube_rhs(P)
 
 
 def reset(P, f):
@@ -131,9 +97,10 @@ def annulus_learned(mgm, args):
     """Learned smoother on the annulus: plain FP64 SOR V-cycles against the
     learned first sweep (FP64 and FP32 inner cycles), network trained on a
     coarser annulus hierarchy (181 per side, 3 levels) and moved over."""
+    from multigrid_meshless_amd import cases
     P, sizes = build_annulus(mgm, args.ann_n, args.ann_levels, 41)
     g = P.geometry(0)
-    f = fourier_rhs(g["x"], g["y"], np.zeros_like(g["x"]))
+    f = cases.fourier_rhs(g["x"], g["y"], np.zeros_like(g["x"]))
     f[g["btag"] != 0] = 0.0
     N = P.info(0)["N"]
 

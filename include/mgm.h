@@ -67,6 +67,12 @@ typedef struct {
     double d_avg;                  /* after calculate_avg_dx (mesh_functions.h:895) */
     double d_avg_bbox;             /* reader's bbox estimate (mesh_functions.h:274-281) */
     int knn_ties;                  /* queries whose stencil needed the kd-order tie-break */
+    /* device layout (no reference counterpart) */
+    int device_rows;               /* rows of the slabs, padding included */
+    int slice_height;              /* rows per ELL slice of the operator slabs */
+    int blocked;                   /* rows ordered in Morton blocks (colour steps take slice lists) */
+    int gather_sorted;             /* lap / lap_Poison also stored gather-sorted */
+    int small_relax;               /* relaxes in one workgroup (at the current MGM_SMALL_ROWS) */
 } mgm_level_info;
 
 typedef struct mgm_points mgm_points;

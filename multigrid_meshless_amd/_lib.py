@@ -24,7 +24,9 @@ class Config(C.Structure):
 class LevelInfo(C.Structure):
     _fields_ = [("N", C.c_int), ("nb", C.c_int), ("m", C.c_int), ("n", C.c_int),
                 ("poly_degree", C.c_int), ("num_colours", C.c_int), ("d_avg", C.c_double),
-                ("d_avg_bbox", C.c_double), ("knn_ties", C.c_int)]
+                ("d_avg_bbox", C.c_double), ("knn_ties", C.c_int), ("device_rows", C.c_int),
+                ("slice_height", C.c_int), ("blocked", C.c_int), ("gather_sorted", C.c_int),
+                ("small_relax", C.c_int)]
 
 
 class NsParams(C.Structure):

@@ -83,3 +83,67 @@ def annulus(n: int, seed: int, jitter: float = 0.2, r_in: float = 0.5, r_out: fl
     btag[:nb] = 1
     zero = np.zeros(N)
     return Points.from_arrays(N, nb, 2 * N, h, x, y, zero, nx, ny, zero, btag, np.zeros(N, np.uint8))
+
+
+def fourier_rhs(x, y, z, seed: int = 2, modes: int = 16):
+    """f = sum_q a_q sin(pi (k_q . x)) with k_q in {1..4}^3 and a_q ~ N(0, 1): the
+    random smooth right-hand side of the 3D cube Poisson workload.  The draws
+    come from SplitMix64 seeded with `seed`: three wave numbers 1 + (u mod 4),
+    then a_q by Box-Muller from two uniforms (top 53 bits of a draw)."""
+    import ctypes as C
+    import math
+
+    from ._lib import lib
+    s = C.c_uint64(seed)
+
+    def u01():
+        return (lib().mgm_splitmix64(C.byref(s)) >> 11) * 2.0 ** -53
+
+    f = np.zeros_like(np.asarray(x, dtype=np.float64))
+    for _ in range(modes):
+        k = [1 + (lib().mgm_splitmix64(C.byref(s)) % 4) for _ in range(3)]
+        u1, u2 = u01(), u01()
+        a = math.sqrt(-2.0 * math.log(max(u1, 1e-300))) * math.cos(2 * math.pi * u2)
+        f += a * np.sin(math.pi * (k[0] * x + k[1] * y + k[2] * z))
+    return f
+
+
+def cube_levels(n: int, levels: int):
+    """Points per axis of a hierarchy that halves its spacing: n, (n+1)//2, ..."""
+    sizes, v = [], n
+    for _ in range(levels):
+        sizes.append(v)
+        v = (v + 1) // 2
+    return sizes
+
+
+def cube_poisson(n: int = 100, levels: int = 5, seed0: int = 1, setup: bool = True):
+    """The 3D cube Poisson hierarchy: a jittered n^3 cloud on [0,1]^3 (jitter
+    0.2 h, seeds seed0, seed0+1, ... per level) and `levels` clouds halving in
+    spacing; PHS r^3 + cubic polynomials, multiplier 3.5 (m = 70, mpn = 90),
+    multicolour SOR with omega 1 and 3 sweeps, transfer rows on every interior
+    node, Dirichlet identity rows on every boundary node of every level.
+    Returns (problem, sizes, setup seconds)."""
+    import time
+
+    from . import ORDER_COLOUR, ROWS_BY_TAG, Points, Problem, config
+    sizes = cube_levels(n, levels)
+    pts = [Points.generate(3, s, seed0 + k, 0.2) for k, s in enumerate(sizes)]
+    cfg = config(dimension=3, poly_degree=3, coarse_poly_degree=3, phs_degree=3, cloud_size_multiplier=3.5,
+                 omega=1.0, num_relax=3, transfer_rows=ROWS_BY_TAG, sweep_order=ORDER_COLOUR)
+    P = Problem(cfg, pts)
+    t = 0.0
+    if setup:
+        t0 = time.perf_counter()
+        P.setup()
+        t = time.perf_counter() - t0
+        P.dirichlet_rows()
+    return P, sizes, t
+
+
+def cube_rhs(P):
+    """The 16-mode Fourier source on level 0's interior nodes, 0 on its boundary."""
+    g = P.geometry(0)
+    f = fourier_rhs(g["x"], g["y"], g["z"])
+    f[g["btag"] != 0] = 0.0
+    return f

@@ -32,7 +32,7 @@ struct DropIn {
     int L = 0;
     mgm_problem *pb = nullptr;
     double multiplier = 0;            // fractional multiplier from flow_parameters.csv
-    std::vector<uint64_t> lapP_hash;  // bytes of host lap_Poison rows at the last upload
+    std::vector<uint64_t> lapP_hash;  // fingerprint of the host lap_Poison rows at the last upload
 };
 DropIn g;
 
@@ -52,12 +52,36 @@ int level_of(const PointStructure *P)
     return (int)(P - g.base);
 }
 
-uint64_t hash_rows(double **rows, int N, int m)
+// Fingerprint of the host lap_Poison rows the init plug-ins may rewrite
+// between solver calls: the boundary rows (set_boundary_rhs,
+// init/init_heat_conduction.c:50-65, writes those).  Hashed a 64-bit word at
+// a time, so at 1M points and m = 70 a call reads the 57k boundary rows (32
+// MB) instead of every row.  MGM_DROPIN_CHECK_ALL_ROWS=1 fingerprints every
+// row, for plug-ins that also rewrite interior rows.
+bool check_all_rows()
 {
+    static const bool all = [] {
+        const char *e = getenv("MGM_DROPIN_CHECK_ALL_ROWS");
+        return e && e[0] == '1';
+    }();
+    return all;
+}
+
+uint64_t hash_rows(double **rows, const PointStructure &P)
+{
+    const int m = P.num_cloud_points;
+    const bool all = check_all_rows();
     uint64_t h = 1469598103934665603ull;
-    for (int i = 0; i < N; i++) {
-        const unsigned char *b = (const unsigned char *)rows[i];
-        for (size_t k = 0; k < sizeof(double) * (size_t)m; k++) h = (h ^ b[k]) * 1099511628211ull;
+    for (int i = 0; i < P.num_nodes; i++) {
+        if (!all && !P.boundary_tag[i]) continue;
+        const double *r = rows[i];
+        for (int j = 0; j < m; j++) {
+            uint64_t w;
+            memcpy(&w, &r[j], sizeof(w));
+            h = (h ^ w) * 1099511628211ull;
+            h ^= h >> 29;
+        }
+        h = (h ^ (uint64_t)i) * 1099511628211ull;
     }
     return h;
 }
@@ -79,7 +103,7 @@ void upload_lapP_if_changed(int l)
 {
     PointStructure &P = g.base[l];
     if (!P.lap_Poison) return;
-    uint64_t h = hash_rows(P.lap_Poison, P.num_nodes, P.num_cloud_points);
+    uint64_t h = hash_rows(P.lap_Poison, P);
     if (h == g.lapP_hash[l]) return;
     std::vector<double> v = flatten(P.lap_Poison, P.num_nodes, P.num_cloud_points);
     MUST(mgm_set_matrix(g.pb, l, MGM_MAT_LAPP, v.data()));
@@ -199,43 +223,78 @@ void free_vector(double *A) { free(A); }
 
 // ---- mesh_functions.h ------------------------------------------------------
 
+// read_flow_parameters (hf/mesh_functions.h:37-100): positional "name,value"
+// lines read with the reference's own conversions (%hd, %lf, %f, %d), so every
+// field of `parameters` holds what the reference's parse leaves there.  One
+// extension: the cloud-size multiplier is read with %lf and kept whole for
+// the stencil size (3.5 -> m = 70 at 20 monomials); parameters.
+// cloud_size_multiplier gets its integer part, which is what the reference's
+// %hd leaves in it (it stops at the '.', and the ".5" then becomes part of the
+// next line's ignored name).  A line that does not parse ends the program
+// (the reference checks only the last line).
 void read_flow_parameters(char *filename)
 {
     FILE *f = fopen(filename, "r");
     if (!f) { printf("Error: Unable to open the file\n"); exit(1); }
     char name[128];
-    double v[20];
-    for (int k = 0; k < 20; k++) {
-        if (fscanf(f, "%127[^,],%lf\n", name, &v[k]) != 2) { printf("Error: Unable to read the file\n"); exit(1); }
-        printf("PARAMETERS: %s = %g\n", name, v[k]);
-    }
+    int line = 0;
+    auto bad = [&]() {
+        printf("Error: Unable to read the file (line %d)\n", line);
+        exit(1);
+    };
+    auto rd_short = [&](short *v) {
+        line++;
+        if (fscanf(f, "%127[^,],%hd\n", name, v) != 2) bad();
+        printf("PARAMETERS: %s = %hd\n", name, *v);
+    };
+    auto rd_double = [&](double *v, const char *fmt) {
+        line++;
+        if (fscanf(f, "%127[^,],%lf\n", name, v) != 2) bad();
+        printf(fmt, name, *v);
+    };
+    auto rd_int = [&](int *v) {
+        line++;
+        if (fscanf(f, "%127[^,],%d\n", name, v) != 2) bad();
+        printf("PARAMETERS: %s = %d\n", name, *v);
+    };
+    rd_short(&parameters.dimension);
+    rd_short(&parameters.poly_degree);
+    rd_short(&parameters.phs_degree);
+    double mult = 0;
+    line++;
+    if (fscanf(f, "%127[^,],%lf\n", name, &mult) != 2 || !(mult > 0) || mult > 32767) bad();
+    parameters.cloud_size_multiplier = (short)mult;
+    g.multiplier = mult;
+    printf("PARAMETERS: %s = %g\n", name, mult);
+    rd_short(&parameters.test);
+    rd_double(&parameters.courant_number, "PARAMETERS: %s = %lf\n");
+    rd_double(&parameters.steady_state_tolerance, "PARAMETERS: %s = %e\n");
+    rd_double(&parameters.poisson_solver_tolerance, "PARAMETERS: %s = %e\n");
+    line++;
+    if (fscanf(f, "%127[^,],%f\n", name, &parameters.omega) != 2) bad();
+    printf("PARAMETERS: %s = %f\n", name, parameters.omega);
+    rd_short(&parameters.num_vcycles);
+    rd_short(&parameters.num_relax);
+    rd_int(&parameters.num_time_steps);
+    rd_int(&parameters.write_interval);
+    int flag = 0;
+    rd_int(&flag);
+    parameters.neumann_flag_boundary = flag == 1;
+    rd_double(&parameters.dt, "PARAMETERS: %s = %lf\n");
+    rd_short(&parameters.iter_momentum);
+    rd_short(&parameters.iter_timple);
+    rd_double(&parameters.Re, "PARAMETERS: %s = %lf\n");
+    rd_double(&parameters.facRe, "PARAMETERS: %s = %lf\n");
+    rd_double(&parameters.facdt, "PARAMETERS: %s = %lf\n");
     fclose(f);
-    parameters.dimension = (short)v[0];
-    parameters.poly_degree = (short)v[1];
-    parameters.phs_degree = (short)v[2];
-    parameters.cloud_size_multiplier = (short)v[3];
-    g.multiplier = v[3];
-    parameters.test = (short)v[4];
-    parameters.courant_number = v[5];
-    parameters.steady_state_tolerance = v[6];
-    parameters.poisson_solver_tolerance = v[7];
-    parameters.omega = (float)v[8];
-    parameters.num_vcycles = (short)v[9];
-    parameters.num_relax = (short)v[10];
-    parameters.num_time_steps = (int)v[11];
-    parameters.write_interval = (int)v[12];
-    parameters.neumann_flag_boundary = ((int)v[13] == 1);
-    parameters.dt = v[14];
-    parameters.iter_momentum = (short)v[15];
-    parameters.iter_timple = (short)v[16];
-    parameters.Re = v[17];
-    parameters.facRe = v[18];
-    parameters.facdt = v[19];
     parameters.rho = 1;
     parameters.mu = parameters.rho / parameters.Re;
     parameters.nu = parameters.mu / parameters.rho;
 }
 
+// read_grid_filenames (hf/mesh_functions.h:102-129): "name,levels" then one
+// mesh file name per level (at most 49 characters, the size of
+// PointStructure.mesh_filename; the reference's %s has no bound).
 void read_grid_filenames(PointStructure **P, char *filename, short *num_levels)
 {
     FILE *f = fopen(filename, "r");
@@ -381,7 +440,7 @@ void create_derivative_matrices(PointStructure *P)
     if (parameters.dimension == 3) fill(MGM_MAT_DZ, &P->Dz);
     fill(MGM_MAT_LAP, &P->lap);
     fill(MGM_MAT_LAPP, &P->lap_Poison);
-    g.lapP_hash[l] = hash_rows(P->lap_Poison, N, m);
+    g.lapP_hash[l] = hash_rows(P->lap_Poison, *P);
 }
 
 // ---- general_functions.h ---------------------------------------------------

@@ -363,8 +363,17 @@ static uint64_t spread_bits(uint64_t v, int stride)
     return r;
 }
 
-// Levels above this many rows are ordered in Morton blocks of 4096 rows.
-constexpr int kBlockedRows = 262144;
+// Levels above this many rows (MGM_BLOCKED_ROWS, default 262,144: vectors
+// that no longer fit an XCD's L2 many times over) are ordered in Morton
+// blocks of MGM_BLOCK_ROWS (4096) rows and read gather-sorted slabs.  Read
+// whenever a level is ordered, so tests can reproduce the large-level layout
+// on small clouds.
+constexpr int kLargeRows = 262144;
+static long blocked_rows()
+{
+    const char *e = getenv("MGM_BLOCKED_ROWS");
+    return e ? atol(e) : kLargeRows;
+}
 
 int small_level_rows();
 
@@ -381,7 +390,7 @@ static int level_slice_height(int m, int N, int ncol, int nblocks)
     const int base = slice_height(m);
     if (getenv("MGM_SLICE_HEIGHT") || base != 8) return base;
     if (N <= small_level_rows()) return 8;
-    if (N <= kBlockedRows) return 4;
+    if (N <= blocked_rows()) return 4;
     const auto padded = [&](int S) { return (long)N + (long)S * ncol * nblocks; };
     if (!slab_sortable(8, m, padded(8)) && slab_sortable(4, m, padded(4))) return 4;
     return 8;
@@ -489,7 +498,8 @@ static void order_by_colour(Level &L, int dim)
         const int v = e ? atoi(e) : 4096;
         return v > 0 ? v : 4096;
     }();
-    const int kBlock = N > kBlockedRows ? block_rows : N;
+    const int kBlock = N > blocked_rows() ? block_rows : N;
+    L.blocked = kBlock < N;
     std::vector<int> block(N);
     for (int r = 0; r < N; r++) block[byk[r]] = r / kBlock;
     std::vector<int> ord(byk);
@@ -561,6 +571,13 @@ static int upload_level(Level &L)
     }
     HIP_TRY(hipStreamSynchronize(g_stream));  // host staging vectors die here
     return 0;
+}
+
+// The level relaxes in one single-workgroup launch (op_relax).
+static bool relax_small_here(const mgm_problem *pb, const Level &L)
+{
+    return pb->cfg.sweep_order != MGM_ORDER_NATURAL && L.Nd <= small_level_rows() &&
+           relax_small_fits(L.Nd, L.m, L.m, L.S);
 }
 
 static bool row_valid(const Level &L, int i, int mode)
@@ -669,7 +686,7 @@ static int level_lap_poisson(mgm_problem *pb, int l)
 }
 
 // Streaming passes over slabs of more than MGM_GATHER_SORT_ROWS rows
-// (default kBlockedRows) read them gather-sorted (EllT::sorted): such a pass
+// (default 262,144) read them gather-sorted (EllT::sorted): such a pass
 // streams its slab from HBM, and the sorted order cuts the cache lines one
 // gather instruction touches.  MGM_GATHER_SORT=0 keeps every slab plain.
 // Results are identical either way.  Both are read whenever a level's slabs
@@ -679,7 +696,7 @@ static bool gather_sort_on(int rows, int S, int m, int ncols)
     const char *e = getenv("MGM_GATHER_SORT");
     if (e && atoi(e) == 0) return false;
     const char *t = getenv("MGM_GATHER_SORT_ROWS");
-    const long min_rows = t ? atol(t) : kBlockedRows;
+    const long min_rows = t ? atol(t) : kLargeRows;
     return rows > min_rows && slab_sortable(S, m, ncols);
 }
 
@@ -987,6 +1004,8 @@ int mgm_level_info_get(const mgm_problem *pb, int lev, mgm_level_info *o)
     const Level &L = pb->lv[lev];
     o->N = L.N; o->nb = L.nb; o->m = L.m; o->n = L.n; o->poly_degree = L.poly;
     o->num_colours = L.ncol; o->d_avg = L.d_avg; o->d_avg_bbox = L.d_avg_bbox; o->knn_ties = L.knn_ties;
+    o->device_rows = L.Nd; o->slice_height = L.S; o->blocked = L.blocked; o->gather_sorted = L.gs;
+    o->small_relax = relax_small_here(pb, L);
     return 0;
 }
 
@@ -1196,8 +1215,7 @@ int op_relax(mgm_problem *pb, int l, int variant)
         if (rc) return rc;
         nrelax--;
     }
-    if (pb->cfg.sweep_order != MGM_ORDER_NATURAL && L.Nd <= small_level_rows() &&
-        relax_small_fits(L.Nd, L.m, L.m, L.S))
+    if (relax_small_here(pb, L))
         return nrelax ? launch_relax_small(L.op_plain(OP_LAPP), L.op_plain(OP_LAP), L.d_col_slices.p,
                                            L.d_col_soff.p, L.ncol, L.Nd,
                                            L.dbtag.p, L.ref0.p, src, p, res, w, c, nrelax, variant, g_stream)

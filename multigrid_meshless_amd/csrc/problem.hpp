@@ -38,6 +38,7 @@ struct Level {
     std::vector<int> col_slices, col_soff;
     DevBuf<int> d_col_slices, d_col_soff;
     bool col_contig = false;           // every colour's slices are consecutive (no list needed)
+    bool blocked = false;              // rows in Morton blocks
     std::vector<int> perm, iperm;      // device -> reference (-1 = padding), reference -> device
     int knn_ties = 0;
     // ---- device, reference order ----

@@ -32,18 +32,6 @@ class _Mesh(C.Structure):
                [("btag", C.POINTER(C.c_uint8)), ("ctag", C.POINTER(C.c_uint8))]
 
 
-class _Params(C.Structure):
-    _fields_ = [("dimension", C.c_int), ("poly_degree", C.c_int), ("phs_degree", C.c_int),
-                ("cloud_size_multiplier", C.c_double), ("test", C.c_int),
-                ("courant_number", C.c_double), ("steady_state_tolerance", C.c_double),
-                ("poisson_solver_tolerance", C.c_double), ("omega", C.c_float),
-                ("num_vcycles", C.c_int), ("num_relax", C.c_int), ("num_time_steps", C.c_int),
-                ("write_interval", C.c_int), ("neumann_flag_boundary", C.c_int),
-                ("dt", C.c_double), ("iter_momentum", C.c_int), ("iter_timple", C.c_int),
-                ("Re", C.c_double), ("facRe", C.c_double), ("facdt", C.c_double),
-                ("rho", C.c_double), ("mu", C.c_double), ("nu", C.c_double)]
-
-
 class Level(C.Structure):
     """orc_level: one multigrid level's arrays for the V-cycle primitives."""
     _fields_ = [("N", C.c_int), ("m", C.c_int),
@@ -63,7 +51,6 @@ def lib():
         L = C.CDLL(path)
         L.orc_read_gmsh.argtypes = [C.c_char_p, C.c_int, C.POINTER(_Mesh)]
         L.orc_free_mesh.argtypes = [C.POINTER(_Mesh)]
-        L.orc_read_flow_parameters.argtypes = [C.c_char_p, C.POINTER(_Params)]
         L.orc_poly_terms.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_find_cloud_index.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _u8, _u8,
                                            C.c_double, C.c_int, _ip]
@@ -120,13 +107,6 @@ def read_gmsh(path: str, dim: int) -> dict:
     out["ctag"] = _np(m.ctag, N, np.uint8)
     lib().orc_free_mesh(C.byref(m))
     return out
-
-
-def read_flow_parameters(path: str) -> dict:
-    p = _Params()
-    if lib().orc_read_flow_parameters(path.encode(), C.byref(p)):
-        raise IOError(path)
-    return {k: getattr(p, k) for k, _ in p._fields_}
 
 
 def poly_terms(dim: int, deg: int):

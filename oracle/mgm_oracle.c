@@ -45,53 +45,6 @@ uint64_t orc_splitmix64(uint64_t *s)
 }
 
 /* ===================================================================== */
-/* flow_parameters.csv: positional, names ignored (hf/mesh_functions.h:37-100).
- * The multiplier is read as a double (the reference's %hd truncates 3.5 to 3;
- * integer-valued files give the same m either way). */
-int orc_read_flow_parameters(const char *path, orc_params *p)
-{
-    FILE *f = fopen(path, "r");
-    if (!f) return -1;
-    char name[128];
-    int t, iv;
-    double dv;
-    memset(p, 0, sizeof(*p));
-#define RDI(field) do { t = fscanf(f, "%127[^,],%d\n", name, &iv); if (t != 2) goto bad; p->field = iv; } while (0)
-#define RDD(field) do { t = fscanf(f, "%127[^,],%lf\n", name, &dv); if (t != 2) goto bad; p->field = dv; } while (0)
-    RDI(dimension);
-    RDI(poly_degree);
-    RDI(phs_degree);
-    RDD(cloud_size_multiplier);
-    RDI(test);
-    RDD(courant_number);
-    RDD(steady_state_tolerance);
-    RDD(poisson_solver_tolerance);
-    { float fv; t = fscanf(f, "%127[^,],%f\n", name, &fv); if (t != 2) goto bad; p->omega = fv; }
-    RDI(num_vcycles);
-    RDI(num_relax);
-    RDI(num_time_steps);
-    RDI(write_interval);
-    RDI(neumann_flag_boundary);
-    p->neumann_flag_boundary = (p->neumann_flag_boundary == 1);
-    RDD(dt);
-    RDI(iter_momentum);
-    RDI(iter_timple);
-    RDD(Re);
-    RDD(facRe);
-    RDD(facdt);
-#undef RDI
-#undef RDD
-    fclose(f);
-    p->rho = 1;                       /* :97-99 */
-    p->mu = p->rho / p->Re;
-    p->nu = p->mu / p->rho;
-    return 0;
-bad:
-    fclose(f);
-    return -2;
-}
-
-/* ===================================================================== */
 /* Monomial exponents: i (x) outer, j (y), k (z) inner (hf/mesh_functions.h:131-170). */
 int orc_poly_terms(int dim, int deg, short *px, short *py, short *pz)
 {

@@ -33,20 +33,6 @@ typedef struct {
 int  orc_read_gmsh(const char *path, int dim, orc_mesh *out);
 void orc_free_mesh(orc_mesh *m);
 
-/* ---- flow_parameters.csv (hf/mesh_functions.h:37-100) ----------------- */
-typedef struct {
-    int dimension, poly_degree, phs_degree;
-    double cloud_size_multiplier;  /* reference: short; float accepted here */
-    int test;
-    double courant_number, steady_state_tolerance, poisson_solver_tolerance;
-    float omega;
-    int num_vcycles, num_relax, num_time_steps, write_interval, neumann_flag_boundary;
-    double dt;
-    int iter_momentum, iter_timple;
-    double Re, facRe, facdt, rho, mu, nu;
-} orc_params;
-int orc_read_flow_parameters(const char *path, orc_params *p);
-
 /* ---- monomial table (hf/mesh_functions.h:131-170) --------------------- */
 int orc_poly_terms(int dim, int deg, short *px, short *py, short *pz);
 

@@ -22,6 +22,7 @@
 #else
 #error "define REF_HEAT, REF_TIMPLE or REF_FS"
 #endif
+#include <string.h>
 #include <unistd.h>
 
 static PointStructure *P = 0;
@@ -188,3 +189,25 @@ double ref_step(void) { return time_implicit_solver(P, F); }
 #elif defined(REF_FS)
 double ref_step(void) { return fractional_step_explicit(P, F); }
 #endif
+
+/* The reference's config readers on a file of the caller's (read_flow_parameters,
+ * hf/mesh_functions.h:37-100; read_grid_filenames, :102-129).  ref_params copies
+ * the global `parameters` the parse left (the struct layout of
+ * hf/structures.h:15-41, which the caller mirrors); ref_grid_filenames returns
+ * the level count and writes the names, 50 bytes each, into names. */
+void ref_params(const char *path, struct parameters *out)
+{
+    memset(&parameters, 0, sizeof(parameters));
+    read_flow_parameters((char *)path);
+    *out = parameters;
+}
+
+int ref_grid_filenames(const char *path, char *names, int max_levels)
+{
+    PointStructure *G = 0;
+    short n = 0;
+    read_grid_filenames(&G, (char *)path, &n);
+    for (int l = 0; l < n && l < max_levels; l++) memcpy(names + 50 * l, G[l].mesh_filename, 50);
+    free(G);
+    return n;
+}

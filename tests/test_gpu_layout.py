@@ -30,14 +30,23 @@ def build(mgm, monkeypatch, sort, dim, sizes, seed=3):
     return mgm.Problem(cfg, pts).setup()
 
 
-@pytest.fixture(params=CASES, ids=["2d", "3d"])
-def pair(request, gpu, monkeypatch):
-    dim, sizes = request.param
+def make_pair(gpu, monkeypatch, dim, sizes):
     monkeypatch.setenv("MGM_SMALL_ROWS", "0")
     print(f"[gpu layout] {dim}D levels {sizes}: plain vs gather-sorted", flush=True)
     A = build(gpu, monkeypatch, False, dim, sizes)
     B = build(gpu, monkeypatch, True, dim, sizes)
     return gpu, A, B
+
+
+@pytest.fixture(params=CASES, ids=["2d", "3d"])
+def pair(request, gpu, monkeypatch):
+    return make_pair(gpu, monkeypatch, *request.param)
+
+
+@pytest.fixture
+def pair_2d(gpu, monkeypatch):
+    """The cavity time steps are two-dimensional: the 2D hierarchy only."""
+    return make_pair(gpu, monkeypatch, *CASES[0])
 
 
 def same_fields(A, B, names=FIELDS):
@@ -88,10 +97,8 @@ def test_solve_bitwise(pair, precision):
     np.testing.assert_array_equal(A.field(0, "p"), B.field(0, "p"))
 
 
-def test_time_steps_bitwise(pair):
-    mgm, A, B = pair
-    if A.cfg.dimension != 2:
-        pytest.skip("the cavity case is two-dimensional")
+def test_time_steps_bitwise(pair_2d):
+    mgm, A, B = pair_2d
     for scheme in ("fs", "timple"):
         errs = []
         for P in (A, B):

@@ -99,13 +99,13 @@ def test_learned_sweep_matches_oracle(gpu, dim, sizes):
 
 
 def test_training_and_learned_solve(gpu):
-    import bench
+    from multigrid_meshless_amd import cases
     P = problem(gpu, 2, (64, 32, 16))
     l0, l1 = P.train_smoother(snapshots=6, samples_per_snapshot=4096, epochs=20, batch=2048, lr=2e-3, seed=5)
     print(f"[gpu learned training] loss {l0:.4f} -> {l1:.4f}", flush=True)
     assert l1 < 0.5 * l0
     g = P.geometry(0)
-    f = bench.fourier_rhs(g["x"], g["y"], 0 * g["x"])
+    f = cases.fourier_rhs(g["x"], g["y"], 0 * g["x"])
     f[g["btag"] != 0] = 0.0
     out = {}
     for learned in (False, True):

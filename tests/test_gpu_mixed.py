@@ -6,19 +6,14 @@ the FP64 residual of the defect-corrected iterate reaches the tolerance, and the
 converged field equals the FP64 multigrid solution -- itself bit-identical to the
 oracle's V-cycle -- to 1e-8 relative (max norm) when both are driven to 1e-12.
 """
-import os
-import sys
-
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
-sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-
 
 def problem(mgm, dim, sizes):
-    import bench
+    from multigrid_meshless_amd import cases
     pts = [mgm.Points.generate(dim, n, 1 + k, 0.2) for k, n in enumerate(sizes)]
     cfg = mgm.config(dimension=dim, poly_degree=3, coarse_poly_degree=3, phs_degree=3,
                      cloud_size_multiplier=3.5 if dim == 3 else 2.0, omega=1.0, num_relax=3,
@@ -26,7 +21,7 @@ def problem(mgm, dim, sizes):
     P = mgm.Problem(cfg, pts).setup()
     P.dirichlet_rows()
     g = P.geometry(0)
-    f = bench.fourier_rhs(g["x"], g["y"], g["z"] if dim == 3 else 0 * g["x"])
+    f = cases.fourier_rhs(g["x"], g["y"], g["z"] if dim == 3 else 0 * g["x"])
     f[g["btag"] != 0] = 0.0
     return P, f
 

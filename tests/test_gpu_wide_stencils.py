@@ -1,0 +1,206 @@
+"""The 3D stencils and device layout of the 1M-point cube against the oracle.
+
+The cube benchmark runs PHS r^3 + cubic polynomials with multiplier 3.5:
+m = 70-point stencils, 90 x 90 local systems.  On its 100^3 hierarchy
+  * level 0 (above 262,144 rows) is ordered in Morton blocks, stored at slice
+    height 8 with gather-sorted copies, and each colour step takes a slice list;
+  * the mid levels use slice height 4 and one launch per colour class;
+  * the small levels relax in one workgroup (sums unrolled to 72 slots);
+  * the weight kernel's autotuner picks 6x16 or 7x16 register tiles.
+The thresholds behind these choices are environment variables, so a 24^3 /
+12^3 / 7^3 hierarchy reproduces the same layout level by level
+(MGM_BLOCKED_ROWS = MGM_GATHER_SORT_ROWS = 8000, MGM_SMALL_ROWS = 1000), and
+the oracle can check it:
+  * weights, lap_Poison and transfer rows under every weight-kernel variant
+    the autotuner may pick: within 1e-11 row-relative of the oracle's pivoted
+    LU (hf/rbf.h:401-421, hf/mesh_functions.h:533-670);
+  * colour-ordered H / T / F V-cycles and the graph-captured solve: bit for bit
+    the oracle's sequential sweep in colour order (hf/heat_conduction.h:25-112,
+    hf/timple_solver.h:235-328, hf/fractional_step_solver.h:168-246).
+Two narrower stencils cover the other unrolled widths of the single-workgroup
+relax: degree-2 polynomials with multiplier 3.5 (m = 35, slice height 16) and
+4 (m = 40, slice height 8).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIZES = (24, 12, 7)
+BENCH_LAYOUT = {"MGM_BLOCKED_ROWS": "8000", "MGM_GATHER_SORT_ROWS": "8000", "MGM_SMALL_ROWS": "1000"}
+PER_COLOUR = dict(BENCH_LAYOUT, MGM_SMALL_ROWS="0")
+# id: (polynomial degree, multiplier, layout)
+CASES = {
+    "m70_bench_layout": (3, 3.5, BENCH_LAYOUT),
+    "m70_per_colour": (3, 3.5, PER_COLOUR),
+    "m35_slice16": (2, 3.5, BENCH_LAYOUT),
+    "m40_slice8": (2, 4.0, BENCH_LAYOUT),
+}
+_ORACLE_CACHE = {}
+
+
+def oracle_rows(oracle, P, key):
+    """The oracle's LU weights, lap_Poison and transfer rows on the GPU's own
+    stencils and geometry (cached per stencil kind: identical across layouts)."""
+    if key in _ORACLE_CACHE:
+        return _ORACLE_CACHE[key]
+    L = P.L
+    lv = []
+    for k in range(L):
+        g = P.geometry(k)
+        info = P.info(k)
+        d = dict(g, cloud=P.cloud(k), nb=info["nb"], poly=info["poly_degree"],
+                 rp=P.points(k, "restriction"), pp=P.points(k, "prolongation"),
+                 ctag=np.zeros(info["N"], np.uint8))
+        lv.append(d)
+    out = []
+    for k, d in enumerate(lv):
+        Dx, Dy, Dz, lap = oracle.weights(d, 3, 3, d["poly"], method=1)
+        rows = dict(Dx=Dx, Dy=Dy, Dz=Dz, lap=lap, lap_Poison=oracle.lap_poisson(d, 3, Dx, Dy, Dz, lap))
+        if k > 0:
+            rows["restr"] = oracle.restriction(lv[k - 1], d, 3, 3, lv[k - 1]["poly"], row_mode=1, method=1)
+        if k < L - 1:
+            rows["prol"] = oracle.prolongation(d, lv[k + 1], 3, 3, lv[k + 1]["poly"], row_mode=1, method=1)
+        out.append(rows)
+    _ORACLE_CACHE[key] = out
+    return out
+
+
+@pytest.fixture(scope="module", params=sorted(CASES))
+def case(request, gpu):
+    poly, mult, env = CASES[request.param]
+    mp = pytest.MonkeyPatch()
+    for k, v in env.items():
+        mp.setenv(k, v)
+    mp.delenv("MGM_LU_VARIANT", raising=False)
+    print(f"[gpu wide stencils] {request.param}: 3D {SIZES}, poly {poly}, multiplier {mult}, {env}", flush=True)
+    cfg = gpu.config(dimension=3, poly_degree=poly, coarse_poly_degree=poly, phs_degree=3,
+                     cloud_size_multiplier=mult, omega=1.0, num_relax=3, transfer_rows=gpu.ROWS_BY_TAG,
+                     sweep_order=gpu.ORDER_COLOUR)
+    P = gpu.Problem(cfg, [gpu.Points.generate(3, n, 1 + k, 0.2) for k, n in enumerate(SIZES)]).setup()
+    yield request.param, P
+    mp.undo()
+
+
+def test_layout_is_the_benchmarks(case):
+    name, P = case
+    i0, i1, i2 = (P.info(k) for k in range(3))
+    assert i0["blocked"] and i0["gather_sorted"] and not i0["small_relax"]
+    assert not i1["blocked"] and not i1["gather_sorted"] and not i1["small_relax"]
+    m = i0["m"]
+    if m > 36:
+        assert (i0["slice_height"], i1["slice_height"]) == (8, 4)
+    else:
+        assert (i0["slice_height"], i1["slice_height"]) == (16, 16)
+    assert i2["small_relax"] == (CASES[name][2]["MGM_SMALL_ROWS"] != "0")
+    assert i0["device_rows"] > i0["N"]          # per-(block, colour) padding
+    print(f"  m={m} mpn={m + i0['n']} colours {[P.info(k)['num_colours'] for k in range(3)]} "
+          f"slice heights {[P.info(k)['slice_height'] for k in range(3)]}", flush=True)
+
+
+def _lu_variants(mpn):
+    t0 = next(t for t in range(2, 9) if 16 * t >= mpn + 4)
+    return [(t, w) for t in (t0, t0 + 1) for w in (4, 1) if t <= 8 and not (w == 4 and t > 6)]
+
+
+def _row_rel(G, W):
+    sc = np.abs(W).max(axis=1)
+    zero = sc == 0
+    assert not np.abs(G[zero]).any(), "rows the oracle leaves empty must stay empty"
+    return (np.abs(G - W).max(axis=1)[~zero] / sc[~zero]).max()
+
+
+def test_weights_every_lu_variant(case, oracle, monkeypatch):
+    name, P = case
+    ref = oracle_rows(oracle, P, name.split("_")[0])
+    mpn = P.info(0)["m"] + P.info(0)["n"]
+    print(f"  autotuned weight kernel at mpn {mpn}: {P.weight_kernel_choice(0)}", flush=True)
+    for tile, wps in _lu_variants(mpn):
+        monkeypatch.setenv("MGM_LU_VARIANT", f"{tile},{wps}")
+        for k in range(P.L):
+            P.rebuild_weights(k)
+        worst = 0.0
+        for k in range(P.L):
+            for key, W in ref[k].items():
+                e = _row_rel(P.matrix(k, key), W)
+                worst = max(worst, e)
+                assert e <= 1e-11, (name, tile, wps, k, key, e)
+        print(f"  tile {tile} wps {wps}: worst row-relative difference from the oracle {worst:.2e}", flush=True)
+    monkeypatch.delenv("MGM_LU_VARIANT")
+    for k in range(P.L):
+        P.rebuild_weights(k)
+
+
+def _oracle_hierarchy(oracle, P):
+    lv = []
+    for k in range(P.L):
+        col = P.colours(k)
+        d = dict(cloud=P.cloud(k), Lp=P.matrix(k, "lap_Poison"), lap=P.matrix(k, "lap"),
+                 btag=P.geometry(k)["btag"], rp=P.points(k, "restriction"), pp=P.points(k, "prolongation"),
+                 order=np.lexsort((np.arange(len(col)), col)).astype(np.int32))
+        if k > 0:
+            d["R"] = P.matrix(k, "restr")
+        if k < P.L - 1:
+            d["P"] = P.matrix(k, "prol")
+        lv.append(d)
+    return oracle.Hierarchy(lv)
+
+
+def _rows(P, dirichlet):
+    """Operator rows of the run: the Neumann rows the weights give, or identity
+    rows on every boundary node (the cube benchmark's Dirichlet problem)."""
+    for k in range(P.L):
+        P.rebuild_weights(k)
+    if dirichlet:
+        P.dirichlet_rows()
+
+
+@pytest.mark.parametrize("variant,dirichlet", [("H", True), ("H", False), ("T", False), ("F", False)])
+def test_colour_vcycles_bitwise(case, oracle, variant, dirichlet):
+    name, P = case
+    import multigrid_meshless_amd as mgm
+    var = getattr(mgm, "VAR_" + variant)
+    _rows(P, dirichlet)
+    H = _oracle_hierarchy(oracle, P)
+    f = np.random.default_rng(11).standard_normal(P.info(0)["N"])
+    if dirichlet:
+        f[P.geometry(0)["btag"] != 0] = 0.0
+    H.lv[0]["s"][:] = f
+    P.zero_fields()
+    P.set_field(0, "source", f)
+    for _ in range(3):
+        H.vcycle(var, 1.0, 3)
+    P.vcycle(var, 3)
+    pf = "pprime" if variant == "T" else "p"
+    for k in range(P.L):
+        np.testing.assert_array_equal(P.field(k, pf), H.lv[k]["p"], err_msg=f"{name} {variant} L{k}")
+        np.testing.assert_array_equal(P.field(k, "res"), H.lv[k]["res"], err_msg=f"{name} {variant} L{k} res")
+        np.testing.assert_array_equal(P.field(k, "source"), H.lv[k]["s"], err_msg=f"{name} {variant} L{k} s")
+    assert np.abs(P.field(0, pf)).max() > 0
+
+
+def test_graph_solve_matches_oracle(case, oracle):
+    """mgm_solve replays one captured V-cycle + fine residual + norm per cycle:
+    its fields equal the oracle's after the same cycles, bit for bit, and its
+    residual history agrees to rounding (the GPU sums the norm as a tree)."""
+    name, P = case
+    _rows(P, True)
+    H = _oracle_hierarchy(oracle, P)
+    btag = P.geometry(0)["btag"]
+    f = np.random.default_rng(12).standard_normal(P.info(0)["N"])
+    f[btag != 0] = 0.0
+    H.lv[0]["s"][:] = f
+    P.zero_fields()
+    P.set_field(0, "source", f)
+    ncyc = 4
+    r = P.solve(1e-300, ncyc)
+    assert r["cycles"] == ncyc
+    fn = oracle.l2_norm_gen(btag, f)
+    hist = []
+    for _ in range(ncyc):
+        H.vcycle(oracle.VAR_H, 1.0, 3)
+        H.residual(oracle.VAR_H, 0)
+        hist.append(oracle.l2_norm_gen(btag, H.lv[0]["res"]) / fn)
+    np.testing.assert_array_equal(P.field(0, "p"), H.lv[0]["p"], err_msg=name)
+    np.testing.assert_allclose(r["history"][1:], hist, rtol=1e-12, atol=0)
+    assert hist[-1] < hist[0]
