@@ -167,6 +167,17 @@ int mgm_l2_norm_gen(const mgm_problem *p, int lev, int field_a, int field_b, dou
  * (multiply_sparse_matrix_vector_gpu for row sets that are not resident.) */
 int mgm_spmv_rows(int n_rows, int n_cols, const double *D, const int32_t *cloud, int ncol,
                   const double *f, double *out);
+/* No counterpart in the reference: mgm_spmv_rows for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and result, bit for bit;
+ * the device staging of D, cloud, f and out lives in a 64 MiB array per GPU
+ * that the library holds until the process ends, so calls whose
+ * 12 * n_rows * n_cols + 8 * (ncol + n_rows) bytes (plus alignment) exceed it
+ * are rejected.  Calls on one GPU share that array: calls that overlap on one
+ * GPU (other host threads, other streams) race.  mgm_init must have selected
+ * the device. */
+int mgm_spmv_rows_static(int n_rows, int n_cols, const double *D, const int32_t *cloud, int ncol,
+                         const double *f, double *out);
 
 /* ---- time steps that call the V-cycle (level-0 fields, resident) ----------- */
 typedef struct {

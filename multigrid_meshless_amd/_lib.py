@@ -86,6 +86,7 @@ SIGNATURES = [
     ("mgm_spmv", _i, [_vp, _i, _i, _i, _i]),
     ("mgm_l2_norm_gen", _i, [_vp, _i, _i, _i, C.POINTER(_d)]),
     ("mgm_spmv_rows", _i, [_i, _i, _vp, _vp, _i, _vp, _vp]),
+    ("mgm_spmv_rows_static", _i, [_i, _i, _vp, _vp, _i, _vp, _vp]),
     ("mgm_ns_set_params", _i, [_vp, _vp]),
     ("mgm_fs_step", _i, [_vp, C.POINTER(_d)]),
     ("mgm_timple_step", _i, [_vp, C.POINTER(_d)]),

@@ -33,6 +33,10 @@ static double now()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+int noalloc_init(int device);
+int spmv_rows_static(int device, int n_rows, int n_cols, const double *D, const int32_t *cloud, int ncol,
+                     const double *f, double *out, hipStream_t s);
+
 static int ensure_init()
 {
     if (g_stream) return 0;
@@ -809,6 +813,8 @@ int mgm_init(int device)
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) MGM_FAIL(-9, "no GPU visible to HIP");
     if (device < 0 || device >= n) MGM_FAIL(-1, "mgm_init: bad device index");
     HIP_TRY(hipSetDevice(device));
+    int rc = noalloc_init(device);   // the allocation-free entry point's scratch address
+    if (rc) return rc;
     if (g_stream) (void)hipStreamDestroy(g_stream);
     HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     g_device = device;
@@ -1373,6 +1379,14 @@ int mgm_spmv_rows(int n_rows, int n_cols, const double *D, const int32_t *cloud,
     HIP_TRY(dy.download(out, n_rows, g_stream));
     HIP_TRY(hipStreamSynchronize(g_stream));
     return 0;
+}
+
+int mgm_spmv_rows_static(int n_rows, int n_cols, const double *D, const int32_t *cloud, int ncol,
+                         const double *f, double *out)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    return spmv_rows_static(g_device, n_rows, n_cols, D, cloud, ncol, f, out, g_stream);
 }
 
 int mgm_l2_norm_gen(const mgm_problem *pbc, int lev, int fa, int fb, double *out)
