@@ -199,6 +199,8 @@ int launch_relax_small(EllT<TW> A, EllT<TW> A2, const int *col_slices, const int
 // single-workgroup relax (its vectors, of vbytes per value, and staging in
 // one workgroup's LDS).
 bool relax_small_fits(int nrows, int m, int m2, int S, int vbytes = 8);
+// Per-device kernel attributes (dynamic LDS limits), set once by mgm_init.
+int stencil_device_init();
 int launch_sor_natural(Ell A, int N, const int *iperm, const double *src, double *p, double w,
                        double c, hipStream_t s);
 // residual forms: 0 = interior, res = s - sum (boundary res = 0)            (H/T)

@@ -497,7 +497,7 @@ static void order_by_colour(Level &L, int dim)
     std::vector<int> byk(N);
     std::iota(byk.begin(), byk.end(), 0);
     std::sort(byk.begin(), byk.end(), [&](int a, int b) { return key[a] != key[b] ? key[a] < key[b] : a < b; });
-    static const int block_rows = [] {
+    const int block_rows = [] {
         const char *e = getenv("MGM_BLOCK_ROWS");
         const int v = e ? atoi(e) : 4096;
         return v > 0 ? v : 4096;
@@ -815,6 +815,7 @@ int mgm_init(int device)
     HIP_TRY(hipSetDevice(device));
     int rc = noalloc_init(device);   // the allocation-free entry point's scratch address
     if (rc) return rc;
+    if ((rc = stencil_device_init())) return rc;
     if (g_stream) (void)hipStreamDestroy(g_stream);
     HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     g_device = device;

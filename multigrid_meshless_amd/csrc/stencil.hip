@@ -818,12 +818,6 @@ static int launch_relax_small_v(EllT<TW> A, EllT<TW> A2, const int *col_slices, 
                                 int nrows, const uint8_t *btag, const int *ref0, const TV *src, TV *p, TV *res,
                                 TV w, TV c, int num_relax, int variant, int nw, size_t shm, hipStream_t s)
 {
-    static bool attr = false;
-    if (!attr) {
-        HIP_TRY(hipFuncSetAttribute((const void *)relax_small_kernel<S, PIN, MT, TW, TV>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSmallLdsMax));
-        attr = true;
-    }
     hipLaunchKernelGGL((relax_small_kernel<S, PIN, MT, TW, TV>), dim3(1), dim3(64 * nw), shm, s, A, A2, col_slices,
                        col_soff, ncol, nrows, btag, ref0, src, p, res, w, c, num_relax, variant);
     HIP_TRY(hipGetLastError());
@@ -892,6 +886,34 @@ RELAX_SMALL_INST(double, double)
 RELAX_SMALL_INST(float, float)
 RELAX_SMALL_INST(bf16, float)
 #undef RELAX_SMALL_INST
+// Dynamic-LDS limit of the single-workgroup relax (up to 159 KB of the 160 KB
+// a workgroup may have), set for every instance once per device by mgm_init:
+// outside any stream capture, so a captured relax launches nothing else.
+template <typename TW, typename TV>
+static int small_lds_attrs()
+{
+    const int mx = (int)kSmallLdsMax;
+#define SMALL_ATTR(S, PIN, MT)                                                         \
+    HIP_TRY(hipFuncSetAttribute((const void *)relax_small_kernel<S, PIN, MT, TW, TV>, \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, mx))
+    SMALL_ATTR(4, false, 24); SMALL_ATTR(4, false, 40); SMALL_ATTR(4, false, 72);
+    SMALL_ATTR(8, false, 24); SMALL_ATTR(8, false, 40); SMALL_ATTR(8, false, 72);
+    SMALL_ATTR(16, false, 24); SMALL_ATTR(16, false, 36);
+    if constexpr (!std::is_same<TW, bf16>::value) {
+        SMALL_ATTR(4, true, 24); SMALL_ATTR(4, true, 40); SMALL_ATTR(4, true, 72);
+        SMALL_ATTR(8, true, 24); SMALL_ATTR(8, true, 40); SMALL_ATTR(8, true, 72);
+        SMALL_ATTR(16, true, 24); SMALL_ATTR(16, true, 36);
+    }
+#undef SMALL_ATTR
+    return 0;
+}
+int stencil_device_init()
+{
+    int rc;
+    if ((rc = small_lds_attrs<double, double>())) return rc;
+    if ((rc = small_lds_attrs<float, float>())) return rc;
+    return small_lds_attrs<bf16, float>();
+}
 int launch_sor_natural(Ell A, int N, const int *iperm, const double *src, double *p, double w,
                        double c, hipStream_t s)
 {
