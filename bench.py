@@ -9,10 +9,10 @@ of multicolour SOR (omega 1, 3 sweeps) until l2_norm_gen(r)/l2_norm_gen(f) <= 1e
 
 Prints one JSON line (rank 0, one GPU).  `roofline` is the FP64 stencil apply
 (lap_Poison over level 0) timed with HIP events on the library stream;
-`cpu_baseline` is the oracle's V-cycle in the same colour order on the same
-operators, the rows of each colour class swept in parallel on every available
-host thread (the 1-thread time is in its sample), timed for a bounded number of
-cycles.
+`cpu_baseline` is the oracle's whole solve to the same tolerance, its V-cycle in
+the same colour order on the same operators, the rows of each colour class
+swept in parallel on every available host thread (the 1-thread rate and a
+sampled CPU weight-build rate are in its sample).
 """
 from __future__ import annotations
 
@@ -170,14 +170,13 @@ def cavity_march(mgm, args):
     return out
 
 
-def cpu_baseline(P, sample_cycles, f):
-    """Seconds per V-cycle (+ fine residual norm) of the oracle's colour-ordered
-    sweep on the same operators, at 1 thread and at every available thread (the
-    rows of a colour class in parallel; same results): {threads: s/cycle}."""
+def cpu_hierarchy(P):
+    """The oracle's hierarchy on the GPU's operators, in the GPU's colour order
+    (the rows of a colour class are swept in parallel, the same bits as the
+    sequential sweep)."""
     import oracle
-    L = P.L
     lv = []
-    for k in range(L):
+    for k in range(P.L):
         col = P.colours(k)
         order = np.lexsort((np.arange(len(col)), col)).astype(np.int32)
         classes = np.searchsorted(col[order], np.arange(col.max() + 2)).astype(np.int32)
@@ -185,25 +184,57 @@ def cpu_baseline(P, sample_cycles, f):
                  rp=P.points(k, "restriction"), pp=P.points(k, "prolongation"), order=order, classes=classes)
         if k > 0:
             d["R"] = P.matrix(k, "restr")
-        if k < L - 1:
+        if k < P.L - 1:
             d["P"] = P.matrix(k, "prol")
         lv.append(d)
-    H = oracle.Hierarchy(lv)
-    fnorm = oracle.l2_norm_gen(lv[0]["btag"], f)
-    out = {}
-    for nt in sorted({1, oracle.num_threads()}):
-        oracle.set_num_threads(nt)
+    return oracle.Hierarchy(lv), lv
+
+
+def cpu_baseline(P, f, tol, max_cycles, weight_sample):
+    """The oracle on the host: (1) the same solve to tol on every available
+    thread -- V-cycle, fine residual and l2_norm_gen norm per cycle, timed
+    whole; (2) two cycles on one thread; (3) the reference's weight build
+    (Gauss-Jordan inverse per node, hf/rbf.h:401-421) of the first
+    weight_sample level-0 systems on every thread, as systems/s."""
+    import oracle
+    H, lv = cpu_hierarchy(P)
+    btag = lv[0]["btag"]
+    fnorm = oracle.l2_norm_gen(btag, f)
+    nt = oracle.num_threads()
+
+    def reset():
         for d in H.lv:
             d["p"][:] = 0
             d["s"][:] = 0
         H.lv[0]["s"][:] = f
-        t0 = time.perf_counter()
-        for _ in range(sample_cycles):
-            H.vcycle(oracle.VAR_H, 1.0, 3)
-            H.residual(oracle.VAR_H, 0)
-            oracle.l2_norm_gen(lv[0]["btag"], H.lv[0]["res"]) / fnorm
-        out[nt] = (time.perf_counter() - t0) / sample_cycles
-    return out
+
+    oracle.set_num_threads(nt)
+    reset()
+    t0 = time.perf_counter()
+    cycles, ratio = 0, 1.0
+    while cycles < max_cycles and ratio > tol:
+        H.vcycle(oracle.VAR_H, 1.0, 3)
+        H.residual(oracle.VAR_H, 0)
+        ratio = oracle.l2_norm_gen(btag, H.lv[0]["res"]) / fnorm
+        cycles += 1
+    t_solve = time.perf_counter() - t0
+    oracle.set_num_threads(1)
+    reset()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        H.vcycle(oracle.VAR_H, 1.0, 3)
+        H.residual(oracle.VAR_H, 0)
+        oracle.l2_norm_gen(btag, H.lv[0]["res"])
+    t1 = (time.perf_counter() - t0) / 2
+    oracle.set_num_threads(nt)
+    g = P.geometry(0)
+    sub = dict(x=g["x"], y=g["y"], z=g["z"], cloud=np.ascontiguousarray(lv[0]["cloud"][:weight_sample]))
+    info = P.info(0)
+    t0 = time.perf_counter()
+    oracle.weights(sub, 3, 3, info["poly_degree"], method=0)
+    t_w = time.perf_counter() - t0
+    return dict(threads=nt, solve_s=t_solve, cycles=cycles, final_ratio=ratio, s_per_cycle_1thread=t1,
+                weight_systems=weight_sample, weight_s=t_w)
 
 
 def main():
@@ -215,7 +246,8 @@ def main():
     ap.add_argument("--levels", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-cycles", type=int, default=400)
-    ap.add_argument("--cpu-cycles", type=int, default=2)
+    ap.add_argument("--cpu-weight-sample", type=int, default=16384,
+                    help="level-0 systems of the CPU weight-build sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-mixed", action="store_true", help="skip the reduced-precision solves")
@@ -370,13 +402,20 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        per = cpu_baseline(P, args.cpu_cycles, f)
-        nt = max(per)
-        cpu = {"value": round(per[nt] * cycles[0], 3), "unit": "s", "cores": nt, "kind": "port",
-               "sample": f"{args.cpu_cycles} V-cycles (+ fine residual norm) of the oracle's colour-ordered sweep "
-                         f"on the same 1M-point operators (rows of a colour class in parallel), "
-                         f"{per[nt]:.3f} s/cycle on {nt} threads, x {cycles[0]} cycles the GPU needed; "
-                         f"1 thread: {per[1]:.3f} s/cycle = {per[1] * cycles[0]:.1f} s"}
+        c = cpu_baseline(P, f, args.tol, 2 * args.max_cycles, args.cpu_weight_sample)
+        cpu = {"value": round(c["solve_s"], 3), "unit": "s", "cores": c["threads"], "kind": "port",
+               "sample": f"the whole solve to {args.tol} on the host: the oracle's colour-ordered V-cycle on the same "
+                         f"1M-point operators (rows of a colour class in parallel on {c['threads']} threads), "
+                         f"{c['cycles']} cycles to {c['final_ratio']:.2e} in {c['solve_s']:.2f} s; 1 thread: "
+                         f"{c['s_per_cycle_1thread']:.3f} s/cycle (2 cycles timed) = "
+                         f"{c['s_per_cycle_1thread'] * c['cycles']:.1f} s for the solve; weight build (the "
+                         f"reference's Gauss-Jordan per node) of {c['weight_systems']} level-0 systems in "
+                         f"{c['weight_s']:.2f} s on {c['threads']} threads = "
+                         f"{c['weight_systems'] / c['weight_s']:.0f} systems/s",
+               "weight_build_systems_per_s": round(c["weight_systems"] / c["weight_s"]),
+               "solve_cycles": c["cycles"]}
+        print(f"[bench] CPU baseline: solve {c['solve_s']:.2f} s ({c['cycles']} cycles, {c['threads']} threads), "
+              f"weights {c['weight_systems'] / c['weight_s']:.0f} systems/s", file=sys.stderr, flush=True)
 
     out = {
         "metric": "time-to-1e-8 residual, 3D 1M-pt MG Poisson (s); stencil-apply HBM GB/s",
