@@ -147,7 +147,18 @@ struct EllT {
     const TW *w0 = nullptr;
     bool sorted = false;
     int shift = 0;
+    // LDS-window form of the full passes (levels in Morton blocks): block b
+    // covers device rows [win_row0[b], win_row0[b+1]); its stencils read those
+    // rows and the halo rows win_halo[win_hoff[b] .. win_hoff[b+1]) (sorted).
+    // win_col is the plain layout's column slab with every column replaced by
+    // its place in the block's window: own rows first (column - row0), then
+    // the halo in list order.  The weights are the plain slab (w0 when sorted,
+    // else w).  win_max bounds a window's values.
+    const int *win_col = nullptr, *win_row0 = nullptr, *win_hoff = nullptr, *win_halo = nullptr;
+    int win_blocks = 0, win_max = 0;
 };
+// Waves per workgroup of the LDS-window full passes (one workgroup per block).
+constexpr int kWindowWaves = 16;
 // Bits of a slot index of a slice of S*m entries (S*m <= 1024).
 inline int slot_bits(int S, int m)
 {

@@ -21,14 +21,15 @@ int op_learned_sweep_mp(mgm_problem *pb, int l, int (*residual_mp)(Level &));
 namespace {
 
 // A level's narrowed operator: gather-sorted where the level has the FP64
-// slab sorted (EllT::sorted; plain is the diagonal source), else plain.
+// slab sorted (EllT::sorted; plain is the diagonal source), else plain; with
+// the level's LDS-window tables where it has them.
 template <typename T>
 EllT<T> narrowed(const Level &L, const DevBuf<T> &plain, const DevBuf<T> &sorted)
 {
     const bool st = stream_slab(L.Nd, L.m, sizeof(T));
     if (L.gs && sorted.p)
-        return EllT<T>{L.gcol.p, sorted.p, L.m, L.S, st, plain.p, true, 32 - slot_bits(L.S, L.m)};
-    return EllT<T>{L.col.p, plain.p, L.m, L.S, st};
+        return L.with_window(EllT<T>{L.gcol.p, sorted.p, L.m, L.S, st, plain.p, true, 32 - slot_bits(L.S, L.m)});
+    return L.with_window(EllT<T>{L.col.p, plain.p, L.m, L.S, st});
 }
 template <typename T>
 EllT<T> narrowed_prol(const Level &F, const DevBuf<T> &plain, const DevBuf<T> &sorted)

@@ -75,13 +75,29 @@ struct Level {
     std::array<DevBuf<double>, NUM_OPS> gslab; // OP_LAP and OP_LAPP only
     DevBuf<double> gpw;
 
-    // The operator as the stencil passes read it: gather-sorted where built.
+    // ---- LDS windows of the full passes (levels in Morton blocks) ----
+    std::vector<int> blk_row0;                 // device row where each Morton block starts (+ end)
+    DevBuf<int> win_col, win_row0, win_hoff, win_halo;
+    int win_blocks = 0, win_max = 0;           // 0 blocks: no windows
+    bool win_tried = false;                    // tables built (or found not to fit) once
+    template <typename T>
+    EllT<T> with_window(EllT<T> A) const
+    {
+        if (win_blocks) {
+            A.win_col = win_col.p; A.win_row0 = win_row0.p; A.win_hoff = win_hoff.p; A.win_halo = win_halo.p;
+            A.win_blocks = win_blocks; A.win_max = win_max;
+        }
+        return A;
+    }
+
+    // The operator as the stencil passes read it: gather-sorted where built,
+    // with the LDS-window tables where built.
     Ell op(int which) const
     {
         if (gs && gslab[which].p)
-            return Ell{gcol.p, gslab[which].p, m, S, stream_slab(Nd, m, 8), slab[which].p, true,
-                       32 - slot_bits(S, m)};
-        return op_plain(which);
+            return with_window(Ell{gcol.p, gslab[which].p, m, S, stream_slab(Nd, m, 8), slab[which].p, true,
+                                   32 - slot_bits(S, m)});
+        return with_window(op_plain(which));
     }
     // The operator in plain slice order (entry j of row r at ell_at(m, S, r, j)).
     Ell op_plain(int which) const { return Ell{col.p, slab[which].p, m, S, stream_slab(Nd, m, 8)}; }

@@ -274,6 +274,94 @@ rows_kernel(EllT<TW> A, const int *__restrict__ slices, int first, int nslices, 
                           prod + (size_t)wave * A.m * S);
 }
 
+// One slice of a full pass in the LDS-window form: as slice_rows on the
+// plain slab order, with x read from the block's window in LDS (win_col holds
+// each entry's place there).  Rows that the mode skips load nothing.
+template <int S, typename TW, typename TV>
+__device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice, int mode,
+                                    const uint8_t *__restrict__ btag, const TV *__restrict__ src, const TV *win,
+                                    TV *y, TV *P)
+{
+    constexpr int LPR = 64 / S;
+    const int lane = threadIdx.x & 63;
+    const int ri = lane % S, jo = lane / S;
+    const int m = A.m;
+    const long r = slice * S + ri;
+    const uint8_t tag = btag[r];
+    const bool act = tag != ROW_PAD, bnd = tag == ROW_BOUNDARY;
+    const bool skip = bnd && skips_boundary(mode);
+    const bool comb = jo == 0 && act;
+    TV sv = 0;
+    if (comb && mode != M_SPMV && !skip) sv = src[r];
+    if (act && !skip) {
+        const size_t base = (size_t)slice * S * m + (size_t)jo * S + ri;
+        const TW *w = wp + base;
+        const int *c = A.win_col + base;
+        TV *pp = P + jo * S + ri;
+        const int nt = (m - jo + LPR - 1) / LPR;
+        if (A.stream) {
+#pragma unroll 4
+            for (int t = 0; t < nt; t++) pp[t * 64] = wv(ldnt(w + t * 64)) * win[ldnt(c + t * 64)];
+        } else {
+#pragma unroll 4
+            for (int t = 0; t < nt; t++) pp[t * 64] = wv(w[t * 64]) * win[c[t * 64]];
+        }
+    }
+    wave_sync();
+    if (comb) {
+        switch (mode) {
+        case M_SPMV:
+            y[r] = ordered_sum<false>(P, S, ri, 0, m, TV(0));
+            break;
+        case M_RES0:
+        case M_RES3:
+            if (bnd) {
+                if (mode == M_RES0) y[r] = 0;
+                break;
+            }
+            y[r] = sv - ordered_sum<false>(P, S, ri, 0, m, TV(0));
+            break;
+        case M_RES1:
+        case M_RES2:
+            if (mode == M_RES1 && bnd) {
+                y[r] = 0;
+                break;
+            }
+            y[r] = ordered_sum<true>(P, S, ri, 0, m, sv);
+            break;
+        }
+    }
+    wave_sync();   // P is reused by the wave's next slice
+}
+
+// Full pass (SpMV or a residual form) over a level in Morton blocks, one
+// workgroup per block: the block's x window -- its own rows, then its halo
+// rows -- is loaded into LDS once (coalesced own rows, one gather per halo
+// row), and every stencil entry reads x there instead of gathering it from
+// memory: a block of 4096 rows gathers ~4k halo values instead of ~290k
+// stencil entries.  The products and ordered sums are those of rows_kernel,
+// so the results are bitwise the same.
+template <int S, typename TW, typename TV>
+__global__ void __launch_bounds__(64 * kWindowWaves)
+window_rows_kernel(EllT<TW> A, int mode, const uint8_t *__restrict__ btag, const TV *__restrict__ src,
+                   const TV *__restrict__ x, TV *y)
+{
+    extern __shared__ unsigned char lds_raw[];
+    constexpr int W = kWindowWaves;
+    const int m = A.m;
+    const int wave = threadIdx.x >> 6;
+    TV *P = reinterpret_cast<TV *>(lds_raw) + (size_t)wave * m * S;
+    TV *win = reinterpret_cast<TV *>(lds_raw) + (size_t)W * m * S;
+    const int b = (int)xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = A.win_row0[b], r1 = A.win_row0[b + 1];
+    const int h0 = A.win_hoff[b], nh = A.win_hoff[b + 1] - h0, nown = r1 - r0;
+    for (int i = threadIdx.x; i < nown; i += 64 * W) win[i] = x[r0 + i];
+    for (int i = threadIdx.x; i < nh; i += 64 * W) win[nown + i] = x[A.win_halo[h0 + i]];
+    __syncthreads();
+    const TW *wp = A.sorted ? A.w0 : A.w;
+    for (int sl = r0 / S + wave; sl < r1 / S; sl += W) window_slice<S>(A, wp, sl, mode, btag, src, win, y, P);
+}
+
 // Weights and columns of this lane's entries j = t*LPR + jo < MT of one
 // slice (entry j of row ri sits at slice*S*m + j*S + ri).  Slots j >= m load
 // entry m-1 again: every load is in the row's range and none is guarded, so
@@ -728,10 +816,37 @@ int launch_rows_w(EllT<TW> A, const int *slices, int first, int nslices, int nro
 // four waves, 0.304 s at two, 0.299 s at one), while the full-pass apply
 // loses 2.5% with smaller workgroups.
 template <typename TW, typename TV>
+size_t window_lds(const EllT<TW> &A)
+{
+    return ((size_t)kWindowWaves * A.m * A.S + (size_t)A.win_max) * sizeof(TV);
+}
+template <typename TW, typename TV>
+int launch_window(EllT<TW> A, int mode, const uint8_t *btag, const TV *src, const TV *x, TV *y, hipStream_t s)
+{
+    const dim3 grid(A.win_blocks), block(64 * kWindowWaves);
+    const size_t shm = window_lds<TW, TV>(A);
+    if (A.S == 4)
+        hipLaunchKernelGGL((window_rows_kernel<4, TW, TV>), grid, block, shm, s, A, mode, btag, src, x, y);
+    else if (A.S == 8)
+        hipLaunchKernelGGL((window_rows_kernel<8, TW, TV>), grid, block, shm, s, A, mode, btag, src, x, y);
+    else if (A.S == 16)
+        hipLaunchKernelGGL((window_rows_kernel<16, TW, TV>), grid, block, shm, s, A, mode, btag, src, x, y);
+    else
+        MGM_FAIL(-1, "unsupported slice height");
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <typename TW, typename TV>
 int launch_rows(EllT<TW> A, const int *slices, int first, int nslices, int nrows, int mode,
                 const uint8_t *btag, const TV *src, const TV *x, TV *y, TV w, TV c, int keep,
                 hipStream_t s)
 {
+    // full SpMV and residual passes over a level with LDS windows
+    if (A.win_blocks && btag && !slices && first == 0 && nslices < 0 &&
+        (mode == M_SPMV || mode == M_RES0 || mode == M_RES1 || mode == M_RES2 || mode == M_RES3) &&
+        window_lds<TW, TV>(A) <= kSmallLdsMax)
+        return launch_window(A, mode, btag, src, x, y, s);
     if (nslices < 0) nslices = (nrows + A.S - 1) / A.S;  // full pass
     if (nslices <= 0 || nrows <= 0) return 0;
     if (mode == M_SOR)
@@ -907,12 +1022,23 @@ static int small_lds_attrs()
 #undef SMALL_ATTR
     return 0;
 }
+template <typename TW, typename TV>
+static int window_lds_attrs()
+{
+    for (const void *f : {(const void *)window_rows_kernel<4, TW, TV>, (const void *)window_rows_kernel<8, TW, TV>,
+                          (const void *)window_rows_kernel<16, TW, TV>})
+        HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSmallLdsMax));
+    return 0;
+}
 int stencil_device_init()
 {
     int rc;
     if ((rc = small_lds_attrs<double, double>())) return rc;
     if ((rc = small_lds_attrs<float, float>())) return rc;
-    return small_lds_attrs<bf16, float>();
+    if ((rc = small_lds_attrs<bf16, float>())) return rc;
+    if ((rc = window_lds_attrs<double, double>())) return rc;
+    if ((rc = window_lds_attrs<float, float>())) return rc;
+    return window_lds_attrs<bf16, float>();
 }
 int launch_sor_natural(Ell A, int N, const int *iperm, const double *src, double *p, double w,
                        double c, hipStream_t s)
