@@ -232,9 +232,23 @@ int mgm_mlp_get_weights(const mgm_problem *p, float *w);
  * mean((f(X) - t)^2) with its gradient (layout of the weights), FP32. */
 int mgm_mlp_forward(mgm_problem *p, int n, const float *X, float *y);
 int mgm_mlp_gradient(mgm_problem *p, int n, const float *X, const float *t, double *loss, float *grad);
+/* Training rows of level-0 nodes (reference-order indices) from caller
+ * fields in reference order: X[b] (inputs values) as the network reads node
+ * nodes[b] with residual field res, and the target t[b] = err[i] a_ii / sigma_i
+ * (0 where sigma_i = 0), unclipped.  FP32 outputs. */
+int mgm_mlp_features(mgm_problem *p, int n, const int32_t *nodes, const double *res, const double *err, float *X,
+                     float *t);
+/* Adam (beta 0.9 / 0.999, eps 1e-8, bias-corrected step size) on the current
+ * weights over caller rows: each epoch shuffles the rows (Fisher-Yates from
+ * seed, splitmix64) and takes floor(n / batch) steps of batch rows; loss over
+ * all rows before and after. */
+int mgm_mlp_fit(mgm_problem *p, int n, const float *X, const float *t, int epochs, int batch, float lr,
+                uint64_t seed, double *loss_first, double *loss_last);
 /* Train on level 0 from (error, residual) pairs of the plain V-cycle (the
- * fields of the problem are overwritten); loss over all samples before and
- * after training. */
+ * fields of the problem are overwritten): the rows of mgm_mlp_features with
+ * targets clipped to |t| <= 8, He-normal initial weights, then mgm_mlp_fit
+ * with the config's epochs, batch, lr and the seed's stream continued; loss
+ * over all samples before and after training. */
 int mgm_train_smoother(mgm_problem *p, const mgm_train_config *cfg, double *loss_first, double *loss_last);
 int mgm_use_learned_smoother(mgm_problem *p, int enable);
 int mgm_learned_fell_back(const mgm_problem *p);      /* 1 if the last solve fell back to SOR */

@@ -313,6 +313,30 @@ class Problem:
                                      g.ctypes.data), "mlp_gradient")
         return loss.value, g
 
+    def mlp_features(self, nodes, res, err):
+        """Network inputs and (unclipped) targets of level-0 nodes from
+        reference-order residual and error fields."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        res = np.ascontiguousarray(res, dtype=np.float64)
+        err = np.ascontiguousarray(err, dtype=np.float64)
+        N = self.info(0)["N"]
+        assert res.size == N and err.size == N
+        X = np.zeros((nodes.size, self.mlp_inputs()), np.float32)
+        t = np.zeros(nodes.size, np.float32)
+        check(lib().mgm_mlp_features(self._h, nodes.size, nodes.ctypes.data, res.ctypes.data, err.ctypes.data,
+                                     X.ctypes.data, t.ctypes.data), "mlp_features")
+        return X, t
+
+    def mlp_fit(self, X, t, epochs=1, batch=4096, lr=1e-3, seed=7):
+        """Adam on the current weights over rows (X, t); (loss before, loss after)."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        assert X.shape == (t.size, self.mlp_inputs())
+        l0, l1 = C.c_double(), C.c_double()
+        check(lib().mgm_mlp_fit(self._h, t.size, X.ctypes.data, t.ctypes.data, epochs, batch, lr, seed,
+                                C.byref(l0), C.byref(l1)), "mlp_fit")
+        return l0.value, l1.value
+
     def train_smoother(self, snapshots=8, samples_per_snapshot=8192, epochs=30, batch=4096, lr=1e-3, seed=7):
         from ._lib import TrainConfig
         cfg = TrainConfig(snapshots, samples_per_snapshot, epochs, batch, lr, seed)

@@ -97,6 +97,8 @@ SIGNATURES = [
     ("mgm_mlp_get_weights", _i, [_vp, _vp]),
     ("mgm_mlp_forward", _i, [_vp, _i, _vp, _vp]),
     ("mgm_mlp_gradient", _i, [_vp, _i, _vp, _vp, C.POINTER(_d), _vp]),
+    ("mgm_mlp_features", _i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    ("mgm_mlp_fit", _i, [_vp, _i, _vp, _vp, _i, _i, _f, C.c_uint64, C.POINTER(_d), C.POINTER(_d)]),
     ("mgm_train_smoother", _i, [_vp, C.POINTER(TrainConfig), C.POINTER(_d), C.POINTER(_d)]),
     ("mgm_use_learned_smoother", _i, [_vp, _i]),
     ("mgm_learned_fell_back", _i, [_vp]),

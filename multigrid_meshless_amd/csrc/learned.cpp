@@ -123,6 +123,78 @@ int set_weights(mgm_problem *pb, LearnedState &S, const float *w)
     return 0;
 }
 
+// Adam on the current weights over device rows X (n x in), targets t:
+// `epochs` passes, each over a fresh shuffle (Fisher-Yates from `seed`) cut
+// into floor(n / batch) batches; loss over all rows before and after.
+int fit(mgm_problem *pb, LearnedState &S, int n, const float *X, const float *t, int epochs, int batch, float lr,
+        uint64_t &seed, double *loss_first, double *loss_last)
+{
+    const int in = S.in;
+    const size_t nw = mlp_weight_count(in);
+    DevBuf<float> M, V, G, H1, H2, y, dy, dZ1, dZ2, Xb, tb, part;
+    HIP_TRY(M.alloc(nw)); HIP_TRY(V.alloc(nw)); HIP_TRY(G.alloc(nw));
+    HIP_TRY(part.alloc(mlp_gradient_workspace(in)));
+    HIP_TRY(hipMemsetAsync(M.p, 0, sizeof(float) * nw, stream()));
+    HIP_TRY(hipMemsetAsync(V.p, 0, sizeof(float) * nw, stream()));
+    const int B = std::min(batch, n);
+    HIP_TRY(H1.alloc((size_t)n * MLP_H)); HIP_TRY(H2.alloc((size_t)n * MLP_H)); HIP_TRY(y.alloc(n));
+    HIP_TRY(dy.alloc(B)); HIP_TRY(dZ1.alloc((size_t)B * MLP_H)); HIP_TRY(dZ2.alloc((size_t)B * MLP_H));
+    HIP_TRY(Xb.alloc((size_t)B * in)); HIP_TRY(tb.alloc(B));
+    DevBuf<int> didx;
+    auto full_loss = [&](double *out) -> int {
+        int r = mlp_forward(n, in, X, S.W.p, H1.p, H2.p, y.p, stream());
+        if (!r) r = mlp_sqerr(n, y.p, t, pb->partial.p, pb->scal.p, stream());
+        if (r) return r;
+        double s = 0;
+        HIP_TRY(hipMemcpyAsync(&s, pb->scal.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
+        HIP_TRY(hipStreamSynchronize(stream()));
+        *out = s / n;
+        return 0;
+    };
+    double l0 = 0, l1 = 0;
+    int rc = full_loss(&l0);
+    if (rc) return rc;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    long step = 0;
+    for (int e = 0; e < epochs; e++) {
+        for (int i = n - 1; i > 0; i--) std::swap(order[i], order[mix(seed) % (uint64_t)(i + 1)]);
+        HIP_TRY(didx.upload(order, stream()));
+        for (int b0 = 0; b0 + B <= n; b0 += B) {
+            step++;
+            // bias-corrected step size: lr sqrt(1 - b2^step) / (1 - b1^step)
+            const float lr_t = lr * (float)(std::sqrt(1.0 - std::pow(b2, step)) / (1.0 - std::pow(b1, step)));
+            rc = mlp_gather_batch(B, in, didx.p + b0, X, t, Xb.p, tb.p, stream());
+            if (!rc) rc = mlp_forward(B, in, Xb.p, S.W.p, H1.p, H2.p, y.p, stream());
+            if (!rc) rc = mlp_gradient(B, in, Xb.p, tb.p, S.W.p, H1.p, H2.p, y.p, dy.p, dZ2.p, dZ1.p, part.p, G.p,
+                                       stream());
+            if (!rc) rc = mlp_adam((int)nw, S.W.p, G.p, M.p, V.p, lr_t, b1, b2, eps, stream());
+            if (rc) return rc;
+        }
+        HIP_TRY(hipStreamSynchronize(stream()));
+    }
+    if ((rc = full_loss(&l1))) return rc;
+    HIP_TRY(S.W.download(S.w.data(), nw, stream()));
+    HIP_TRY(hipStreamSynchronize(stream()));
+    for (float v : S.w)
+        if (!std::isfinite(v)) MGM_FAIL(-11, "train_smoother: training diverged");
+    S.prepared = false;
+    if (loss_first) *loss_first = l0;
+    if (loss_last) *loss_last = l1;
+    return 0;
+}
+
+// Level-0 reference-order field as a device-order vector (padding rows 0).
+int upload_ref_field(const Level &L, const double *v, DevBuf<double> &out)
+{
+    std::vector<double> t(L.Nd, 0.0);
+    for (int d = 0; d < L.Nd; d++)
+        if (L.perm[d] >= 0) t[d] = v[L.perm[d]];
+    HIP_TRY(out.upload(t, stream()));
+    return 0;
+}
+
 }  // namespace
 
 namespace mgm {
@@ -332,57 +404,57 @@ int mgm_train_smoother(mgm_problem *pb, const mgm_train_config *cfg, double *los
         for (int i = 0; i < MLP_H; i++) w3[i] = (float)(normal(seed) * std::sqrt(1.0 / MLP_H));
     }
     if ((rc = set_weights(pb, S, w.data()))) return rc;
-    DevBuf<float> M, V, G, H1, H2, y, dy, dZ1, dZ2, Xb, tb, part;
-    HIP_TRY(M.alloc(nw)); HIP_TRY(V.alloc(nw)); HIP_TRY(G.alloc(nw));
-    HIP_TRY(part.alloc(mlp_gradient_workspace(in)));
-    HIP_TRY(hipMemsetAsync(M.p, 0, sizeof(float) * nw, stream()));
-    HIP_TRY(hipMemsetAsync(V.p, 0, sizeof(float) * nw, stream()));
-    const int B = std::min(cfg->batch, n);
-    HIP_TRY(H1.alloc((size_t)n * MLP_H)); HIP_TRY(H2.alloc((size_t)n * MLP_H)); HIP_TRY(y.alloc(n));
-    HIP_TRY(dy.alloc(B)); HIP_TRY(dZ1.alloc((size_t)B * MLP_H)); HIP_TRY(dZ2.alloc((size_t)B * MLP_H));
-    HIP_TRY(Xb.alloc((size_t)B * in)); HIP_TRY(tb.alloc(B));
-    DevBuf<int> didx;
-    auto full_loss = [&](double *out) -> int {
-        int r = mlp_forward(n, in, X.p, S.W.p, H1.p, H2.p, y.p, stream());
-        if (!r) r = mlp_sqerr(n, y.p, t.p, pb->partial.p, pb->scal.p, stream());
-        if (r) return r;
-        double s = 0;
-        HIP_TRY(hipMemcpyAsync(&s, pb->scal.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
-        HIP_TRY(hipStreamSynchronize(stream()));
-        *out = s / n;
-        return 0;
-    };
-    double l0 = 0, l1 = 0;
-    if ((rc = full_loss(&l0))) return rc;
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    std::vector<int> order(n);
-    for (int i = 0; i < n; i++) order[i] = i;
-    long step = 0;
-    for (int e = 0; e < cfg->epochs; e++) {
-        for (int i = n - 1; i > 0; i--) std::swap(order[i], order[mix(seed) % (uint64_t)(i + 1)]);
-        HIP_TRY(didx.upload(order, stream()));
-        for (int b0 = 0; b0 + B <= n; b0 += B) {
-            step++;
-            const float lr_t = cfg->lr * (float)(std::sqrt(1.0 - std::pow(b2, step)) / (1.0 - std::pow(b1, step)));
-            rc = mlp_gather_batch(B, in, didx.p + b0, X.p, t.p, Xb.p, tb.p, stream());
-            if (!rc) rc = mlp_forward(B, in, Xb.p, S.W.p, H1.p, H2.p, y.p, stream());
-            if (!rc) rc = mlp_gradient(B, in, Xb.p, tb.p, S.W.p, H1.p, H2.p, y.p, dy.p, dZ2.p, dZ1.p, part.p, G.p,
-                                       stream());
-            if (!rc) rc = mlp_adam((int)nw, S.W.p, G.p, M.p, V.p, lr_t, b1, b2, eps, stream());
-            if (rc) return rc;
-        }
-        HIP_TRY(hipStreamSynchronize(stream()));
-    }
-    if ((rc = full_loss(&l1))) return rc;
-    HIP_TRY(S.W.download(S.w.data(), nw, stream()));
-    HIP_TRY(hipStreamSynchronize(stream()));
-    for (float v : S.w)
-        if (!std::isfinite(v)) MGM_FAIL(-11, "train_smoother: training diverged");
-    S.prepared = false;
+    rc = fit(pb, S, n, X.p, t.p, cfg->epochs, cfg->batch, cfg->lr, seed, loss_first, loss_last);
+    if (rc) return rc;
     S.enabled = was_enabled;
     if (S.enabled && (rc = prepare(pb, S))) return rc;
-    if (loss_first) *loss_first = l0;
-    if (loss_last) *loss_last = l1;
+    return 0;
+}
+
+int mgm_mlp_features(mgm_problem *pb, int n, const int32_t *nodes, const double *res, const double *err, float *X,
+                     float *t)
+{
+    if (!pb || !pb->ready) MGM_FAIL(-1, "problem is not set up");
+    if (n <= 0 || !nodes || !res || !err || !X || !t) MGM_FAIL(-1, "mlp_features: bad arguments");
+    LearnedState &S = state(pb);
+    Level &L0 = pb->lv[0];
+    if (L0.m > MLP_MAX_M) MGM_FAIL(-1, "learned smoother: stencil wider than supported");
+    int rc = ensure_coords(pb, S);
+    if (rc) return rc;
+    std::vector<int> rows(n);
+    for (int b = 0; b < n; b++) {
+        if (nodes[b] < 0 || nodes[b] >= L0.N) MGM_FAIL(-1, "mlp_features: node index out of range");
+        rows[b] = L0.iperm[nodes[b]];
+    }
+    DevBuf<int> drows;
+    DevBuf<double> dres, derr;
+    DevBuf<float> dX, dt;
+    HIP_TRY(drows.upload(rows, stream()));
+    if ((rc = upload_ref_field(L0, res, dres)) || (rc = upload_ref_field(L0, err, derr))) return rc;
+    HIP_TRY(dX.alloc((size_t)n * S.in));
+    HIP_TRY(dt.alloc(n));
+    if ((rc = mlp_features(n, L0.op_plain(OP_LAPP), pb->cfg.dimension, L0.d_avg, S.dx[0].p, S.dy[0].p, S.dz[0].p,
+                           drows.p, nullptr, L0.Nd, dres.p, derr.p, dX.p, dt.p, stream())))
+        return rc;
+    HIP_TRY(dX.download(X, (size_t)n * S.in, stream()));
+    HIP_TRY(dt.download(t, n, stream()));
+    HIP_TRY(hipStreamSynchronize(stream()));
+    return 0;
+}
+
+int mgm_mlp_fit(mgm_problem *pb, int n, const float *X, const float *t, int epochs, int batch, float lr,
+                uint64_t seed, double *loss_first, double *loss_last)
+{
+    if (!pb || !pb->ready) MGM_FAIL(-1, "problem is not set up");
+    if (n <= 0 || !X || !t || epochs < 1 || batch < 1 || !(lr > 0)) MGM_FAIL(-1, "mlp_fit: bad arguments");
+    LearnedState &S = state(pb);
+    if (!S.have) MGM_FAIL(-1, "learned smoother: no weights (set them first)");
+    DevBuf<float> dX, dt;
+    HIP_TRY(dX.upload(X, (size_t)n * S.in, stream()));
+    HIP_TRY(dt.upload(t, n, stream()));
+    int rc = fit(pb, S, n, dX.p, dt.p, epochs, batch, lr, seed, loss_first, loss_last);
+    if (rc) return rc;
+    if (S.enabled && (rc = prepare(pb, S))) return rc;
     return 0;
 }
 

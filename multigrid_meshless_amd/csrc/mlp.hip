@@ -244,7 +244,8 @@ __device__ inline float stencil_residual(const Ell &A, int r, const double *__re
 }
 
 // Training rows: X[b] = [offsets, scaled residuals] of device row rows[b] in
-// snapshot snap[b]; target t[b] = e * a_ii / sigma (0 where sigma = 0).
+// snapshot snap[b] (snapshot 0 when snap is null); target t[b] = e * a_ii /
+// sigma (0 where sigma = 0).
 __global__ void __launch_bounds__(BLK)
 features_kernel(int n, Ell A, int dim, double h, const double *__restrict__ dx, const double *__restrict__ dy,
                 const double *__restrict__ dz, const int *__restrict__ rows, const int *__restrict__ snap,
@@ -255,7 +256,8 @@ features_kernel(int n, Ell A, int dim, double h, const double *__restrict__ dx, 
     if (b >= n) return;
     const int m = A.m, in = m * dim + m;
     const int r = rows[b];
-    const double *res = res_snaps + (size_t)snap[b] * Nd;
+    const size_t so = snap ? (size_t)snap[b] * Nd : 0;
+    const double *res = res_snaps + so;
     float *x = X + (size_t)b * in;
     const double *crd[3] = {dx, dy, dz};
     for (int j = 0; j < m; j++) {
@@ -264,7 +266,7 @@ features_kernel(int n, Ell A, int dim, double h, const double *__restrict__ dx, 
     }
     const float sigma = stencil_residual(A, r, res, x + m * dim);
     const double diag = A.w[ell_at(m, A.S, r, 0)];
-    t[b] = sigma > 0.f ? (float)(err_snaps[(size_t)snap[b] * Nd + r] * diag / (double)sigma) : 0.f;
+    t[b] = sigma > 0.f ? (float)(err_snaps[so + r] * diag / (double)sigma) : 0.f;
 }
 
 // Per-row constant part of layer 1: b1[k] + sum_{j,d} W1[k][j*dim+d] off_jd,

@@ -8,6 +8,10 @@ float64 numpy restatement defines the network the GPU kernels implement
 (multigrid_meshless_amd/csrc/mlp.hip) and is itself pinned by central finite
 differences of the loss (tests/test_oracle_mlp.py).
 
+Training: mean squared loss, its gradient, and Adam over shuffled batches
+(adam_fit), pinned by finite differences and closed forms of the first
+steps (tests/test_oracle_mlp.py).
+
 Weight vector layout: W1[H][in], b1[H], W2[H][H], b2[H], w3[H], b3 (H = 64).
 Node features on a level with stencil width m (dimension dim):
   offsets (x_cj - x_i) / h for j < m and each axis, then r_cj / sigma_i for
@@ -70,6 +74,54 @@ def loss_grad(X: np.ndarray, t: np.ndarray, w: np.ndarray):
     db1 = dZ1.sum(0)
     g = np.concatenate([dW1.ravel(), db1, dW2.ravel(), db2, dw3, [db3]])
     return float(np.mean((y - t) ** 2)), g
+
+
+# Adam constants as the GPU holds them (FP32 literals)
+ADAM_B1 = float(np.float32(0.9))
+ADAM_B2 = float(np.float32(0.999))
+ADAM_EPS = float(np.float32(1e-8))
+
+
+def shuffles(n: int, epochs: int, draws) -> list:
+    """Row order of each epoch: Fisher-Yates from the last row down, swap
+    partner draw % (i + 1), with `draws` the consecutive splitmix64 outputs
+    of the seed (epochs * (n - 1) of them)."""
+    order = list(range(n))
+    out, q = [], 0
+    for _ in range(epochs):
+        for i in range(n - 1, 0, -1):
+            j = int(draws[q]) % (i + 1)
+            q += 1
+            order[i], order[j] = order[j], order[i]
+        out.append(list(order))
+    return out
+
+
+def adam_fit(X, t, w, epochs: int, batch: int, lr: float, draws):
+    """Adam on the MSE loss over rows (X, t): each epoch takes floor(n / batch)
+    steps over consecutive batches of its shuffle; step k moves
+    w -= lr sqrt(1 - b2^k) / (1 - b1^k) m / (sqrt(v) + eps) with
+    m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2 from zero.  Returns the
+    final weights and every step's gradient."""
+    X = np.asarray(X, dtype=np.float64)
+    t = np.asarray(t, dtype=np.float64)
+    w = np.asarray(w, dtype=np.float64).copy()
+    n = X.shape[0]
+    B = min(batch, n)
+    M = np.zeros_like(w)
+    V = np.zeros_like(w)
+    grads, k = [], 0
+    for order in shuffles(n, epochs, draws):
+        for b0 in range(0, n - B + 1, B):
+            k += 1
+            idx = order[b0:b0 + B]
+            _, g = loss_grad(X[idx], t[idx], w)
+            M = ADAM_B1 * M + (1 - ADAM_B1) * g
+            V = ADAM_B2 * V + (1 - ADAM_B2) * g * g
+            lr_t = lr * np.sqrt(1 - ADAM_B2 ** k) / (1 - ADAM_B1 ** k)
+            w -= lr_t * M / (np.sqrt(V) + ADAM_EPS)
+            grads.append(g)
+    return w, grads
 
 
 def bf16_round(a: np.ndarray) -> np.ndarray:

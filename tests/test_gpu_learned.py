@@ -2,7 +2,10 @@
 
 No reference counterpart (the reference smooths with SOR only).  Bars:
   * forward outputs, loss and loss gradient of the FP32 training kernels equal
-    the float64 oracle to 1e-5 / 1e-4 relative;
+    the float64 oracle to 1e-5 / 1e-4 relative, at 60 and 280 inputs;
+  * the training rows (offsets, scaled residuals, targets) equal the oracle's
+    node features to FP32 rounding, and Adam over shuffled batches moves the
+    weights as the oracle's Adam does;
   * one learned sweep changes p on every interior node by the oracle's
     correction (BF16 matrix cores: BF16 weights and activations, FP32
     accumulation) to 1e-3 of the largest correction, and leaves boundary
@@ -48,10 +51,12 @@ def he_weights(n_in, seed):
     return w
 
 
-def test_forward_and_gradient_match_oracle(gpu):
-    P = problem(gpu, 2, (24, 12))
+@pytest.mark.parametrize("dim,sizes,width", [(2, (24, 12), 20 * 2 + 20), (3, (16, 8), 70 * 3 + 70)])
+def test_forward_and_gradient_match_oracle(gpu, dim, sizes, width):
+    """2D m = 20 (60 inputs) and the 3D cube's m = 70 (280 inputs)."""
+    P = problem(gpu, dim, sizes)
     n_in = P.mlp_inputs()
-    assert n_in == 20 * 2 + 20
+    assert n_in == width
     w = he_weights(n_in, 1)
     P.set_mlp_weights(w)
     np.testing.assert_array_equal(P.mlp_weights(), w)
@@ -65,6 +70,76 @@ def test_forward_and_gradient_match_oracle(gpu):
     loss_ref, g_ref = mlp.loss_grad(X, t.astype(np.float64), w.astype(np.float64))
     assert abs(loss - loss_ref) <= 1e-5 * loss_ref
     assert np.abs(g - g_ref).max() <= 1e-4 * np.abs(g_ref).max()
+
+
+def test_training_features_match_oracle(gpu):
+    """mgm_mlp_features (the builder the trainer uses) on the 3D m = 70 cloud:
+    offsets / h, residuals / sigma and the target e a_ii / sigma of every
+    sampled node equal oracle.mlp.node_features up to FP32 rounding, and a
+    node whose stencil residual vanishes gets zero residual inputs and target."""
+    P = problem(gpu, 3, (16, 8))
+    geo = P.geometry(0)
+    N = geo["x"].size
+    cl, Lp = P.cloud(0), P.matrix(0, "lap_Poison")
+    m = cl.shape[1]
+    rng = np.random.default_rng(8)
+    res = rng.standard_normal(N)
+    err = rng.standard_normal(N)
+    interior = np.flatnonzero(geo["btag"] == 0)
+    nodes = np.concatenate([rng.choice(interior, 400, replace=False), np.flatnonzero(geo["btag"] != 0)[:8]])
+    quiet = nodes[0]
+    res[cl[quiet]] = 0.0
+    X, t = P.mlp_features(nodes, res, err)
+    h = P.info(0)["d_avg"]
+    Xw = np.zeros(X.shape)
+    tw = np.zeros(t.shape)
+    for b, i in enumerate(nodes):
+        f, sigma = mlp.node_features(geo["x"], geo["y"], geo["z"], 3, cl, h, res, i)
+        Xw[b] = f
+        tw[b] = err[i] * Lp[i, 0] / sigma if sigma > 0 else 0.0
+    assert not X[0, m * 3:].any() and t[0] == 0
+    np.testing.assert_allclose(X[:, :m * 3], Xw[:, :m * 3], rtol=0, atol=2e-7 * np.abs(Xw[:, :m * 3]).max())
+    np.testing.assert_allclose(X[:, m * 3:], Xw[:, m * 3:], rtol=3e-7, atol=1e-30)
+    np.testing.assert_allclose(t, tw, rtol=3e-7, atol=1e-30)
+    print(f"  {nodes.size} rows of {X.shape[1]} inputs, max |t| {np.abs(t).max():.3g}", flush=True)
+
+
+@pytest.mark.parametrize("epochs,batch", [(2, 512), (1, 128)])
+def test_adam_fit_matches_oracle(gpu, oracle, epochs, batch):
+    """mgm_mlp_fit (FP32 Adam over shuffled batches, as the trainer runs it)
+    against oracle.mlp.adam_fit (float64) at the 280-input width.  Each step
+    moves a weight by about lr times the sign of its averaged gradient, so
+    weights whose gradient is near zero at some step (below 1e-4 of the
+    largest, about 100x the FP32 gradient error) may move differently: every
+    other weight -- over 95% of them -- must move as the oracle's to 0.1% of
+    lr, and those few by at most 2 lr per step."""
+    P = problem(gpu, 3, (16, 8))
+    n_in = P.mlp_inputs()
+    w0 = he_weights(n_in, 5)
+    P.set_mlp_weights(w0)
+    rng = np.random.default_rng(9)
+    n = 512
+    X = rng.standard_normal((n, n_in)).astype(np.float32)
+    t = rng.standard_normal(n).astype(np.float32)
+    lr, seed = 1e-4, 17
+    l0, l1 = P.mlp_fit(X, t, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    w = P.mlp_weights().astype(np.float64)
+    draws = oracle.splitmix64(seed, epochs * (n - 1))
+    w_ref, grads = mlp.adam_fit(X, t.astype(np.float64), w0.astype(np.float64), epochs, batch, lr, draws)
+    steps = len(grads)
+    assert steps == epochs * (n // batch)
+    robust = np.ones(w0.size, bool)
+    for g in grads:
+        robust &= np.abs(g) > 1e-4 * np.abs(g).max()
+    d, d_ref = w - w0, w_ref - w0
+    print(f"  {steps} steps, {robust.mean():.4f} of weights with a clear gradient sign, "
+          f"loss {l0:.5f} -> {l1:.5f}", flush=True)
+    assert robust.mean() > 0.95
+    assert np.abs(d - d_ref)[robust].max() <= 1e-3 * lr
+    assert np.abs(d[~robust]).max(initial=0) <= 2 * lr * steps
+    assert abs(l0 - mlp.loss(X, t, w0.astype(np.float64))) <= 1e-5 * l0
+    assert abs(l1 - mlp.loss(X, t, w)) <= 1e-5 * l1
+    assert l1 < l0
 
 
 @pytest.mark.parametrize("dim,sizes", [(2, (40, 20, 10)), (3, (16, 8))])

@@ -67,3 +67,68 @@ def test_correction_bf16_activation_path(monkeypatch):
     monkeypatch.setattr(mlp, "bf16_round", lambda a: np.asarray(a, dtype=np.float64))
     same = mlp.correction(*args, 5, bf16_weights=False, bf16_activations=True)
     assert abs(same - exact) <= 1e-6 * abs(exact)
+
+
+def test_shuffles_known_answer_and_uniform():
+    # hand-worked: draws 7, 3, 10, 1 on 5 rows swap (4, 2), (3, 3), (2, 1), (1, 1)
+    assert mlp.shuffles(5, 1, [7, 3, 10, 1]) == [[0, 4, 1, 3, 2]]
+    # successive epochs continue from the previous order
+    two = mlp.shuffles(5, 2, [7, 3, 10, 1, 7, 3, 10, 1])
+    assert two[0] == [0, 4, 1, 3, 2] and two[1] == [0, 2, 4, 3, 1]
+    # Fisher-Yates reaches every permutation of 4 rows about equally often
+    # (a swap partner drawn below i instead of i + 1 would give cycles only)
+    rng = np.random.default_rng(5)
+    counts = {}
+    for _ in range(4800):
+        p = tuple(mlp.shuffles(4, 1, rng.integers(0, 2 ** 62, 3))[0])
+        counts[p] = counts.get(p, 0) + 1
+    assert len(counts) == 24
+    assert min(counts.values()) > 120 and max(counts.values()) < 280
+
+
+def test_adam_constant_gradient_closed_form(monkeypatch):
+    """With a constant gradient g the moments are m_k = (1 - b1^k) g and
+    v_k = (1 - b2^k) g^2, so step k moves w by lr g / (|g| + eps / sqrt(1 - b2^k))."""
+    g = np.array([1e-3, -2.0, 5e-7, 0.0, 3e-9])
+    monkeypatch.setattr(mlp, "loss_grad", lambda X, t, w: (0.0, g.copy()))
+    w0 = np.array([0.5, -0.25, 1.0, 2.0, 0.0])
+    lr = 1e-2
+    w, grads = mlp.adam_fit(np.zeros((8, 1)), np.zeros(8), w0, epochs=3, batch=4, lr=lr, draws=np.arange(21))
+    assert len(grads) == 6
+    want = w0.copy()
+    for k in range(1, 7):
+        want -= lr * g / (np.abs(g) + mlp.ADAM_EPS / np.sqrt(1 - mlp.ADAM_B2 ** k))
+    np.testing.assert_allclose(w, want, rtol=1e-12, atol=1e-15)
+
+
+def test_adam_batches_follow_the_shuffle(monkeypatch):
+    seen = []
+
+    def fake(X, t, w):
+        seen.append(X[:, 0].astype(int).tolist())
+        return 0.0, np.zeros_like(w)
+
+    monkeypatch.setattr(mlp, "loss_grad", fake)
+    X = np.arange(10, dtype=float)[:, None]
+    draws = np.random.default_rng(1).integers(0, 2 ** 62, 18)
+    mlp.adam_fit(X, np.zeros(10), np.zeros(3), epochs=2, batch=4, lr=1.0, draws=draws)
+    o = mlp.shuffles(10, 2, draws)
+    # two full batches per epoch; the last two rows of each shuffle are left out
+    assert seen == [o[0][0:4], o[0][4:8], o[1][0:4], o[1][4:8]]
+
+
+def test_node_features_scaling():
+    x, y, z, dim, cloud, h, Lp, res, w = _node_case(seed=4)
+    i = 7
+    f, sigma = mlp.node_features(x, y, z, dim, cloud, h, res, i)
+    m = cloud.shape[1]
+    off = f[:m * dim].reshape(m, dim)
+    np.testing.assert_array_equal(off[0], 0.0)                      # the node itself leads its stencil
+    np.testing.assert_allclose(off[3], [(x[cloud[i, 3]] - x[i]) / h, (y[cloud[i, 3]] - y[i]) / h], rtol=1e-15)
+    u = f[m * dim:]
+    assert np.isclose(np.sqrt(np.mean(u * u)), 1.0, rtol=1e-14)     # residuals over their RMS
+    assert np.isclose(sigma, np.sqrt(np.mean(res[cloud[i]] ** 2)), rtol=1e-15)
+    res0 = res.copy()
+    res0[cloud[i]] = 0.0
+    f0, s0 = mlp.node_features(x, y, z, dim, cloud, h, res0, i)
+    assert s0 == 0 and not f0[m * dim:].any()
