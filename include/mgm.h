@@ -116,10 +116,14 @@ int mgm_setup_timings(const mgm_problem *p, double *t6);
 /* Rebuild the weight rows of one level (batched local solves) and return the
  * kernel time in seconds and the number of systems solved. */
 int mgm_rebuild_weights(mgm_problem *p, int lev, double *seconds, long *systems);
-/* No reference counterpart: the variant of the batched weight kernel the
- * autotuner chose at the first build of systems of size mpn (= m + n): *tile
- * x 16 rows and columns of register tiles, *waves_per_simd the register budget
- * (1: unconstrained).  Returns -1 if no system of that size was built yet. */
+/* No reference counterpart: the variant of the batched weight kernel that
+ * ran the last build of systems of size mpn (= m + n; the autotuner's choice
+ * unless the environment forced one): the
+ * FP64 Gauss-Jordan kernel with *tile x 16 rows and columns of register tiles
+ * and *waves_per_simd its register budget (1: unconstrained), or, when
+ * *waves_per_simd is 0, the matrix-core kernel (FP32 LU on MFMA with FP64
+ * iterative refinement, *tile tiles of 16 x 16).  Returns -1 if no system of
+ * that size was built yet. */
 int mgm_weight_kernel_choice(int mpn, int *tile, int *waves_per_simd);
 
 /* ---- problem: data exchange (reference node order) ----------------------- */

@@ -147,7 +147,11 @@ class Problem:
         return s.value, n.value
 
     def weight_kernel_choice(self, lev: int = 0):
-        """(tile, waves_per_simd) the autotuner chose for this level's system size, or None."""
+        """(tile, waves_per_simd) of the weight kernel that ran the last build of
+        this level's system size (the autotuner's choice unless forced), or None.
+        waves_per_simd 0 is the matrix-core kernel (FP32 LU on MFMA + FP64
+        refinement, tile = its 16 x 16 tile count); otherwise the FP64
+        Gauss-Jordan register-tile kernel."""
         info = self.info(lev)
         t, w = C.c_int(), C.c_int()
         if lib().mgm_weight_kernel_choice(info["m"] + info["n"], C.byref(t), C.byref(w)) != 0:

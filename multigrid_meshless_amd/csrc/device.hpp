@@ -110,6 +110,7 @@ struct WeightsArgs {
     int *piv;           // mpn*batch
     int batch;
     int *status;        // nonzero if a zero pivot was met
+    int rhs_max;        // most right-hand sides of one system (operators + targets)
 };
 // Systems with mpn + 4 <= 128 run one workgroup per system in registers
 // (weights_tile(mpn) > 0) and need no workspace; wider ones use ws/rhs/piv.
@@ -118,6 +119,10 @@ int launch_weights(const WeightsArgs &a, hipStream_t s);
 // Tile grid and register budget the autotuner chose for systems of size mpn
 // (-1 before the first build of that size).
 int weights_kernel_choice(int mpn, int *tile, int *waves_per_simd);
+// The matrix-core form of the same solves (lu32.hip): FP32 LU on MFMA, FP64
+// refinement; systems up to 128 unknowns (lu32_tiles(mpn) > 0).
+int lu32_tiles(int mpn);
+int launch_weights_lu32(const WeightsArgs &a, hipStream_t s);
 
 // ---- stencil kernels (stencil.hip) ----------------------------------------
 // Fixed-width rows in sliced-ELL layout: rows are grouped in slices of S

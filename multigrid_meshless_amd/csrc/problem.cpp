@@ -671,6 +671,12 @@ static int level_weights(mgm_problem *pb, int l, double *kernel_seconds)
     if (l + 1 < pb->L) { a.rt_off = d_roff.p; a.rt_xyz = d_rxyz.p; a.rt_row = d_rrow.p; a.R_out = pb->lv[l + 1].Rrows.p; }
     if (l > 0) { a.pt_off = d_poff.p; a.pt_xyz = d_pxyz.p; a.pt_row = d_prow.p; a.P_out = pb->lv[l - 1].Prows.p; }
     a.ws = ws.p; a.rhs = rhs.p; a.piv = piv.p; a.batch = (int)batch; a.status = d_status.p;
+    {
+        const int nops = 3 + (dim == 3 ? 1 : 0);
+        int most = 0;
+        for (int i = 0; i < N; i++) most = std::max(most, (roff[i + 1] - roff[i]) + (poff[i + 1] - poff[i]));
+        a.rhs_max = nops + most;
+    }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0)); HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, g_stream));

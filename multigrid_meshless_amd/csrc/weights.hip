@@ -19,43 +19,17 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <utility>
 #include <vector>
 
 #include "device.hpp"
+#include "rbf_entry.hpp"
 
 namespace mgm {
 
 namespace {
-
-__device__ inline double rbf_phs(double sum, int phs)
-{
-    if (sum == 0) return 0;
-    if (phs & 1) {  // r^phs = sum^((phs-1)/2) * sqrt(sum)
-        double r = sqrt(sum), v = r;
-        for (int k = 0; k < (phs - 1) / 2; k++) v = v * sum;
-        return v;
-    }
-    return pow(sum, phs * 0.5);
-}
-
-__device__ inline double ipow(double d, int p)
-{
-    double v = 1;
-    for (int k = 0; k < p; k++) v = (k == 0) ? d : v * d;
-    return v;
-}
-
-__device__ inline double dist_sum(double ax, double ay, double az, double bx, double by, double bz,
-                                  int dim)
-{
-    double s = 0, d;
-    d = ax - bx; s += d * d;
-    d = ay - by; s += d * d;
-    if (dim == 3) { d = az - bz; s += d * d; }
-    return s;
-}
 
 __global__ void __launch_bounds__(64)
 weights_kernel(WeightsArgs a, int base, int count)
@@ -270,39 +244,12 @@ __device__ __attribute__((noinline)) double lu_entry(PolyTerms a, int dim, int p
             return i == j ? 0.0 : rbf_phs(dist_sum(cx[i], cy[i], cz[i], cx[j], cy[j], cz[j], dim), phs);
         if (i >= m && j >= m) return 0;
         const int pt = i < m ? i : j, k = (i < m ? j : i) - m;
-        const double dx = cx[pt] - sx, dy = cy[pt] - sy, dz = cz[pt] - sz;
-        double w = 1;
-        if (a.px[k]) w *= ipow(dx, a.px[k]);
-        if (a.py[k]) w *= ipow(dy, a.py[k]);
-        if (a.pz[k]) w *= ipow(dz, a.pz[k]);
-        return w;
+        return monomial(cx[pt] - sx, cy[pt] - sy, cz[pt] - sz, a.px[k], a.py[k], a.pz[k]);
     }
     if (j >= mpn + nrhs) return 0;
     const int r = j - mpn, q = q0 + r;
-    if (q < nops) {
-        // row 0 of the operator's right-hand-side matrix (x = seed)
-        const int op = ops[q];
-        if (i < m) {
-            const double s2 = dist_sum(sx, sy, sz, cx[i], cy[i], cz[i], dim);
-            const double rb = rbf_phs(s2, phs - 2);
-            if (op < 3) {
-                const double d = (op == 0) ? sx - cx[i] : (op == 1 ? sy - cy[i] : sz - cz[i]);
-                return phs * d * rb;
-            }
-            return (phs * phs) * rb;
-        }
-        const int k = i - m, ex = a.px[k], ey = a.py[k], ez = a.pz[k];
-        if (op == 0 && ex == 1 && ey == 0 && ez == 0) return 1;
-        if (op == 1 && ex == 0 && ey == 1 && ez == 0) return 1;
-        if (op == 2 && ex == 0 && ey == 0 && ez == 1) return 1;
-        if (op == 3 && ((ex == 2 && ey == 0 && ez == 0) || (ex == 0 && ey == 2 && ez == 0) ||
-                        (ex == 0 && ey == 0 && ez == 2)))
-            return 2;
-        return 0;
-    }
-    if (i < m) return rbf_phs(dist_sum(tgt[r][0], tgt[r][1], tgt[r][2], cx[i], cy[i], cz[i], dim), phs);
-    const int k = i - m;
-    return ipow(tgt[r][0] - sx, a.px[k]) * ipow(tgt[r][1] - sy, a.py[k]) * ipow(tgt[r][2] - sz, a.pz[k]);
+    if (q < nops) return operator_rhs(ops[q], i, m, dim, phs, a.px, a.py, a.pz, cx, cy, cz, sx, sy, sz);
+    return target_rhs(i, m, dim, phs, a.px, a.py, a.pz, cx, cy, cz, sx, sy, sz, tgt[r][0], tgt[r][1], tgt[r][2]);
 }
 
 // WPS: waves per SIMD the register budget aims at (1: no constraint).  Four
@@ -518,17 +465,42 @@ static int lu_run(LuFn f, const WeightsArgs &a, int base, int count, hipStream_t
     return 0;
 }
 
-// Autotuner of the weight kernel: per system size mpn, the tile grid (the
-// smallest that holds [A | 4 RHS] and the next) and the register budget (four
-// waves per SIMD or unconstrained) are timed on the first 8192 systems of the
-// first build -- every variant computes identical results, so those systems
-// are simply recomputed by the winner -- and the fastest is kept for the
-// process.  MGM_LU_TUNE=0 takes the smallest grid and the four-wave budget.
+// Autotuner of the weight kernel: per system size mpn, the register-tile
+// Gauss-Jordan kernel (FP64; the tile grid that holds [A | 4 RHS] and the
+// next, register budget of four waves per SIMD or unconstrained) and the
+// matrix-core kernel (lu32.hip: FP32 LU on MFMA, FP64 refinement) are timed
+// on the first 8192 systems of the first build -- every variant writes the
+// same rows to within the 1e-11 bar, and the sample is simply recomputed by
+// the winner -- and the fastest is kept for the process.  MGM_LU_TUNE=0 takes
+// the smallest Gauss-Jordan grid and the four-wave budget; MGM_LU_KERNEL=gj or
+// mfma restricts the choice to one kernel; MGM_LU_VARIANT=tile,wps forces a
+// Gauss-Jordan variant.  A choice with wps == 0 is the matrix-core kernel
+// (tile = its 16 x 16 tile count).
 static std::map<int, std::pair<int, int>> g_lu_choice;
+static std::map<int, std::pair<int, int>> g_lu_last;   // variant of the last build per size
+
+static int run_choice(std::pair<int, int> c, const WeightsArgs &a, int count, hipStream_t s)
+{
+    if (c.second == 0) {
+        WeightsArgs b = a;
+        b.nsys = count;
+        return launch_weights_lu32(b, s);
+    }
+    return lu_run(lu_fn(c.first, c.second), a, 0, count, s);
+}
 
 static int lu_choose(const WeightsArgs &a, hipStream_t s, int *tile, int *wps)
 {
-    const int mpn = a.m + a.n, t0 = weights_tile(mpn);
+    const int mpn = a.m + a.n, t0 = weights_tile(mpn), t32 = lu32_tiles(mpn);
+    const char *kind = getenv("MGM_LU_KERNEL");
+    const bool only_gj = kind && strcmp(kind, "gj") == 0, only_mfma = kind && strcmp(kind, "mfma") == 0;
+    if (kind && !only_gj && !only_mfma) MGM_FAIL(-1, "MGM_LU_KERNEL: expected gj or mfma");
+    if (only_mfma) {
+        if (!t32) MGM_FAIL(-1, "MGM_LU_KERNEL=mfma: systems wider than 128 unknowns");
+        *tile = t32;
+        *wps = 0;
+        return 0;
+    }
     // MGM_LU_VARIANT=tile,wps forces a variant (tests compare them), uncached
     if (const char *v = getenv("MGM_LU_VARIANT")) {
         int t = 0, w = 0;
@@ -539,26 +511,29 @@ static int lu_choose(const WeightsArgs &a, hipStream_t s, int *tile, int *wps)
         }
         MGM_FAIL(-1, "MGM_LU_VARIANT: expected tile,wps with tile >= the smallest grid and wps 1 or 4");
     }
-    auto it = g_lu_choice.find(mpn);
+    const int key = only_gj ? -mpn : mpn;
+    auto it = g_lu_choice.find(key);
     if (it != g_lu_choice.end()) { *tile = it->second.first; *wps = it->second.second; return 0; }
-    std::pair<int, int> best{t0, t0 <= 6 ? 4 : 1};
+    if (!t0 && only_gj) MGM_FAIL(-1, "MGM_LU_KERNEL=gj: systems too wide for the register-tile kernel");
+    // systems too wide for the Gauss-Jordan register tiles go to the matrix cores
+    std::pair<int, int> best = t0 ? std::pair<int, int>{t0, t0 <= 6 ? 4 : 1} : std::pair<int, int>{t32, 0};
     const char *e = getenv("MGM_LU_TUNE");
     const int sample = a.nsys < 8192 ? a.nsys : 8192;
-    if (!(e && e[0] == '0') && sample >= 1024) {
+    if (t0 && !(e && e[0] == '0') && sample >= 1024) {
         std::vector<std::pair<int, int>> cand;
         for (int t = t0; t <= t0 + 1 && t <= 8; t++) {
             if (t <= 6) cand.push_back({t, 4});
             cand.push_back({t, 1});
         }
+        if (t32 && !only_gj) cand.push_back({t32, 0});
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         float best_ms = 1e30f;
         for (auto c : cand) {
-            LuFn f = lu_fn(c.first, c.second);
-            int rc = lu_run(f, a, 0, sample, s);   // first launch loads the code object
+            int rc = run_choice(c, a, sample, s);   // first launch loads the code object
             if (!rc) HIP_TRY(hipEventRecord(e0, s));
-            if (!rc) rc = lu_run(f, a, 0, sample, s);
+            if (!rc) rc = run_choice(c, a, sample, s);
             if (!rc) HIP_TRY(hipEventRecord(e1, s));
             if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
             HIP_TRY(hipEventSynchronize(e1));
@@ -569,7 +544,7 @@ static int lu_choose(const WeightsArgs &a, hipStream_t s, int *tile, int *wps)
         HIP_TRY(hipEventDestroy(e0));
         HIP_TRY(hipEventDestroy(e1));
     }
-    g_lu_choice[mpn] = best;
+    g_lu_choice[key] = best;
     *tile = best.first;
     *wps = best.second;
     return 0;
@@ -577,8 +552,8 @@ static int lu_choose(const WeightsArgs &a, hipStream_t s, int *tile, int *wps)
 
 int weights_kernel_choice(int mpn, int *tile, int *wps)
 {
-    auto it = g_lu_choice.find(mpn);
-    if (it == g_lu_choice.end()) return -1;
+    auto it = g_lu_last.find(mpn);
+    if (it == g_lu_last.end()) return -1;
     *tile = it->second.first;
     *wps = it->second.second;
     return 0;
@@ -586,10 +561,11 @@ int weights_kernel_choice(int mpn, int *tile, int *wps)
 
 int launch_weights(const WeightsArgs &a, hipStream_t s)
 {
-    if (weights_tile(a.m + a.n)) {
+    if (weights_tile(a.m + a.n) || lu32_tiles(a.m + a.n)) {
         int tile = 0, wps = 0, rc;
         if ((rc = lu_choose(a, s, &tile, &wps))) return rc;
-        return lu_run(lu_fn(tile, wps), a, 0, a.nsys, s);
+        g_lu_last[a.m + a.n] = {tile, wps};
+        return run_choice({tile, wps}, a, a.nsys, s);
     }
     if (!a.ws) MGM_FAIL(-1, "weights: workspace missing for wide systems");
     for (int base = 0; base < a.nsys; base += a.batch) {
