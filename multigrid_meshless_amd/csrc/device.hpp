@@ -116,13 +116,28 @@ struct WeightsArgs {
 // (weights_tile(mpn) > 0) and need no workspace; wider ones use ws/rhs/piv.
 int weights_tile(int mpn);
 int launch_weights(const WeightsArgs &a, hipStream_t s);
+// A batch of systems from several levels (uneven sizes allowed) for one
+// persistent launch: lv[l] describes level l (its nsys / sys fields unused),
+// list[t] = (level, node) of system t.  Workgroups take systems from a
+// work counter until the list is exhausted.
+struct WeightsBatch {
+    const WeightsArgs *lv;   // device array
+    const int2 *list;        // device array of nsys entries
+    int nsys;
+    int *next;               // work counter, zero at launch
+    int *status;             // nonzero if a zero pivot was met
+};
+// One launch over every system of the given levels (host descriptors; their
+// device arrays must stay valid until the stream reaches the end of the
+// launch).  Systems larger than 128 unknowns are not accepted here.
+int launch_weights_levels(const std::vector<WeightsArgs> &levels, hipStream_t s);
 // Tile grid and register budget the autotuner chose for systems of size mpn
 // (-1 before the first build of that size).
 int weights_kernel_choice(int mpn, int *tile, int *waves_per_simd);
 // The matrix-core form of the same solves (lu32.hip): FP32 LU on MFMA, FP64
 // refinement; systems up to 128 unknowns (lu32_tiles(mpn) > 0).
 int lu32_tiles(int mpn);
-int launch_weights_lu32(const WeightsArgs &a, hipStream_t s);
+int launch_weights_lu32(const WeightsBatch &b, int tiles, int rmax, hipStream_t s);
 
 // ---- stencil kernels (stencil.hip) ----------------------------------------
 // Fixed-width rows in sliced-ELL layout: rows are grouped in slices of S

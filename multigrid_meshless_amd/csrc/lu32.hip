@@ -456,22 +456,24 @@ __device__ __attribute__((noinline)) void fill_system(const WeightsArgs &a, int 
 // WPS: waves per SIMD the register budget aims at.
 template <int T, int WPS>
 __global__ void __launch_bounds__(64, WPS)
-lu32_kernel(WeightsArgs a, double *scratch, int rmax, int *next, int nref)
+lu32_kernel(WeightsBatch bt, double *scratch, int rmax, int nref)
 {
     __shared__ Lu32Smem<T> sm;
     constexpr int NC = 16 * T;
     const int lane = threadIdx.x & 63, li = lane & 15, g = lane >> 4;
-    const int m = a.m, n = a.n, mpn = m + n, dim = a.dim, phs = a.phs;
     double *full = scratch + (size_t)blockIdx.x * lu32_slot(T, rmax);   // [j][i], NC x NC
     double *rb = full + (size_t)NC * NC;                                // [r][i], rmax x NC
-    const unsigned opmask = (a.Dx ? 1u : 0u) | (a.Dy ? 2u : 0u) | ((a.Dz && dim == 3) ? 4u : 0u) | (a.lap ? 8u : 0u);
-    const int nops = __builtin_popcount(opmask);
     for (;;) {
         int t = 0;
-        if (lane == 0) t = atomicAdd(next, 1);
+        if (lane == 0) t = atomicAdd(bt.next, 1);
         t = bcast(t, 0);
-        if (t >= a.nsys) break;
-        const int s = uni(a.sys[t]);
+        if (t >= bt.nsys) break;
+        const int2 e = bt.list[t];
+        const int s = uni(e.y);
+        const WeightsArgs &a = bt.lv[uni(e.x)];
+        const int m = a.m, n = a.n, mpn = m + n, dim = a.dim;
+        const unsigned opmask = (a.Dx ? 1u : 0u) | (a.Dy ? 2u : 0u) | ((a.Dz && dim == 3) ? 4u : 0u) | (a.lap ? 8u : 0u);
+        const int nops = __builtin_popcount(opmask);
         const int *cl = a.cloud + (size_t)s * m;
         for (int i = lane; i < NC; i += 64) {
             const bool in = i < m;
@@ -509,7 +511,7 @@ lu32_kernel(WeightsArgs a, double *scratch, int rmax, int *next, int nref)
             __syncthreads();
         }
         if (!lu_factor<T>(A, pos, mpn, sm)) {
-            if (lane == 0) atomicOr(a.status, 1);
+            if (lane == 0) atomicOr(bt.status, 1);
             __syncthreads();
             continue;
         }
@@ -584,7 +586,7 @@ lu32_kernel(WeightsArgs a, double *scratch, int rmax, int *next, int nref)
     }
 }
 
-using Lu32Fn = void (*)(WeightsArgs, double *, int, int *, int);
+using Lu32Fn = void (*)(WeightsBatch, double *, int, int);
 
 }  // namespace
 
@@ -621,21 +623,19 @@ static int lu32_refinements()
     return r < 0 ? 0 : (r > 8 ? 8 : r);
 }
 
-int launch_weights_lu32(const WeightsArgs &a, hipStream_t s)
+int launch_weights_lu32(const WeightsBatch &b, int tiles, int rmax, hipStream_t s)
 {
-    const int mpn = a.m + a.n, tiles = lu32_tiles(mpn);
-    if (!tiles) MGM_FAIL(-1, "weights: system wider than 128 unknowns for the matrix-core kernel");
+    if (tiles < 2 || tiles > 8) MGM_FAIL(-1, "weights: the matrix-core kernel takes 2 to 8 tiles");
     int dev = 0, cus = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const int waves_per_cu = tiles <= 6 ? 8 : 4;
     const int slots = cus * waves_per_cu;
-    const int rmax = a.rhs_max > 0 ? a.rhs_max : 4;
-    // scratch slots and the work counter, kept for later builds (grown when a
-    // build needs more; never freed, like the other per-process buffers)
+    if (rmax < 1) rmax = 4;
+    // scratch slots, kept for later builds (grown when a build needs more;
+    // never freed, like the other per-process buffers)
     static double *scratch = nullptr;
     static size_t scratch_n = 0;
-    static int *next = nullptr;
     const size_t need = (size_t)slots * lu32_slot(tiles, rmax);
     if (scratch_n < need) {
         if (scratch) HIP_TRY(hipFree(scratch));   // synchronises: no launch still reads it
@@ -644,9 +644,8 @@ int launch_weights_lu32(const WeightsArgs &a, hipStream_t s)
         HIP_TRY(hipMalloc((void **)&scratch, sizeof(double) * need));
         scratch_n = need;
     }
-    if (!next) HIP_TRY(hipMalloc((void **)&next, sizeof(int)));
-    HIP_TRY(hipMemsetAsync(next, 0, sizeof(int), s));
-    hipLaunchKernelGGL(lu32_fn(tiles), dim3(slots), dim3(64), 0, s, a, scratch, rmax, next, lu32_refinements());
+    HIP_TRY(hipMemsetAsync(b.next, 0, sizeof(int), s));
+    hipLaunchKernelGGL(lu32_fn(tiles), dim3(slots), dim3(64), 0, s, b, scratch, rmax, lu32_refinements());
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -597,8 +597,17 @@ static bool row_valid(const Level &L, int i, int mode)
     return mode == MGM_ROWS_REFERENCE ? (i >= L.nb) : (L.btag[i] == 0);
 }
 
-// Batched weight + transfer-row solves for the systems of level l.
-static int level_weights(mgm_problem *pb, int l, double *kernel_seconds)
+// Inputs of the batched weight + transfer-row solves of one level: its
+// systems, the interpolation targets mapped onto them, and the output rows
+// (zeroed).  The buffers stay alive until the launch has finished.
+struct WeightsPrep {
+    DevBuf<int> d_sys, d_roff, d_poff, d_rrow, d_prow;
+    DevBuf<double> d_rxyz, d_pxyz, ws, rhs;
+    DevBuf<int> piv;
+    WeightsArgs a{};
+};
+
+static int prepare_weights(mgm_problem *pb, int l, WeightsPrep &w, int *d_status)
 {
     Level &L = pb->lv[l];
     const int dim = pb->cfg.dimension, mode = pb->cfg.transfer_rows;
@@ -644,44 +653,63 @@ static int level_weights(mgm_problem *pb, int l, double *kernel_seconds)
         HIP_TRY(F.Prows.alloc((size_t)F.N * m));
         HIP_TRY(hipMemsetAsync(F.Prows.p, 0, sizeof(double) * F.N * m, g_stream));
     }
-    DevBuf<int> d_sys, d_roff, d_poff, d_rrow, d_prow, d_status;
-    DevBuf<double> d_rxyz, d_pxyz, ws, rhs;
-    DevBuf<int> piv;
-    HIP_TRY(d_sys.upload(sys, g_stream));
-    HIP_TRY(d_roff.upload(roff, g_stream)); HIP_TRY(d_poff.upload(poff, g_stream));
-    if (!rrow.empty()) { HIP_TRY(d_rrow.upload(rrow, g_stream)); HIP_TRY(d_rxyz.upload(rxyz, g_stream)); }
-    if (!prow.empty()) { HIP_TRY(d_prow.upload(prow, g_stream)); HIP_TRY(d_pxyz.upload(pxyz, g_stream)); }
-    HIP_TRY(d_status.alloc(1));
-    HIP_TRY(hipMemsetAsync(d_status.p, 0, sizeof(int), g_stream));
+    HIP_TRY(w.d_sys.upload(sys, g_stream));
+    HIP_TRY(w.d_roff.upload(roff, g_stream)); HIP_TRY(w.d_poff.upload(poff, g_stream));
+    if (!rrow.empty()) { HIP_TRY(w.d_rrow.upload(rrow, g_stream)); HIP_TRY(w.d_rxyz.upload(rxyz, g_stream)); }
+    if (!prow.empty()) { HIP_TRY(w.d_prow.upload(prow, g_stream)); HIP_TRY(w.d_pxyz.upload(pxyz, g_stream)); }
     // batch sized for ~4 GiB of factor workspace (only the wide-system kernel uses one)
     long batch = (long)((4.0 * 1024 * 1024 * 1024) / (8.0 * mpn * mpn));
     batch = std::max(64L, std::min(batch, (long)sys.size()));
     batch = (batch + 63) / 64 * 64;
-    if (!weights_tile(mpn)) {
-        HIP_TRY(ws.alloc((size_t)mpn * mpn * batch));
-        HIP_TRY(rhs.alloc((size_t)4 * mpn * batch));
-        HIP_TRY(piv.alloc((size_t)mpn * batch));
+    if (!weights_tile(mpn) && !lu32_tiles(mpn)) {
+        HIP_TRY(w.ws.alloc((size_t)mpn * mpn * batch));
+        HIP_TRY(w.rhs.alloc((size_t)4 * mpn * batch));
+        HIP_TRY(w.piv.alloc((size_t)mpn * batch));
     }
-    WeightsArgs a{};
-    a.nsys = (int)sys.size(); a.sys = d_sys.p; a.m = m; a.n = L.n; a.dim = dim; a.phs = pb->cfg.phs_degree;
+    WeightsArgs &a = w.a;
+    a.nsys = (int)sys.size(); a.sys = w.d_sys.p; a.m = m; a.n = L.n; a.dim = dim; a.phs = pb->cfg.phs_degree;
     a.px = L.d_px.p; a.py = L.d_py.p; a.pz = L.d_pz.p;
     a.x = L.rx.p; a.y = L.ry.p; a.z = L.rz.p; a.cloud = L.rcloud.p;
     a.Dx = L.rows[OP_DX].p; a.Dy = L.rows[OP_DY].p; a.Dz = (dim == 3) ? L.rows[OP_DZ].p : nullptr;
     a.lap = L.rows[OP_LAP].p;
-    if (l + 1 < pb->L) { a.rt_off = d_roff.p; a.rt_xyz = d_rxyz.p; a.rt_row = d_rrow.p; a.R_out = pb->lv[l + 1].Rrows.p; }
-    if (l > 0) { a.pt_off = d_poff.p; a.pt_xyz = d_pxyz.p; a.pt_row = d_prow.p; a.P_out = pb->lv[l - 1].Prows.p; }
-    a.ws = ws.p; a.rhs = rhs.p; a.piv = piv.p; a.batch = (int)batch; a.status = d_status.p;
-    {
-        const int nops = 3 + (dim == 3 ? 1 : 0);
-        int most = 0;
-        for (int i = 0; i < N; i++) most = std::max(most, (roff[i + 1] - roff[i]) + (poff[i + 1] - poff[i]));
-        a.rhs_max = nops + most;
+    if (l + 1 < pb->L) { a.rt_off = w.d_roff.p; a.rt_xyz = w.d_rxyz.p; a.rt_row = w.d_rrow.p; a.R_out = pb->lv[l + 1].Rrows.p; }
+    if (l > 0) { a.pt_off = w.d_poff.p; a.pt_xyz = w.d_pxyz.p; a.pt_row = w.d_prow.p; a.P_out = pb->lv[l - 1].Prows.p; }
+    a.ws = w.ws.p; a.rhs = w.rhs.p; a.piv = w.piv.p; a.batch = (int)batch; a.status = d_status;
+    int most = 0;
+    for (int i = 0; i < N; i++) most = std::max(most, (roff[i + 1] - roff[i]) + (poff[i + 1] - poff[i]));
+    a.rhs_max = 3 + (dim == 3 ? 1 : 0) + most;
+    return 0;
+}
+
+// Weights and transfer rows of levels l0 .. l1-1: one persistent launch over
+// all their systems (uneven sizes included) when every system fits the
+// register kernels (at most 128 unknowns), else one launch per level.
+static int build_weights(mgm_problem *pb, int l0, int l1, double *kernel_seconds)
+{
+    std::vector<WeightsPrep> prep(l1 - l0);
+    DevBuf<int> d_status;
+    HIP_TRY(d_status.alloc(1));
+    HIP_TRY(hipMemsetAsync(d_status.p, 0, sizeof(int), g_stream));
+    bool batch = true;
+    int rc;
+    for (int l = l0; l < l1; l++) {
+        if ((rc = prepare_weights(pb, l, prep[l - l0], d_status.p))) return rc;
+        const int mpn = pb->lv[l].m + pb->lv[l].n;
+        batch = batch && (weights_tile(mpn) || lu32_tiles(mpn));
     }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0)); HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, g_stream));
-    int rc = launch_weights(a, g_stream);
-    if (rc) return rc;
+    if (batch) {
+        std::vector<WeightsArgs> args;
+        for (auto &w : prep) args.push_back(w.a);
+        rc = launch_weights_levels(args, g_stream);
+    } else {
+        rc = 0;
+        for (auto &w : prep)
+            if (!rc) rc = launch_weights(w.a, g_stream);
+    }
+    if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
     HIP_TRY(hipEventRecord(e1, g_stream));
     int status = 0;
     HIP_TRY(hipMemcpyAsync(&status, d_status.p, sizeof(int), hipMemcpyDeviceToHost, g_stream));
@@ -1072,10 +1100,9 @@ int mgm_problem_setup(mgm_problem *pb)
     }
     tr += now() - a;
     a = now();
-    for (int l = 0; l < pb->L; l++) {
-        if ((rc = level_weights(pb, l, nullptr))) return rc;
+    if ((rc = build_weights(pb, 0, pb->L, nullptr))) return rc;
+    for (int l = 0; l < pb->L; l++)
         if ((rc = level_lap_poisson(pb, l))) return rc;
-    }
     HIP_TRY(hipStreamSynchronize(g_stream));
     tw = now() - a;
     a = now();
@@ -1120,7 +1147,7 @@ int mgm_setup_timings(const mgm_problem *pb, double *t6)
 int mgm_rebuild_weights(mgm_problem *pb, int lev, double *seconds, long *systems)
 {
     CHECK_READY(pb, lev);
-    int rc = level_weights(pb, lev, seconds);
+    int rc = build_weights(pb, lev, lev + 1, seconds);
     if (rc) return rc;
     if ((rc = level_lap_poisson(pb, lev))) return rc;
     for (int l = std::max(0, lev - 1); l <= std::min(pb->L - 1, lev + 1); l++)

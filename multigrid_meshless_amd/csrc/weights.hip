@@ -257,9 +257,15 @@ __device__ __attribute__((noinline)) double lu_entry(PolyTerms a, int dim, int p
 // faster than the unconstrained three at TILE 6; five spills inside the
 // elimination loop).  The launcher picks TILE and WPS per system size by
 // timing (lu_choose).
+//
+// One launch covers a batch: workgroup b solves system (level, node) =
+// list[b], so every level and uneven system sizes (mpn + 4 <= 16 TILE) share
+// it; the hardware dispatcher hands the workgroups to CUs as slots free up.
+// (A persistent loop over a work counter instead costs registers: 39 spills
+// at TILE 6 and 8% of the build rate, measured.)
 template <int TILE, int WPS>
 __global__ void __launch_bounds__(LU_T, WPS)
-lu_kernel(WeightsArgs a, int base)
+lu_kernel(WeightsBatch bt)
 {
     constexpr int NC = 16 * TILE;
     __shared__ double cx[NC], cy[NC], cz[NC];
@@ -268,9 +274,12 @@ lu_kernel(WeightsArgs a, int base)
     __shared__ double *outp[NC];
     __shared__ double tgt[NC][3];
     __shared__ int s_bad;
-    const int s = a.sys[base + blockIdx.x];
-    const int m = a.m, n = a.n, mpn = m + n, dim = a.dim, phs = a.phs;
     const int tid = threadIdx.x, ti = tid % 16, tj = tid / 16;
+    {
+    const int2 e = bt.list[blockIdx.x];
+    const int s = __builtin_amdgcn_readfirstlane(e.y);
+    const WeightsArgs &a = bt.lv[__builtin_amdgcn_readfirstlane(e.x)];
+    const int m = a.m, n = a.n, mpn = m + n, dim = a.dim, phs = a.phs;
     const int *cl = a.cloud + (size_t)s * m;
     for (int i = tid; i < m; i += LU_T) {
         cx[i] = a.x[cl[i]];
@@ -405,7 +414,7 @@ lu_kernel(WeightsArgs a, int base)
             if (s_bad) break;
         }
         if (s_bad) {
-            if (tid == 0) atomicOr(a.status, 1);
+            if (tid == 0) atomicOr(bt.status, 1);
             return;
         }
         // ---- solution: unknown k is the right-hand-side entries of its pivot
@@ -426,6 +435,7 @@ lu_kernel(WeightsArgs a, int base)
         }
         __syncthreads();   // LDS tables are rewritten by the next pass
     }
+    }
 }
 
 }  // namespace
@@ -437,7 +447,7 @@ int weights_tile(int mpn)
     return 0;
 }
 
-using LuFn = void (*)(WeightsArgs, int);
+using LuFn = void (*)(WeightsBatch);
 
 template <int W>
 static LuFn lu_fn_w(int tile)
@@ -454,14 +464,12 @@ static LuFn lu_fn_w(int tile)
 }
 static LuFn lu_fn(int tile, int wps) { return wps == 4 ? lu_fn_w<4>(tile) : lu_fn_w<1>(tile); }
 
-static int lu_run(LuFn f, const WeightsArgs &a, int base, int count, hipStream_t s)
+// One workgroup per system of the batch.
+static int lu_run(LuFn f, WeightsBatch b, hipStream_t s)
 {
-    // one workgroup per system; launches of at most 2^20 systems
-    for (int b = base; b < base + count; b += 1 << 20) {
-        const int c = (base + count - b < (1 << 20)) ? base + count - b : (1 << 20);
-        hipLaunchKernelGGL(f, dim3(c), dim3(LU_T), 0, s, a, b);
-        HIP_TRY(hipGetLastError());
-    }
+    if (b.nsys <= 0) return 0;
+    hipLaunchKernelGGL(f, dim3(b.nsys), dim3(LU_T), 0, s, b);
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 
@@ -479,19 +487,15 @@ static int lu_run(LuFn f, const WeightsArgs &a, int base, int count, hipStream_t
 static std::map<int, std::pair<int, int>> g_lu_choice;
 static std::map<int, std::pair<int, int>> g_lu_last;   // variant of the last build per size
 
-static int run_choice(std::pair<int, int> c, const WeightsArgs &a, int count, hipStream_t s)
+static int run_choice(std::pair<int, int> c, const WeightsBatch &b, int rmax, hipStream_t s)
 {
-    if (c.second == 0) {
-        WeightsArgs b = a;
-        b.nsys = count;
-        return launch_weights_lu32(b, s);
-    }
-    return lu_run(lu_fn(c.first, c.second), a, 0, count, s);
+    if (c.second == 0) return launch_weights_lu32(b, c.first, rmax, s);
+    return lu_run(lu_fn(c.first, c.second), b, s);
 }
 
-static int lu_choose(const WeightsArgs &a, hipStream_t s, int *tile, int *wps)
+static int lu_choose(int mpn, const WeightsBatch &b, int rmax, hipStream_t s, int *tile, int *wps)
 {
-    const int mpn = a.m + a.n, t0 = weights_tile(mpn), t32 = lu32_tiles(mpn);
+    const int t0 = weights_tile(mpn), t32 = lu32_tiles(mpn);
     const char *kind = getenv("MGM_LU_KERNEL");
     const bool only_gj = kind && strcmp(kind, "gj") == 0, only_mfma = kind && strcmp(kind, "mfma") == 0;
     if (kind && !only_gj && !only_mfma) MGM_FAIL(-1, "MGM_LU_KERNEL: expected gj or mfma");
@@ -504,7 +508,7 @@ static int lu_choose(const WeightsArgs &a, hipStream_t s, int *tile, int *wps)
     // MGM_LU_VARIANT=tile,wps forces a variant (tests compare them), uncached
     if (const char *v = getenv("MGM_LU_VARIANT")) {
         int t = 0, w = 0;
-        if (sscanf(v, "%d,%d", &t, &w) == 2 && t >= t0 && t <= 8 && (w == 1 || (w == 4 && t <= 6))) {
+        if (sscanf(v, "%d,%d", &t, &w) == 2 && t0 && t >= t0 && t <= 8 && (w == 1 || (w == 4 && t <= 6))) {
             *tile = t;
             *wps = w;
             return 0;
@@ -518,8 +522,9 @@ static int lu_choose(const WeightsArgs &a, hipStream_t s, int *tile, int *wps)
     // systems too wide for the Gauss-Jordan register tiles go to the matrix cores
     std::pair<int, int> best = t0 ? std::pair<int, int>{t0, t0 <= 6 ? 4 : 1} : std::pair<int, int>{t32, 0};
     const char *e = getenv("MGM_LU_TUNE");
-    const int sample = a.nsys < 8192 ? a.nsys : 8192;
-    if (t0 && !(e && e[0] == '0') && sample >= 1024) {
+    WeightsBatch sample = b;
+    sample.nsys = b.nsys < 8192 ? b.nsys : 8192;
+    if (t0 && !(e && e[0] == '0') && sample.nsys >= 1024) {
         std::vector<std::pair<int, int>> cand;
         for (int t = t0; t <= t0 + 1 && t <= 8; t++) {
             if (t <= 6) cand.push_back({t, 4});
@@ -531,9 +536,9 @@ static int lu_choose(const WeightsArgs &a, hipStream_t s, int *tile, int *wps)
         HIP_TRY(hipEventCreate(&e1));
         float best_ms = 1e30f;
         for (auto c : cand) {
-            int rc = run_choice(c, a, sample, s);   // first launch loads the code object
+            int rc = run_choice(c, sample, rmax, s);   // first launch loads the code object
             if (!rc) HIP_TRY(hipEventRecord(e0, s));
-            if (!rc) rc = run_choice(c, a, sample, s);
+            if (!rc) rc = run_choice(c, sample, rmax, s);
             if (!rc) HIP_TRY(hipEventRecord(e1, s));
             if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
             HIP_TRY(hipEventSynchronize(e1));
@@ -559,14 +564,66 @@ int weights_kernel_choice(int mpn, int *tile, int *wps)
     return 0;
 }
 
+namespace {
+__global__ void __launch_bounds__(256) fill_list_kernel(int2 *list, int n, const int *sys, int lvl)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) list[i] = make_int2(lvl, sys[i]);
+}
+}  // namespace
+
+int launch_weights_levels(const std::vector<WeightsArgs> &levels, hipStream_t s)
+{
+    int mpn = 0, rmax = 0;
+    long total = 0;
+    for (const WeightsArgs &a : levels) {
+        mpn = a.m + a.n > mpn ? a.m + a.n : mpn;
+        rmax = a.rhs_max > rmax ? a.rhs_max : rmax;
+        total += a.nsys;
+    }
+    if (levels.empty() || total == 0) return 0;
+    if (!weights_tile(mpn) && !lu32_tiles(mpn)) MGM_FAIL(-1, "weights: batch with systems over 128 unknowns");
+    if (total >= (1L << 31)) MGM_FAIL(-1, "weights: too many systems for one batch");
+    // descriptors, the (level, node) list and the work counter; kept between
+    // builds and grown when a batch needs more
+    static WeightsArgs *d_lv = nullptr;
+    static size_t n_lv = 0;
+    static int2 *d_list = nullptr;
+    static size_t n_list = 0;
+    static int *d_next = nullptr;
+    if (n_lv < levels.size()) {
+        if (d_lv) HIP_TRY(hipFree(d_lv));   // synchronises: no launch still reads it
+        d_lv = nullptr;
+        HIP_TRY(hipMalloc((void **)&d_lv, sizeof(WeightsArgs) * levels.size()));
+        n_lv = levels.size();
+    }
+    if (n_list < (size_t)total) {
+        if (d_list) HIP_TRY(hipFree(d_list));
+        d_list = nullptr;
+        HIP_TRY(hipMalloc((void **)&d_list, sizeof(int2) * total));
+        n_list = total;
+    }
+    if (!d_next) HIP_TRY(hipMalloc((void **)&d_next, sizeof(int)));
+    HIP_TRY(hipMemcpyAsync(d_lv, levels.data(), sizeof(WeightsArgs) * levels.size(), hipMemcpyHostToDevice, s));
+    long off = 0;
+    for (size_t l = 0; l < levels.size(); l++) {
+        const int n = levels[l].nsys;
+        if (n > 0) hipLaunchKernelGGL(fill_list_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_list + off, n,
+                                      levels[l].sys, (int)l);
+        HIP_TRY(hipGetLastError());
+        off += n;
+    }
+    WeightsBatch b{d_lv, d_list, (int)total, d_next, levels[0].status};
+    int tile = 0, wps = 0, rc;
+    if ((rc = lu_choose(mpn, b, rmax, s, &tile, &wps))) return rc;
+    for (const WeightsArgs &a : levels) g_lu_last[a.m + a.n] = {tile, wps};
+    g_lu_last[mpn] = {tile, wps};
+    return run_choice({tile, wps}, b, rmax, s);
+}
+
 int launch_weights(const WeightsArgs &a, hipStream_t s)
 {
-    if (weights_tile(a.m + a.n) || lu32_tiles(a.m + a.n)) {
-        int tile = 0, wps = 0, rc;
-        if ((rc = lu_choose(a, s, &tile, &wps))) return rc;
-        g_lu_last[a.m + a.n] = {tile, wps};
-        return run_choice({tile, wps}, a, a.nsys, s);
-    }
+    if (weights_tile(a.m + a.n) || lu32_tiles(a.m + a.n)) return launch_weights_levels({a}, s);
     if (!a.ws) MGM_FAIL(-1, "weights: workspace missing for wide systems");
     for (int base = 0; base < a.nsys; base += a.batch) {
         int count = (a.nsys - base < a.batch) ? a.nsys - base : a.batch;

@@ -102,3 +102,35 @@ def _oracle_rows(oracle, P, dim, phs):
             rows["prol"] = oracle.prolongation(d, lv[k + 1], dim, phs, lv[k + 1]["poly"], row_mode=1, method=1)
         out.append(rows)
     return out
+
+
+@pytest.mark.parametrize("kernel", ["gj", "mfma"])
+def test_one_launch_over_levels_of_uneven_size(kernel, gpu, oracle, monkeypatch):
+    """Setup builds every level's weights and transfer rows in one launch over
+    all systems; here the levels' systems differ in size (PHS r^5 with degree-4
+    polynomials on level 0: 45 unknowns; degree 3 below: 30).  Each level's
+    rows equal those of a build of that level alone, bit for bit, and the
+    oracle's LU to 1e-11."""
+    monkeypatch.setenv("MGM_LU_KERNEL", kernel)
+    monkeypatch.delenv("MGM_LU_VARIANT", raising=False)
+    dim, sizes, phs = 2, (65, 33, 17), 5
+    cfg = gpu.config(dimension=dim, poly_degree=4, coarse_poly_degree=3, phs_degree=phs,
+                     cloud_size_multiplier=2.0, omega=1.0, num_relax=3, transfer_rows=gpu.ROWS_BY_TAG,
+                     sweep_order=gpu.ORDER_COLOUR)
+    P = gpu.Problem(cfg, [gpu.Points.generate(dim, n, 1 + k, 0.2) for k, n in enumerate(sizes)]).setup()
+    assert [P.info(k)["m"] + P.info(k)["n"] for k in range(P.L)] == [45, 30, 30]
+    keys = ["Dx", "Dy", "lap", "lap_Poison", "restr", "prol"]
+    batched = [{key: P.matrix(k, key).copy() for key in keys
+                if not (key == "restr" and k == 0) and not (key == "prol" and k == P.L - 1)} for k in range(P.L)]
+    ref = _oracle_rows(oracle, P, dim, phs)
+    for k in range(P.L):
+        for key, W in ref[k].items():
+            e = _row_rel(batched[k][key], W).max()
+            assert e <= 1e-11, (kernel, k, key, e)
+    # level by level: transfer rows of level k come from the systems of level
+    # k - 1 (restriction) and k + 1 (prolongation), so rebuild all first
+    for k in range(P.L):
+        P.rebuild_weights(k)
+    for k in range(P.L):
+        for key, G in batched[k].items():
+            np.testing.assert_array_equal(P.matrix(k, key), G, err_msg=f"{kernel} L{k} {key}")
