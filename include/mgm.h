@@ -60,6 +60,15 @@ typedef struct {
     int num_relax;
     int transfer_rows;             /* MGM_ROWS_* */
     int sweep_order;               /* MGM_ORDER_* */
+    /* No reference counterpart: wider stencils on boundary nodes.  0 (the
+     * default) gives every node m = floor(cloud_size_multiplier * n) points,
+     * as the reference does (kdtree_functions.h:139-149).  A larger value
+     * gives each boundary node a second stencil of m_b = floor(value * n)
+     * points (itself, then its m_b - 1 nearest interior nodes, the same rule
+     * as the reference's boundary stencils) and wide derivative operators on
+     * it (mgm_get_boundary_rows, mgm_spmv_wide); every other operator keeps
+     * the m-point stencils.  m_b + n <= 128. */
+    double boundary_cloud_size_multiplier;
 } mgm_config;
 
 typedef struct {
@@ -114,7 +123,8 @@ int mgm_level_info_get(const mgm_problem *p, int lev, mgm_level_info *out);
  * [3] weights+transfer rows [4] slab assembly [5] total */
 int mgm_setup_timings(const mgm_problem *p, double *t6);
 /* Rebuild the weight rows of one level (batched local solves) and return the
- * kernel time in seconds and the number of systems solved. */
+ * kernel time in seconds and the number of systems solved (the wide boundary
+ * systems included). */
 int mgm_rebuild_weights(mgm_problem *p, int lev, double *seconds, long *systems);
 /* No reference counterpart: the variant of the batched weight kernel that
  * ran the last build of systems of size mpn (= m + n; the autotuner's choice
@@ -165,6 +175,20 @@ int mgm_prolong(mgm_problem *p, int lev, int variant);       /* lev -> lev-1 */
 int mgm_vcycle(mgm_problem *p, int variant, int ncycles);
 /* out = M * in over fixed-width rows (M: MGM_MAT_DX..MGM_MAT_LAPP). */
 int mgm_spmv(mgm_problem *p, int lev, int which_matrix, int in_field, int out_field);
+/* No reference counterpart (wide boundary stencils, boundary_cloud_size_multiplier):
+ * the number of boundary nodes with a wide stencil and its width m_b (0, 0 when
+ * the level has none). */
+int mgm_boundary_info(const mgm_problem *p, int lev, int *nbw, int *mb);
+/* No reference counterpart: the wide boundary rows of Dx, Dy, Dz or lap
+ * (MGM_MAT_DX .. MGM_MAT_LAP): nodes[nbw] (reference order), cloud[nbw*m_b]
+ * (node 0 of each row is the node itself) and w[nbw*m_b]; null outputs are
+ * skipped.  Solved in the same batched weight build as the m-point systems. */
+int mgm_get_boundary_rows(const mgm_problem *p, int lev, int which, int *nodes, int *cloud, double *w);
+/* No reference counterpart: mgm_spmv of Dx, Dy, Dz or lap with the boundary
+ * rows taken from the wide stencils (ordered sums, as
+ * multiply_sparse_matrix_vector, header_files/mat_lib.h:70-89, over the wider
+ * rows). */
+int mgm_spmv_wide(mgm_problem *p, int lev, int which_matrix, int in_field, int out_field);
 int mgm_l2_norm_gen(const mgm_problem *p, int lev, int field_a, int field_b, double *out);
 /* out = D f over caller-supplied fixed-width rows (uploaded for the call):
  * D and cloud are n_rows x n_cols row-major, f has ncol entries.

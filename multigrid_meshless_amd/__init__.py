@@ -248,6 +248,27 @@ class Problem:
     def spmv(self, lev, matrix="lap_Poison", src="p", dst="dpdn"):
         check(lib().mgm_spmv(self._h, lev, MAT[matrix], FIELD[src], FIELD[dst]), "spmv")
 
+    def spmv_wide(self, lev, matrix="Dx", src="p", dst="dpdx"):
+        """spmv with the boundary rows taken from the wide boundary stencils."""
+        check(lib().mgm_spmv_wide(self._h, lev, MAT[matrix], FIELD[src], FIELD[dst]), "spmv_wide")
+
+    def boundary_info(self, lev: int):
+        """(number of boundary nodes with a wide stencil, its width m_b); (0, 0) if none."""
+        nbw, mb = C.c_int(), C.c_int()
+        check(lib().mgm_boundary_info(self._h, lev, C.byref(nbw), C.byref(mb)), "boundary_info")
+        return nbw.value, mb.value
+
+    def boundary_rows(self, lev: int, name: str = "Dx") -> dict:
+        """The wide boundary rows of Dx, Dy, Dz or lap: nodes (reference order),
+        cloud and weights (nbw x m_b)."""
+        nbw, mb = self.boundary_info(lev)
+        nodes = np.zeros(nbw, np.int32)
+        cloud = np.zeros((nbw, mb), np.int32)
+        w = np.zeros((nbw, mb))
+        check(lib().mgm_get_boundary_rows(self._h, lev, MAT[name], nodes.ctypes.data, cloud.ctypes.data,
+                                          w.ctypes.data), "get_boundary_rows")
+        return dict(nodes=nodes, cloud=cloud, W=w)
+
     def l2_norm_gen(self, lev, a, b=None) -> float:
         out = C.c_double()
         check(lib().mgm_l2_norm_gen(self._h, lev, FIELD[a], -1 if b is None else FIELD[b],

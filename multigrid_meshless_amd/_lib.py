@@ -18,7 +18,7 @@ class Config(C.Structure):
     _fields_ = [("dimension", C.c_int), ("poly_degree", C.c_int), ("coarse_poly_degree", C.c_int),
                 ("phs_degree", C.c_int), ("cloud_size_multiplier", C.c_double),
                 ("omega", C.c_float), ("num_relax", C.c_int), ("transfer_rows", C.c_int),
-                ("sweep_order", C.c_int)]
+                ("sweep_order", C.c_int), ("boundary_cloud_size_multiplier", C.c_double)]
 
 
 class LevelInfo(C.Structure):
@@ -84,6 +84,9 @@ SIGNATURES = [
     ("mgm_prolong", _i, [_vp, _i, _i]),
     ("mgm_vcycle", _i, [_vp, _i, _i]),
     ("mgm_spmv", _i, [_vp, _i, _i, _i, _i]),
+    ("mgm_spmv_wide", _i, [_vp, _i, _i, _i, _i]),
+    ("mgm_boundary_info", _i, [_vp, _i, _vp, _vp]),
+    ("mgm_get_boundary_rows", _i, [_vp, _i, _i, _vp, _vp, _vp]),
     ("mgm_l2_norm_gen", _i, [_vp, _i, _i, _i, C.POINTER(_d)]),
     ("mgm_spmv_rows", _i, [_i, _i, _vp, _vp, _i, _vp, _vp]),
     ("mgm_spmv_rows_static", _i, [_i, _i, _vp, _vp, _i, _vp, _vp]),

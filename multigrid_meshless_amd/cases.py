@@ -117,20 +117,23 @@ def cube_levels(n: int, levels: int):
     return sizes
 
 
-def cube_poisson(n: int = 100, levels: int = 5, seed0: int = 1, setup: bool = True):
+def cube_poisson(n: int = 100, levels: int = 5, seed0: int = 1, setup: bool = True,
+                 boundary_multiplier: float = 0.0):
     """The 3D cube Poisson hierarchy: a jittered n^3 cloud on [0,1]^3 (jitter
     0.2 h, seeds seed0, seed0+1, ... per level) and `levels` clouds halving in
     spacing; PHS r^3 + cubic polynomials, multiplier 3.5 (m = 70, mpn = 90),
     multicolour SOR with omega 1 and 3 sweeps, transfer rows on every interior
     node, Dirichlet identity rows on every boundary node of every level.
-    Returns (problem, sizes, setup seconds)."""
+    boundary_multiplier > 3.5 adds wide boundary stencils (5.25: m_b = 105,
+    125-unknown systems).  Returns (problem, sizes, setup seconds)."""
     import time
 
     from . import ORDER_COLOUR, ROWS_BY_TAG, Points, Problem, config
     sizes = cube_levels(n, levels)
     pts = [Points.generate(3, s, seed0 + k, 0.2) for k, s in enumerate(sizes)]
     cfg = config(dimension=3, poly_degree=3, coarse_poly_degree=3, phs_degree=3, cloud_size_multiplier=3.5,
-                 omega=1.0, num_relax=3, transfer_rows=ROWS_BY_TAG, sweep_order=ORDER_COLOUR)
+                 omega=1.0, num_relax=3, transfer_rows=ROWS_BY_TAG, sweep_order=ORDER_COLOUR,
+                 boundary_cloud_size_multiplier=boundary_multiplier)
     P = Problem(cfg, pts)
     t = 0.0
     if setup:

@@ -89,6 +89,8 @@ struct KdOrder {
 // out[q*K_out + j], j < K_out, receives ids (K_out <= K).  When `query_ids`
 // is non-null only those queries run.  tie_flag[q] is set when equal
 // distances occur among the K+1 best (order then needs the kd rule).
+// Per-device kNN kernel attribute (dynamic LDS up to the 128-point stencils), set once by mgm_init.
+int knn_device_init();
 int launch_knn(const KnnGrid &g, const KdOrder *kd, int nq, const double *qx, const double *qy,
                const double *qz, const int *query_ids, int K, int K_out, int slot_offset,
                int *out, int out_stride, uint8_t *tie_flag, hipStream_t s);
@@ -112,7 +114,7 @@ struct WeightsArgs {
     int *status;        // nonzero if a zero pivot was met
     int rhs_max;        // most right-hand sides of one system (operators + targets)
 };
-// Systems with mpn + 4 <= 128 run one workgroup per system in registers
+// Systems with mpn + 4 <= 144 run one workgroup per system in registers
 // (weights_tile(mpn) > 0) and need no workspace; wider ones use ws/rhs/piv.
 int weights_tile(int mpn);
 int launch_weights(const WeightsArgs &a, hipStream_t s);
@@ -280,5 +282,9 @@ int launch_lap_poisson(int N, int m, int dim, const double *Dx, const double *Dy
                        const double *lap, const double *nx, const double *ny, const double *nz,
                        const uint8_t *btag, double *lapP, hipStream_t s);
 int launch_dirichlet_rows(int N, int m, const uint8_t *btag, double *rows, hipStream_t s);
+// y[iperm[bnodes[t]]] = sum_j w[t*mb + j] x[iperm[cloud_b[t*mb + j]]], in order
+// from 0.0 (the wide boundary rows over a device-order vector).
+int launch_wide_rows(int nbw, int mb, const int *bnodes, const int *cloud_b, const double *w, const int *iperm,
+                     const double *x, double *y, hipStream_t s);
 
 }  // namespace mgm

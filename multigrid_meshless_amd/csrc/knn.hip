@@ -145,13 +145,24 @@ knn_kernel(KnnGrid g, KdOrder kd, int use_kd, int nq, const double *__restrict__
 
 }  // namespace
 
+// Each query keeps its K best in LDS: 12 bytes x K x 64 lanes, 96 KiB at the
+// widest stencils (128 points), above the 64 KiB default.
+constexpr int kKnnMaxK = 128;
+
+int knn_device_init()
+{
+    HIP_TRY(hipFuncSetAttribute((const void *)knn_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)((size_t)kKnnMaxK * KNN_BLOCK * (sizeof(double) + sizeof(int)))));
+    return 0;
+}
+
 int launch_knn(const KnnGrid &g, const KdOrder *kd, int nq, const double *qx, const double *qy,
                const double *qz, const int *query_ids, int K, int K_out, int slot_offset, int *out,
                int out_stride, uint8_t *tie_flag, hipStream_t s)
 {
     if (nq <= 0) return 0;
     size_t shm = (size_t)K * KNN_BLOCK * (sizeof(double) + sizeof(int));
-    if (shm > 64 * 1024) MGM_FAIL(-1, "knn: stencil larger than 84 points does not fit the 64 KiB LDS budget");
+    if (K > kKnnMaxK) MGM_FAIL(-1, "knn: stencils of more than 128 points are not supported");
     KdOrder k0{};
     int blocks = (nq + KNN_BLOCK - 1) / KNN_BLOCK;
     hipLaunchKernelGGL(knn_kernel, dim3(blocks), dim3(KNN_BLOCK), shm, s, g, kd ? *kd : k0,

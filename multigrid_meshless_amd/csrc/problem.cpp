@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <thread>
 
@@ -240,6 +241,24 @@ static int level_knn(Level &L, int dim)
     HIP_TRY(dcloud.download(L.cloud.data(), (size_t)N * m, g_stream));
     HIP_TRY(hipStreamSynchronize(g_stream));
     for (int i = 0; i < nb; i++) L.cloud[(size_t)i * m] = i;
+    // wide boundary stencils: the node itself, then its mb - 1 nearest
+    // interior nodes (file order here; level_bfs maps them)
+    L.cloud_b.clear();
+    if (L.mb) {
+        const int mb = L.mb;
+        DevBuf<int> dwide;
+        HIP_TRY(dwide.alloc((size_t)nb * mb));
+        HIP_TRY(hipMemsetAsync(dwide.p, 0, sizeof(int) * (size_t)nb * mb, g_stream));
+        int tW = 0;
+        rc = knn_pass(P.x.data(), P.y.data(), P.z.data(), N, dx.p, dy.p, dz.p, candB, dim, dx.p, dy.p,
+                      dz.p, N, qB, mb, mb - 1, 1, dwide.p, mb, &tW);
+        if (rc) return rc;
+        L.knn_ties += tW;
+        L.cloud_b.resize((size_t)nb * mb);
+        HIP_TRY(dwide.download(L.cloud_b.data(), (size_t)nb * mb, g_stream));
+        HIP_TRY(hipStreamSynchronize(g_stream));
+        for (int i = 0; i < nb; i++) L.cloud_b[(size_t)i * mb] = i;
+    }
     return 0;
 }
 
@@ -296,6 +315,21 @@ static int level_bfs(Level &L)
         }
     });
     L.cloud.swap(nc);
+    // wide boundary stencils in the new order: one row per boundary node with a
+    // system (the reference skips nodes whose corner tag, in file order, is set)
+    L.bnodes.clear();
+    std::vector<int> wide;
+    if (L.mb) {
+        const int mb = L.mb;
+        for (int i = 0; i < N; i++)
+            if (L.btag[i] && !L.ctag[i]) L.bnodes.push_back(i);
+        wide.resize(L.bnodes.size() * (size_t)mb);
+        for (size_t t = 0; t < L.bnodes.size(); t++) {
+            const int *src = &L.cloud_b[(size_t)q1[L.bnodes[t]] * mb];
+            for (int j = 0; j < mb; j++) wide[t * mb + j] = q2[src[j]];
+        }
+    }
+    L.cloud_b.swap(wide);
     return 0;
 }
 
@@ -556,6 +590,10 @@ static int upload_level(Level &L)
     HIP_TRY(L.rnx.upload(L.nx, g_stream)); HIP_TRY(L.rny.upload(L.ny, g_stream)); HIP_TRY(L.rnz.upload(L.nz, g_stream));
     HIP_TRY(L.rbtag.upload(L.btag, g_stream));
     HIP_TRY(L.rcloud.upload(L.cloud, g_stream));
+    if (L.mb) {
+        HIP_TRY(L.rcloud_b.upload(L.cloud_b, g_stream));
+        HIP_TRY(L.d_bnodes.upload(L.bnodes, g_stream));
+    }
     HIP_TRY(L.d_iperm.upload(L.iperm, g_stream));
     HIP_TRY(L.d_rp.upload(L.rp, g_stream));
     HIP_TRY(L.d_pp.upload(L.pp, g_stream));
@@ -605,6 +643,11 @@ struct WeightsPrep {
     DevBuf<double> d_rxyz, d_pxyz, ws, rhs;
     DevBuf<int> piv;
     WeightsArgs a{};
+    // wide boundary systems (mb > 0): rows 0 .. nbw-1 of the wide stencils,
+    // the four derivative operators only
+    DevBuf<int> d_rows;
+    WeightsArgs wide{};
+    bool has_wide = false;
 };
 
 static int prepare_weights(mgm_problem *pb, int l, WeightsPrep &w, int *d_status)
@@ -678,6 +721,27 @@ static int prepare_weights(mgm_problem *pb, int l, WeightsPrep &w, int *d_status
     int most = 0;
     for (int i = 0; i < N; i++) most = std::max(most, (roff[i + 1] - roff[i]) + (poff[i + 1] - poff[i]));
     a.rhs_max = 3 + (dim == 3 ? 1 : 0) + most;
+    w.has_wide = L.mb > 0 && !L.bnodes.empty();
+    if (w.has_wide) {
+        const int nbw = (int)L.bnodes.size(), mb = L.mb;
+        std::vector<int> rows(nbw);
+        for (int t = 0; t < nbw; t++) rows[t] = t;
+        HIP_TRY(w.d_rows.upload(rows, g_stream));
+        for (int o = 0; o < 4; o++) {
+            if (o == OP_DZ && dim != 3) continue;
+            HIP_TRY(L.rows_b[o].alloc((size_t)nbw * mb));
+            HIP_TRY(hipMemsetAsync(L.rows_b[o].p, 0, sizeof(double) * nbw * mb, g_stream));
+        }
+        WeightsArgs &b = w.wide;
+        b = a;
+        b.nsys = nbw; b.sys = w.d_rows.p; b.m = mb; b.cloud = L.rcloud_b.p;
+        b.Dx = L.rows_b[OP_DX].p; b.Dy = L.rows_b[OP_DY].p; b.Dz = (dim == 3) ? L.rows_b[OP_DZ].p : nullptr;
+        b.lap = L.rows_b[OP_LAP].p;
+        b.rt_off = nullptr; b.rt_xyz = nullptr; b.rt_row = nullptr; b.R_out = nullptr;
+        b.pt_off = nullptr; b.pt_xyz = nullptr; b.pt_row = nullptr; b.P_out = nullptr;
+        b.ws = nullptr; b.rhs = nullptr; b.piv = nullptr;
+        b.rhs_max = 3 + (dim == 3 ? 1 : 0);
+    }
     return 0;
 }
 
@@ -694,20 +758,33 @@ static int build_weights(mgm_problem *pb, int l0, int l1, double *kernel_seconds
     int rc;
     for (int l = l0; l < l1; l++) {
         if ((rc = prepare_weights(pb, l, prep[l - l0], d_status.p))) return rc;
-        const int mpn = pb->lv[l].m + pb->lv[l].n;
-        batch = batch && (weights_tile(mpn) || lu32_tiles(mpn));
+        const int mpn = pb->lv[l].m + pb->lv[l].n, mpnb = pb->lv[l].mb + pb->lv[l].n;
+        batch = batch && (weights_tile(mpn) || lu32_tiles(mpn)) && (!pb->lv[l].mb || weights_tile(mpnb) || lu32_tiles(mpnb));
     }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0)); HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, g_stream));
     if (batch) {
-        std::vector<WeightsArgs> args;
-        for (auto &w : prep) args.push_back(w.a);
-        rc = launch_weights_levels(args, g_stream);
+        // one launch per register-tile class: every level's systems (and the
+        // wide boundary systems where their tile shape is the same) together
+        std::map<int, std::vector<WeightsArgs>> cls;
+        auto klass = [](const WeightsArgs &x) {
+            const int mpn = x.m + x.n, t = weights_tile(mpn);
+            return t ? t : 100 + lu32_tiles(mpn);
+        };
+        for (auto &w : prep) {
+            cls[klass(w.a)].push_back(w.a);
+            if (w.has_wide) cls[klass(w.wide)].push_back(w.wide);
+        }
+        rc = 0;
+        for (auto &c : cls)
+            if (!rc) rc = launch_weights_levels(c.second, g_stream);
     } else {
         rc = 0;
-        for (auto &w : prep)
+        for (auto &w : prep) {
             if (!rc) rc = launch_weights(w.a, g_stream);
+            if (!rc && w.has_wide) rc = launch_weights(w.wide, g_stream);
+        }
     }
     if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
     HIP_TRY(hipEventRecord(e1, g_stream));
@@ -938,6 +1015,7 @@ int mgm_init(int device)
     int rc = noalloc_init(device);   // the allocation-free entry point's scratch address
     if (rc) return rc;
     if ((rc = stencil_device_init())) return rc;
+    if ((rc = knn_device_init())) return rc;
     if (g_stream) (void)hipStreamDestroy(g_stream);
     HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     g_device = device;
@@ -955,6 +1033,7 @@ void mgm_config_default(mgm_config *c)
     c->num_relax = 3;
     c->transfer_rows = MGM_ROWS_REFERENCE;
     c->sweep_order = MGM_ORDER_COLOUR;
+    c->boundary_cloud_size_multiplier = 0.0;
 }
 
 int mgm_poly_terms(int dim, int deg, short *px, short *py, short *pz)
@@ -1077,6 +1156,14 @@ int mgm_problem_setup(mgm_problem *pb)
         L.N = L.in.N; L.nb = L.in.nb; L.d_avg_bbox = L.in.d_avg;
         if (L.m < 2 || L.m > 127) MGM_FAIL(-1, "stencil size must be in [2,127]");
         if (L.m + L.n > 256) MGM_FAIL(-1, "local system larger than 256");
+        L.mb = 0;
+        if (pb->cfg.boundary_cloud_size_multiplier > 0) {
+            const int mb = (int)floor(pb->cfg.boundary_cloud_size_multiplier * L.n + 1e-9);
+            if (mb > L.m) {
+                if (mb + L.n > 128) MGM_FAIL(-1, "wide boundary stencils: m_b + n must not exceed 128");
+                L.mb = mb;
+            }
+        }
         if (L.N < L.m + 1) MGM_FAIL(-1, "level has fewer nodes than the stencil size");
         double a = now();
         if ((rc = level_knn(L, dim))) return rc;
@@ -1156,6 +1243,7 @@ int mgm_rebuild_weights(mgm_problem *pb, int lev, double *seconds, long *systems
     if (systems) {
         long s = 0;
         for (int i = 0; i < pb->lv[lev].N; i++) s += !pb->lv[lev].ctag[i];
+        if (pb->lv[lev].mb) s += (long)pb->lv[lev].bnodes.size();   // the wide boundary systems
         *systems = s;
     }
     return 0;
@@ -1236,6 +1324,33 @@ int mgm_get_matrix(const mgm_problem *pb, int lev, int which, double *out)
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(out, buf, 8 * cnt, hipMemcpyDeviceToHost, g_stream));
     HIP_TRY(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int mgm_boundary_info(const mgm_problem *pb, int lev, int *nbw, int *mb)
+{
+    CHECK_READY(pb, lev);
+    if (!nbw || !mb) MGM_FAIL(-1, "null argument");
+    const Level &L = pb->lv[lev];
+    *mb = L.mb;
+    *nbw = L.mb ? (int)L.bnodes.size() : 0;
+    return 0;
+}
+
+int mgm_get_boundary_rows(const mgm_problem *pb, int lev, int which, int *nodes, int *cloud, double *w)
+{
+    CHECK_READY(pb, lev);
+    const Level &L = pb->lv[lev];
+    if (!L.mb) MGM_FAIL(-1, "boundary_rows: this level has no wide boundary stencils");
+    if (which < MGM_MAT_DX || which > MGM_MAT_LAP || (which == MGM_MAT_DZ && pb->cfg.dimension != 3))
+        MGM_FAIL(-1, "boundary_rows: Dx, Dy, Dz (3D) or lap only");
+    const size_t nbw = L.bnodes.size();
+    if (nodes) std::copy(L.bnodes.begin(), L.bnodes.end(), nodes);
+    if (cloud) std::copy(L.cloud_b.begin(), L.cloud_b.end(), cloud);
+    if (w) {
+        HIP_TRY(hipMemcpyAsync(w, L.rows_b[which].p, sizeof(double) * nbw * L.mb, hipMemcpyDeviceToHost, g_stream));
+        HIP_TRY(hipStreamSynchronize(g_stream));
+    }
     return 0;
 }
 
@@ -1471,6 +1586,26 @@ int mgm_spmv(mgm_problem *pb, int lev, int which_matrix, int in_field, int out_f
     Level &L = pb->lv[lev];
     if (!L.slab[which_matrix].p) MGM_FAIL(-1, "spmv: matrix not built");
     OP_SYNC(launch_spmv(L.op(which_matrix), L.Nd, L.dbtag.p, L.f[in_field].p, L.f[out_field].p, g_stream));
+}
+
+int mgm_spmv_wide(mgm_problem *pb, int lev, int which_matrix, int in_field, int out_field)
+{
+    CHECK_READY(pb, lev);
+    if (which_matrix < MGM_MAT_DX || which_matrix > MGM_MAT_LAP ||
+        (which_matrix == MGM_MAT_DZ && pb->cfg.dimension != 3))
+        MGM_FAIL(-1, "spmv_wide: Dx, Dy, Dz (3D) or lap only");
+    if (in_field < 0 || in_field >= MGM_NUM_FIELDS || out_field < 0 || out_field >= MGM_NUM_FIELDS ||
+        in_field == out_field)
+        MGM_FAIL(-1, "spmv_wide: bad field ids");
+    Level &L = pb->lv[lev];
+    if (!L.mb) MGM_FAIL(-1, "spmv_wide: this level has no wide boundary stencils");
+    int rc = launch_spmv(L.op(which_matrix), L.Nd, L.dbtag.p, L.f[in_field].p, L.f[out_field].p, g_stream);
+    if (rc) return rc;
+    rc = launch_wide_rows((int)L.bnodes.size(), L.mb, L.d_bnodes.p, L.rcloud_b.p, L.rows_b[which_matrix].p,
+                          L.d_iperm.p, L.f[in_field].p, L.f[out_field].p, g_stream);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    return 0;
 }
 
 int mgm_spmv_rows(int n_rows, int n_cols, const double *D, const int32_t *cloud, int ncol,

@@ -27,6 +27,10 @@ struct Level {
     std::vector<double> x, y, z, nx, ny, nz;
     std::vector<uint8_t> btag, ctag;   // ctag is NOT permuted, as in the reference
     std::vector<int> cloud;            // N*m
+    // Wide boundary stencils (mb > 0): the boundary nodes with a system
+    // (ascending, reference order) and their mb-point stencils, nbw x mb.
+    int mb = 0;
+    std::vector<int> bnodes, cloud_b;
     std::vector<int> rp, pp;           // restriction / prolongation points
     std::vector<int> colour;
     int ncol = 0;
@@ -47,6 +51,8 @@ struct Level {
     DevBuf<int> rcloud, d_iperm, d_rp, d_pp;
     DevBuf<short> d_px, d_py, d_pz;
     std::array<DevBuf<double>, NUM_OPS> rows;  // row-major N*m
+    DevBuf<int> rcloud_b, d_bnodes;            // wide boundary stencils (mb > 0)
+    std::array<DevBuf<double>, 4> rows_b;      // wide Dx, Dy, Dz, lap rows: bnodes.size() * mb
     DevBuf<double> Rrows;                      // this level as coarse: N * m_{l-1}
     DevBuf<double> Prows;                      // this level as fine:   N * m_{l+1}
     // ---- device, device order ----

@@ -778,6 +778,23 @@ lap_poisson_kernel(int N, int m, int dim, const double *__restrict__ Dx, const d
     lapP[t] = v;
 }
 
+// Rows with wide boundary stencils (reference order: node bnodes[t], its mb
+// points cloud_b[t * mb ..], weights w[t * mb ..]) applied to a device-order
+// vector: the ordered sum from 0.0 of the rounded products, one thread per row.
+__global__ void __launch_bounds__(BLK)
+wide_rows_kernel(int nbw, int mb, const int *__restrict__ bnodes, const int *__restrict__ cloud_b,
+                 const double *__restrict__ w, const int *__restrict__ iperm, const double *__restrict__ x,
+                 double *__restrict__ y)
+{
+    const int t = blockIdx.x * BLK + threadIdx.x;
+    if (t >= nbw) return;
+    const int *c = cloud_b + (size_t)t * mb;
+    const double *ww = w + (size_t)t * mb;
+    double s = 0.0;
+    for (int j = 0; j < mb; j++) s += ww[j] * x[iperm[c[j]]];
+    y[iperm[bnodes[t]]] = s;
+}
+
 __global__ void __launch_bounds__(BLK)
 dirichlet_rows_kernel(int N, int m, const uint8_t *__restrict__ btag, double *__restrict__ rows)
 {
@@ -1161,6 +1178,15 @@ int launch_lap_poisson(int N, int m, int dim, const double *Dx, const double *Dy
     LAUNCH(lap_poisson_kernel, (long)N * m, N, m, dim, Dx, Dy, Dz, lap, nx, ny, nz, btag, lapP);
     return 0;
 }
+int launch_wide_rows(int nbw, int mb, const int *bnodes, const int *cloud_b, const double *w, const int *iperm,
+                     const double *x, double *y, hipStream_t s)
+{
+    if (nbw <= 0) return 0;
+    hipLaunchKernelGGL(wide_rows_kernel, dim3(nblk(nbw)), dim3(BLK), 0, s, nbw, mb, bnodes, cloud_b, w, iperm, x, y);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 int launch_dirichlet_rows(int N, int m, const uint8_t *btag, double *rows, hipStream_t s)
 {
     LAUNCH(dirichlet_rows_kernel, (long)N * m, N, m, btag, rows);

@@ -442,7 +442,7 @@ lu_kernel(WeightsBatch bt)
 
 int weights_tile(int mpn)
 {
-    for (int t = 2; t <= 8; t++)
+    for (int t = 2; t <= 9; t++)
         if (16 * t >= mpn + 4) return t;
     return 0;
 }
@@ -462,7 +462,13 @@ static LuFn lu_fn_w(int tile)
     default: return lu_kernel<8, W>;
     }
 }
-static LuFn lu_fn(int tile, int wps) { return wps == 4 ? lu_fn_w<4>(tile) : lu_fn_w<1>(tile); }
+// 9 x 16 tiles (systems of up to 140 unknowns, the wide boundary stencils):
+// unconstrained register budget only
+static LuFn lu_fn(int tile, int wps)
+{
+    if (tile == 9) return lu_kernel<9, 1>;
+    return wps == 4 ? lu_fn_w<4>(tile) : lu_fn_w<1>(tile);
+}
 
 // One workgroup per system of the batch.
 static int lu_run(LuFn f, WeightsBatch b, hipStream_t s)
@@ -508,7 +514,7 @@ static int lu_choose(int mpn, const WeightsBatch &b, int rmax, hipStream_t s, in
     // MGM_LU_VARIANT=tile,wps forces a variant (tests compare them), uncached
     if (const char *v = getenv("MGM_LU_VARIANT")) {
         int t = 0, w = 0;
-        if (sscanf(v, "%d,%d", &t, &w) == 2 && t0 && t >= t0 && t <= 8 && (w == 1 || (w == 4 && t <= 6))) {
+        if (sscanf(v, "%d,%d", &t, &w) == 2 && t0 && t >= t0 && t <= 9 && (w == 1 || (w == 4 && t <= 6))) {
             *tile = t;
             *wps = w;
             return 0;
@@ -526,7 +532,7 @@ static int lu_choose(int mpn, const WeightsBatch &b, int rmax, hipStream_t s, in
     sample.nsys = b.nsys < 8192 ? b.nsys : 8192;
     if (t0 && !(e && e[0] == '0') && sample.nsys >= 1024) {
         std::vector<std::pair<int, int>> cand;
-        for (int t = t0; t <= t0 + 1 && t <= 8; t++) {
+        for (int t = t0; t <= t0 + 1 && t <= 9; t++) {
             if (t <= 6) cand.push_back({t, 4});
             cand.push_back({t, 1});
         }
