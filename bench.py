@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_FLOPS = 78.6e12
+FP32_MFMA_PEAK_FLOPS = 157.3e12
 COPY_BYTES = 1 << 30
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stencil_apply_3d_1m_traffic.md")
 
@@ -60,6 +61,72 @@ def set_rhs(P):
 def reset(P, f):
     P.zero_fields()
     P.set_field(0, "source", f)
+
+
+def lu32_mfma_count(mpn, tiles):
+    """v_mfma_f32_16x16x4_f32 instructions of one factorisation in the matrix-core
+    weight kernel: panels of four columns, each updating every active tile
+    column (T - tc of them, less the panel's own once its four columns are done)
+    in each of the T tile rows."""
+    count = 0
+    for c in range((mpn + 3) // 4):
+        tc, gc = divmod(c, 4)
+        count += tiles * ((tiles - tc) - (1 if gc == 3 else 0))
+    return count
+
+
+def matrix_core_build(P, mpn):
+    """Level-0 weight build with the matrix-core kernel forced (FP32 LU on MFMA,
+    FP64 refinement): systems/s and the matrix-core flop rate of its
+    factorisations against the FP32 matrix peak."""
+    os.environ["MGM_LU_KERNEL"] = "mfma"
+    try:
+        t, n = P.rebuild_weights(0)
+    finally:
+        del os.environ["MGM_LU_KERNEL"]
+    P.rebuild_weights(0)                # restore the autotuned kernel's rows
+    tiles = -(-mpn // 16)
+    flops = lu32_mfma_count(mpn, tiles) * 16 * 16 * 4 * 2
+    out = {"kernel_s": round(t, 4), "systems_per_s": round(n / t), "tiles": tiles,
+           "mfma_per_system": lu32_mfma_count(mpn, tiles),
+           "mfma_tflops": round(n * flops / t / 1e12, 3),
+           "frac_fp32_matrix_peak": round(n * flops / t / FP32_MFMA_PEAK_FLOPS, 4)}
+    print(f"[bench] weights level 0, matrix-core kernel: {n / t:.0f} systems/s, "
+          f"{out['mfma_tflops']} TFLOP/s on MFMA", file=sys.stderr, flush=True)
+    return out
+
+
+def uneven_build(mgm, args):
+    """The uneven batched build: the cube's hierarchy with wide boundary stencils
+    (boundary multiplier 5.25: m_b = 105, 125 unknowns; 70 points and 90
+    unknowns inside), every level's systems built in one launch per tile class.
+    Level-0 rebuild rate and the whole hierarchy's weight phase in setup."""
+    from multigrid_meshless_amd import cases
+    P, sizes, t_setup = cases.cube_poisson(args.n, args.levels, 1, boundary_multiplier=5.25)
+    systems = sum(P.info(k)["N"] for k in range(P.L)) + sum(P.boundary_info(k)[0] for k in range(P.L))
+    t_all = P.timings()["weights"]
+    t0, n0 = P.rebuild_weights(0)
+    nbw, mb = P.boundary_info(0)
+    out = {"workload": f"{sizes[0]}^3 cube, {len(sizes)} levels, m=70 inside (90 unknowns), "
+                       f"m_b={mb} on the boundary ({mb + 20} unknowns)",
+           "level0_systems": n0, "level0_wide_systems": nbw, "level0_kernel_s": round(t0, 4),
+           "level0_systems_per_s": round(n0 / t0),
+           "hierarchy_systems": systems, "hierarchy_weight_phase_s": round(t_all, 4),
+           "kernel_90": dict(zip(("tile", "waves_per_simd"), P.weight_kernel_choice(0) or (None, None))),
+           "kernel_125": dict(zip(("tile", "waves_per_simd"),
+                                  mgm_kernel_choice(mgm, mb + 20) or (None, None)))}
+    print(f"[bench] uneven weight build: level 0 {n0} systems ({nbw} wide) in {t0:.3f}s = {n0 / t0:.0f} "
+          f"systems/s; hierarchy {systems} systems, weight phase {t_all:.3f}s", file=sys.stderr, flush=True)
+    del P
+    return out
+
+
+def mgm_kernel_choice(mgm, mpn):
+    import ctypes as C
+    t, w = C.c_int(), C.c_int()
+    if mgm._lib.lib().mgm_weight_kernel_choice(mpn, C.byref(t), C.byref(w)) != 0:
+        return None
+    return t.value, w.value
 
 
 def build_cavity(mgm, n, levels, Re, courant, num_relax, num_vcycles, seed0=31):
@@ -259,6 +326,8 @@ def main():
     ap.add_argument("--ldc-re", type=float, default=100.0)
     ap.add_argument("--ldc-steps", type=int, default=2000)
     ap.add_argument("--no-annulus", action="store_true", help="skip the learned smoother on the 2D annulus")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven weight build (wide boundary stencils, a second hierarchy)")
     ap.add_argument("--ann-n", type=int, default=714, help="annulus grid points per side (714: 301k points)")
     ap.add_argument("--ann-levels", type=int, default=4)
     ap.add_argument("--ann-max-cycles", type=int, default=3000)
@@ -396,6 +465,10 @@ def main():
                "autotuned_kernel": dict(zip(("tile", "waves_per_simd"), P.weight_kernel_choice(0) or (None, None)))}
     print(f"[bench] weights level 0: {n_sys} systems in {t_w:.3f}s = {n_sys / t_w:.0f} systems/s, "
           f"{weights['gflops']} GFLOP/s", file=sys.stderr, flush=True)
+    weights["matrix_core_kernel"] = matrix_core_build(P, mpn)
+    P.dirichlet_rows()
+    if not args.no_uneven:
+        weights["uneven_wide_boundary"] = uneven_build(mgm, args)
 
     cavity = None if args.no_ldc else cavity_march(mgm, args)
     annulus = None if (args.no_annulus or args.no_learned) else annulus_learned(mgm, args)
