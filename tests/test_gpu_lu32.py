@@ -106,11 +106,12 @@ def _oracle_rows(oracle, P, dim, phs):
 
 @pytest.mark.parametrize("kernel", ["gj", "mfma"])
 def test_one_launch_over_levels_of_uneven_size(kernel, gpu, oracle, monkeypatch):
-    """Setup builds every level's weights and transfer rows in one launch over
-    all systems; here the levels' systems differ in size (PHS r^5 with degree-4
-    polynomials on level 0: 45 unknowns; degree 3 below: 30).  Each level's
-    rows equal those of a build of that level alone, bit for bit, and the
-    oracle's LU to 1e-11."""
+    """Setup solves the systems of all levels in one launch per tile class
+    (16 t >= unknowns + 4 for the Gauss-Jordan grid of t tiles); here the
+    levels' systems differ in size (PHS r^5 with degree-4 polynomials on level
+    0: 45 unknowns, class 4; degree 3 below: 30 unknowns, levels 1 and 2
+    together in class 3).  Each level's rows equal those of a build of that
+    level alone, bit for bit, and the oracle's LU to 1e-11."""
     monkeypatch.setenv("MGM_LU_KERNEL", kernel)
     monkeypatch.delenv("MGM_LU_VARIANT", raising=False)
     dim, sizes, phs = 2, (65, 33, 17), 5
@@ -134,3 +135,50 @@ def test_one_launch_over_levels_of_uneven_size(kernel, gpu, oracle, monkeypatch)
     for k in range(P.L):
         for key, G in batched[k].items():
             np.testing.assert_array_equal(P.matrix(k, key), G, err_msg=f"{kernel} L{k} {key}")
+
+
+@pytest.mark.parametrize("kernel", ["gj", "mfma"])
+def test_interior_and_boundary_systems_in_one_launch(kernel, gpu, oracle, monkeypatch):
+    """Interior systems (m = 70: 90 unknowns) and wide boundary systems
+    (m_b = 72: 92 unknowns) of both levels share a tile class (6 tiles of 16),
+    so setup solves all of them in one launch, each at its own size.  Every
+    row within 1e-11 of the oracle's LU on the same stencils, and bit for bit
+    the rows of per-level rebuilds."""
+    monkeypatch.setenv("MGM_LU_KERNEL", kernel)
+    monkeypatch.delenv("MGM_LU_VARIANT", raising=False)
+    dim, sizes, phs = 3, (16, 8), 3
+    print(f"[gpu lu32] interior 90 + boundary 92 unknowns in one launch, kernel {kernel}", flush=True)
+    cfg = gpu.config(dimension=dim, poly_degree=3, coarse_poly_degree=3, phs_degree=phs,
+                     cloud_size_multiplier=3.5, omega=1.0, num_relax=3, transfer_rows=gpu.ROWS_BY_TAG,
+                     sweep_order=gpu.ORDER_COLOUR, boundary_cloud_size_multiplier=3.6)
+    P = gpu.Problem(cfg, [gpu.Points.generate(dim, n, 1 + k, 0.2) for k, n in enumerate(sizes)]).setup()
+    ops = ["Dx", "Dy", "Dz", "lap"]
+    for k in range(P.L):
+        assert P.info(k)["m"] + P.info(k)["n"] == 90 and P.boundary_info(k)[1] == 72
+    keys = ops + ["lap_Poison", "restr", "prol"]
+    batched = [{key: P.matrix(k, key).copy() for key in keys
+                if not (key == "restr" and k == 0) and not (key == "prol" and k == P.L - 1)} for k in range(P.L)]
+    wide = [{key: P.boundary_rows(k, key) for key in ops} for k in range(P.L)]
+    ref = _oracle_rows(oracle, P, dim, phs)
+    worst = 0.0
+    for k in range(P.L):
+        for key, W in ref[k].items():
+            e = _row_rel(batched[k][key], W).max()
+            worst = max(worst, e)
+            assert e <= 1e-11, (kernel, k, key, e)
+        geo = P.geometry(k)
+        lv = dict(x=geo["x"], y=geo["y"], z=geo["z"], cloud=wide[k]["Dx"]["cloud"])
+        Wo = dict(zip(ops, oracle.weights(lv, dim, phs, P.info(k)["poly_degree"], method=1)))
+        for key in ops:
+            e = _row_rel(wide[k][key]["W"], Wo[key]).max()
+            worst = max(worst, e)
+            assert e <= 1e-11, (kernel, k, "wide " + key, e)
+    for k in range(P.L):
+        P.rebuild_weights(k)
+    for k in range(P.L):
+        for key, G in batched[k].items():
+            np.testing.assert_array_equal(P.matrix(k, key), G, err_msg=f"{kernel} L{k} {key}")
+        for key in ops:
+            np.testing.assert_array_equal(P.boundary_rows(k, key)["W"], wide[k][key]["W"],
+                                          err_msg=f"{kernel} L{k} wide {key}")
+    print(f"  worst row-relative difference from the oracle's LU: {worst:.2e}", flush=True)
