@@ -5,7 +5,10 @@ HBM): jittered 100^3 cube (seed 1; 998,816 nodes after corner/edge removal),
 5 levels 100^3/50^3/25^3/13^3/7^3 (seeds 1..5), PHS r^3 + cubic polynomials,
 m = 70 stencils (multiplier 3.5), Dirichlet-zero identity rows on every level,
 RHS = sum of 16 Fourier modes (amplitudes N(0,1), SplitMix64 seed 2), V-cycles
-of multicolour SOR (omega 1, 3 sweeps) until l2_norm_gen(r)/l2_norm_gen(f) <= 1e-8.
+of multicolour SOR (3 sweeps) until l2_norm_gen(r)/l2_norm_gen(f) <= 1e-8.  The
+SOR factor omega is autotuned before the timed solves (mgm_tune_omega: the
+fewest cycles to the tolerance on this source; --omega fixes it instead); the
+omega = 1 solve is timed once beside it.
 
 Prints one JSON line (rank 0, one GPU).  `roofline` is the FP64 stencil apply
 (lap_Poison over level 0) timed with HIP events on the library stream;
@@ -257,7 +260,7 @@ def cpu_hierarchy(P):
     return oracle.Hierarchy(lv), lv
 
 
-def cpu_baseline(P, f, tol, max_cycles, weight_sample):
+def cpu_baseline(P, f, tol, max_cycles, weight_sample, omega):
     """The oracle on the host: (1) the same solve to tol on every available
     thread -- V-cycle, fine residual and l2_norm_gen norm per cycle, timed
     whole; (2) two cycles on one thread; (3) the reference's weight build
@@ -280,7 +283,7 @@ def cpu_baseline(P, f, tol, max_cycles, weight_sample):
     t0 = time.perf_counter()
     cycles, ratio = 0, 1.0
     while cycles < max_cycles and ratio > tol:
-        H.vcycle(oracle.VAR_H, 1.0, 3)
+        H.vcycle(oracle.VAR_H, omega, 3)
         H.residual(oracle.VAR_H, 0)
         ratio = oracle.l2_norm_gen(btag, H.lv[0]["res"]) / fnorm
         cycles += 1
@@ -289,7 +292,7 @@ def cpu_baseline(P, f, tol, max_cycles, weight_sample):
     reset()
     t0 = time.perf_counter()
     for _ in range(2):
-        H.vcycle(oracle.VAR_H, 1.0, 3)
+        H.vcycle(oracle.VAR_H, omega, 3)
         H.residual(oracle.VAR_H, 0)
         oracle.l2_norm_gen(btag, H.lv[0]["res"])
     t1 = (time.perf_counter() - t0) / 2
@@ -313,6 +316,7 @@ def main():
     ap.add_argument("--levels", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-cycles", type=int, default=400)
+    ap.add_argument("--omega", default="tune", help="SOR factor of the cube solves, or 'tune' (autotuned)")
     ap.add_argument("--cpu-weight-sample", type=int, default=16384,
                     help="level-0 systems of the CPU weight-build sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -360,6 +364,23 @@ def main():
     print(f"[bench] setup {t_setup:.2f}s levels={sizes} N0={N} m={m} colours0={info0['num_colours']} "
           f"phases={ {k: round(v, 3) for k, v in P.timings().items()} }", file=sys.stderr, flush=True)
 
+    # the SOR factor: autotuned on this source (untimed), and the omega = 1 solve for the record
+    reset(P, f)
+    P.set_smoother(1.0, 3)
+    P.solve(args.tol, args.max_cycles)
+    reset(P, f)
+    r1 = P.solve(args.tol, args.max_cycles)
+    omega1 = {"s": round(r1["seconds"], 5), "cycles": r1["cycles"]}
+    if args.omega == "tune":
+        reset(P, f)
+        t0 = time.perf_counter()
+        tuned = P.tune_omega(args.tol, args.max_cycles)
+        omega, omega_tuned = tuned["omega"], True
+        print(f"[bench] omega tuned in {time.perf_counter() - t0:.2f}s: {omega:.4f} ({tuned['cycles']} cycles; "
+              f"omega 1: {r1['cycles']} cycles {r1['seconds']:.4f}s)", file=sys.stderr, flush=True)
+    else:
+        omega, omega_tuned = float(np.float32(float(args.omega))), False
+    P.set_smoother(omega, 3)
     for _ in range(args.warmup):
         reset(P, f)
         r = P.solve(args.tol, args.max_cycles)
@@ -423,7 +444,10 @@ def main():
         P.set_mlp_weights(Pt.mlp_weights())
         del Pt
         P.use_learned_smoother(True)
-        learned = {"train_s": round(t_train, 2), "train_loss": [round(l0, 4), round(l1, 4)]}
+        learned = {"train_s": round(t_train, 2), "train_loss": [round(l0, 4), round(l1, 4)], "omega": omega}
+        if args.omega == "tune":
+            reset(P, f)
+            learned["omega"] = P.tune_omega(args.tol, args.max_cycles)["omega"]
         for prec, tag in ((0, "fp64"), (1, "fp32"), (2, "fp32_bf16_coarse")):
             if prec and args.no_mixed:
                 continue
@@ -446,6 +470,7 @@ def main():
         print(f"[bench] learned smoother trained in {t_train:.1f}s, loss {l0:.3f} -> {l1:.3f}", file=sys.stderr,
               flush=True)
         P.use_learned_smoother(False)
+        P.set_smoother(omega, 3)
 
     # stencil apply roofline: lap_Poison over level 0, FP64 weights + int32 columns
     t_apply = P.time_spmv(0, args.spmv_reps)
@@ -475,10 +500,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        c = cpu_baseline(P, f, args.tol, 2 * args.max_cycles, args.cpu_weight_sample)
+        c = cpu_baseline(P, f, args.tol, 2 * args.max_cycles, args.cpu_weight_sample, omega)
         cpu = {"value": round(c["solve_s"], 3), "unit": "s", "cores": c["threads"], "kind": "port",
-               "sample": f"the whole solve to {args.tol} on the host: the oracle's colour-ordered V-cycle on the same "
-                         f"1M-point operators (rows of a colour class in parallel on {c['threads']} threads), "
+               "sample": f"the whole solve to {args.tol} on the host: the oracle's colour-ordered V-cycle (SOR factor "
+                         f"{omega:.4g}) on the same 1M-point operators (rows of a colour class in parallel on {c['threads']} threads), "
                          f"{c['cycles']} cycles to {c['final_ratio']:.2e} in {c['solve_s']:.2f} s; 1 thread: "
                          f"{c['s_per_cycle_1thread']:.3f} s/cycle (2 cycles timed) = "
                          f"{c['s_per_cycle_1thread'] * c['cycles']:.1f} s for the solve; weight build (the "
@@ -506,6 +531,7 @@ def main():
         "config": {"workload": f"3D cube Poisson, {sizes[0]}^3 jittered cloud ({N} nodes), "
                                f"{len(sizes)} levels {sizes}, m={m}, FP64 multicolour SOR V-cycle, "
                                f"tol {args.tol}",
+                   "omega": omega, "omega_tuned": omega_tuned, "num_relax": 3, "omega_1": omega1,
                    "cycles": cycles[0], "converged": converged,
                    "vcycles_per_s": round(cycles[0] / t_solve, 1),
                    "final_ratio": finals[0], "setup_s": round(t_setup, 2),

@@ -292,6 +292,17 @@ int mgm_learned_fell_back(const mgm_problem *p);      /* 1 if the last solve fel
  *            1 and 2 support the H variant in the multicolour order. */
 int mgm_solve(mgm_problem *p, int variant, int precision, double tol, int max_cycles,
               int *cycles_done, double *history, double *seconds);
+/* SOR factor autotuner (no reference counterpart: the reference reads omega
+ * from flow_parameters.csv, hf/mesh_functions.h:37-100).  Runs mgm_solve from
+ * the current level-0 iterate and source for omega = lo, lo + step, ..., hi,
+ * then at the midpoints on either side of the best; keeps the omega that
+ * reaches tol in the fewest cycles (ties: the smaller final residual) and sets
+ * it as the smoother's factor.  A candidate whose solve diverges or needs more
+ * cycles than the best so far is stopped early.  The level-0 iterate and
+ * source are restored afterwards and every other field is left zero.  Fails
+ * (-12, omega unchanged) when no candidate reaches tol within max_cycles. */
+int mgm_tune_omega(mgm_problem *p, int variant, int precision, double tol, int max_cycles, float lo, float hi,
+                   float step, float *omega, int *cycles);
 /* Seconds per copy of `bytes` (rounded down to 16) between two device buffers
  * (a streaming-copy kernel): the measured HBM reference of the roofline. */
 int mgm_stream_copy(size_t bytes, int reps, double *seconds);

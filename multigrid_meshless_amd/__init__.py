@@ -229,6 +229,16 @@ class Problem:
     def set_smoother(self, omega: float, num_relax: int):
         check(lib().mgm_set_smoother(self._h, omega, num_relax), "set_smoother")
 
+    def tune_omega(self, tol=1e-8, max_cycles=400, variant=VAR_H, precision=0, lo=0.5, hi=1.0, step=0.05):
+        """Pick the SOR factor that solves the current level-0 source (from the
+        current iterate) in the fewest cycles: a grid lo..hi by step, then the
+        midpoints beside the best.  Sets it as the smoother's factor; the
+        iterate and source are restored, every other field left zero."""
+        w, n = C.c_float(), C.c_int()
+        check(lib().mgm_tune_omega(self._h, variant, precision, tol, max_cycles, lo, hi, step, C.byref(w),
+                                   C.byref(n)), "tune_omega")
+        return dict(omega=w.value, cycles=n.value)
+
     # ---- operators (reference variant semantics) ----
     def relax(self, lev, variant=VAR_H):
         check(lib().mgm_relax(self._h, lev, variant), "relax")

@@ -106,6 +106,8 @@ SIGNATURES = [
     ("mgm_use_learned_smoother", _i, [_vp, _i]),
     ("mgm_learned_fell_back", _i, [_vp]),
     ("mgm_solve", _i, [_vp, _i, _i, _d, _i, C.POINTER(_i), _vp, C.POINTER(_d)]),
+    ("mgm_tune_omega", _i, [_vp, _i, _i, _d, _i, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
+                            C.POINTER(_i)]),
     ("mgm_stream_copy", _i, [C.c_size_t, _i, C.POINTER(_d)]),
     ("mgm_time_spmv", _i, [_vp, _i, _i, C.POINTER(_d)]),
 ]

@@ -166,6 +166,78 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
     return 0;
 }
 
+int mgm_tune_omega(mgm_problem *pb, int variant, int precision, double tol, int max_cycles, float lo, float hi,
+                   float step, float *omega, int *cycles)
+{
+    if (!pb || !pb->ready) MGM_FAIL(-1, "problem is not set up");
+    if (!(lo > 0.0f && hi < 2.0f && lo <= hi && step > 0.0f)) MGM_FAIL(-1, "tune_omega: need 0 < lo <= hi < 2, step > 0");
+    if ((hi - lo) / step > 64.0f) MGM_FAIL(-1, "tune_omega: more than 65 candidates");
+    if (max_cycles < 1) MGM_FAIL(-1, "tune_omega: max_cycles < 1");
+    Level &L0 = pb->lv[0];
+    const int pf = variant == MGM_VAR_T ? MGM_F_PPRIME : MGM_F_P;
+    const size_t nb = 8 * (size_t)L0.Nd;
+    // every candidate starts from the caller's level-0 iterate and source with
+    // every other field zero (a diverged candidate leaves non-finite values
+    // behind); the same state is restored at the end
+    DevBuf<double> saved;
+    HIP_TRY(saved.alloc(2 * (size_t)L0.Nd));
+    HIP_TRY(hipMemcpyAsync(saved.p, L0.f[pf].p, nb, hipMemcpyDeviceToDevice, stream()));
+    HIP_TRY(hipMemcpyAsync(saved.p + L0.Nd, L0.f[MGM_F_SOURCE].p, nb, hipMemcpyDeviceToDevice, stream()));
+    auto restore = [&]() -> int {
+        for (auto &L : pb->lv)
+            for (auto &f : L.f) HIP_TRY(hipMemsetAsync(f.p, 0, 8 * (size_t)L.Nd, stream()));
+        HIP_TRY(hipMemcpyAsync(L0.f[pf].p, saved.p, nb, hipMemcpyDeviceToDevice, stream()));
+        HIP_TRY(hipMemcpyAsync(L0.f[MGM_F_SOURCE].p, saved.p + L0.Nd, nb, hipMemcpyDeviceToDevice, stream()));
+        return 0;
+    };
+    const float omega0 = pb->cfg.omega;
+    std::vector<double> hist(max_cycles + 1);
+    float best = 0.0f;
+    int best_cycles = max_cycles + 1;
+    double best_ratio = 0.0;
+    // cycles of one candidate; a solve that cannot beat the best so far stops
+    // at the best's count, one that diverges counts as never converging
+    auto trial = [&](float w) -> int {
+        pb->cfg.omega = w;
+        int rc = restore();
+        if (rc) return rc;
+        const int cap = best_cycles <= max_cycles ? best_cycles : max_cycles;
+        int done = 0;
+        rc = mgm_solve(pb, variant, precision, tol, cap, &done, hist.data(), nullptr);
+        if (rc == -11) return 0;   // diverged
+        if (rc) return rc;
+        const double ratio = hist[done];
+        if (!(ratio <= tol)) return 0;
+        if (done < best_cycles || (done == best_cycles && ratio < best_ratio)) {
+            best = w;
+            best_cycles = done;
+            best_ratio = ratio;
+        }
+        return 0;
+    };
+    int rc = 0;
+    // a grid, then the midpoints on either side of the best point
+    const int n = (int)((hi - lo) / step + 0.5f);
+    for (int i = 0; i <= n && !rc; i++) rc = trial(lo + i * step);
+    if (!rc && best_cycles <= max_cycles) {
+        const float b = best;
+        if (b - 0.5f * step > 0.0f) rc = trial(b - 0.5f * step);
+        if (!rc && b + 0.5f * step < 2.0f) rc = trial(b + 0.5f * step);
+    }
+    const int rc2 = restore();
+    HIP_TRY(hipStreamSynchronize(stream()));
+    if (!rc) rc = rc2;
+    if (rc) { pb->cfg.omega = omega0; return rc; }
+    if (best_cycles > max_cycles) {
+        pb->cfg.omega = omega0;
+        MGM_FAIL(-12, "tune_omega: no candidate reached the tolerance within max_cycles");
+    }
+    pb->cfg.omega = best;
+    if (omega) *omega = best;
+    if (cycles) *cycles = best_cycles;
+    return 0;
+}
+
 int mgm_stream_copy(size_t bytes, int reps, double *seconds)
 {
     if (bytes < 16 || reps < 1 || !seconds) MGM_FAIL(-1, "stream_copy: bad arguments");

@@ -155,11 +155,14 @@ def _rows(P, dirichlet):
         P.dirichlet_rows()
 
 
+# SOR factor 1, and 0.85: under-relaxation, the factor the benchmark's autotuner picks
+@pytest.mark.parametrize("omega", [1.0, 0.85])
 @pytest.mark.parametrize("variant,dirichlet", [("H", True), ("H", False), ("T", False), ("F", False)])
-def test_colour_vcycles_bitwise(case, oracle, variant, dirichlet):
+def test_colour_vcycles_bitwise(case, oracle, variant, dirichlet, omega):
     name, P = case
     import multigrid_meshless_amd as mgm
     var = getattr(mgm, "VAR_" + variant)
+    P.set_smoother(omega, 3)
     _rows(P, dirichlet)
     H = _oracle_hierarchy(oracle, P)
     f = np.random.default_rng(11).standard_normal(P.info(0)["N"])
@@ -169,7 +172,7 @@ def test_colour_vcycles_bitwise(case, oracle, variant, dirichlet):
     P.zero_fields()
     P.set_field(0, "source", f)
     for _ in range(3):
-        H.vcycle(var, 1.0, 3)
+        H.vcycle(var, omega, 3)
     P.vcycle(var, 3)
     pf = "pprime" if variant == "T" else "p"
     for k in range(P.L):
@@ -179,11 +182,13 @@ def test_colour_vcycles_bitwise(case, oracle, variant, dirichlet):
     assert np.abs(P.field(0, pf)).max() > 0
 
 
-def test_graph_solve_matches_oracle(case, oracle):
+@pytest.mark.parametrize("omega", [1.0, 0.85])
+def test_graph_solve_matches_oracle(case, oracle, omega):
     """mgm_solve replays one captured V-cycle + fine residual + norm per cycle:
     its fields equal the oracle's after the same cycles, bit for bit, and its
     residual history agrees to rounding (the GPU sums the norm as a tree)."""
     name, P = case
+    P.set_smoother(omega, 3)
     _rows(P, True)
     H = _oracle_hierarchy(oracle, P)
     btag = P.geometry(0)["btag"]
@@ -198,9 +203,35 @@ def test_graph_solve_matches_oracle(case, oracle):
     fn = oracle.l2_norm_gen(btag, f)
     hist = []
     for _ in range(ncyc):
-        H.vcycle(oracle.VAR_H, 1.0, 3)
+        H.vcycle(oracle.VAR_H, omega, 3)
         H.residual(oracle.VAR_H, 0)
         hist.append(oracle.l2_norm_gen(btag, H.lv[0]["res"]) / fn)
     np.testing.assert_array_equal(P.field(0, "p"), H.lv[0]["p"], err_msg=name)
     np.testing.assert_allclose(r["history"][1:], hist, rtol=1e-12, atol=0)
     assert hist[-1] < hist[0]
+
+
+def test_tune_omega(case):
+    """mgm_tune_omega keeps the factor with the fewest cycles to the tolerance:
+    no more than omega = 1 needs, equal to a plain solve at the chosen factor;
+    the iterate and source are restored and the factor stays set."""
+    name, P = case
+    P.set_smoother(1.0, 3)
+    _rows(P, True)
+    btag = P.geometry(0)["btag"]
+    f = np.random.default_rng(13).standard_normal(P.info(0)["N"])
+    f[btag != 0] = 0.0
+    P.zero_fields()
+    P.set_field(0, "source", f)
+    r1 = P.solve(1e-8, 300)
+    P.zero_fields()
+    P.set_field(0, "source", f)
+    t = P.tune_omega(1e-8, 300, lo=0.6, hi=1.0, step=0.1)
+    # the grid 0.6 .. 1.0, then the midpoints beside the best (so 0.55 .. 1.05)
+    assert 0.55 - 1e-6 <= t["omega"] <= 1.05 + 1e-6 and t["cycles"] <= r1["cycles"], (t, r1["cycles"])
+    np.testing.assert_array_equal(P.field(0, "source"), f)
+    assert not np.abs(P.field(0, "p")).any()
+    r = P.solve(1e-8, 300)
+    assert r["cycles"] == t["cycles"]
+    print(f"  {name}: omega {t['omega']:.3f} {t['cycles']} cycles (omega 1: {r1['cycles']})", flush=True)
+    P.set_smoother(1.0, 3)
