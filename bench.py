@@ -6,9 +6,12 @@ HBM): jittered 100^3 cube (seed 1; 998,816 nodes after corner/edge removal),
 m = 70 stencils (multiplier 3.5), Dirichlet-zero identity rows on every level,
 RHS = sum of 16 Fourier modes (amplitudes N(0,1), SplitMix64 seed 2), V-cycles
 of multicolour SOR (3 sweeps) until l2_norm_gen(r)/l2_norm_gen(f) <= 1e-8.  The
-SOR factor omega is autotuned before the timed solves (mgm_tune_omega: the
-fewest cycles to the tolerance on this source; --omega fixes it instead); the
-omega = 1 solve is timed once beside it.
+SOR factors are autotuned before the timed solves, on another source (the
+16-mode sum drawn with seed 3): one factor for every level (mgm_tune_omega),
+then a factor per level by coordinate descent from it (mgm_tune_omega_levels);
+each keeps the fewest cycles to the tolerance.  The timed solves use the
+per-level factors; the omega = 1 and single-factor solves are timed beside
+them.  --omega fixes one factor instead.
 
 Prints one JSON line (rank 0, one GPU).  `roofline` is the FP64 stencil apply
 (lap_Poison over level 0) timed with HIP events on the library stream;
@@ -56,9 +59,9 @@ def build_problem(mgm, n, levels, seed0=1):
     return cases.cube_poisson(n, levels, seed0)
 
 
-def set_rhs(P):
+def set_rhs(P, seed=2):
     from multigrid_meshless_amd import cases
-    return cases.cube_rhs(P)
+    return cases.cube_rhs(P, seed)
 
 
 def reset(P, f):
@@ -364,23 +367,34 @@ def main():
     print(f"[bench] setup {t_setup:.2f}s levels={sizes} N0={N} m={m} colours0={info0['num_colours']} "
           f"phases={ {k: round(v, 3) for k, v in P.timings().items()} }", file=sys.stderr, flush=True)
 
-    # the SOR factor: autotuned on this source (untimed), and the omega = 1 solve for the record
-    reset(P, f)
-    P.set_smoother(1.0, 3)
-    P.solve(args.tol, args.max_cycles)
-    reset(P, f)
-    r1 = P.solve(args.tol, args.max_cycles)
-    omega1 = {"s": round(r1["seconds"], 5), "cycles": r1["cycles"]}
-    if args.omega == "tune":
+    # SOR factors: autotuned on another source (untimed); the omega = 1 and
+    # single-factor solves of the benchmark's source are timed for the record
+    def timed_once():
         reset(P, f)
+        P.solve(args.tol, args.max_cycles)
+        reset(P, f)
+        r = P.solve(args.tol, args.max_cycles)
+        return {"s": round(r["seconds"], 5), "cycles": r["cycles"]}
+
+    P.set_smoother(1.0, 3)
+    omega1 = timed_once()
+    single = None
+    if args.omega == "tune":
+        f_tune = set_rhs(P, seed=3)
+        reset(P, f_tune)
         t0 = time.perf_counter()
         tuned = P.tune_omega(args.tol, args.max_cycles)
-        omega, omega_tuned = tuned["omega"], True
-        print(f"[bench] omega tuned in {time.perf_counter() - t0:.2f}s: {omega:.4f} ({tuned['cycles']} cycles; "
-              f"omega 1: {r1['cycles']} cycles {r1['seconds']:.4f}s)", file=sys.stderr, flush=True)
+        single = dict(omega=tuned["omega"], **timed_once())
+        reset(P, f_tune)
+        tl = P.tune_omega_levels(args.tol, args.max_cycles, lo=0.5, hi=1.5, step=0.1, sweeps=3)
+        omega, omega_tuned = tl["omega"], True
+        print(f"[bench] SOR factors tuned in {time.perf_counter() - t0:.1f}s on the seed-3 source: single "
+              f"{single['omega']:.3f} ({single['cycles']} cycles {single['s']:.4f}s here), per level "
+              f"{[round(w, 3) for w in omega]}; omega 1: {omega1['cycles']} cycles {omega1['s']:.4f}s",
+              file=sys.stderr, flush=True)
     else:
-        omega, omega_tuned = float(np.float32(float(args.omega))), False
-    P.set_smoother(omega, 3)
+        omega, omega_tuned = [float(np.float32(float(args.omega)))] * P.L, False
+    P.set_smoother_levels(omega)
     for _ in range(args.warmup):
         reset(P, f)
         r = P.solve(args.tol, args.max_cycles)
@@ -446,8 +460,9 @@ def main():
         P.use_learned_smoother(True)
         learned = {"train_s": round(t_train, 2), "train_loss": [round(l0, 4), round(l1, 4)], "omega": omega}
         if args.omega == "tune":
-            reset(P, f)
-            learned["omega"] = P.tune_omega(args.tol, args.max_cycles)["omega"]
+            reset(P, set_rhs(P, seed=3))
+            learned["omega"] = P.tune_omega_levels(args.tol, args.max_cycles, lo=0.5, hi=1.5, step=0.1,
+                                                   sweeps=3)["omega"]
         for prec, tag in ((0, "fp64"), (1, "fp32"), (2, "fp32_bf16_coarse")):
             if prec and args.no_mixed:
                 continue
@@ -470,7 +485,7 @@ def main():
         print(f"[bench] learned smoother trained in {t_train:.1f}s, loss {l0:.3f} -> {l1:.3f}", file=sys.stderr,
               flush=True)
         P.use_learned_smoother(False)
-        P.set_smoother(omega, 3)
+        P.set_smoother_levels(omega)
 
     # stencil apply roofline: lap_Poison over level 0, FP64 weights + int32 columns
     t_apply = P.time_spmv(0, args.spmv_reps)
@@ -502,8 +517,8 @@ def main():
     if not args.no_cpu:
         c = cpu_baseline(P, f, args.tol, 2 * args.max_cycles, args.cpu_weight_sample, omega)
         cpu = {"value": round(c["solve_s"], 3), "unit": "s", "cores": c["threads"], "kind": "port",
-               "sample": f"the whole solve to {args.tol} on the host: the oracle's colour-ordered V-cycle (SOR factor "
-                         f"{omega:.4g}) on the same 1M-point operators (rows of a colour class in parallel on {c['threads']} threads), "
+               "sample": f"the whole solve to {args.tol} on the host: the oracle's colour-ordered V-cycle (SOR factors "
+                         f"{[round(w, 3) for w in omega]}) on the same 1M-point operators (rows of a colour class in parallel on {c['threads']} threads), "
                          f"{c['cycles']} cycles to {c['final_ratio']:.2e} in {c['solve_s']:.2f} s; 1 thread: "
                          f"{c['s_per_cycle_1thread']:.3f} s/cycle (2 cycles timed) = "
                          f"{c['s_per_cycle_1thread'] * c['cycles']:.1f} s for the solve; weight build (the "
@@ -531,7 +546,8 @@ def main():
         "config": {"workload": f"3D cube Poisson, {sizes[0]}^3 jittered cloud ({N} nodes), "
                                f"{len(sizes)} levels {sizes}, m={m}, FP64 multicolour SOR V-cycle, "
                                f"tol {args.tol}",
-                   "omega": omega, "omega_tuned": omega_tuned, "num_relax": 3, "omega_1": omega1,
+                   "omega_per_level": omega, "omega_tuned": omega_tuned, "num_relax": 3,
+                   "omega_1": omega1, "omega_single_tuned": single,
                    "cycles": cycles[0], "converged": converged,
                    "vcycles_per_s": round(cycles[0] / t_solve, 1),
                    "final_ratio": finals[0], "setup_s": round(t_setup, 2),

@@ -164,7 +164,8 @@ int mgm_write_solution(const mgm_problem *p, int lev, const char *path, int with
 int mgm_write_history(const char *path, int n, const double *values, int scientific);
 
 /* Smoother parameters used by subsequent relaxations (parameters.omega and
- * parameters.num_relax in the reference, header_files/structures.h:26,31). */
+ * parameters.num_relax in the reference, header_files/structures.h:26,31);
+ * omega applies to every level (per-level factors are cleared). */
 int mgm_set_smoother(mgm_problem *p, float omega, int num_relax);
 
 /* ---- problem: operators (reference variant semantics) --------------------- */
@@ -303,6 +304,21 @@ int mgm_solve(mgm_problem *p, int variant, int precision, double tol, int max_cy
  * (-12, omega unchanged) when no candidate reaches tol within max_cycles. */
 int mgm_tune_omega(mgm_problem *p, int variant, int precision, double tol, int max_cycles, float lo, float hi,
                    float step, float *omega, int *cycles);
+/* A SOR factor per level (no reference counterpart: the reference uses one
+ * factor on every level).  n = the number of levels, or 0 to return to the
+ * single factor of mgm_set_smoother (which also clears per-level factors).
+ * mgm_get_smoother_levels writes the factor each level uses. */
+int mgm_set_smoother_levels(mgm_problem *p, const float *omega, int n);
+int mgm_get_smoother_levels(const mgm_problem *p, float *omega);
+/* Per-level autotuner: coordinate descent from the current factors -- each
+ * level's factor over lo, lo + step, ..., hi with the others held, up to
+ * `sweeps` passes over the levels (stopping after a pass that changes
+ * nothing) -- keeping the candidate that reaches tol in the fewest cycles
+ * (ties: the smaller final residual), with the same early stop and state
+ * handling as mgm_tune_omega.  omega receives the per-level factors (number
+ * of levels), which stay set. */
+int mgm_tune_omega_levels(mgm_problem *p, int variant, int precision, double tol, int max_cycles, float lo,
+                          float hi, float step, int sweeps, float *omega, int *cycles);
 /* Seconds per copy of `bytes` (rounded down to 16) between two device buffers
  * (a streaming-copy kernel): the measured HBM reference of the roofline. */
 int mgm_stream_copy(size_t bytes, int reps, double *seconds);

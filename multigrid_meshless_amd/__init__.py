@@ -239,6 +239,27 @@ class Problem:
                                    C.byref(n)), "tune_omega")
         return dict(omega=w.value, cycles=n.value)
 
+    def tune_omega_levels(self, tol=1e-8, max_cycles=400, variant=VAR_H, precision=0, lo=0.5, hi=1.5, step=0.1,
+                          sweeps=2):
+        """Per-level SOR factors by coordinate descent from the current ones
+        (fewest cycles to tol on the current level-0 source); they stay set."""
+        w = np.zeros(self.L, np.float32)
+        n = C.c_int()
+        check(lib().mgm_tune_omega_levels(self._h, variant, precision, tol, max_cycles, lo, hi, step, sweeps,
+                                          w.ctypes.data, C.byref(n)), "tune_omega_levels")
+        return dict(omega=[float(v) for v in w], cycles=n.value)
+
+    def set_smoother_levels(self, omega):
+        """A SOR factor per level (None or [] returns to the single factor)."""
+        w = np.ascontiguousarray(omega if omega is not None else [], dtype=np.float32)
+        check(lib().mgm_set_smoother_levels(self._h, w.ctypes.data if len(w) else None, len(w)),
+              "set_smoother_levels")
+
+    def smoother_levels(self):
+        w = np.zeros(self.L, np.float32)
+        check(lib().mgm_get_smoother_levels(self._h, w.ctypes.data), "smoother_levels")
+        return [float(v) for v in w]
+
     # ---- operators (reference variant semantics) ----
     def relax(self, lev, variant=VAR_H):
         check(lib().mgm_relax(self._h, lev, variant), "relax")

@@ -108,6 +108,10 @@ SIGNATURES = [
     ("mgm_solve", _i, [_vp, _i, _i, _d, _i, C.POINTER(_i), _vp, C.POINTER(_d)]),
     ("mgm_tune_omega", _i, [_vp, _i, _i, _d, _i, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
                             C.POINTER(_i)]),
+    ("mgm_tune_omega_levels", _i, [_vp, _i, _i, _d, _i, C.c_float, C.c_float, C.c_float, _i, _vp,
+                                   C.POINTER(_i)]),
+    ("mgm_set_smoother_levels", _i, [_vp, _vp, _i]),
+    ("mgm_get_smoother_levels", _i, [_vp, _vp]),
     ("mgm_stream_copy", _i, [C.c_size_t, _i, C.POINTER(_d)]),
     ("mgm_time_spmv", _i, [_vp, _i, _i, C.POINTER(_d)]),
 ]

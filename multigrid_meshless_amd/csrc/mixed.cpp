@@ -68,8 +68,8 @@ int residual_mp(Level &L);
 int relax_mp(mgm_problem *pb, int l)
 {
     Level &L = pb->lv[l];
-    const float w = pb->cfg.omega;
-    const float c = 1.0f - pb->cfg.omega;
+    const float w = pb->omega(l);
+    const float c = 1.0f - w;
     int it0 = 0;
     if (learned_active(pb, l, MGM_VAR_H) && pb->cfg.num_relax > 0) {
         int rc = op_learned_sweep_mp(pb, l, residual_mp);
@@ -145,8 +145,8 @@ const float *srcl(const Level &L, int l) { return l == 0 ? L.srcl32.p : L.src32.
 int relax_ns_mp(mgm_problem *pb, int l, int variant)
 {
     Level &L = pb->lv[l];
-    const float w = pb->cfg.omega;
-    const float c = 1.0f - pb->cfg.omega;
+    const float w = pb->omega(l);
+    const float c = 1.0f - w;
     // (level 0 of variant F has its own residual source: per-colour launches)
     if (small_relax(L) && !(variant == MGM_VAR_F && l == 0) && pb->cfg.num_relax > 0)
         return launch_relax_small(plain_op(L, L.lapp32), plain_op(L, L.lap32), L.d_col_slices.p, L.d_col_soff.p,

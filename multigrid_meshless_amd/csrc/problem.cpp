@@ -1412,6 +1412,25 @@ int mgm_set_smoother(mgm_problem *pb, float omega, int num_relax)
     if (num_relax < 0 || num_relax > 64) MGM_FAIL(-1, "num_relax must be in [0, 64]");
     pb->cfg.omega = omega;
     pb->cfg.num_relax = num_relax;
+    pb->omega_lv.clear();
+    return 0;
+}
+
+int mgm_set_smoother_levels(mgm_problem *pb, const float *omega, int n)
+{
+    if (!pb) MGM_FAIL(-1, "null problem");
+    if (n != 0 && n != pb->L) MGM_FAIL(-1, "set_smoother_levels: n must be 0 or the number of levels");
+    if (n && !omega) MGM_FAIL(-1, "set_smoother_levels: null omega");
+    for (int l = 0; l < n; l++)
+        if (!(omega[l] > 0.0f && omega[l] < 2.0f)) MGM_FAIL(-1, "omega must be in (0, 2)");
+    pb->omega_lv.assign(omega, omega + n);
+    return 0;
+}
+
+int mgm_get_smoother_levels(const mgm_problem *pb, float *omega)
+{
+    if (!pb || !omega) MGM_FAIL(-1, "null argument");
+    for (int l = 0; l < pb->L; l++) omega[l] = pb->omega(l);
     return 0;
 }
 
@@ -1447,8 +1466,9 @@ static int pfield(int variant) { return variant == MGM_VAR_T ? MGM_F_PPRIME : MG
 int op_relax(mgm_problem *pb, int l, int variant)
 {
     Level &L = pb->lv[l];
-    const double w = pb->cfg.omega;
-    const double c = (float)(1 - pb->cfg.omega);  // (1-omega) evaluated in float, as the reference
+    const float wl = pb->omega(l);
+    const double w = wl;
+    const double c = (float)(1 - wl);  // (1-omega) evaluated in float, as the reference
     double *p = L.f[pfield(variant)].p, *src = L.f[MGM_F_SOURCE].p, *res = L.f[MGM_F_RES].p;
     Ell A = L.op(OP_LAPP);
     int nrelax = pb->cfg.num_relax;

@@ -159,5 +159,7 @@ struct mgm_problem {
     bool mp_ns_built = false;      // FP32 lap copies of the time steps' pressure cycle are current
     bool res0_current = false;     // level 0's RES holds s - lap_Poison p (form 0) whenever a cycle starts
     std::unique_ptr<mgm::LearnedState> ml;
+    std::vector<float> omega_lv;   // SOR factor per level; empty: cfg.omega on every level
+    float omega(int l) const { return l < (int)omega_lv.size() ? omega_lv[l] : cfg.omega; }
     ~mgm_problem();
 };

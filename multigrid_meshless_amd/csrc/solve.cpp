@@ -6,6 +6,7 @@
 // l2_norm_gen norm (hf/general_functions.h:181-192) is compared with the
 // source's.  The per-cycle launch sequence is captured once into a hipGraph
 // and replayed.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -166,45 +167,60 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
     return 0;
 }
 
-int mgm_tune_omega(mgm_problem *pb, int variant, int precision, double tol, int max_cycles, float lo, float hi,
-                   float step, float *omega, int *cycles)
-{
-    if (!pb || !pb->ready) MGM_FAIL(-1, "problem is not set up");
-    if (!(lo > 0.0f && hi < 2.0f && lo <= hi && step > 0.0f)) MGM_FAIL(-1, "tune_omega: need 0 < lo <= hi < 2, step > 0");
-    if ((hi - lo) / step > 64.0f) MGM_FAIL(-1, "tune_omega: more than 65 candidates");
-    if (max_cycles < 1) MGM_FAIL(-1, "tune_omega: max_cycles < 1");
-    Level &L0 = pb->lv[0];
-    const int pf = variant == MGM_VAR_T ? MGM_F_PPRIME : MGM_F_P;
-    const size_t nb = 8 * (size_t)L0.Nd;
-    // every candidate starts from the caller's level-0 iterate and source with
-    // every other field zero (a diverged candidate leaves non-finite values
-    // behind); the same state is restored at the end
+}  // extern "C"
+
+namespace {
+
+// Search state of the SOR factor autotuners: every candidate solves from the
+// caller's level-0 iterate and source with every other field zero (a diverged
+// candidate leaves non-finite values behind); the same state is restored at
+// the end.  A candidate's solve is capped at the best cycle count so far.
+struct OmegaSearch {
+    mgm_problem *pb;
+    int variant, precision, max_cycles;
+    double tol;
+    int pf;
+    size_t nb;
     DevBuf<double> saved;
-    HIP_TRY(saved.alloc(2 * (size_t)L0.Nd));
-    HIP_TRY(hipMemcpyAsync(saved.p, L0.f[pf].p, nb, hipMemcpyDeviceToDevice, stream()));
-    HIP_TRY(hipMemcpyAsync(saved.p + L0.Nd, L0.f[MGM_F_SOURCE].p, nb, hipMemcpyDeviceToDevice, stream()));
-    auto restore = [&]() -> int {
+    std::vector<double> hist;
+    std::vector<float> best;
+    int best_cycles;
+    double best_ratio = 0.0;
+
+    OmegaSearch(mgm_problem *p, int var, int prec, double t, int maxc)
+        : pb(p), variant(var), precision(prec), max_cycles(maxc), tol(t),
+          pf(var == MGM_VAR_T ? MGM_F_PPRIME : MGM_F_P), nb(8 * (size_t)p->lv[0].Nd), hist(maxc + 1),
+          best_cycles(maxc + 1) {}
+
+    int save()
+    {
+        Level &L0 = pb->lv[0];
+        HIP_TRY(saved.alloc(2 * (size_t)L0.Nd));
+        HIP_TRY(hipMemcpyAsync(saved.p, L0.f[pf].p, nb, hipMemcpyDeviceToDevice, stream()));
+        HIP_TRY(hipMemcpyAsync(saved.p + L0.Nd, L0.f[MGM_F_SOURCE].p, nb, hipMemcpyDeviceToDevice, stream()));
+        return 0;
+    }
+    int restore()
+    {
+        Level &L0 = pb->lv[0];
         for (auto &L : pb->lv)
             for (auto &f : L.f) HIP_TRY(hipMemsetAsync(f.p, 0, 8 * (size_t)L.Nd, stream()));
         HIP_TRY(hipMemcpyAsync(L0.f[pf].p, saved.p, nb, hipMemcpyDeviceToDevice, stream()));
         HIP_TRY(hipMemcpyAsync(L0.f[MGM_F_SOURCE].p, saved.p + L0.Nd, nb, hipMemcpyDeviceToDevice, stream()));
         return 0;
-    };
-    const float omega0 = pb->cfg.omega;
-    std::vector<double> hist(max_cycles + 1);
-    float best = 0.0f;
-    int best_cycles = max_cycles + 1;
-    double best_ratio = 0.0;
-    // cycles of one candidate; a solve that cannot beat the best so far stops
-    // at the best's count, one that diverges counts as never converging
-    auto trial = [&](float w) -> int {
-        pb->cfg.omega = w;
+    }
+    // one candidate (a factor per level); kept if it reaches tol in fewer
+    // cycles than the best so far, or in as many with a smaller final ratio;
+    // a diverged solve counts as never converging
+    int trial(const std::vector<float> &w)
+    {
+        pb->omega_lv = w;
         int rc = restore();
         if (rc) return rc;
         const int cap = best_cycles <= max_cycles ? best_cycles : max_cycles;
         int done = 0;
         rc = mgm_solve(pb, variant, precision, tol, cap, &done, hist.data(), nullptr);
-        if (rc == -11) return 0;   // diverged
+        if (rc == -11) return 0;
         if (rc) return rc;
         const double ratio = hist[done];
         if (!(ratio <= tol)) return 0;
@@ -214,27 +230,88 @@ int mgm_tune_omega(mgm_problem *pb, int variant, int precision, double tol, int 
             best_ratio = ratio;
         }
         return 0;
-    };
-    int rc = 0;
+    }
+};
+
+int check_omega_range(float lo, float hi, float step)
+{
+    if (!(lo > 0.0f && hi < 2.0f && lo <= hi && step > 0.0f)) MGM_FAIL(-1, "tune_omega: need 0 < lo <= hi < 2, step > 0");
+    if ((hi - lo) / step > 64.0f) MGM_FAIL(-1, "tune_omega: more than 65 candidates");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mgm_tune_omega(mgm_problem *pb, int variant, int precision, double tol, int max_cycles, float lo, float hi,
+                   float step, float *omega, int *cycles)
+{
+    if (!pb || !pb->ready) MGM_FAIL(-1, "problem is not set up");
+    if (int rc = check_omega_range(lo, hi, step)) return rc;
+    if (max_cycles < 1) MGM_FAIL(-1, "tune_omega: max_cycles < 1");
+    const std::vector<float> omega_lv0 = pb->omega_lv;
+    OmegaSearch S(pb, variant, precision, tol, max_cycles);
+    int rc = S.save();
+    if (rc) return rc;
+    auto uniform = [&](float w) { return std::vector<float>(pb->L, w); };
     // a grid, then the midpoints on either side of the best point
     const int n = (int)((hi - lo) / step + 0.5f);
-    for (int i = 0; i <= n && !rc; i++) rc = trial(lo + i * step);
-    if (!rc && best_cycles <= max_cycles) {
-        const float b = best;
-        if (b - 0.5f * step > 0.0f) rc = trial(b - 0.5f * step);
-        if (!rc && b + 0.5f * step < 2.0f) rc = trial(b + 0.5f * step);
+    for (int i = 0; i <= n && !rc; i++) rc = S.trial(uniform(lo + i * step));
+    if (!rc && S.best_cycles <= max_cycles) {
+        const float b = S.best[0];
+        if (b - 0.5f * step > 0.0f) rc = S.trial(uniform(b - 0.5f * step));
+        if (!rc && b + 0.5f * step < 2.0f) rc = S.trial(uniform(b + 0.5f * step));
     }
-    const int rc2 = restore();
+    const int rc2 = S.restore();
     HIP_TRY(hipStreamSynchronize(stream()));
     if (!rc) rc = rc2;
-    if (rc) { pb->cfg.omega = omega0; return rc; }
-    if (best_cycles > max_cycles) {
-        pb->cfg.omega = omega0;
-        MGM_FAIL(-12, "tune_omega: no candidate reached the tolerance within max_cycles");
+    pb->omega_lv = omega_lv0;
+    if (rc) return rc;
+    if (S.best_cycles > max_cycles) MGM_FAIL(-12, "tune_omega: no candidate reached the tolerance within max_cycles");
+    pb->cfg.omega = S.best[0];
+    pb->omega_lv.clear();
+    if (omega) *omega = S.best[0];
+    if (cycles) *cycles = S.best_cycles;
+    return 0;
+}
+
+int mgm_tune_omega_levels(mgm_problem *pb, int variant, int precision, double tol, int max_cycles, float lo,
+                          float hi, float step, int sweeps, float *omega, int *cycles)
+{
+    if (!pb || !pb->ready) MGM_FAIL(-1, "problem is not set up");
+    if (int rc = check_omega_range(lo, hi, step)) return rc;
+    if (max_cycles < 1 || sweeps < 1 || sweeps > 8) MGM_FAIL(-1, "tune_omega_levels: max_cycles >= 1, sweeps 1..8");
+    const std::vector<float> omega_lv0 = pb->omega_lv;
+    OmegaSearch S(pb, variant, precision, tol, max_cycles);
+    int rc = S.save();
+    if (rc) return rc;
+    std::vector<float> start(pb->L);
+    for (int l = 0; l < pb->L; l++) start[l] = pb->omega(l);
+    rc = S.trial(start);
+    // coordinate descent: each level's factor over the grid, the others held
+    const int n = (int)((hi - lo) / step + 0.5f);
+    for (int sw = 0; sw < sweeps && !rc; sw++) {
+        const std::vector<float> before = S.best;
+        for (int l = 0; l < pb->L && !rc; l++)
+            for (int i = 0; i <= n && !rc; i++) {
+                std::vector<float> w = S.best.empty() ? start : S.best;
+                w[l] = lo + i * step;
+                rc = S.trial(w);
+            }
+        if (S.best == before) break;
     }
-    pb->cfg.omega = best;
-    if (omega) *omega = best;
-    if (cycles) *cycles = best_cycles;
+    const int rc2 = S.restore();
+    HIP_TRY(hipStreamSynchronize(stream()));
+    if (!rc) rc = rc2;
+    if (rc || S.best_cycles > max_cycles) {
+        pb->omega_lv = omega_lv0;
+        if (rc) return rc;
+        MGM_FAIL(-12, "tune_omega_levels: no candidate reached the tolerance within max_cycles");
+    }
+    pb->omega_lv = S.best;
+    if (omega) std::copy(S.best.begin(), S.best.end(), omega);
+    if (cycles) *cycles = S.best_cycles;
     return 0;
 }
 

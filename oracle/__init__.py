@@ -77,6 +77,7 @@ def lib():
         L.orc_restrict.argtypes = [C.c_int, C.POINTER(Level), C.POINTER(Level)]
         L.orc_prolong.argtypes = [C.c_int, C.POINTER(Level), C.POINTER(Level)]
         L.orc_vcycle.argtypes = [C.c_int, C.c_int, C.POINTER(Level), C.c_float, C.c_int]
+        L.orc_vcycle_levels.argtypes = [C.c_int, C.c_int, C.POINTER(Level), C.c_void_p, C.c_int]
         L.orc_l2_norm_gen.argtypes = [C.c_int, _u8, _dp, C.c_void_p]
         L.orc_l2_norm_gen.restype = C.c_double
         L.orc_spmv.argtypes = [C.c_int, C.c_int, _dp, _ip, _dp, _dp]
@@ -300,4 +301,10 @@ class Hierarchy:
         lib().orc_prolong(var, C.byref(self.arr[k - 1]), C.byref(self.arr[k]))
 
     def vcycle(self, var, omega, num_relax):
-        lib().orc_vcycle(var, len(self.lv), self.arr, omega, num_relax)
+        """One V-cycle; omega is one factor, or a sequence of one per level."""
+        if np.ndim(omega):
+            w = np.ascontiguousarray(omega, dtype=np.float32)
+            assert len(w) == len(self.lv)
+            lib().orc_vcycle_levels(var, len(self.lv), self.arr, w.ctypes.data, num_relax)
+        else:
+            lib().orc_vcycle(var, len(self.lv), self.arr, omega, num_relax)

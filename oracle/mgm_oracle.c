@@ -801,6 +801,21 @@ void orc_prolong(int variant, orc_level *f, orc_level *c)
         if (variant == ORC_VAR_T || !c->btag[i]) c->p[i] = 0.0;
 }
 
+/* One V-cycle with a SOR factor per level (no reference counterpart: the
+ * reference's cycle below uses one factor; with equal factors this is it). */
+void orc_vcycle_levels(int variant, int nlev, orc_level *lv, const float *omega, int num_relax)
+{
+    for (int l = 0; l < nlev; l++) {
+        orc_relax(variant, &lv[l], omega[l], num_relax);
+        orc_residual(variant, &lv[l]);
+        if (l != nlev - 1) orc_restrict(variant, &lv[l], &lv[l + 1]);
+    }
+    for (int l = nlev - 1; l > 0; l--) {
+        orc_prolong(variant, &lv[l - 1], &lv[l]);
+        if (l != 1) orc_relax(variant, &lv[l - 1], omega[l - 1], num_relax);
+    }
+}
+
 /* One V-cycle: mg_heat_conduction.c:64-75, hf/timple_solver.h:195-207,
  * hf/fractional_step_solver.h:152-163 (no post-smoothing on level 0). */
 void orc_vcycle(int variant, int nlev, orc_level *lv, float omega, int num_relax)

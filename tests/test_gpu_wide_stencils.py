@@ -155,14 +155,26 @@ def _rows(P, dirichlet):
         P.dirichlet_rows()
 
 
-# SOR factor 1, and 0.85: under-relaxation, the factor the benchmark's autotuner picks
-@pytest.mark.parametrize("omega", [1.0, 0.85])
+# SOR factor 1; 0.85, under-relaxation, the single factor the benchmark's
+# autotuner picks; and a factor per level
+OMEGAS = [1.0, 0.85, (0.7, 0.6, 1.2)]
+
+
+def _set_omega(P, omega):
+    if isinstance(omega, tuple):
+        P.set_smoother_levels(list(omega))
+        assert P.smoother_levels() == [float(np.float32(w)) for w in omega]
+    else:
+        P.set_smoother(omega, 3)
+
+
+@pytest.mark.parametrize("omega", OMEGAS)
 @pytest.mark.parametrize("variant,dirichlet", [("H", True), ("H", False), ("T", False), ("F", False)])
 def test_colour_vcycles_bitwise(case, oracle, variant, dirichlet, omega):
     name, P = case
     import multigrid_meshless_amd as mgm
     var = getattr(mgm, "VAR_" + variant)
-    P.set_smoother(omega, 3)
+    _set_omega(P, omega)
     _rows(P, dirichlet)
     H = _oracle_hierarchy(oracle, P)
     f = np.random.default_rng(11).standard_normal(P.info(0)["N"])
@@ -172,7 +184,7 @@ def test_colour_vcycles_bitwise(case, oracle, variant, dirichlet, omega):
     P.zero_fields()
     P.set_field(0, "source", f)
     for _ in range(3):
-        H.vcycle(var, omega, 3)
+        H.vcycle(var, list(omega) if isinstance(omega, tuple) else omega, 3)
     P.vcycle(var, 3)
     pf = "pprime" if variant == "T" else "p"
     for k in range(P.L):
@@ -182,13 +194,13 @@ def test_colour_vcycles_bitwise(case, oracle, variant, dirichlet, omega):
     assert np.abs(P.field(0, pf)).max() > 0
 
 
-@pytest.mark.parametrize("omega", [1.0, 0.85])
+@pytest.mark.parametrize("omega", OMEGAS)
 def test_graph_solve_matches_oracle(case, oracle, omega):
     """mgm_solve replays one captured V-cycle + fine residual + norm per cycle:
     its fields equal the oracle's after the same cycles, bit for bit, and its
     residual history agrees to rounding (the GPU sums the norm as a tree)."""
     name, P = case
-    P.set_smoother(omega, 3)
+    _set_omega(P, omega)
     _rows(P, True)
     H = _oracle_hierarchy(oracle, P)
     btag = P.geometry(0)["btag"]
@@ -203,7 +215,7 @@ def test_graph_solve_matches_oracle(case, oracle, omega):
     fn = oracle.l2_norm_gen(btag, f)
     hist = []
     for _ in range(ncyc):
-        H.vcycle(oracle.VAR_H, omega, 3)
+        H.vcycle(oracle.VAR_H, list(omega) if isinstance(omega, tuple) else omega, 3)
         H.residual(oracle.VAR_H, 0)
         hist.append(oracle.l2_norm_gen(btag, H.lv[0]["res"]) / fn)
     np.testing.assert_array_equal(P.field(0, "p"), H.lv[0]["p"], err_msg=name)
@@ -235,3 +247,29 @@ def test_tune_omega(case):
     assert r["cycles"] == t["cycles"]
     print(f"  {name}: omega {t['omega']:.3f} {t['cycles']} cycles (omega 1: {r1['cycles']})", flush=True)
     P.set_smoother(1.0, 3)
+
+
+def test_tune_omega_levels(case):
+    """mgm_tune_omega_levels: coordinate descent from the single tuned factor
+    never needs more cycles than it; a plain solve at the chosen per-level
+    factors takes the reported count; iterate and source are restored."""
+    name, P = case
+    P.set_smoother(1.0, 3)
+    _rows(P, True)
+    btag = P.geometry(0)["btag"]
+    f = np.random.default_rng(14).standard_normal(P.info(0)["N"])
+    f[btag != 0] = 0.0
+    P.zero_fields()
+    P.set_field(0, "source", f)
+    t1 = P.tune_omega(1e-8, 300, lo=0.6, hi=1.0, step=0.1)
+    t = P.tune_omega_levels(1e-8, 300, lo=0.6, hi=1.4, step=0.2, sweeps=2)
+    assert len(t["omega"]) == P.L and t["cycles"] <= t1["cycles"], (t, t1)
+    assert P.smoother_levels() == t["omega"]
+    np.testing.assert_array_equal(P.field(0, "source"), f)
+    assert not np.abs(P.field(0, "p")).any()
+    r = P.solve(1e-8, 300)
+    assert r["cycles"] == t["cycles"]
+    print(f"  {name}: per-level omega {t['omega']} {t['cycles']} cycles (single {t1['omega']:.3f}: "
+          f"{t1['cycles']})", flush=True)
+    P.set_smoother(1.0, 3)
+    assert P.smoother_levels() == [1.0] * P.L

@@ -214,6 +214,52 @@ def test_trivial_colouring_is_natural_order(oracle):
         np.testing.assert_array_equal(Ha.lv[k]["p"], Hb.lv[k]["p"])
 
 
+@pytest.mark.parametrize("var_name", ["H", "T", "F"])
+def test_vcycle_with_level_factors(var_name, oracle):
+    """The V-cycle with a SOR factor per level (orc_vcycle_levels, no reference
+    counterpart): with equal factors it is the reference cycle (orc_vcycle)
+    bit for bit; with unequal ones it is the reference's sequence of level
+    operations (mg_heat_conduction.c:64-75: relax, residual, restrict down;
+    prolong, relax up, no post-smoothing on level 0) with each relax taking
+    its own level's factor."""
+    g = load_golden("gen2d")
+    lv = oracle_setup(oracle, g)
+    L = num_levels(g)
+    var = getattr(oracle, "VAR_" + var_name)
+    src = np.random.default_rng(3).standard_normal(lv[0]["N"])
+    orders = [np.lexsort((np.arange(lv[k]["N"]), oracle.greedy_colour(lv[k]["cloud"])[1])).astype(np.int32)
+              for k in range(L)]
+    H = [_hier(oracle, g, lv, var, orders)[0] for _ in range(3)]
+    for h in H:
+        h.lv[0]["s"][:] = src
+    for _ in range(2):
+        H[0].vcycle(var, 0.85, 3)
+        H[1].vcycle(var, [0.85] * L, 3)
+    for k in range(L):
+        np.testing.assert_array_equal(H[0].lv[k]["p"], H[1].lv[k]["p"])
+    w = [0.7, 1.1, 0.9, 1.2][:L]
+    H2 = H[2]
+    H2.vcycle(var, w, 3)
+    M = _hier(oracle, g, lv, var, orders)[0]
+    M.lv[0]["s"][:] = src
+    for k in range(L):
+        M.relax(var, k, w[k], 3)
+        M.residual(var, k)
+        if k < L - 1:
+            M.restrict(var, k)
+    for k in range(L - 1, 0, -1):
+        M.prolong(var, k)
+        if k != 1:
+            M.relax(var, k - 1, w[k - 1], 3)
+    for k in range(L):
+        np.testing.assert_array_equal(H2.lv[k]["p"], M.lv[k]["p"], err_msg=f"{var_name} L{k}")
+        np.testing.assert_array_equal(H2.lv[k]["res"], M.lv[k]["res"])
+    U = _hier(oracle, g, lv, var, orders)[0]
+    U.lv[0]["s"][:] = src
+    U.vcycle(var, 0.85, 3)
+    assert not np.array_equal(H2.lv[0]["p"], U.lv[0]["p"])
+
+
 def test_l2_norm_gen_closed_form(oracle):
     rng = np.random.default_rng(2)
     a = rng.standard_normal(1000)
