@@ -11,7 +11,7 @@ SOR factors are autotuned before the timed solves, on another source (the
 then a factor per level by coordinate descent from it (mgm_tune_omega_levels);
 each keeps the fewest cycles to the tolerance.  The timed solves use the
 per-level factors; the omega = 1 and single-factor solves are timed beside
-them.  --omega fixes one factor instead.
+them.  --omega fixes the factors instead (one, or one per level).
 
 Prints one JSON line (rank 0, one GPU).  `roofline` is the FP64 stencil apply
 (lap_Poison over level 0) timed with HIP events on the library stream;
@@ -319,7 +319,8 @@ def main():
     ap.add_argument("--levels", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-cycles", type=int, default=400)
-    ap.add_argument("--omega", default="tune", help="SOR factor of the cube solves, or 'tune' (autotuned)")
+    ap.add_argument("--omega", default="tune",
+                    help="SOR factors of the cube solves: 'tune' (autotuned), one factor, or one per level (w0,w1,...)")
     ap.add_argument("--cpu-weight-sample", type=int, default=16384,
                     help="level-0 systems of the CPU weight-build sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -393,7 +394,8 @@ def main():
               f"{[round(w, 3) for w in omega]}; omega 1: {omega1['cycles']} cycles {omega1['s']:.4f}s",
               file=sys.stderr, flush=True)
     else:
-        omega, omega_tuned = [float(np.float32(float(args.omega)))] * P.L, False
+        ws = [float(np.float32(float(v))) for v in args.omega.split(",")]
+        omega, omega_tuned = (ws * P.L if len(ws) == 1 else ws), False
     P.set_smoother_levels(omega)
     for _ in range(args.warmup):
         reset(P, f)
