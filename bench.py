@@ -93,8 +93,12 @@ def matrix_core_build(P, mpn):
     P.rebuild_weights(0)                # restore the autotuned kernel's rows
     tiles = -(-mpn // 16)
     flops = lu32_mfma_count(mpn, tiles) * 16 * 16 * 4 * 2
+    # the flop rate counts the factorisations only; each solve adds 8 T^2 FP32
+    # MFMA (block substitutions) and each refinement step 4 T^2 FP64 MFMA
+    # (the residual), their number depending on how fast the refinement converges
     out = {"kernel_s": round(t, 4), "systems_per_s": round(n / t), "tiles": tiles,
            "mfma_per_system": lu32_mfma_count(mpn, tiles),
+           "solve_mfma_f32_per_solve": 8 * tiles * tiles, "residual_mfma_f64_per_step": 4 * tiles * tiles,
            "mfma_tflops": round(n * flops / t / 1e12, 3),
            "frac_fp32_matrix_peak": round(n * flops / t / FP32_MFMA_PEAK_FLOPS, 4)}
     print(f"[bench] weights level 0, matrix-core kernel: {n / t:.0f} systems/s, "

@@ -26,14 +26,21 @@
 // panel's four pivot rows then take their triangular update in LDS, and one
 // MFMA per 16 x 16 tile applies the rank-4 update
 //     A[r][J] -= sum_q L[r][4c+q] U[q][J]
-// to every trailing tile.  Solves walk the pivot order (forward: rows not yet
-// pivot, backward: earlier pivots), one right-hand side per lane group.
+// to every trailing tile.  The factors then go to the wave's scratch slot,
+// with the inverses of the 16 x 16 diagonal blocks of L and U in pivot order.
+//
+// Solves: up to 16 right-hand sides at once, one per lane li, in blocks of 16
+// steps of the pivot order.  Forward, for block kt: the block's pivot rows of
+// the running residual are gathered through LDS, multiplied by the inverse of
+// L's diagonal block (four MFMA), and every row takes -L[:, kt] y_kt (one MFMA
+// per tile and column group); backward the same with U from the last block.
+// No step walks the pivots one by one.
 //
 // Refinement: x = 0, r = b; repeat { solve LU d = fl32(r); x += d; r = b - A x }
 // until every right-hand side's correction is below 1e-12 of its largest
 // weight (typically three refinements for PHS r^3 stencils, five for r^5).
-// Phi and P are kept in FP64 in a per-wave global scratch slot (written once
-// per system), so residuals stream them instead of recomputing square roots.
+// Phi and P are kept in FP64 in the slot (written once per system); the FP64
+// residual runs on the FP64 matrix cores (v_mfma_f64_16x16x4_f64).
 //
 // Work distribution: a persistent grid of one-wave workgroups takes systems
 // from a global counter; each workgroup owns one scratch slot.
@@ -52,20 +59,26 @@ namespace {
 constexpr int kNotPivot = 1 << 20;
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-// Per-wave LDS of a system of up to 16T unknowns.
+// Per-wave LDS of a system of up to 16T unknowns; the phases of a system
+// share one buffer.
 template <int T>
 struct Lu32Smem {
-    // xs, bs and rs are contiguous: the matrix assembly stages 16 rows of
-    // 16T + 4 floats there
-    double cx[16 * T], cy[16 * T], cz[16 * T];
-    double xs[4][16 * T];   // FP64 solution of each right-hand side, by unknown
-    double bs[4][16 * T];   // FP64 right-hand sides, by row
-    float rs[4][16 * T];    // FP32 residual, by row
-    float U[4][16 * T];     // the panel's pivot rows
-    f4 L[T][16];            // the panel's multipliers of each row (0 on pivot rows)
-    float diag[16 * T];     // pivot value of each step
-    int piv[16 * T];        // pivot row of each step
-    double *outp[4];
+    union {
+        struct {                          // matrix assembly
+            double cx[16 * T], cy[16 * T], cz[16 * T];
+            float stage[16 * (16 * T + 4)];   // 16 rows of the FP32 matrix
+        } as;
+        struct {                          // factorisation
+            float U[4][16 * T];           // the panel's pivot rows
+            f4 L[T][16];                  // the panel's multipliers of each row (0 on pivot rows)
+        } fa;
+        float G[T * 256];                 // diagonal blocks of the factors in pivot order
+        float R[16 * (16 * T + 4)];       // residual rows by right-hand side (solve gathers)
+    };
+    f4 ys[T * 64];                        // forward results of a solve, by block and lane
+    float diag[16 * T];                   // pivot value of each step
+    int piv[16 * T];                      // pivot row of each step
+    double *outp[16];
 };
 
 __device__ inline float bcast(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
@@ -218,14 +231,14 @@ __device__ __attribute__((always_inline)) inline bool lu_panel(f4 (&a)[T][T], in
             if (16 * ti + li == pq[q]) {
 #pragma unroll
                 for (int tj = 0; tj < T; tj++)
-                    *reinterpret_cast<f4 *>(&sm.U[q][16 * tj + 4 * g]) = a[ti][tj];
+                    *reinterpret_cast<f4 *>(&sm.fa.U[q][16 * tj + 4 * g]) = a[ti][tj];
             }
     // the panel's multipliers, 0 on rows that are pivots by now
     if (g == gc) {
 #pragma unroll
         for (int ti = 0; ti < T; ti++) {
             const f4 z = {0.f, 0.f, 0.f, 0.f};
-            sm.L[ti][li] = pos[ti] > 4 * c + 3 ? a[ti][0] : z;
+            sm.fa.L[ti][li] = pos[ti] > 4 * c + 3 ? a[ti][0] : z;
         }
     }
     __syncthreads();
@@ -234,17 +247,17 @@ __device__ __attribute__((always_inline)) inline bool lu_panel(f4 (&a)[T][T], in
     // L11[q][q'] = U[q][4gc + q']
     const int jb = 4 * gc;
     for (int j = jb + 4 + lane; j < 16 * nt; j += 64) {
-        const float u0 = sm.U[0][j];
+        const float u0 = sm.fa.U[0][j];
         if (nq > 1) {
-            const float u1 = __builtin_fmaf(-sm.U[1][jb], u0, sm.U[1][j]);
-            sm.U[1][j] = u1;
+            const float u1 = __builtin_fmaf(-sm.fa.U[1][jb], u0, sm.fa.U[1][j]);
+            sm.fa.U[1][j] = u1;
             if (nq > 2) {
-                const float u2 = __builtin_fmaf(-sm.U[2][jb + 1], u1, __builtin_fmaf(-sm.U[2][jb], u0, sm.U[2][j]));
-                sm.U[2][j] = u2;
+                const float u2 = __builtin_fmaf(-sm.fa.U[2][jb + 1], u1, __builtin_fmaf(-sm.fa.U[2][jb], u0, sm.fa.U[2][j]));
+                sm.fa.U[2][j] = u2;
                 if (nq > 3)
-                    sm.U[3][j] = __builtin_fmaf(-sm.U[3][jb + 2], u2,
-                                                __builtin_fmaf(-sm.U[3][jb + 1], u1,
-                                                               __builtin_fmaf(-sm.U[3][jb], u0, sm.U[3][j])));
+                    sm.fa.U[3][j] = __builtin_fmaf(-sm.fa.U[3][jb + 2], u2,
+                                                __builtin_fmaf(-sm.fa.U[3][jb + 1], u1,
+                                                               __builtin_fmaf(-sm.fa.U[3][jb], u0, sm.fa.U[3][j])));
             }
         }
     }
@@ -259,21 +272,21 @@ __device__ __attribute__((always_inline)) inline bool lu_panel(f4 (&a)[T][T], in
 #pragma unroll
                 for (int tj = 0; tj < T; tj++)
                     if ((tj > 0 && tj < nt) || (tj == 0 && g > gc))
-                        a[ti][tj] = *reinterpret_cast<const f4 *>(&sm.U[q][16 * tj + 4 * g]);
+                        a[ti][tj] = *reinterpret_cast<const f4 *>(&sm.fa.U[q][16 * tj + 4 * g]);
             }
     // rank-4 update of the trailing tiles on the matrix cores:
     // C'[x][y] = A[16ti + y][16tj + x] -= sum_k U[k][16tj + x] L[16ti + y][4c + k]
     float bop[T];
 #pragma unroll
     for (int ti = 0; ti < T; ti++) {
-        const f4 l4 = sm.L[ti][li];
+        const f4 l4 = sm.fa.L[ti][li];
         bop[ti] = group_sel(l4[0], l4[1], l4[2], l4[3], mg);
     }
 #pragma unroll
     for (int tj = 0; tj < T; tj++) {
         if (tj >= nt || (tj == 0 && gc == 3)) continue;
         const int j = 16 * tj + li;
-        const float aop = (j >= jb + 4 && g < nq) ? -sm.U[g][j] : 0.f;
+        const float aop = (j >= jb + 4 && g < nq) ? -sm.fa.U[g][j] : 0.f;
 #pragma unroll
         for (int ti = 0; ti < T; ti++) a[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop[ti], a[ti][tj], 0, 0, 0);
     }
@@ -295,92 +308,135 @@ __device__ __attribute__((always_inline)) inline bool lu_factor(f4 (&a)[T][T], i
     return true;
 }
 
-// Columns 16 kt + 4 gk + q (q = 0..3) of the factors, the tile column in
-// position 0, for the rows of this lane (16 ti + li): from the lanes of group gk.
+// Right-hand sides and residuals in the solve layout: for the rows of tile
+// ti, lane 16 g + li holds in register r the value of row 16 ti + 4 g + r of
+// right-hand side li (the MFMA result layout), up to 16 right-hand sides.
+
+// The rows piv(16 kt + 4 g + r) (packed pb, a byte per r) of the residual v
+// in the solve layout, through LDS.
 template <int T>
-__device__ __attribute__((always_inline)) inline void factor_cols(const f4 (&a)[T][T], int gk, float (&col)[4][T])
+__device__ __attribute__((always_inline)) inline f4 gather_rows(const f4 (&v)[T], unsigned pb, float *R)
 {
-    const int src = (threadIdx.x & 15) + 16 * gk;
+    constexpr int RS = 16 * T + 4;
+    const int lane = threadIdx.x & 63, li = lane & 15, g = lane >> 4;
+    __syncthreads();   // the previous gather's reads are done
 #pragma unroll
-    for (int q = 0; q < 4; q++)
+    for (int ti = 0; ti < T; ti++) *reinterpret_cast<f4 *>(&R[li * RS + 16 * ti + 4 * g]) = v[ti];
+    __syncthreads();
+    f4 out;
 #pragma unroll
-        for (int ti = 0; ti < T; ti++) col[q][ti] = __shfl(a[ti][0][q], src);
+    for (int r = 0; r < 4; r++) out[r] = R[li * RS + ((pb >> (8 * r)) & 255u)];
+    return out;
 }
 
-// Forward elimination of group g's right-hand side (rr: rows 16 ti + li):
-// step k, rows not yet pivot take -= L[r][k] y_k, y_k the value of row piv[k].
-template <int T>
-__device__ __attribute__((always_inline)) inline void solve_fwd(f4 (&a)[T][T], const int (&pos)[T], float (&rr)[T],
-                                                                int mpn, const Lu32Smem<T> &sm)
+// One solve LU d = r of up to 16 right-hand sides by 16 x 16 blocks of the
+// pivot order, on the matrix cores.  Forward: for block kt, gather the block's
+// pivot rows of the running residual, multiply by the inverse of the unit lower
+// diagonal block (y_kt), subtract L[:, kt] y_kt from every row (one MFMA per
+// tile and column group).  Backward: the same with the upper diagonal block's
+// inverse, from the last block, subtracting U[:, kt] x_kt.  Rows a step has
+// consumed are updated too; they are never gathered again.  The correction of
+// block kt (unknowns 16 kt + 4 g + r) is passed to emit(kt, x).
+template <int T, typename Emit>
+__device__ __attribute__((always_inline)) inline void solve_blocks(const f4 *fac, f4 (&rr)[T], const unsigned (&pb)[T],
+                                                                   const f4 *linv, const f4 *uinv, float *R, f4 *ys,
+                                                                   Emit emit)
 {
     const int lane = threadIdx.x & 63, li = lane & 15, g = lane >> 4;
-    const unsigned mg[4] = {lane_mask(g == 0), lane_mask(g == 1), lane_mask(g == 2), lane_mask(g == 3)};
-#pragma unroll 1
+#pragma unroll
     for (int kt = 0; kt < T; kt++) {
-#pragma unroll 1
-        for (int gk = 0; gk < 4; gk++) {
-            const int k0 = 16 * kt + 4 * gk;
-            if (k0 >= mpn) break;
-            float col[4][T];
-            factor_cols<T>(a, gk, col);
+        __builtin_amdgcn_sched_barrier(0);
+        f4 col[T];   // tile column kt of the factors
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int k = k0 + q;
-                if (k < mpn) {
-                    const int p = uni(sm.piv[k]);
-                    const float y = row_value<T>(rr, p, li, mg);
+        for (int ti = 0; ti < T; ti++) col[ti] = fac[(ti * T + kt) * 64 + lane];
+        const f4 lf = linv[kt * 64 + 4 * li + g];   // row li, columns 4g .. 4g+3
+        const f4 yt = gather_rows<T>(rr, pb[kt], R);
+        f4 y = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                    for (int ti = 0; ti < T; ti++) {
-                        const float nv = __builtin_fmaf(-col[q][ti], y, rr[ti]);
-                        rr[ti] = pos[ti] > k ? nv : rr[ti];
-                    }
-                }
-            }
+        for (int s2 = 0; s2 < 4; s2++) y = __builtin_amdgcn_mfma_f32_16x16x4f32(lf[s2], yt[s2], y, 0, 0, 0);
+        ys[kt * 64 + lane] = y;
+        if (kt + 1 < T) {
+            const f4 ny = -y;
+#pragma unroll
+            for (int ti = 0; ti < T; ti++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) rr[ti] = __builtin_amdgcn_mfma_f32_16x16x4f32(col[ti][q], ny[q], rr[ti], 0, 0, 0);
         }
-        rot_left<T>(a);
+    }
+#pragma unroll
+    for (int ti = 0; ti < T; ti++) rr[ti] = (f4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = T - 1; kt >= 0; kt--) {
+        __builtin_amdgcn_sched_barrier(0);
+        f4 col[T];
+        if (kt > 0) {
+#pragma unroll
+            for (int ti = 0; ti < T; ti++) col[ti] = fac[(ti * T + kt) * 64 + lane];
+        }
+        const f4 uf = uinv[kt * 64 + 4 * li + g];
+        f4 yt = ys[kt * 64 + lane];
+        if (kt + 1 < T) yt += gather_rows<T>(rr, pb[kt], R);
+        f4 x = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s2 = 0; s2 < 4; s2++) x = __builtin_amdgcn_mfma_f32_16x16x4f32(uf[s2], yt[s2], x, 0, 0, 0);
+        if (kt > 0) {
+            const f4 nx = -x;
+#pragma unroll
+            for (int ti = 0; ti < T; ti++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) rr[ti] = __builtin_amdgcn_mfma_f32_16x16x4f32(col[ti][q], nx[q], rr[ti], 0, 0, 0);
+        }
+        emit(kt, x);
     }
 }
 
-// Back substitution: last step first, x_k = (value of row piv[k]) / diag[k]
-// is written to that row, and earlier pivot rows take -= U[r][k] x_k.
+// Inverses of the diagonal 16 x 16 blocks of the factors in pivot order,
+// G_kt[i][j] = A[piv(16 kt + i)][16 kt + j] (L strictly below the diagonal,
+// with a unit diagonal; U on and above it): one lane per (block, column),
+// substitution against the unit vector.  Written row-major, 256 floats per
+// block: the inverses of L's blocks, then of U's.
 template <int T>
-__device__ __attribute__((always_inline)) inline void solve_bwd(f4 (&a)[T][T], const int (&pos)[T], float (&rr)[T],
-                                                                int mpn, const Lu32Smem<T> &sm)
+__device__ __attribute__((always_inline)) inline void block_inverses(const float *G, float *inv)
 {
-    const int lane = threadIdx.x & 63, li = lane & 15, g = lane >> 4;
-    const unsigned mg[4] = {lane_mask(g == 0), lane_mask(g == 1), lane_mask(g == 2), lane_mask(g == 3)};
+    int t0 = threadIdx.x & 63;
+    asm volatile("" : "+v"(t0));   // keeps the unit-vector selects inside the loop over systems
 #pragma unroll 1
-    for (int kt = T - 1; kt >= 0; kt--) {
-        rot_right<T>(a);
-#pragma unroll 1
-        for (int gk = 3; gk >= 0; gk--) {
-            const int k0 = 16 * kt + 4 * gk;
-            if (k0 >= mpn) continue;
-            float col[4][T];
-            factor_cols<T>(a, gk, col);
+    for (int t = t0; t < 16 * T; t += 64) {
+        const float *Gb = G + (t >> 4) * 256;
+        const int j = t & 15;
+        float y[16];
 #pragma unroll
-            for (int q = 3; q >= 0; q--) {
-                const int k = k0 + q;
-                if (k < mpn) {
-                    const int p = uni(sm.piv[k]);
-                    const float x = row_value<T>(rr, p, li, mg) / sm.diag[k];
+        for (int i = 0; i < 16; i++) {
+            float v = i == j ? 1.f : 0.f;
 #pragma unroll
-                    for (int ti = 0; ti < T; ti++) {
-                        const float nv = __builtin_fmaf(-col[q][ti], x, rr[ti]);
-                        rr[ti] = 16 * ti + li == p ? x : (pos[ti] < k ? nv : rr[ti]);
-                    }
-                }
-            }
+            for (int k = 0; k < i; k++) v = __builtin_fmaf(-Gb[i * 16 + k], y[k], v);
+            y[i] = v;
         }
+        float *Li = inv + (t >> 4) * 256 + j;
+#pragma unroll
+        for (int i = 0; i < 16; i++) Li[i * 16] = y[i];
+#pragma unroll
+        for (int i = 15; i >= 0; i--) {
+            float v = i == j ? 1.f : 0.f;
+#pragma unroll
+            for (int k = i + 1; k < 16; k++) v = __builtin_fmaf(-Gb[i * 16 + k], y[k], v);
+            y[i] = v * __builtin_amdgcn_rcpf(Gb[i * 17]);   // FP32 factors are refined anyway
+        }
+        float *Ui = inv + (T + (t >> 4)) * 256 + j;
+#pragma unroll
+        for (int i = 0; i < 16; i++) Ui[i * 16] = y[i];
     }
 }
 
 // Scratch slot of a wave, in doubles: the FP64 system A padded to 16T x 16T
-// with zeros, column-major (A[i][j] at j * 16T + i), then the right-hand
-// sides of the system [r][i] (rmax x 16T).
+// with zeros, column-major (A[i][j] at j * 16T + i); the right-hand sides of
+// the system [r][i] (rmax x 16T); the FP64 solution of the current pass in
+// the solve layout (T x 64 lanes x 4); the inverses of the diagonal blocks
+// (2T x 256 floats); the FP32 factors, tile (ti, tj) as 64 lanes x 4 floats
+// (T^2 x 256 floats).
 __host__ __device__ inline long lu32_slot(int tiles, int rmax)
 {
-    return (long)(16 * tiles) * (16 * tiles + rmax);
+    return (long)(16 * tiles) * (16 * tiles + rmax) + 512L * tiles + 128L * tiles * tiles;
 }
 
 // The q-th operator present in mask (bit 0 Dx, 1 Dy, 2 Dz, 3 Laplacian).
@@ -423,17 +479,16 @@ __device__ __attribute__((noinline)) void fill_system(const WeightsArgs &a, int 
     constexpr int NC = 16 * T;
     const int lane = threadIdx.x & 63;
     const int m = a.m, n = a.n, mpn = m + n, dim = a.dim, phs = a.phs;
-    const double sx = sm.cx[0], sy = sm.cy[0], sz = sm.cz[0];
-    // the FP64 system and its right-hand sides to the scratch slot
+    const double *cx = sm.as.cx, *cy = sm.as.cy, *cz = sm.as.cz;
+    const double sx = cx[0], sy = cy[0], sz = cz[0];
     for (int e = lane; e < NC * NC; e += 64) {
         const int j = e / NC, i = e - j * NC;
         double v = 0.0;
         if (i < m && j < m) {
-            if (i != j)
-                v = rbf_phs(dist_sum(sm.cx[i], sm.cy[i], sm.cz[i], sm.cx[j], sm.cy[j], sm.cz[j], dim), phs);
+            if (i != j) v = rbf_phs(dist_sum(cx[i], cy[i], cz[i], cx[j], cy[j], cz[j], dim), phs);
         } else if (i < mpn && j < mpn && (i < m || j < m)) {
             const int pt = i < m ? i : j, k = (i < m ? j : i) - m;
-            v = monomial(sm.cx[pt] - sx, sm.cy[pt] - sy, sm.cz[pt] - sz, a.px[k], a.py[k], a.pz[k]);
+            v = monomial(cx[pt] - sx, cy[pt] - sy, cz[pt] - sz, a.px[k], a.py[k], a.pz[k]);
         }
         full[e] = v;
     }
@@ -444,14 +499,14 @@ __device__ __attribute__((noinline)) void fill_system(const WeightsArgs &a, int 
             const double *tg;
             const double *o = rhs_out(a, s, q, nops, nr, opmask, &tg);
             (void)o;
-            v = q < nops ? operator_rhs(nth_op(opmask, q), i, m, dim, phs, a.px, a.py, a.pz, sm.cx, sm.cy, sm.cz,
-                                        sx, sy, sz)
-                         : target_rhs(i, m, dim, phs, a.px, a.py, a.pz, sm.cx, sm.cy, sm.cz, sx, sy, sz, tg[0],
-                                      tg[1], tg[2]);
+            v = q < nops ? operator_rhs(nth_op(opmask, q), i, m, dim, phs, a.px, a.py, a.pz, cx, cy, cz, sx, sy, sz)
+                         : target_rhs(i, m, dim, phs, a.px, a.py, a.pz, cx, cy, cz, sx, sy, sz, tg[0], tg[1], tg[2]);
         }
         rb[e] = v;
     }
 }
+
+typedef double d4 __attribute__((ext_vector_type(4)));
 
 // WPS: waves per SIMD the register budget aims at.
 template <int T, int WPS>
@@ -461,9 +516,17 @@ lu32_kernel(WeightsBatch bt, double *scratch, int rmax, int nref)
     __shared__ Lu32Smem<T> sm;
     constexpr int NC = 16 * T;
     const int lane = threadIdx.x & 63, li = lane & 15, g = lane >> 4;
-    double *full = scratch + (size_t)blockIdx.x * lu32_slot(T, rmax);   // [j][i], NC x NC
-    double *rb = full + (size_t)NC * NC;                                // [r][i], rmax x NC
+    double *const slot = scratch + (size_t)blockIdx.x * lu32_slot(T, rmax);
     for (;;) {
+        // the slot's addresses are formed per system: hoisted out of this loop
+        // (they are loop-invariant) they would not fit the registers
+        double *full = slot;                                            // [j][i], NC x NC
+        asm volatile("" : "+s"(full));
+        double *rb = full + (size_t)NC * NC;                            // [r][i], rmax x NC
+        d4 *xg = reinterpret_cast<d4 *>(rb + (size_t)rmax * NC);        // [kt][lane], solve layout
+        float *inv = reinterpret_cast<float *>(xg + 64 * T);            // 2T blocks of 16 x 16
+        const f4 *linv = reinterpret_cast<const f4 *>(inv), *uinv = linv + 64 * T;
+        f4 *fac = reinterpret_cast<f4 *>(inv + 512 * T);                // [ti][tj][lane]
         int t = 0;
         if (lane == 0) t = atomicAdd(bt.next, 1);
         t = bcast(t, 0);
@@ -478,12 +541,11 @@ lu32_kernel(WeightsBatch bt, double *scratch, int rmax, int nref)
         for (int i = lane; i < NC; i += 64) {
             const bool in = i < m;
             const int c = in ? cl[i] : 0;
-            sm.cx[i] = in ? a.x[c] : 0.0;
-            sm.cy[i] = in ? a.y[c] : 0.0;
-            sm.cz[i] = in ? a.z[c] : 0.0;
+            sm.as.cx[i] = in ? a.x[c] : 0.0;
+            sm.as.cy[i] = in ? a.y[c] : 0.0;
+            sm.as.cz[i] = in ? a.z[c] : 0.0;
         }
         __syncthreads();
-        const double sx = sm.cx[0], sy = sm.cy[0], sz = sm.cz[0];
         const int nr = a.rt_off ? a.rt_off[s + 1] - a.rt_off[s] : 0;
         const int np = a.pt_off ? a.pt_off[s + 1] - a.pt_off[s] : 0;
         const int total = nops + nr + np;
@@ -491,18 +553,18 @@ lu32_kernel(WeightsBatch bt, double *scratch, int rmax, int nref)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // no stale cache lines of the slot
-        // FP32 matrix in registers, one tile row at a time through LDS (xs, bs
-        // and rs are free until the solves): rows 16 ti .. 16 ti + 15 staged
-        // row-major with a padded stride, then each lane reads its 4T entries
+        // FP32 matrix in registers, one tile row at a time through LDS: rows
+        // 16 ti .. 16 ti + 15 staged row-major with a padded stride, then each
+        // lane reads its 4T entries
         f4 A[T][T];
         int pos[T];
-        float *stage = reinterpret_cast<float *>(&sm.xs[0][0]);
+        float *stage = sm.as.stage;
         constexpr int SST = NC + 4;
 #pragma unroll
         for (int ti = 0; ti < T; ti++) {
             pos[ti] = 16 * ti + li < mpn ? kNotPivot : -1;
-            for (int e = lane; e < 16 * NC; e += 64) {
-                const int j = e >> 4, r = e & 15;
+            for (int e2 = lane; e2 < 16 * NC; e2 += 64) {
+                const int j = e2 >> 4, r = e2 & 15;
                 stage[r * SST + j] = (float)full[(size_t)j * NC + 16 * ti + r];
             }
             __syncthreads();
@@ -515,71 +577,120 @@ lu32_kernel(WeightsBatch bt, double *scratch, int rmax, int nref)
             __syncthreads();
             continue;
         }
-        // right-hand sides in passes of four (one per lane group)
-        for (int q0 = 0; q0 < total; q0 += 4) {
-            const int nrhs = total - q0 < 4 ? total - q0 : 4;
-            if (lane < nrhs) {
+        // padding rows and columns are zero: step k >= mpn takes row k
+#pragma unroll
+        for (int ti = 0; ti < T; ti++)
+            if (pos[ti] < 0) pos[ti] = 16 * ti + li;
+        if (lane < NC - mpn) sm.piv[mpn + lane] = mpn + lane;
+        // the diagonal blocks in pivot order to LDS, their inverses to the slot
+#pragma unroll
+        for (int ti = 0; ti < T; ti++) {
+            const int kb = pos[ti] >> 4, i = pos[ti] & 15;
+#pragma unroll
+            for (int tj = 0; tj < T; tj++)
+                if (kb == tj) *reinterpret_cast<f4 *>(&sm.G[tj * 256 + i * 16 + 4 * g]) = A[ti][tj];
+        }
+        __syncthreads();
+        if (lane < NC - mpn) {
+            const int k = mpn + lane;
+            sm.G[(k >> 4) * 256 + (k & 15) * 17] = 1.f;   // unit pivots of the padding steps
+        }
+        // the factors leave the registers: the solves read them tile column by
+        // tile column from the slot
+#pragma unroll
+        for (int ti = 0; ti < T; ti++)
+#pragma unroll
+            for (int tj = 0; tj < T; tj++) fac[(ti * T + tj) * 64 + lane] = A[ti][tj];
+        __syncthreads();
+        block_inverses<T>(sm.G, inv);
+        unsigned pb[T];   // pivot rows of steps 16 kt + 4 g + r, a byte per r
+#pragma unroll
+        for (int kt = 0; kt < T; kt++) {
+            const int4 pv = *reinterpret_cast<const int4 *>(&sm.piv[16 * kt + 4 * g]);
+            pb[kt] = (unsigned)pv.x | ((unsigned)pv.y << 8) | ((unsigned)pv.z << 16) | ((unsigned)pv.w << 24);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the inverses, written by other lanes
+        // right-hand sides in passes of 16 (one per lane li)
+        for (int q0 = 0; q0 < total; q0 += 16) {
+            const int nrhs = total - q0 < 16 ? total - q0 : 16;
+            const bool has = li < nrhs;
+            if (g == 0 && has) {
                 const double *tg;
-                sm.outp[lane] = rhs_out(a, s, q0 + lane, nops, nr, opmask, &tg);
+                sm.outp[li] = rhs_out(a, s, q0 + li, nops, nr, opmask, &tg);
             }
-            for (int e = lane; e < 4 * NC; e += 64) {
-                const int c = e / NC, i = e - c * NC;
-                const double v = c < nrhs ? rb[(size_t)(q0 + c) * NC + i] : 0.0;
-                sm.bs[c][i] = v;
-                sm.xs[c][i] = 0.0;
-                sm.rs[c][i] = (float)v;
+            const d4 *b4 = reinterpret_cast<const d4 *>(rb + (size_t)(q0 + (has ? li : 0)) * NC);
+            f4 rr[T];
+#pragma unroll
+            for (int ti = 0; ti < T; ti++) {
+                const d4 bv = has ? b4[4 * ti + g] : (d4){0.0, 0.0, 0.0, 0.0};
+                rr[ti] = (f4){(float)bv[0], (float)bv[1], (float)bv[2], (float)bv[3]};
+                xg[ti * 64 + lane] = (d4){0.0, 0.0, 0.0, 0.0};
             }
-            __syncthreads();
             for (int it = 0;; it++) {
-                // d = (LU)^-1 fl32(r), right-hand side g in lane group g
-                float rr[T];
-#pragma unroll
-                for (int ti = 0; ti < T; ti++) rr[ti] = sm.rs[g][16 * ti + li];
-                solve_fwd<T>(A, pos, rr, mpn, sm);
-                solve_bwd<T>(A, pos, rr, mpn, sm);
-                // x += d (row r holds unknown pos[r]); the largest correction
-                // and the largest weight of each right-hand side
+                // the slot's addresses are formed here, each step: hoisted out
+                // of the loop over systems they would not fit the registers
+                const f4 *fac_i = fac, *linv_i = linv, *uinv_i = uinv;
+                const double *full_i = full;
+                asm volatile("" : "+s"(fac_i), "+s"(linv_i), "+s"(uinv_i), "+s"(full_i));
+                // x += (LU)^-1 fl32(r); the largest correction and the largest
+                // weight of each right-hand side (first m unknowns)
                 double dmax = 0.0, xmax = 0.0;
+                solve_blocks<T>(fac_i, rr, pb, linv_i, uinv_i, sm.R, sm.ys, [&](int kt, f4 x) {
+                    const d4 xo = xg[kt * 64 + lane];
+                    d4 xn;
 #pragma unroll
-                for (int ti = 0; ti < T; ti++)
-                    if (16 * ti + li < mpn) {
-                        const double xn = sm.xs[g][pos[ti]] + (double)rr[ti];
-                        sm.xs[g][pos[ti]] = xn;
-                        if (pos[ti] < m) {
-                            dmax = fmax(dmax, fabs((double)rr[ti]));
-                            xmax = fmax(xmax, fabs(xn));
+                    for (int r = 0; r < 4; r++) {
+                        xn[r] = xo[r] + (double)x[r];
+                        if (16 * kt + 4 * g + r < m) {
+                            dmax = fmax(dmax, fabs((double)x[r]));
+                            xmax = fmax(xmax, fabs(xn[r]));
                         }
                     }
-#pragma unroll
-                for (int o = 1; o < 16; o <<= 1) {
-                    dmax = fmax(dmax, __shfl_xor(dmax, o));
-                    xmax = fmax(xmax, __shfl_xor(xmax, o));
-                }
-                __syncthreads();
+                    xg[kt * 64 + lane] = xn;
+                });
+                dmax = fmax(dmax, __shfl_xor(dmax, 16));
+                dmax = fmax(dmax, __shfl_xor(dmax, 32));
+                xmax = fmax(xmax, __shfl_xor(xmax, 16));
+                xmax = fmax(xmax, __shfl_xor(xmax, 32));
                 // done when every right-hand side's last correction is below
                 // 1e-12 of its largest weight (the next would be smaller still
                 // by the contraction of the refinement), or after nref steps
                 if (it == nref || __all(dmax <= 1e-12 * xmax)) break;
-                // r = b - A x in FP64, one row per lane
-                for (int i = lane; i < mpn; i += 64) {
-                    double r0 = sm.bs[0][i], r1 = sm.bs[1][i], r2 = sm.bs[2][i], r3 = sm.bs[3][i];
-                    const double *col = full + i;
-#pragma unroll 8
-                    for (int j = 0; j < mpn; j++) {
-                        const double v = col[(size_t)j * NC];
-                        r0 = __builtin_fma(-v, sm.xs[0][j], r0);
-                        r1 = __builtin_fma(-v, sm.xs[1][j], r1);
-                        r2 = __builtin_fma(-v, sm.xs[2][j], r2);
-                        r3 = __builtin_fma(-v, sm.xs[3][j], r3);
-                    }
-                    sm.rs[0][i] = (float)r0; sm.rs[1][i] = (float)r1; sm.rs[2][i] = (float)r2; sm.rs[3][i] = (float)r3;
+                // r = b - A x in FP64 on the matrix cores: for row tile bi,
+                // k-slices of four unknowns 16 kt + 4 g + r (r fixed).  The
+                // FP64 MFMA leaves result row i in lane group i mod 4, register
+                // i / 4, so the operand's row li is matrix row
+                // 16 bi + 4 (li mod 4) + li / 4: the result is then in the solve
+                // layout (row 16 bi + 4 g + r in lane group g, register r)
+                d4 xk[T];
+#pragma unroll
+                for (int kt = 0; kt < T; kt++) xk[kt] = -xg[kt * 64 + lane];
+#pragma unroll
+                for (int bi = 0; bi < T; bi++) {
+                    __builtin_amdgcn_sched_barrier(0);   // one row tile's loads in flight at a time
+                    d4 acc = has ? b4[4 * bi + g] : (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int kt = 0; kt < T; kt++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const double av = full_i[(size_t)(16 * kt + 4 * g + r) * NC + 16 * bi + 4 * (li & 3) + (li >> 2)];
+                            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xk[kt][r], acc, 0, 0, 0);
+                        }
+                    rr[bi] = (f4){(float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]};
                 }
-                __syncthreads();
             }
             // the first m unknowns are the weights
-            for (int e = lane; e < 4 * m; e += 64) {
-                const int c = e / m, j = e - c * m;
-                if (c < nrhs) sm.outp[c][j] = sm.xs[c][j];
+            __syncthreads();
+#pragma unroll
+            for (int kt = 0; kt < T; kt++) {
+                const d4 xv = xg[kt * 64 + lane];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int j = 16 * kt + 4 * g + r;
+                    if (has && j < m) sm.outp[li][j] = xv[r];
+                }
             }
             __syncthreads();
         }
