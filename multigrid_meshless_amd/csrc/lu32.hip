@@ -23,10 +23,11 @@
 // magnitude among rows not yet used as pivots), compute the multipliers and
 // update the panel's remaining columns.  Rows are permuted logically (a row
 // keeps its place; pos[] records the step that used it as pivot).  The
-// panel's four pivot rows then take their triangular update in LDS, and one
-// MFMA per 16 x 16 tile applies the rank-4 update
+// panel's four pivot rows are copied to LDS and take their triangular update
+// there, giving the operand U; one MFMA per 16 x 16 tile then applies
 //     A[r][J] -= sum_q L[r][4c+q] U[q][J]
-// to every trailing tile.  The factors then go to the wave's scratch slot,
+// to every trailing tile, the panel's pivot rows included: their L entries
+// are the multipliers among the panel's pivots, which leaves U in place.  The factors then go to the wave's scratch slot,
 // with the inverses of the 16 x 16 diagonal blocks of L and U in pivot order.
 //
 // Solves: up to 16 right-hand sides at once, one per lane li, in blocks of 16
@@ -84,6 +85,15 @@ struct Lu32Smem {
 __device__ inline float bcast(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
 __device__ inline int bcast(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 __device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// The workgroup is one wave: LDS accesses of a wave complete in program
+// order, so ordering them needs only a compiler barrier, not s_barrier.
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Branch-free selections written with bit masks: the compiler turns select
 // chains over a register array (on a uniform or a per-lane index alike) into
@@ -233,18 +243,24 @@ __device__ __attribute__((always_inline)) inline bool lu_panel(f4 (&a)[T][T], in
                 for (int tj = 0; tj < T; tj++)
                     *reinterpret_cast<f4 *>(&sm.fa.U[q][16 * tj + 4 * g]) = a[ti][tj];
             }
-    // the panel's multipliers, 0 on rows that are pivots by now
+    // the panel's multipliers of each row: column q of the panel where the
+    // row was not yet pivot at step 4c + q, else 0.  The panel's own pivot
+    // rows keep the multipliers of the earlier pivots among them, so the
+    // rank-4 update below also turns their trailing entries into those of U
+    // (the triangular solve with the panel's unit lower block).
     if (g == gc) {
 #pragma unroll
         for (int ti = 0; ti < T; ti++) {
-            const f4 z = {0.f, 0.f, 0.f, 0.f};
-            sm.fa.L[ti][li] = pos[ti] > 4 * c + 3 ? a[ti][0] : z;
+            f4 l;
+#pragma unroll
+            for (int q = 0; q < 4; q++) l[q] = pos[ti] > 4 * c + q ? a[ti][0][q] : 0.f;
+            sm.fa.L[ti][li] = l;
         }
     }
-    __syncthreads();
-    // triangular update of the pivot rows' trailing columns (storage columns
-    // 4gc+4 .. 16nt-1): U[q][j] -= sum_{q' < q} L11[q][q'] U[q'][j],
-    // L11[q][q'] = U[q][4gc + q']
+    wave_sync();
+    // triangular update of the pivot rows' trailing columns in LDS (storage
+    // columns 4gc+4 .. 16nt-1): U[q][j] -= sum_{q' < q} L11[q][q'] U[q'][j],
+    // L11[q][q'] = U[q][4gc + q'], the operand of the rank-4 update
     const int jb = 4 * gc;
     for (int j = jb + 4 + lane; j < 16 * nt; j += 64) {
         const float u0 = sm.fa.U[0][j];
@@ -261,19 +277,7 @@ __device__ __attribute__((always_inline)) inline bool lu_panel(f4 (&a)[T][T], in
             }
         }
     }
-    __syncthreads();
-    // pivot rows take their updated trailing columns back (the 4 lanes of
-    // each pivot row, a branch per tile row)
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-#pragma unroll
-        for (int ti = 0; ti < T; ti++)
-            if (16 * ti + li == pq[q]) {
-#pragma unroll
-                for (int tj = 0; tj < T; tj++)
-                    if ((tj > 0 && tj < nt) || (tj == 0 && g > gc))
-                        a[ti][tj] = *reinterpret_cast<const f4 *>(&sm.fa.U[q][16 * tj + 4 * g]);
-            }
+    wave_sync();
     // rank-4 update of the trailing tiles on the matrix cores:
     // C'[x][y] = A[16ti + y][16tj + x] -= sum_k U[k][16tj + x] L[16ti + y][4c + k]
     float bop[T];
@@ -290,7 +294,7 @@ __device__ __attribute__((always_inline)) inline bool lu_panel(f4 (&a)[T][T], in
 #pragma unroll
         for (int ti = 0; ti < T; ti++) a[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop[ti], a[ti][tj], 0, 0, 0);
     }
-    __syncthreads();   // sm.U / sm.L are rewritten by the next panel
+    wave_sync();   // sm.U / sm.L are rewritten by the next panel
     return true;
 }
 
@@ -319,10 +323,10 @@ __device__ __attribute__((always_inline)) inline f4 gather_rows(const f4 (&v)[T]
 {
     constexpr int RS = 16 * T + 4;
     const int lane = threadIdx.x & 63, li = lane & 15, g = lane >> 4;
-    __syncthreads();   // the previous gather's reads are done
+    wave_sync();   // the previous gather's reads are done
 #pragma unroll
     for (int ti = 0; ti < T; ti++) *reinterpret_cast<f4 *>(&R[li * RS + 16 * ti + 4 * g]) = v[ti];
-    __syncthreads();
+    wave_sync();
     f4 out;
 #pragma unroll
     for (int r = 0; r < 4; r++) out[r] = R[li * RS + ((pb >> (8 * r)) & 255u)];
@@ -545,13 +549,13 @@ lu32_kernel(WeightsBatch bt, double *scratch, int rmax, int nref)
             sm.as.cy[i] = in ? a.y[c] : 0.0;
             sm.as.cz[i] = in ? a.z[c] : 0.0;
         }
-        __syncthreads();
+        wave_sync();
         const int nr = a.rt_off ? a.rt_off[s + 1] - a.rt_off[s] : 0;
         const int np = a.pt_off ? a.pt_off[s + 1] - a.pt_off[s] : 0;
         const int total = nops + nr + np;
         fill_system<T>(a, s, nops, nr, total, opmask, full, rb, sm);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __syncthreads();
+        wave_sync();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // no stale cache lines of the slot
         // FP32 matrix in registers, one tile row at a time through LDS: rows
         // 16 ti .. 16 ti + 15 staged row-major with a padded stride, then each
@@ -567,14 +571,14 @@ lu32_kernel(WeightsBatch bt, double *scratch, int rmax, int nref)
                 const int j = e2 >> 4, r = e2 & 15;
                 stage[r * SST + j] = (float)full[(size_t)j * NC + 16 * ti + r];
             }
-            __syncthreads();
+            wave_sync();
 #pragma unroll
             for (int tj = 0; tj < T; tj++) A[ti][tj] = *reinterpret_cast<const f4 *>(&stage[li * SST + 16 * tj + 4 * g]);
-            __syncthreads();
+            wave_sync();
         }
         if (!lu_factor<T>(A, pos, mpn, sm)) {
             if (lane == 0) atomicOr(bt.status, 1);
-            __syncthreads();
+            wave_sync();
             continue;
         }
         // padding rows and columns are zero: step k >= mpn takes row k
@@ -590,7 +594,7 @@ lu32_kernel(WeightsBatch bt, double *scratch, int rmax, int nref)
             for (int tj = 0; tj < T; tj++)
                 if (kb == tj) *reinterpret_cast<f4 *>(&sm.G[tj * 256 + i * 16 + 4 * g]) = A[ti][tj];
         }
-        __syncthreads();
+        wave_sync();
         if (lane < NC - mpn) {
             const int k = mpn + lane;
             sm.G[(k >> 4) * 256 + (k & 15) * 17] = 1.f;   // unit pivots of the padding steps
@@ -601,7 +605,7 @@ lu32_kernel(WeightsBatch bt, double *scratch, int rmax, int nref)
         for (int ti = 0; ti < T; ti++)
 #pragma unroll
             for (int tj = 0; tj < T; tj++) fac[(ti * T + tj) * 64 + lane] = A[ti][tj];
-        __syncthreads();
+        wave_sync();
         block_inverses<T>(sm.G, inv);
         unsigned pb[T];   // pivot rows of steps 16 kt + 4 g + r, a byte per r
 #pragma unroll
@@ -610,7 +614,7 @@ lu32_kernel(WeightsBatch bt, double *scratch, int rmax, int nref)
             pb[kt] = (unsigned)pv.x | ((unsigned)pv.y << 8) | ((unsigned)pv.z << 16) | ((unsigned)pv.w << 24);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __syncthreads();
+        wave_sync();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the inverses, written by other lanes
         // right-hand sides in passes of 16 (one per lane li)
         for (int q0 = 0; q0 < total; q0 += 16) {
@@ -682,7 +686,7 @@ lu32_kernel(WeightsBatch bt, double *scratch, int rmax, int nref)
                 }
             }
             // the first m unknowns are the weights
-            __syncthreads();
+            wave_sync();
 #pragma unroll
             for (int kt = 0; kt < T; kt++) {
                 const d4 xv = xg[kt * 64 + lane];
@@ -692,7 +696,7 @@ lu32_kernel(WeightsBatch bt, double *scratch, int rmax, int nref)
                     if (has && j < m) sm.outp[li][j] = xv[r];
                 }
             }
-            __syncthreads();
+            wave_sync();
         }
     }
 }
