@@ -29,6 +29,14 @@ import pytest
 from conftest import ROOT
 from multigrid_meshless_amd import cases
 
+
+@pytest.fixture(autouse=True)
+def fixed_weight_kernel(monkeypatch):
+    """The drivers run in processes of their own, each with its own weight-kernel
+    autotuner; the Gauss-Jordan kernel (whose variants give the same bits) is
+    pinned here and in the drivers, so both build the same operator rows."""
+    monkeypatch.setenv("MGM_LU_KERNEL", "gj")
+
 DRIVERS = os.path.join(ROOT, "tests", "dropin")
 BUILDS = {"heat": ("heat_driver.c", []), "timple": ("ns_driver.c", []),
           "fs": ("ns_driver.c", ["-DFRACTIONAL_STEP"])}
