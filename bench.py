@@ -6,12 +6,13 @@ HBM): jittered 100^3 cube (seed 1; 998,816 nodes after corner/edge removal),
 m = 70 stencils (multiplier 3.5), Dirichlet-zero identity rows on every level,
 RHS = sum of 16 Fourier modes (amplitudes N(0,1), SplitMix64 seed 2), V-cycles
 of multicolour SOR (3 sweeps) until l2_norm_gen(r)/l2_norm_gen(f) <= 1e-8.  The
-SOR factors are autotuned before the timed solves, on another source (the
-16-mode sum drawn with seed 3): one factor for every level (mgm_tune_omega),
-then a factor per level by coordinate descent from it (mgm_tune_omega_levels);
-each keeps the fewest cycles to the tolerance.  The timed solves use the
-per-level factors; the omega = 1 and single-factor solves are timed beside
-them.  --omega fixes the factors instead (one, or one per level).
+smoother is autotuned before the timed solves, on another source (the 16-mode
+sum drawn with seed 3): one SOR factor for every level (mgm_tune_omega), then
+each level's smoother -- multicolour SOR or weighted Jacobi -- and factor by
+coordinate descent from it (mgm_tune_smoother); each keeps the fewest cycles
+to the tolerance.  The timed solves use the per-level choice; the SOR omega = 1
+and single-factor solves are timed beside them.  --omega fixes SOR factors
+instead (one, or one per level); --smoother fixes the kinds (s/j per level).
 
 Prints one JSON line (rank 0, one GPU).  `roofline` is the FP64 stencil apply
 (lap_Poison over level 0) timed with HIP events on the library stream;
@@ -267,7 +268,7 @@ def cpu_hierarchy(P):
     return oracle.Hierarchy(lv), lv
 
 
-def cpu_baseline(P, f, tol, max_cycles, weight_sample, omega):
+def cpu_baseline(P, f, tol, max_cycles, weight_sample, omega, kinds):
     """The oracle on the host: (1) the same solve to tol on every available
     thread -- V-cycle, fine residual and l2_norm_gen norm per cycle, timed
     whole; (2) two cycles on one thread; (3) the reference's weight build
@@ -290,7 +291,7 @@ def cpu_baseline(P, f, tol, max_cycles, weight_sample, omega):
     t0 = time.perf_counter()
     cycles, ratio = 0, 1.0
     while cycles < max_cycles and ratio > tol:
-        H.vcycle(oracle.VAR_H, omega, 3)
+        H.vcycle(oracle.VAR_H, omega, 3, kinds=kinds)
         H.residual(oracle.VAR_H, 0)
         ratio = oracle.l2_norm_gen(btag, H.lv[0]["res"]) / fnorm
         cycles += 1
@@ -299,7 +300,7 @@ def cpu_baseline(P, f, tol, max_cycles, weight_sample, omega):
     reset()
     t0 = time.perf_counter()
     for _ in range(2):
-        H.vcycle(oracle.VAR_H, omega, 3)
+        H.vcycle(oracle.VAR_H, omega, 3, kinds=kinds)
         H.residual(oracle.VAR_H, 0)
         oracle.l2_norm_gen(btag, H.lv[0]["res"])
     t1 = (time.perf_counter() - t0) / 2
@@ -324,7 +325,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-cycles", type=int, default=400)
     ap.add_argument("--omega", default="tune",
-                    help="SOR factors of the cube solves: 'tune' (autotuned), one factor, or one per level (w0,w1,...)")
+                    help="smoother factors of the cube solves: 'tune' (autotuned), one, or one per level (w0,w1,...)")
+    ap.add_argument("--smoother", default=None,
+                    help="with fixed --omega: smoother per level, s (SOR) or j (Jacobi), e.g. jjjjj")
     ap.add_argument("--cpu-weight-sample", type=int, default=16384,
                     help="level-0 systems of the CPU weight-build sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -391,16 +394,18 @@ def main():
         tuned = P.tune_omega(args.tol, args.max_cycles)
         single = dict(omega=tuned["omega"], **timed_once())
         reset(P, f_tune)
-        tl = P.tune_omega_levels(args.tol, args.max_cycles, lo=0.5, hi=1.5, step=0.1, sweeps=3)
-        omega, omega_tuned = tl["omega"], True
-        print(f"[bench] SOR factors tuned in {time.perf_counter() - t0:.1f}s on the seed-3 source: single "
+        tl = P.tune_smoother(args.tol, args.max_cycles, lo=0.5, hi=1.9, step=0.1, sweeps=3)
+        omega, kinds, omega_tuned = tl["omega"], tl["kind"], True
+        print(f"[bench] smoother tuned in {time.perf_counter() - t0:.1f}s on the seed-3 source: single SOR factor "
               f"{single['omega']:.3f} ({single['cycles']} cycles {single['s']:.4f}s here), per level "
-              f"{[round(w, 3) for w in omega]}; omega 1: {omega1['cycles']} cycles {omega1['s']:.4f}s",
-              file=sys.stderr, flush=True)
+              f"{''.join('sj'[k] for k in kinds)} {[round(w, 3) for w in omega]}; SOR omega 1: {omega1['cycles']} "
+              f"cycles {omega1['s']:.4f}s", file=sys.stderr, flush=True)
     else:
         ws = [float(np.float32(float(v))) for v in args.omega.split(",")]
         omega, omega_tuned = (ws * P.L if len(ws) == 1 else ws), False
+        kinds = [0] * P.L if not args.smoother else ["sj".index(ch) for ch in args.smoother]
     P.set_smoother_levels(omega)
+    P.set_smoother_kinds(kinds)
     for _ in range(args.warmup):
         reset(P, f)
         r = P.solve(args.tol, args.max_cycles)
@@ -467,8 +472,8 @@ def main():
         learned = {"train_s": round(t_train, 2), "train_loss": [round(l0, 4), round(l1, 4)], "omega": omega}
         if args.omega == "tune":
             reset(P, set_rhs(P, seed=3))
-            learned["omega"] = P.tune_omega_levels(args.tol, args.max_cycles, lo=0.5, hi=1.5, step=0.1,
-                                                   sweeps=3)["omega"]
+            tl = P.tune_smoother(args.tol, args.max_cycles, lo=0.5, hi=1.9, step=0.1, sweeps=3)
+            learned["omega"], learned["smoother"] = tl["omega"], "".join("sj"[k] for k in tl["kind"])
         for prec, tag in ((0, "fp64"), (1, "fp32"), (2, "fp32_bf16_coarse")):
             if prec and args.no_mixed:
                 continue
@@ -492,6 +497,7 @@ def main():
               flush=True)
         P.use_learned_smoother(False)
         P.set_smoother_levels(omega)
+        P.set_smoother_kinds(kinds)
 
     # stencil apply roofline: lap_Poison over level 0, FP64 weights + int32 columns
     t_apply = P.time_spmv(0, args.spmv_reps)
@@ -521,10 +527,11 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        c = cpu_baseline(P, f, args.tol, 2 * args.max_cycles, args.cpu_weight_sample, omega)
+        c = cpu_baseline(P, f, args.tol, 2 * args.max_cycles, args.cpu_weight_sample, omega, kinds)
         cpu = {"value": round(c["solve_s"], 3), "unit": "s", "cores": c["threads"], "kind": "port",
-               "sample": f"the whole solve to {args.tol} on the host: the oracle's colour-ordered V-cycle (SOR factors "
-                         f"{[round(w, 3) for w in omega]}) on the same 1M-point operators (rows of a colour class in parallel on {c['threads']} threads), "
+               "sample": f"the whole solve to {args.tol} on the host: the oracle's V-cycle with the same smoothers "
+                         f"({''.join('sj'[k] for k in kinds)}, factors {[round(w, 3) for w in omega]}; SOR in the "
+                         f"GPU's colour order) on the same 1M-point operators (rows of a colour class in parallel on {c['threads']} threads), "
                          f"{c['cycles']} cycles to {c['final_ratio']:.2e} in {c['solve_s']:.2f} s; 1 thread: "
                          f"{c['s_per_cycle_1thread']:.3f} s/cycle (2 cycles timed) = "
                          f"{c['s_per_cycle_1thread'] * c['cycles']:.1f} s for the solve; weight build (the "
@@ -550,9 +557,10 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"3D cube Poisson, {sizes[0]}^3 jittered cloud ({N} nodes), "
-                               f"{len(sizes)} levels {sizes}, m={m}, FP64 multicolour SOR V-cycle, "
-                               f"tol {args.tol}",
-                   "omega_per_level": omega, "omega_tuned": omega_tuned, "num_relax": 3,
+                               f"{len(sizes)} levels {sizes}, m={m}, FP64 V-cycle (per level: multicolour "
+                               f"SOR or weighted Jacobi, 3 sweeps), tol {args.tol}",
+                   "smoother_per_level": "".join("sj"[k] for k in kinds), "omega_per_level": omega,
+                   "omega_tuned": omega_tuned, "num_relax": 3,
                    "omega_1": omega1, "omega_single_tuned": single,
                    "cycles": cycles[0], "converged": converged,
                    "vcycles_per_s": round(cycles[0] / t_solve, 1),

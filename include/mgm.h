@@ -37,6 +37,7 @@ extern "C" {
 enum { MGM_VAR_H = 0, MGM_VAR_T = 1, MGM_VAR_F = 2 };  /* heat / TIMPLE / fractional-step semantics */
 enum { MGM_ROWS_REFERENCE = 0, MGM_ROWS_BY_TAG = 1 };  /* transfer rows for i>=nb, or for !boundary */
 enum { MGM_ORDER_COLOUR = 0, MGM_ORDER_NATURAL = 1 };  /* SOR sweep order */
+enum { MGM_SMOOTH_SOR = 0, MGM_SMOOTH_JACOBI = 1 };    /* smoother of a level (mgm_set_smoother_kinds) */
 enum {
     MGM_MAT_DX = 0, MGM_MAT_DY = 1, MGM_MAT_DZ = 2, MGM_MAT_LAP = 3, MGM_MAT_LAPP = 4,
     MGM_MAT_RESTR = 5, MGM_MAT_PROL = 6
@@ -310,6 +311,15 @@ int mgm_tune_omega(mgm_problem *p, int variant, int precision, double tol, int m
  * mgm_get_smoother_levels writes the factor each level uses. */
 int mgm_set_smoother_levels(mgm_problem *p, const float *omega, int n);
 int mgm_get_smoother_levels(const mgm_problem *p, float *omega);
+/* The smoother of each level (no reference counterpart: the reference relaxes
+ * by SOR, hf/heat_conduction.h:25-56): MGM_SMOOTH_SOR (the default, in the
+ * configured sweep order) or MGM_SMOOTH_JACOBI -- each sweep updates every row
+ * from the previous iterate with the SOR formula, so a sweep is one full pass
+ * instead of one launch per colour class.  Jacobi applies to the H variant
+ * (mgm_vcycle, mgm_solve, the relaxation of the H variant); the T and F
+ * variants keep SOR.  n = the number of levels, or 0 for SOR everywhere. */
+int mgm_set_smoother_kinds(mgm_problem *p, const int *kind, int n);
+int mgm_get_smoother_kinds(const mgm_problem *p, int *kind);
 /* Per-level autotuner: coordinate descent from the current factors -- each
  * level's factor over lo, lo + step, ..., hi with the others held, up to
  * `sweeps` passes over the levels (stopping after a pass that changes
@@ -319,6 +329,14 @@ int mgm_get_smoother_levels(const mgm_problem *p, float *omega);
  * of levels), which stay set. */
 int mgm_tune_omega_levels(mgm_problem *p, int variant, int precision, double tol, int max_cycles, float lo,
                           float hi, float step, int sweeps, float *omega, int *cycles);
+/* As mgm_tune_omega_levels, with each level's smoother kind searched too:
+ * for each level, every allowed kind (bit 1 << MGM_SMOOTH_*) over the factor
+ * grid.  With both kinds allowed the candidates differ in their cost per
+ * cycle, so they are ranked by measured solve time (a candidate must be 0.5%
+ * faster to win), not by cycles.  omega and kind receive the per-level
+ * choice, which stays set. */
+int mgm_tune_smoother(mgm_problem *p, int variant, int precision, double tol, int max_cycles, float lo, float hi,
+                      float step, int sweeps, int kinds_allowed, float *omega, int *kind, int *cycles);
 /* Seconds per copy of `bytes` (rounded down to 16) between two device buffers
  * (a streaming-copy kernel): the measured HBM reference of the roofline. */
 int mgm_stream_copy(size_t bytes, int reps, double *seconds);

@@ -17,6 +17,7 @@ import numpy as np
 from ._lib import Config, LevelInfo, MgmError, check, lib
 
 VAR_H, VAR_T, VAR_F = 0, 1, 2
+SMOOTH_SOR, SMOOTH_JACOBI = 0, 1
 ROWS_REFERENCE, ROWS_BY_TAG = 0, 1
 ORDER_COLOUR, ORDER_NATURAL = 0, 1
 MAT = dict(Dx=0, Dy=1, Dz=2, lap=3, lap_Poison=4, restr=5, prol=6)
@@ -25,7 +26,7 @@ FIELD = dict(p=0, source=1, res=2, pprime=3, u=4, v=5, w=6, u_new=7, v_new=8, w_
 
 __all__ = ["Config", "Points", "Problem", "init", "device_count", "MgmError", "config", "stream_copy_seconds",
            "write_history",
-           "VAR_H", "VAR_T", "VAR_F", "MAT", "FIELD"]
+           "VAR_H", "VAR_T", "VAR_F", "SMOOTH_SOR", "SMOOTH_JACOBI", "MAT", "FIELD"]
 
 
 def device_count() -> int:
@@ -259,6 +260,30 @@ class Problem:
         w = np.zeros(self.L, np.float32)
         check(lib().mgm_get_smoother_levels(self._h, w.ctypes.data), "smoother_levels")
         return [float(v) for v in w]
+
+    def set_smoother_kinds(self, kinds):
+        """SMOOTH_SOR or SMOOTH_JACOBI per level (None or [] returns to SOR everywhere)."""
+        k = np.ascontiguousarray(kinds if kinds is not None else [], dtype=np.int32)
+        check(lib().mgm_set_smoother_kinds(self._h, k.ctypes.data if len(k) else None, len(k)),
+              "set_smoother_kinds")
+
+    def smoother_kinds(self):
+        k = np.zeros(self.L, np.int32)
+        check(lib().mgm_get_smoother_kinds(self._h, k.ctypes.data), "smoother_kinds")
+        return [int(v) for v in k]
+
+    def tune_smoother(self, tol=1e-8, max_cycles=400, variant=VAR_H, precision=0, lo=0.5, hi=1.5, step=0.1,
+                      sweeps=3, kinds=(SMOOTH_SOR, SMOOTH_JACOBI)):
+        """Per-level smoother kind and factor by coordinate descent from the
+        current ones (fewest cycles to tol on the current level-0 source); they
+        stay set."""
+        w = np.zeros(self.L, np.float32)
+        k = np.zeros(self.L, np.int32)
+        n = C.c_int()
+        mask = sum(1 << kd for kd in kinds)
+        check(lib().mgm_tune_smoother(self._h, variant, precision, tol, max_cycles, lo, hi, step, sweeps, mask,
+                                      w.ctypes.data, k.ctypes.data, C.byref(n)), "tune_smoother")
+        return dict(omega=[float(v) for v in w], kind=[int(v) for v in k], cycles=n.value)
 
     # ---- operators (reference variant semantics) ----
     def relax(self, lev, variant=VAR_H):

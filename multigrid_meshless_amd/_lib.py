@@ -112,6 +112,10 @@ SIGNATURES = [
                                    C.POINTER(_i)]),
     ("mgm_set_smoother_levels", _i, [_vp, _vp, _i]),
     ("mgm_get_smoother_levels", _i, [_vp, _vp]),
+    ("mgm_set_smoother_kinds", _i, [_vp, _vp, _i]),
+    ("mgm_get_smoother_kinds", _i, [_vp, _vp]),
+    ("mgm_tune_smoother", _i, [_vp, _i, _i, _d, _i, C.c_float, C.c_float, C.c_float, _i, _i, _vp, _vp,
+                               C.POINTER(_i)]),
     ("mgm_stream_copy", _i, [C.c_size_t, _i, C.POINTER(_d)]),
     ("mgm_time_spmv", _i, [_vp, _i, _i, C.POINTER(_d)]),
 ]

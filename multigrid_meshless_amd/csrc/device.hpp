@@ -220,6 +220,11 @@ int launch_spmv(EllT<TW> A, int nrows, const uint8_t *btag, const TV *x, TV *y, 
 template <typename TW, typename TV>
 int launch_sor_slices(EllT<TW> A, const int *slices, int first, int nslices, const uint8_t *btag,
                       const TV *src, TV *p, TV w, TV c, hipStream_t s);
+// One weighted-Jacobi sweep over all N rows: y = the SOR update of x with
+// every neighbour from x (x and y distinct); LDS windows on blocked levels.
+template <typename TW, typename TV>
+int launch_jacobi(EllT<TW> A, int N, const uint8_t *btag, const TV *src, const TV *x, TV *y, TV w, TV c,
+                  hipStream_t s);
 // The whole relaxation of a small level (all sweeps, all colour classes, and
 // for the T / F variants the pin and residual after each sweep, A2 being the
 // operator of the F residual) in one single-workgroup launch.  Weights TW

@@ -76,6 +76,18 @@ int relax_mp(mgm_problem *pb, int l)
         if (rc) return rc;
         it0 = 1;
     }
+    if (pb->jacobi(l, MGM_VAR_H))
+        return with_op(L, [&](auto A) {
+            const float *in = L.p32.p;
+            float *out = L.res32.p;
+            for (int it = it0; it < pb->cfg.num_relax; it++) {
+                int rc = launch_jacobi(A, L.Nd, L.dbtag.p, (const float *)L.src32.p, in, out, w, c, stream());
+                if (rc) return rc;
+                in = out;
+                out = out == L.res32.p ? L.p32.p : L.res32.p;
+            }
+            return in == L.p32.p ? 0 : launch_stream_copy((long)L.Nd * 4 / 16, in, L.p32.p, stream());
+        });
     if (small_relax(L) && pb->cfg.num_relax > it0) {
         const int n = pb->cfg.num_relax - it0;
         if (L.w16) {

@@ -1434,6 +1434,24 @@ int mgm_get_smoother_levels(const mgm_problem *pb, float *omega)
     return 0;
 }
 
+int mgm_set_smoother_kinds(mgm_problem *pb, const int *kind, int n)
+{
+    if (!pb) MGM_FAIL(-1, "null problem");
+    if (n != 0 && n != pb->L) MGM_FAIL(-1, "set_smoother_kinds: n must be 0 or the number of levels");
+    if (n && !kind) MGM_FAIL(-1, "set_smoother_kinds: null kind");
+    for (int l = 0; l < n; l++)
+        if (kind[l] != MGM_SMOOTH_SOR && kind[l] != MGM_SMOOTH_JACOBI) MGM_FAIL(-1, "unknown smoother kind");
+    pb->smoother_lv.assign(kind, kind + n);
+    return 0;
+}
+
+int mgm_get_smoother_kinds(const mgm_problem *pb, int *kind)
+{
+    if (!pb || !kind) MGM_FAIL(-1, "null argument");
+    for (int l = 0; l < pb->L; l++) kind[l] = l < (int)pb->smoother_lv.size() ? pb->smoother_lv[l] : MGM_SMOOTH_SOR;
+    return 0;
+}
+
 int mgm_zero_fields(mgm_problem *pb)
 {
     CHECK_READY(pb, 0);
@@ -1477,6 +1495,20 @@ int op_relax(mgm_problem *pb, int l, int variant)
         int rc = op_learned_sweep(pb, l);
         if (rc) return rc;
         nrelax--;
+    }
+    if (pb->jacobi(l, variant)) {
+        // weighted Jacobi: sweeps alternate between p and res (free until
+        // the residual that follows the relaxation); an odd count ends in
+        // res and is copied back
+        const double *in = p;
+        double *out = res;
+        for (int it = 0; it < nrelax; it++) {
+            int rc = launch_jacobi(A, L.Nd, L.dbtag.p, src, in, out, w, c, g_stream);
+            if (rc) return rc;
+            in = out;
+            out = out == res ? p : res;
+        }
+        return in == p ? 0 : launch_stream_copy((long)L.Nd * 8 / 16, in, p, g_stream);
     }
     if (relax_small_here(pb, L))
         return nrelax ? launch_relax_small(L.op_plain(OP_LAPP), L.op_plain(OP_LAP), L.d_col_slices.p,

@@ -160,6 +160,12 @@ struct mgm_problem {
     bool res0_current = false;     // level 0's RES holds s - lap_Poison p (form 0) whenever a cycle starts
     std::unique_ptr<mgm::LearnedState> ml;
     std::vector<float> omega_lv;   // SOR factor per level; empty: cfg.omega on every level
+    std::vector<int> smoother_lv;  // MGM_SMOOTH_* per level; empty: SOR on every level
     float omega(int l) const { return l < (int)omega_lv.size() ? omega_lv[l] : cfg.omega; }
+    // the H variant may relax a level by weighted Jacobi; T and F keep SOR
+    bool jacobi(int l, int variant) const
+    {
+        return variant == MGM_VAR_H && l < (int)smoother_lv.size() && smoother_lv[l] == MGM_SMOOTH_JACOBI;
+    }
     ~mgm_problem();
 };

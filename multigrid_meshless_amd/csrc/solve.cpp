@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 
@@ -171,10 +172,13 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
 
 namespace {
 
-// Search state of the SOR factor autotuners: every candidate solves from the
+// Search state of the smoother autotuners: every candidate solves from the
 // caller's level-0 iterate and source with every other field zero (a diverged
 // candidate leaves non-finite values behind); the same state is restored at
-// the end.  A candidate's solve is capped at the best cycle count so far.
+// the end.  Candidates that differ only in their factors cost the same per
+// cycle and are ranked by cycles (a solve is capped at the best count so
+// far); candidates that differ in their smoother kinds are ranked by solve
+// time (capped at three times the best count).
 struct OmegaSearch {
     mgm_problem *pb;
     int variant, precision, max_cycles;
@@ -184,8 +188,11 @@ struct OmegaSearch {
     DevBuf<double> saved;
     std::vector<double> hist;
     std::vector<float> best;
+    std::vector<int> best_kind;
     int best_cycles;
-    double best_ratio = 0.0;
+    double best_ratio = 0.0, best_s = 1e30;
+    bool by_time = false;
+    const bool trace = getenv("MGM_TUNE_TRACE") != nullptr;   // one stderr line per candidate
 
     OmegaSearch(mgm_problem *p, int var, int prec, double t, int maxc)
         : pb(p), variant(var), precision(prec), max_cycles(maxc), tol(t),
@@ -212,20 +219,37 @@ struct OmegaSearch {
     // one candidate (a factor per level); kept if it reaches tol in fewer
     // cycles than the best so far, or in as many with a smaller final ratio;
     // a diverged solve counts as never converging
-    int trial(const std::vector<float> &w)
+    int trial(const std::vector<float> &w) { return trial(w, pb->smoother_lv); }
+    int trial(const std::vector<float> &w, const std::vector<int> &kind)
     {
         pb->omega_lv = w;
+        pb->smoother_lv = kind;
         int rc = restore();
         if (rc) return rc;
-        const int cap = best_cycles <= max_cycles ? best_cycles : max_cycles;
+        int cap = best_cycles <= max_cycles ? best_cycles : max_cycles;
+        if (by_time && best_cycles <= max_cycles && 3 * best_cycles < max_cycles) cap = 3 * best_cycles;
+        if (by_time && best_cycles > max_cycles) cap = max_cycles;
         int done = 0;
-        rc = mgm_solve(pb, variant, precision, tol, cap, &done, hist.data(), nullptr);
+        double sec = 0.0;
+        rc = mgm_solve(pb, variant, precision, tol, cap, &done, hist.data(), &sec);
         if (rc == -11) return 0;
         if (rc) return rc;
         const double ratio = hist[done];
+        if (trace) {
+            fprintf(stderr, "[tune] kinds");
+            for (int l = 0; l < pb->L; l++) fprintf(stderr, " %d", l < (int)kind.size() ? kind[l] : 0);
+            fprintf(stderr, " omega");
+            for (int l = 0; l < pb->L; l++) fprintf(stderr, " %.3f", pb->omega(l));
+            fprintf(stderr, ": %d cycles (cap %d) ratio %.3e %.4f s\n", done, cap, ratio, sec);
+        }
         if (!(ratio <= tol)) return 0;
-        if (done < best_cycles || (done == best_cycles && ratio < best_ratio)) {
+        // by time: a candidate must be 0.5% faster (timing noise)
+        const bool better = by_time ? sec < 0.995 * best_s
+                                    : (done < best_cycles || (done == best_cycles && ratio < best_ratio));
+        if (better) {
+            best_s = sec;
             best = w;
+            best_kind = kind;
             best_cycles = done;
             best_ratio = ratio;
         }
@@ -251,6 +275,7 @@ int mgm_tune_omega(mgm_problem *pb, int variant, int precision, double tol, int 
     if (int rc = check_omega_range(lo, hi, step)) return rc;
     if (max_cycles < 1) MGM_FAIL(-1, "tune_omega: max_cycles < 1");
     const std::vector<float> omega_lv0 = pb->omega_lv;
+    const std::vector<int> kind0 = pb->smoother_lv;
     OmegaSearch S(pb, variant, precision, tol, max_cycles);
     int rc = S.save();
     if (rc) return rc;
@@ -267,6 +292,7 @@ int mgm_tune_omega(mgm_problem *pb, int variant, int precision, double tol, int 
     HIP_TRY(hipStreamSynchronize(stream()));
     if (!rc) rc = rc2;
     pb->omega_lv = omega_lv0;
+    pb->smoother_lv = kind0;
     if (rc) return rc;
     if (S.best_cycles > max_cycles) MGM_FAIL(-12, "tune_omega: no candidate reached the tolerance within max_cycles");
     pb->cfg.omega = S.best[0];
@@ -276,43 +302,85 @@ int mgm_tune_omega(mgm_problem *pb, int variant, int precision, double tol, int 
     return 0;
 }
 
-int mgm_tune_omega_levels(mgm_problem *pb, int variant, int precision, double tol, int max_cycles, float lo,
-                          float hi, float step, int sweeps, float *omega, int *cycles)
+}  // extern "C"
+
+namespace {
+
+// Coordinate descent over the levels' (kind, factor): from the current
+// choice, each level in turn over every allowed kind (kinds_allowed 0: its
+// current kind) and the factor grid, the other levels held; up to `sweeps`
+// passes, stopping after one that changes nothing.
+int tune_levels(mgm_problem *pb, int variant, int precision, double tol, int max_cycles, float lo, float hi,
+                float step, int sweeps, int kinds_allowed, float *omega, int *kind, int *cycles)
 {
     if (!pb || !pb->ready) MGM_FAIL(-1, "problem is not set up");
     if (int rc = check_omega_range(lo, hi, step)) return rc;
-    if (max_cycles < 1 || sweeps < 1 || sweeps > 8) MGM_FAIL(-1, "tune_omega_levels: max_cycles >= 1, sweeps 1..8");
+    if (max_cycles < 1 || sweeps < 1 || sweeps > 8) MGM_FAIL(-1, "tune: max_cycles >= 1, sweeps 1..8");
+    if (kinds_allowed && !(kinds_allowed & 3)) MGM_FAIL(-1, "tune: no smoother kind allowed");
     const std::vector<float> omega_lv0 = pb->omega_lv;
+    const std::vector<int> kind0 = pb->smoother_lv;
     OmegaSearch S(pb, variant, precision, tol, max_cycles);
+    S.by_time = kinds_allowed != 0 && (kinds_allowed & 3) == 3;   // kinds differ in their cost per cycle
     int rc = S.save();
     if (rc) return rc;
-    std::vector<float> start(pb->L);
-    for (int l = 0; l < pb->L; l++) start[l] = pb->omega(l);
-    rc = S.trial(start);
-    // coordinate descent: each level's factor over the grid, the others held
+    std::vector<float> w0(pb->L);
+    std::vector<int> k0(pb->L);
+    for (int l = 0; l < pb->L; l++) {
+        w0[l] = pb->omega(l);
+        k0[l] = l < (int)kind0.size() ? kind0[l] : MGM_SMOOTH_SOR;
+    }
+    rc = S.trial(w0, k0);
     const int n = (int)((hi - lo) / step + 0.5f);
     for (int sw = 0; sw < sweeps && !rc; sw++) {
         const std::vector<float> before = S.best;
+        const std::vector<int> before_k = S.best_kind;
         for (int l = 0; l < pb->L && !rc; l++)
-            for (int i = 0; i <= n && !rc; i++) {
-                std::vector<float> w = S.best.empty() ? start : S.best;
-                w[l] = lo + i * step;
-                rc = S.trial(w);
+            for (int kd = 0; kd < 2 && !rc; kd++) {
+                const int cur = (S.best_kind.empty() ? k0 : S.best_kind)[l];
+                if (kinds_allowed ? !(kinds_allowed & (1 << kd)) : kd != cur) continue;
+                for (int i = 0; i <= n && !rc; i++) {
+                    std::vector<float> w = S.best.empty() ? w0 : S.best;
+                    std::vector<int> k = S.best_kind.empty() ? k0 : S.best_kind;
+                    w[l] = lo + i * step;
+                    k[l] = kd;
+                    rc = S.trial(w, k);
+                }
             }
-        if (S.best == before) break;
+        if (S.best == before && S.best_kind == before_k) break;
     }
     const int rc2 = S.restore();
     HIP_TRY(hipStreamSynchronize(stream()));
     if (!rc) rc = rc2;
     if (rc || S.best_cycles > max_cycles) {
         pb->omega_lv = omega_lv0;
+        pb->smoother_lv = kind0;
         if (rc) return rc;
-        MGM_FAIL(-12, "tune_omega_levels: no candidate reached the tolerance within max_cycles");
+        MGM_FAIL(-12, "tune: no candidate reached the tolerance within max_cycles");
     }
     pb->omega_lv = S.best;
+    pb->smoother_lv = S.best_kind;
     if (omega) std::copy(S.best.begin(), S.best.end(), omega);
+    if (kind) std::copy(S.best_kind.begin(), S.best_kind.end(), kind);
     if (cycles) *cycles = S.best_cycles;
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mgm_tune_omega_levels(mgm_problem *pb, int variant, int precision, double tol, int max_cycles, float lo,
+                          float hi, float step, int sweeps, float *omega, int *cycles)
+{
+    // the factors only: each level keeps its smoother kind
+    return tune_levels(pb, variant, precision, tol, max_cycles, lo, hi, step, sweeps, 0, omega, nullptr, cycles);
+}
+
+int mgm_tune_smoother(mgm_problem *pb, int variant, int precision, double tol, int max_cycles, float lo, float hi,
+                      float step, int sweeps, int kinds_allowed, float *omega, int *kind, int *cycles)
+{
+    return tune_levels(pb, variant, precision, tol, max_cycles, lo, hi, step, sweeps, kinds_allowed, omega, kind,
+                       cycles);
 }
 
 int mgm_stream_copy(size_t bytes, int reps, double *seconds)

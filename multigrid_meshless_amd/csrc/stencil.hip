@@ -48,6 +48,7 @@ enum Mode {
     M_RES3 = 5,      // interior s - sum, bnd kept     hf/fractional_step_solver.h:236-246
     M_RESTRICT = 6,  // interior sum, bnd 0 or kept    hf/heat_conduction.h:58-77 (+T, F)
     M_PROLONG = 7,   // interior p += sum              hf/heat_conduction.h:82-92
+    M_JACOBI = 8,    // the SOR update of every row from the previous iterate x, into y
 };
 
 __device__ inline bool skips_boundary(int mode)
@@ -203,7 +204,8 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
     if (comb && mode != M_SPMV && mode != M_RESTRICT && mode != M_PROLONG && !(bnd && skips_boundary(mode)))
         sv = src[r];
     if (comb && (mode == M_SOR || (mode == M_PROLONG && !bnd))) yv = y[r];
-    if (comb && mode == M_SOR) dg = wv((A.sorted ? A.w0 : A.w)[base]);
+    if (comb && mode == M_JACOBI) yv = x[r];
+    if (comb && (mode == M_SOR || mode == M_JACOBI)) dg = wv((A.sorted ? A.w0 : A.w)[base]);
     // a gather-sorted slice mixes its rows over the lanes: every lane loads
     if (need || A.sorted) stage_products<S, NIT, TW, TV>(A, base, ri, jo, x, P);
     wave_sync();
@@ -212,7 +214,8 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
         case M_SPMV:
             y[r] = ordered_sum<false>(P, S, ri, 0, m, TV(0));
             break;
-        case M_SOR: {
+        case M_SOR:
+        case M_JACOBI: {
             const TV sum = ordered_sum<false>(P, S, ri, 1, m, TV(0));
             y[r] = w * ((sv - sum) / dg) + c * yv;
             break;
@@ -280,7 +283,7 @@ rows_kernel(EllT<TW> A, const int *__restrict__ slices, int first, int nslices, 
 template <int S, typename TW, typename TV>
 __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice, int mode,
                                     const uint8_t *__restrict__ btag, const TV *__restrict__ src, const TV *win,
-                                    TV *y, TV *P)
+                                    int r0, TV *y, TV *P, TV wf, TV cf)
 {
     constexpr int LPR = 64 / S;
     const int lane = threadIdx.x & 63;
@@ -291,8 +294,12 @@ __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice,
     const bool act = tag != ROW_PAD, bnd = tag == ROW_BOUNDARY;
     const bool skip = bnd && skips_boundary(mode);
     const bool comb = jo == 0 && act;
-    TV sv = 0;
+    TV sv = 0, yv = 0, dg = 1;
     if (comb && mode != M_SPMV && !skip) sv = src[r];
+    if (comb && mode == M_JACOBI) {
+        yv = win[r - r0];   // the row's own previous value: own rows lead the window
+        dg = wv(wp[(size_t)slice * S * m + ri]);
+    }
     if (act && !skip) {
         const size_t base = (size_t)slice * S * m + (size_t)jo * S + ri;
         const TW *w = wp + base;
@@ -329,6 +336,11 @@ __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice,
             }
             y[r] = ordered_sum<true>(P, S, ri, 0, m, sv);
             break;
+        case M_JACOBI: {
+            const TV sum = ordered_sum<false>(P, S, ri, 1, m, TV(0));
+            y[r] = wf * ((sv - sum) / dg) + cf * yv;
+            break;
+        }
         }
     }
     wave_sync();   // P is reused by the wave's next slice
@@ -344,7 +356,7 @@ __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice,
 template <int S, typename TW, typename TV>
 __global__ void __launch_bounds__(64 * kWindowWaves)
 window_rows_kernel(EllT<TW> A, int mode, const uint8_t *__restrict__ btag, const TV *__restrict__ src,
-                   const TV *__restrict__ x, TV *y)
+                   const TV *__restrict__ x, TV *y, TV wf, TV cf)
 {
     extern __shared__ unsigned char lds_raw[];
     constexpr int W = kWindowWaves;
@@ -359,7 +371,8 @@ window_rows_kernel(EllT<TW> A, int mode, const uint8_t *__restrict__ btag, const
     for (int i = threadIdx.x; i < nh; i += 64 * W) win[nown + i] = x[A.win_halo[h0 + i]];
     __syncthreads();
     const TW *wp = A.sorted ? A.w0 : A.w;
-    for (int sl = r0 / S + wave; sl < r1 / S; sl += W) window_slice<S>(A, wp, sl, mode, btag, src, win, y, P);
+    for (int sl = r0 / S + wave; sl < r1 / S; sl += W)
+        window_slice<S>(A, wp, sl, mode, btag, src, win, r0, y, P, wf, cf);
 }
 
 // Weights and columns of this lane's entries j = t*LPR + jo < MT of one
@@ -838,16 +851,17 @@ size_t window_lds(const EllT<TW> &A)
     return ((size_t)kWindowWaves * A.m * A.S + (size_t)A.win_max) * sizeof(TV);
 }
 template <typename TW, typename TV>
-int launch_window(EllT<TW> A, int mode, const uint8_t *btag, const TV *src, const TV *x, TV *y, hipStream_t s)
+int launch_window(EllT<TW> A, int mode, const uint8_t *btag, const TV *src, const TV *x, TV *y, TV w, TV c,
+                  hipStream_t s)
 {
     const dim3 grid(A.win_blocks), block(64 * kWindowWaves);
     const size_t shm = window_lds<TW, TV>(A);
     if (A.S == 4)
-        hipLaunchKernelGGL((window_rows_kernel<4, TW, TV>), grid, block, shm, s, A, mode, btag, src, x, y);
+        hipLaunchKernelGGL((window_rows_kernel<4, TW, TV>), grid, block, shm, s, A, mode, btag, src, x, y, w, c);
     else if (A.S == 8)
-        hipLaunchKernelGGL((window_rows_kernel<8, TW, TV>), grid, block, shm, s, A, mode, btag, src, x, y);
+        hipLaunchKernelGGL((window_rows_kernel<8, TW, TV>), grid, block, shm, s, A, mode, btag, src, x, y, w, c);
     else if (A.S == 16)
-        hipLaunchKernelGGL((window_rows_kernel<16, TW, TV>), grid, block, shm, s, A, mode, btag, src, x, y);
+        hipLaunchKernelGGL((window_rows_kernel<16, TW, TV>), grid, block, shm, s, A, mode, btag, src, x, y, w, c);
     else
         MGM_FAIL(-1, "unsupported slice height");
     HIP_TRY(hipGetLastError());
@@ -861,9 +875,10 @@ int launch_rows(EllT<TW> A, const int *slices, int first, int nslices, int nrows
 {
     // full SpMV and residual passes over a level with LDS windows
     if (A.win_blocks && btag && !slices && first == 0 && nslices < 0 &&
-        (mode == M_SPMV || mode == M_RES0 || mode == M_RES1 || mode == M_RES2 || mode == M_RES3) &&
+        (mode == M_SPMV || mode == M_RES0 || mode == M_RES1 || mode == M_RES2 || mode == M_RES3 ||
+         mode == M_JACOBI) &&
         window_lds<TW, TV>(A) <= kSmallLdsMax)
-        return launch_window(A, mode, btag, src, x, y, s);
+        return launch_window(A, mode, btag, src, x, y, w, c, s);
     if (nslices < 0) nslices = (nrows + A.S - 1) / A.S;  // full pass
     if (nslices <= 0 || nrows <= 0) return 0;
     if (mode == M_SOR)
@@ -910,6 +925,14 @@ template <typename TW, typename TV>
 int launch_spmv(EllT<TW> A, int nrows, const uint8_t *btag, const TV *x, TV *y, hipStream_t s)
 {
     return launch_rows<TW, TV>(A, nullptr, 0, -1, nrows, M_SPMV, btag, nullptr, x, y, TV(0), TV(0), 0, s);
+}
+// One weighted-Jacobi sweep over every row: y = the SOR update of x with
+// every neighbour taken from x (x and y distinct).
+template <typename TW, typename TV>
+int launch_jacobi(EllT<TW> A, int N, const uint8_t *btag, const TV *src, const TV *x, TV *y, TV w, TV c,
+                  hipStream_t s)
+{
+    return launch_rows<TW, TV>(A, nullptr, 0, -1, N, M_JACOBI, btag, src, x, y, w, c, 0, s);
 }
 template <typename TW, typename TV>
 int launch_sor_slices(EllT<TW> A, const int *slices, int first, int nslices, const uint8_t *btag,
@@ -1096,6 +1119,8 @@ int launch_zero(int N, const uint8_t *btag, int interior_only, TV *p, hipStream_
     template int launch_spmv<TW, TV>(EllT<TW>, int, const uint8_t *, const TV *, TV *, hipStream_t); \
     template int launch_sor_slices<TW, TV>(EllT<TW>, const int *, int, int, const uint8_t *,         \
                                            const TV *, TV *, TV, TV, hipStream_t);                   \
+    template int launch_jacobi<TW, TV>(EllT<TW>, int, const uint8_t *, const TV *, const TV *, TV *, \
+                                       TV, TV, hipStream_t);                                         \
     template int launch_residual<TW, TV>(EllT<TW>, int, int, const uint8_t *, const TV *,            \
                                          const TV *, TV *, hipStream_t);                             \
     template int launch_restrict<TW, TV>(EllT<TW>, int, const uint8_t *, int, const TV *, TV *,      \

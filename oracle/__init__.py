@@ -78,6 +78,7 @@ def lib():
         L.orc_prolong.argtypes = [C.c_int, C.POINTER(Level), C.POINTER(Level)]
         L.orc_vcycle.argtypes = [C.c_int, C.c_int, C.POINTER(Level), C.c_float, C.c_int]
         L.orc_vcycle_levels.argtypes = [C.c_int, C.c_int, C.POINTER(Level), C.c_void_p, C.c_int]
+        L.orc_vcycle_smooth.argtypes = [C.c_int, C.POINTER(Level), C.c_void_p, C.c_void_p, C.c_int]
         L.orc_l2_norm_gen.argtypes = [C.c_int, _u8, _dp, C.c_void_p]
         L.orc_l2_norm_gen.restype = C.c_double
         L.orc_spmv.argtypes = [C.c_int, C.c_int, _dp, _ip, _dp, _dp]
@@ -300,8 +301,15 @@ class Hierarchy:
     def prolong(self, var, k):
         lib().orc_prolong(var, C.byref(self.arr[k - 1]), C.byref(self.arr[k]))
 
-    def vcycle(self, var, omega, num_relax):
-        """One V-cycle; omega is one factor, or a sequence of one per level."""
+    def vcycle(self, var, omega, num_relax, kinds=None):
+        """One V-cycle; omega is one factor, or a sequence of one per level;
+        kinds (H variant only): 0 SOR or 1 weighted Jacobi per level."""
+        if kinds is not None:
+            assert var == VAR_H and len(kinds) == len(self.lv)
+            w = np.ascontiguousarray(np.broadcast_to(np.asarray(omega, np.float32), (len(self.lv),)))
+            k = np.ascontiguousarray(kinds, dtype=np.int32)
+            lib().orc_vcycle_smooth(len(self.lv), self.arr, w.ctypes.data, k.ctypes.data, num_relax)
+            return
         if np.ndim(omega):
             w = np.ascontiguousarray(omega, dtype=np.float32)
             assert len(w) == len(self.lv)

@@ -801,6 +801,43 @@ void orc_prolong(int variant, orc_level *f, orc_level *c)
         if (variant == ORC_VAR_T || !c->btag[i]) c->p[i] = 0.0;
 }
 
+/* One weighted-Jacobi sweep (no reference counterpart): sor_row's update of
+ * every row with every neighbour from the previous iterate, staged in res. */
+static void jacobi_sweep(orc_level *L, float omega)
+{
+    const int N = L->N, m = L->m;
+    const double w = omega, c = (float)(1 - omega);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; i++) {
+        const double *Wr = L->Lp + (size_t)i * m;
+        const int *cr = L->cloud + (size_t)i * m;
+        double sum = 0.0;
+        for (int j = 1; j < m; j++) sum += Wr[j] * L->p[cr[j]];
+        L->res[i] = w * ((L->s[i] - sum) / Wr[0]) + c * L->p[i];
+    }
+    memcpy(L->p, L->res, sizeof(double) * (size_t)N);
+}
+
+/* One V-cycle of the H variant with a smoother kind (0 SOR, 1 Jacobi) and a
+ * factor per level (no reference counterpart; with SOR everywhere this is
+ * orc_vcycle_levels). */
+void orc_vcycle_smooth(int nlev, orc_level *lv, const float *omega, const int *kind, int num_relax)
+{
+    for (int l = 0; l < nlev; l++) {
+        if (kind[l]) for (int it = 0; it < num_relax; it++) jacobi_sweep(&lv[l], omega[l]);
+        else orc_relax(ORC_VAR_H, &lv[l], omega[l], num_relax);
+        orc_residual(ORC_VAR_H, &lv[l]);
+        if (l != nlev - 1) orc_restrict(ORC_VAR_H, &lv[l], &lv[l + 1]);
+    }
+    for (int l = nlev - 1; l > 0; l--) {
+        orc_prolong(ORC_VAR_H, &lv[l - 1], &lv[l]);
+        if (l != 1) {
+            if (kind[l - 1]) for (int it = 0; it < num_relax; it++) jacobi_sweep(&lv[l - 1], omega[l - 1]);
+            else orc_relax(ORC_VAR_H, &lv[l - 1], omega[l - 1], num_relax);
+        }
+    }
+}
+
 /* One V-cycle with a SOR factor per level (no reference counterpart: the
  * reference's cycle below uses one factor; with equal factors this is it). */
 void orc_vcycle_levels(int variant, int nlev, orc_level *lv, const float *omega, int num_relax)

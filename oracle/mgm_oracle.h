@@ -106,6 +106,7 @@ void   orc_restrict(int variant, orc_level *f, orc_level *c);
 void   orc_prolong(int variant, orc_level *f, orc_level *c);
 void   orc_vcycle(int variant, int nlev, orc_level *levs, float omega, int num_relax);
 void   orc_vcycle_levels(int variant, int nlev, orc_level *levs, const float *omega, int num_relax);
+void   orc_vcycle_smooth(int nlev, orc_level *levs, const float *omega, const int *kind, int num_relax);
 double orc_l2_norm_gen(int N, const uint8_t *btag, const double *a, const double *b);
 void   orc_spmv(int N, int m, const double *W, const int *cloud, const double *f, double *out);
 

@@ -273,3 +273,117 @@ def test_tune_omega_levels(case):
           f"{t1['cycles']})", flush=True)
     P.set_smoother(1.0, 3)
     assert P.smoother_levels() == [1.0] * P.L
+
+
+# weighted Jacobi on every level, and Jacobi / SOR alternating (H variant)
+JACOBI = [((1, 1, 1), (0.7, 0.8, 1.6)), ((1, 0, 1), (0.8, 0.6, 1.2))]
+
+
+@pytest.mark.parametrize("kinds,omega", JACOBI, ids=["all", "mixed"])
+@pytest.mark.parametrize("dirichlet", [True, False])
+def test_jacobi_vcycles_bitwise(case, oracle, kinds, omega, dirichlet):
+    """Weighted-Jacobi levels (one full pass per sweep: LDS windows on the
+    blocked level 0, slabs elsewhere, FP64) against the oracle's cycle with
+    the same kinds and factors: every level's p, residual and source bit for
+    bit after three cycles."""
+    name, P = case
+    import multigrid_meshless_amd as mgm
+    P.set_smoother_levels(list(omega))
+    P.set_smoother_kinds(list(kinds))
+    try:
+        _rows(P, dirichlet)
+        H = _oracle_hierarchy(oracle, P)
+        f = np.random.default_rng(15).standard_normal(P.info(0)["N"])
+        if dirichlet:
+            f[P.geometry(0)["btag"] != 0] = 0.0
+        H.lv[0]["s"][:] = f
+        P.zero_fields()
+        P.set_field(0, "source", f)
+        for _ in range(3):
+            H.vcycle(oracle.VAR_H, list(omega), 3, kinds=list(kinds))
+        P.vcycle(mgm.VAR_H, 3)
+        for k in range(P.L):
+            np.testing.assert_array_equal(P.field(k, "p"), H.lv[k]["p"], err_msg=f"{name} {kinds} L{k}")
+            np.testing.assert_array_equal(P.field(k, "res"), H.lv[k]["res"], err_msg=f"{name} {kinds} L{k} res")
+            np.testing.assert_array_equal(P.field(k, "source"), H.lv[k]["s"], err_msg=f"{name} {kinds} L{k} s")
+        assert np.abs(P.field(0, "p")).max() > 0
+    finally:
+        P.set_smoother_kinds(None)
+        P.set_smoother(1.0, 3)
+
+
+def test_jacobi_graph_solve_matches_oracle(case, oracle):
+    """The captured solve with Jacobi levels: fields bit for bit the oracle's
+    after the same cycles, residual history to rounding."""
+    name, P = case
+    kinds, omega = JACOBI[0]
+    P.set_smoother_levels(list(omega))
+    P.set_smoother_kinds(list(kinds))
+    try:
+        _rows(P, True)
+        H = _oracle_hierarchy(oracle, P)
+        btag = P.geometry(0)["btag"]
+        f = np.random.default_rng(16).standard_normal(P.info(0)["N"])
+        f[btag != 0] = 0.0
+        H.lv[0]["s"][:] = f
+        P.zero_fields()
+        P.set_field(0, "source", f)
+        ncyc = 4
+        r = P.solve(1e-300, ncyc)
+        fn = oracle.l2_norm_gen(btag, f)
+        hist = []
+        for _ in range(ncyc):
+            H.vcycle(oracle.VAR_H, list(omega), 3, kinds=list(kinds))
+            H.residual(oracle.VAR_H, 0)
+            hist.append(oracle.l2_norm_gen(btag, H.lv[0]["res"]) / fn)
+        np.testing.assert_array_equal(P.field(0, "p"), H.lv[0]["p"], err_msg=name)
+        np.testing.assert_allclose(r["history"][1:], hist, rtol=1e-12, atol=0)
+        assert hist[-1] < hist[0]
+    finally:
+        P.set_smoother_kinds(None)
+        P.set_smoother(1.0, 3)
+
+
+def _timed(P, f):
+    P.zero_fields()
+    P.set_field(0, "source", f)
+    return P.solve(1e-8, 300)["seconds"]
+
+
+def test_tune_smoother(case):
+    """mgm_tune_smoother searches SOR and Jacobi per level, ranked by solve
+    time: no slower than the per-level SOR factors (10% for timing noise); a
+    plain solve at its choice takes the reported count; FP32 inner cycles with
+    the same choice reach 1e-10."""
+    name, P = case
+    P.set_smoother(1.0, 3)
+    P.set_smoother_kinds(None)
+    _rows(P, True)
+    btag = P.geometry(0)["btag"]
+    f = np.random.default_rng(17).standard_normal(P.info(0)["N"])
+    f[btag != 0] = 0.0
+    try:
+        P.zero_fields()
+        P.set_field(0, "source", f)
+        ts = P.tune_omega_levels(1e-8, 300, lo=0.6, hi=1.4, step=0.2, sweeps=2)
+        t_sor = min(_timed(P, f) for _ in range(3))
+        P.zero_fields()          # the tuner starts from the current iterate
+        P.set_field(0, "source", f)
+        t = P.tune_smoother(1e-8, 300, lo=0.6, hi=1.4, step=0.2, sweeps=2)
+        assert P.smoother_kinds() == t["kind"] and P.smoother_levels() == t["omega"]
+        P.zero_fields()
+        P.set_field(0, "source", f)
+        assert P.solve(1e-8, 300)["cycles"] == t["cycles"]
+        t_tuned = min(_timed(P, f) for _ in range(3))
+        assert t_tuned <= 1.1 * t_sor, (t_tuned, t_sor)
+        p64 = P.field(0, "p")
+        P.zero_fields()
+        P.set_field(0, "source", f)
+        r32 = P.solve(1e-10, 300, precision=1)
+        assert r32["history"][-1] <= 1e-10
+        print(f"  {name}: kinds {t['kind']} omega {[round(w, 2) for w in t['omega']]} {t['cycles']} cycles "
+              f"(SOR per level: {ts['cycles']}); FP32 inner to 1e-10 in {r32['cycles']}", flush=True)
+        assert np.abs(P.field(0, "p") - p64).max() <= 1e-6 * np.abs(p64).max()
+    finally:
+        P.set_smoother_kinds(None)
+        P.set_smoother(1.0, 3)

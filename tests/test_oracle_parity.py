@@ -260,6 +260,64 @@ def test_vcycle_with_level_factors(var_name, oracle):
     assert not np.array_equal(H2.lv[0]["p"], U.lv[0]["p"])
 
 
+def _jacobi_numpy(d, omega, sweeps):
+    """The weighted-Jacobi sweep restated in numpy: every row's SOR update
+    (hf/heat_conduction.h:29-34: w (s - sum_{j>=1} a_ij p_j) / a_i0 + (1-w) p_i,
+    the sum from 0.0 in stencil order, (1-w) in float) from the previous iterate."""
+    W, cl = d["Lp"], d["cloud"]
+    w, c = float(np.float32(omega)), float(np.float32(1 - np.float32(omega)))
+    for _ in range(sweeps):
+        p = d["p"].copy()
+        acc = np.zeros(len(p))
+        for j in range(1, W.shape[1]):
+            acc = acc + W[:, j] * p[cl[:, j]]
+        d["p"][:] = w * ((d["s"] - acc) / W[:, 0]) + c * p
+
+
+def test_vcycle_with_jacobi_levels(oracle):
+    """orc_vcycle_smooth (no reference counterpart): with SOR on every level it
+    is the per-level-factor cycle bit for bit; a Jacobi level relaxes exactly
+    as the numpy restatement of the sweep, and a cycle with mixed kinds is the
+    reference's sequence of level operations with those sweeps."""
+    g = load_golden("gen2d")
+    lv = oracle_setup(oracle, g)
+    L = num_levels(g)
+    src = np.random.default_rng(4).standard_normal(lv[0]["N"])
+    orders = [np.lexsort((np.arange(lv[k]["N"]), oracle.greedy_colour(lv[k]["cloud"])[1])).astype(np.int32)
+              for k in range(L)]
+    w = [0.7, 0.9, 1.1, 0.8][:L]
+    A, B = (_hier(oracle, g, lv, oracle.VAR_H, orders)[0] for _ in range(2))
+    for h in (A, B):
+        h.lv[0]["s"][:] = src
+    A.vcycle(oracle.VAR_H, w, 3)
+    B.vcycle(oracle.VAR_H, w, 3, kinds=[0] * L)
+    for k in range(L):
+        np.testing.assert_array_equal(A.lv[k]["p"], B.lv[k]["p"])
+    kinds = [1, 0, 1, 0][:L]
+    J, M = (_hier(oracle, g, lv, oracle.VAR_H, orders)[0] for _ in range(2))
+    for h in (J, M):
+        h.lv[0]["s"][:] = src
+    J.vcycle(oracle.VAR_H, w, 3, kinds=kinds)
+
+    def relax(k):
+        if kinds[k]:
+            _jacobi_numpy(M.lv[k], w[k], 3)
+        else:
+            M.relax(oracle.VAR_H, k, w[k], 3)
+    for k in range(L):
+        relax(k)
+        M.residual(oracle.VAR_H, k)
+        if k < L - 1:
+            M.restrict(oracle.VAR_H, k)
+    for k in range(L - 1, 0, -1):
+        M.prolong(oracle.VAR_H, k)
+        if k != 1:
+            relax(k - 1)
+    for k in range(L):
+        np.testing.assert_array_equal(J.lv[k]["p"], M.lv[k]["p"], err_msg=f"L{k}")
+    assert not np.array_equal(J.lv[0]["p"], A.lv[0]["p"])
+
+
 def test_l2_norm_gen_closed_form(oracle):
     rng = np.random.default_rng(2)
     a = rng.standard_normal(1000)
