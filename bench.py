@@ -361,7 +361,11 @@ def main():
     P, sizes, t_setup = build_problem(mgm, args.n, args.levels)
     info0 = P.info(0)
     N, m = info0["N"], info0["m"]
-    bytes_apply = N * (m * (8 + 4) + 8 + 8)
+    # per row: m FP64 weights and m column entries (16-bit window places on a
+    # level with LDS windows, else int32 columns), x[r] of the window's own
+    # rows (the halo values are the 2-3% re-read that the counters show) and y[r]
+    col_bytes = 2 if info0["window_blocks"] else 4
+    bytes_apply = N * (m * (8 + col_bytes) + 8 + 8)
     if args.spmv_only:
         # the roofline kernel and the copy of known size that calibrates HBM counters
         t_apply = P.time_spmv(0, args.spmv_reps)
@@ -499,7 +503,7 @@ def main():
         P.set_smoother_levels(omega)
         P.set_smoother_kinds(kinds)
 
-    # stencil apply roofline: lap_Poison over level 0, FP64 weights + int32 columns
+    # stencil apply roofline: lap_Poison over level 0, FP64 weights + column places
     t_apply = P.time_spmv(0, args.spmv_reps)
     achieved = bytes_apply / t_apply / 1e9
     t_copy = mgm.stream_copy_seconds(COPY_BYTES, args.spmv_reps)

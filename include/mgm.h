@@ -83,6 +83,8 @@ typedef struct {
     int blocked;                   /* rows ordered in Morton blocks (colour steps take slice lists) */
     int gather_sorted;             /* lap / lap_Poison also stored gather-sorted */
     int small_relax;               /* relaxes in one workgroup (at the current MGM_SMALL_ROWS) */
+    int window_blocks;             /* full passes read x from LDS windows of this many blocks (0: none) */
+    int window_max;                /* largest window (values); places are stored as 16-bit */
 } mgm_level_info;
 
 typedef struct mgm_points mgm_points;
@@ -282,6 +284,10 @@ int mgm_mlp_fit(mgm_problem *p, int n, const float *X, const float *t, int epoch
 int mgm_train_smoother(mgm_problem *p, const mgm_train_config *cfg, double *loss_first, double *loss_last);
 int mgm_use_learned_smoother(mgm_problem *p, int enable);
 int mgm_learned_fell_back(const mgm_problem *p);      /* 1 if the last solve fell back to SOR */
+/* The levels whose first sweep the network replaces (use[l] != 0), n = the
+ * number of levels; n = 0: every level whose stencils fit the network (the
+ * default).  The others keep their plain smoother in every sweep. */
+int mgm_set_learned_levels(mgm_problem *p, const int *use, int n);
 
 /* ---- resident solve -------------------------------------------------------- */
 /* V-cycles on level-0 (field P, SOURCE) until l2(r)/l2(f) <= tol (l2_norm_gen

@@ -267,6 +267,12 @@ class Problem:
         check(lib().mgm_set_smoother_kinds(self._h, k.ctypes.data if len(k) else None, len(k)),
               "set_smoother_kinds")
 
+    def set_learned_levels(self, use):
+        """Levels whose first sweep the learned network replaces (None: all that fit)."""
+        u = np.ascontiguousarray([1 if x else 0 for x in use] if use is not None else [], dtype=np.int32)
+        check(lib().mgm_set_learned_levels(self._h, u.ctypes.data if len(u) else None, len(u)),
+              "set_learned_levels")
+
     def smoother_kinds(self):
         k = np.zeros(self.L, np.int32)
         check(lib().mgm_get_smoother_kinds(self._h, k.ctypes.data), "smoother_kinds")

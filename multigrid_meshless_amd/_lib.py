@@ -26,7 +26,7 @@ class LevelInfo(C.Structure):
                 ("poly_degree", C.c_int), ("num_colours", C.c_int), ("d_avg", C.c_double),
                 ("d_avg_bbox", C.c_double), ("knn_ties", C.c_int), ("device_rows", C.c_int),
                 ("slice_height", C.c_int), ("blocked", C.c_int), ("gather_sorted", C.c_int),
-                ("small_relax", C.c_int)]
+                ("small_relax", C.c_int), ("window_blocks", C.c_int), ("window_max", C.c_int)]
 
 
 class NsParams(C.Structure):
@@ -113,6 +113,7 @@ SIGNATURES = [
     ("mgm_set_smoother_levels", _i, [_vp, _vp, _i]),
     ("mgm_get_smoother_levels", _i, [_vp, _vp]),
     ("mgm_set_smoother_kinds", _i, [_vp, _vp, _i]),
+    ("mgm_set_learned_levels", _i, [_vp, _vp, _i]),
     ("mgm_get_smoother_kinds", _i, [_vp, _vp]),
     ("mgm_tune_smoother", _i, [_vp, _i, _i, _d, _i, C.c_float, C.c_float, C.c_float, _i, _i, _vp, _vp,
                                C.POINTER(_i)]),

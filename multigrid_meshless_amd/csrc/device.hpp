@@ -174,9 +174,11 @@ struct EllT {
     // rows and the halo rows win_halo[win_hoff[b] .. win_hoff[b+1]) (sorted).
     // win_col is the plain layout's column slab with every column replaced by
     // its place in the block's window: own rows first (column - row0), then
-    // the halo in list order.  The weights are the plain slab (w0 when sorted,
-    // else w).  win_max bounds a window's values.
-    const int *win_col = nullptr, *win_row0 = nullptr, *win_hoff = nullptr, *win_halo = nullptr;
+    // the halo in list order, as 16-bit places (a window fits in LDS, so it
+    // holds fewer than 65,536 values).  The weights are the plain slab (w0
+    // when sorted, else w).  win_max bounds a window's values.
+    const uint16_t *win_col = nullptr;
+    const int *win_row0 = nullptr, *win_hoff = nullptr, *win_halo = nullptr;
     int win_blocks = 0, win_max = 0;
 };
 // Waves per workgroup of the LDS-window full passes (one workgroup per block).
@@ -210,6 +212,15 @@ inline size_t ell_size(int N, int m, int S) { return (size_t)((N + S - 1) / S) *
 // Device row tags (dbtag): 0 interior, 1 boundary, 2 padding row.  Row counts
 // passed to the full-pass launchers are device row counts (padding included).
 enum { ROW_INTERIOR = 0, ROW_BOUNDARY = 1, ROW_PAD = 2 };
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Renumber them so XCD i takes a contiguous range of blocks of rows: neighbouring
+// rows (and the x entries their stencils gather) then stay in one L2.
+__device__ inline long xcd_block(long b, long nb)
+{
+    const long xcd = b % 8, q = nb / 8, rem = nb % 8, idx = b / 8;
+    return (xcd < rem) ? xcd * (q + 1) + idx : rem * (q + 1) + (xcd - rem) * q + idx;
+}
+
 // The stencil launchers are instantiated for (TW, TV) = (double, double),
 // (float, float) and (bf16, float); TV is the vector/accumulation type.
 // y = A x over rows [0, nrows); rows tagged ROW_PAD in btag (nullable) are skipped.

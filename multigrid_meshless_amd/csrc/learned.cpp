@@ -100,7 +100,7 @@ int prepare(mgm_problem *pb, LearnedState &S)
     for (int l = 0; l < pb->L; l++) {
         Level &L = pb->lv[l];
         if (L.m != m0 || L.m > MLP_MAX_M) continue;   // network input width differs: SOR only
-        HIP_TRY(S.G[l].alloc((size_t)L.Nd * MLP_H));
+        HIP_TRY(S.G[l].alloc(mlp_geometry_size(L.Nd)));
         if ((rc = mlp_geometry(L.Nd, L.op_plain(OP_LAPP), dim, L.d_avg, S.dx[l].p, S.dy[l].p, S.dz[l].p, L.dbtag.p,
                                S.W.p, S.G[l].p, stream())))
             return rc;
@@ -212,7 +212,7 @@ int op_learned_sweep(mgm_problem *pb, int l)
                                  (const double *)L.f[MGM_F_P].p, L.f[MGM_F_RES].p, stream());
         if (rc) return rc;
     }
-    return mlp_correct(L.Nd, L.op_plain(OP_LAPP), L.dbtag.p, S.G[l].p, S.W1r.p, S.W2.p, S.b2.p, S.w3.p, S.b3,
+    return mlp_correct(L.Nd, L.with_window(L.op_plain(OP_LAPP)), L.dbtag.p, S.G[l].p, S.W1r.p, S.W2.p, S.b2.p, S.w3.p, S.b3,
                        L.f[MGM_F_RES].p, L.f[MGM_F_P].p, stream());
 }
 
@@ -231,14 +231,15 @@ int op_learned_sweep_mp(mgm_problem *pb, int l, int (*residual_mp)(Level &))
         if (rc) return rc;
         r = L.res32.p;
     }
-    return mlp_correct(L.Nd, L.op_plain(OP_LAPP), L.dbtag.p, S.G[l].p, S.W1r.p, S.W2.p, S.b2.p, S.w3.p, S.b3,
+    return mlp_correct(L.Nd, L.with_window(L.op_plain(OP_LAPP)), L.dbtag.p, S.G[l].p, S.W1r.p, S.W2.p, S.b2.p, S.w3.p, S.b3,
                        r, L.p32.p, stream());
 }
 
 bool learned_active(const mgm_problem *pb, int l, int variant)
 {
     return pb->ml && pb->ml->enabled && pb->ml->prepared && variant == MGM_VAR_H &&
-           pb->cfg.sweep_order == MGM_ORDER_COLOUR && l < (int)pb->ml->lev_ok.size() && pb->ml->lev_ok[l];
+           pb->cfg.sweep_order == MGM_ORDER_COLOUR && l < (int)pb->ml->lev_ok.size() && pb->ml->lev_ok[l] &&
+           (pb->ml->lev_use.empty() || (l < (int)pb->ml->lev_use.size() && pb->ml->lev_use[l]));
 }
 
 }  // namespace mgm
@@ -476,6 +477,17 @@ int mgm_use_learned_smoother(mgm_problem *pb, int enable)
 int mgm_learned_fell_back(const mgm_problem *pb)
 {
     return (pb && pb->ml) ? pb->ml->fell_back : 0;
+}
+
+int mgm_set_learned_levels(mgm_problem *pb, const int *use, int n)
+{
+    if (!pb || !pb->ready) MGM_FAIL(-1, "problem is not set up");
+    if (n != 0 && n != pb->L) MGM_FAIL(-1, "set_learned_levels: n must be 0 or the number of levels");
+    if (n && !use) MGM_FAIL(-1, "set_learned_levels: null use");
+    if (!pb->ml) pb->ml.reset(new LearnedState());
+    pb->ml->lev_use.assign(n, 0);
+    for (int l = 0; l < n; l++) pb->ml->lev_use[l] = use[l] != 0;
+    return 0;
 }
 
 }  // extern "C"

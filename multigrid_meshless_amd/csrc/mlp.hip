@@ -270,8 +270,13 @@ features_kernel(int n, Ell A, int dim, double h, const double *__restrict__ dx, 
 }
 
 // Per-row constant part of layer 1: b1[k] + sum_{j,d} W1[k][j*dim+d] off_jd,
-// stored hidden-unit-major at G[k*Nd + r] (the correction reads four
-// consecutive rows of one hidden unit per lane).
+// stored in 16-row tiles, hidden-unit-major inside a tile: row r, unit k at
+// G[geo_at(r, k)].  The correction reads four consecutive rows of one hidden
+// unit per lane (16 bytes), and a 16-row tile's 64 units are 4 KB in one run.
+__host__ __device__ inline size_t geo_at(long r, int k)
+{
+    return (size_t)(r >> 4) * (16 * MLP_H) + (size_t)k * 16 + (size_t)(r & 15);
+}
 __global__ void __launch_bounds__(BLK)
 geom_kernel(int Nd, Ell A, int dim, double h, const double *__restrict__ dx, const double *__restrict__ dy,
             const double *__restrict__ dz, const uint8_t *__restrict__ btag, const float *__restrict__ W,
@@ -295,7 +300,7 @@ geom_kernel(int Nd, Ell A, int dim, double h, const double *__restrict__ dx, con
         }
     }
 #pragma unroll
-    for (int k = 0; k < MLP_H; k++) G[(size_t)k * Nd + r] = acc[k];
+    for (int k = 0; k < MLP_H; k++) G[geo_at(r, k)] = acc[k];
 }
 
 // ---- inference ----------------------------------------------------------------
@@ -319,6 +324,7 @@ constexpr int MF_KS2 = MLP_H / 32;              // K steps of layer 2
 constexpr int MF_NT = MLP_H / 16;               // 16-wide column tiles of a hidden layer
 constexpr int MF_WAVES = BLK / 64;
 constexpr int MF_TPW = 4;                       // node tiles per wave
+constexpr int MF_COLS = MF_KS1 * 8;             // residual columns a lane holds per node
 
 __device__ inline __bf16 bits_bf(bf16 v) { return __builtin_bit_cast(__bf16, v.bits); }
 
@@ -331,6 +337,112 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The network's weights in LDS, B-operand layout (f1: residual columns of
+// W1, f2: W2), with b2 and BF16(w3).  Threads tid = 0 .. nthr-1 of the block
+// share the copy; the caller synchronises the block afterwards.
+struct NetLds {
+    bf16x8 f1[MF_KS1][MF_NT][64];
+    bf16x8 f2[MF_KS2][MF_NT][64];
+    float b2[MLP_H], w3[MLP_H];
+};
+__device__ inline void load_net(NetLds &n, int m, const bf16 *__restrict__ W1rT, const bf16 *__restrict__ W2,
+                                const float *__restrict__ b2, const bf16 *__restrict__ w3, int tid, int nthr)
+{
+    for (int q = tid; q < MF_KS1 * MF_NT * 64; q += nthr) {
+        const int l = q % 64, t = (q / 64) % MF_NT, ks = q / (64 * MF_NT);
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int k = ks * 32 + (l / 16) * 8 + e, n = t * 16 + l % 16;   // residual column k, hidden unit n
+            v[e] = k < m ? bits_bf(W1rT[k * MLP_H + n]) : (__bf16)0.f;
+        }
+        n.f1[ks][t][l] = v;
+    }
+    for (int q = tid; q < MF_KS2 * MF_NT * 64; q += nthr) {
+        const int l = q % 64, t = (q / 64) % MF_NT, ks = q / (64 * MF_NT);
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int k = ks * 32 + (l / 16) * 8 + e, n = t * 16 + l % 16;   // layer-2 input k, output n
+            v[e] = bits_bf(W2[n * MLP_H + k]);
+        }
+        n.f2[ks][t][l] = v;
+    }
+    if (tid < MLP_H) {
+        n.b2[tid] = b2[tid];
+        n.w3[tid] = bf(w3[tid]);
+    }
+}
+
+// The geometric part of layer 1 for the four rows n4 .. n4+3 (a multiple of
+// 4, so the four sit in one 16-row tile of G) of the hidden units
+// t*16 + li: one aligned 16-byte load per column tile, zero when !have.
+__device__ inline void load_geo(f32x4 (&c)[MF_NT], const float *__restrict__ G, long n4, bool have, int li)
+{
+#pragma unroll
+    for (int t = 0; t < MF_NT; t++)
+        c[t] = have ? *reinterpret_cast<const f32x4 *>(G + geo_at(n4, t * 16 + li))
+                    : (f32x4){0.f, 0.f, 0.f, 0.f};
+}
+
+// Both layers of one 16-node tile from the layer-1 accumulators c (geometric
+// part) and this lane's scaled residual operands u; returns the network
+// output before b3 of node l%16, in every lane.  hb / yb are the wave's own
+// LDS rows; the caller syncs the wave before they are written again.
+__device__ inline float net_tile(f32x4 (&c)[MF_NT], const bf16x8 (&u)[MF_KS1], const NetLds &n,
+                                 __bf16 (*hb)[MLP_H], float *yb, int l)
+{
+    const int li = l % 16, g = l / 16;
+#pragma unroll
+    for (int ks = 0; ks < MF_KS1; ks++)
+#pragma unroll
+        for (int t = 0; t < MF_NT; t++)
+            c[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(u[ks], n.f1[ks][t][l], c[t], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < MF_NT; t++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) hb[g * 4 + v][t * 16 + li] = (__bf16)relu(c[t][v]);
+    wave_sync();
+    f32x4 c2[MF_NT];
+#pragma unroll
+    for (int t = 0; t < MF_NT; t++) {
+        const float bb = n.b2[t * 16 + li];
+        c2[t] = (f32x4){bb, bb, bb, bb};
+    }
+#pragma unroll
+    for (int ks = 0; ks < MF_KS2; ks++) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&hb[li][ks * 32 + g * 8]);
+#pragma unroll
+        for (int t = 0; t < MF_NT; t++)
+            c2[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, n.f2[ks][t][l], c2[t], 0, 0, 0);
+    }
+    float sv[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        float acc = 0.f;
+#pragma unroll
+        for (int t = 0; t < MF_NT; t++) acc += n.w3[t * 16 + li] * relu(c2[t][v]);
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) acc += __shfl_xor(acc, o);   // the 16 lanes of group g
+        sv[v] = acc;
+    }
+    if (li == 0)
+#pragma unroll
+        for (int v = 0; v < 4; v++) yb[g * 4 + v] = sv[v];
+    wave_sync();
+    return yb[li];
+}
+
+// sigma = RMS of the m stencil residuals of node l%16 (this lane holds
+// MF_COLS of them, zero beyond m or on rows without a correction), summed in
+// column order per lane, then across the four lane groups.
+__device__ inline double stencil_sigma(double s2, int m)
+{
+    s2 += __shfl_xor(s2, 16);
+    s2 += __shfl_xor(s2, 32);
+    return sqrt(s2 / m);
+}
+
 // Three waves per SIMD: the 24 residual gathers a lane keeps in flight need
 // the registers (a cap of four spills).
 template <typename TV>
@@ -339,36 +451,11 @@ correct_kernel(int Nd, Ell A, const uint8_t *__restrict__ btag, const float *__r
                const bf16 *__restrict__ W1rT, const bf16 *__restrict__ W2, const float *__restrict__ b2,
                const bf16 *__restrict__ w3, float b3, const TV *__restrict__ res, TV *__restrict__ p)
 {
-    __shared__ bf16x8 f1[MF_KS1][MF_NT][64];
-    __shared__ bf16x8 f2[MF_KS2][MF_NT][64];
-    __shared__ float sb2[MLP_H], sw3[MLP_H];
+    __shared__ NetLds net;
     __shared__ __attribute__((aligned(16))) __bf16 hb[MF_WAVES][16][MLP_H];
     __shared__ float yb[MF_WAVES][16];
     const int m = A.m;
-    for (int q = threadIdx.x; q < MF_KS1 * MF_NT * 64; q += BLK) {
-        const int l = q % 64, t = (q / 64) % MF_NT, ks = q / (64 * MF_NT);
-        bf16x8 v;
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int k = ks * 32 + (l / 16) * 8 + e, n = t * 16 + l % 16;   // residual column k, hidden unit n
-            v[e] = k < m ? bits_bf(W1rT[k * MLP_H + n]) : (__bf16)0.f;
-        }
-        f1[ks][t][l] = v;
-    }
-    for (int q = threadIdx.x; q < MF_KS2 * MF_NT * 64; q += BLK) {
-        const int l = q % 64, t = (q / 64) % MF_NT, ks = q / (64 * MF_NT);
-        bf16x8 v;
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int k = ks * 32 + (l / 16) * 8 + e, n = t * 16 + l % 16;   // layer-2 input k, output n
-            v[e] = bits_bf(W2[n * MLP_H + k]);
-        }
-        f2[ks][t][l] = v;
-    }
-    if (threadIdx.x < MLP_H) {
-        sb2[threadIdx.x] = b2[threadIdx.x];
-        sw3[threadIdx.x] = bf(w3[threadIdx.x]);
-    }
+    load_net(net, m, W1rT, W2, b2, w3, threadIdx.x, BLK);
     __syncthreads();
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, li = l % 16, g = l / 16;
     const long ntiles = ((long)Nd + 15) / 16;
@@ -378,7 +465,7 @@ correct_kernel(int Nd, Ell A, const uint8_t *__restrict__ btag, const float *__r
         const long nb = tile * 16, r = nb + li;
         const bool valid = r < Nd && btag[r] == ROW_INTERIOR;
         // this lane's residual columns of node li: j = ks*32 + g*8 + e
-        double vq[MF_KS1 * 8];
+        double vq[MF_COLS];
         double s2 = 0;
 #pragma unroll
         for (int ks = 0; ks < MF_KS1; ks++)
@@ -390,63 +477,90 @@ correct_kernel(int Nd, Ell A, const uint8_t *__restrict__ btag, const float *__r
                 vq[ks * 8 + e] = v;
                 s2 += v * v;
             }
-        s2 += __shfl_xor(s2, 16);
-        s2 += __shfl_xor(s2, 32);
-        const double sigma = sqrt(s2 / m);
+        const double sigma = stencil_sigma(s2, m);
         const double inv = sigma > 0 ? 1.0 / sigma : 0.0;
-        // the geometric part of this lane's four nodes nb + 4g .. nb + 4g + 3:
-        // one 16-byte load per column tile (Nd and nb are multiples of 4, so
-        // the four are all rows or all beyond Nd, and the load is aligned)
+        // Nd and nb are multiples of 4: the lane's four rows are all rows or all beyond Nd
         f32x4 c[MF_NT];
-        const long n4 = nb + g * 4;
+        load_geo(c, G, nb + g * 4, nb + g * 4 < Nd, li);
+        bf16x8 u[MF_KS1];
 #pragma unroll
-        for (int t = 0; t < MF_NT; t++)
-            c[t] = n4 < Nd ? *reinterpret_cast<const f32x4 *>(G + (size_t)(t * 16 + li) * Nd + n4)
-                           : (f32x4){0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < MF_KS1; ks++)
 #pragma unroll
-        for (int ks = 0; ks < MF_KS1; ks++) {
-            bf16x8 a;
-#pragma unroll
-            for (int e = 0; e < 8; e++) a[e] = (__bf16)(float)(vq[ks * 8 + e] * inv);
-#pragma unroll
-            for (int t = 0; t < MF_NT; t++)
-                c[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, f1[ks][t][l], c[t], 0, 0, 0);
-        }
-#pragma unroll
-        for (int t = 0; t < MF_NT; t++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) hb[wave][g * 4 + v][t * 16 + li] = (__bf16)relu(c[t][v]);
-        wave_sync();
-        f32x4 c2[MF_NT];
-#pragma unroll
-        for (int t = 0; t < MF_NT; t++) {
-            const float bb = sb2[t * 16 + li];
-            c2[t] = (f32x4){bb, bb, bb, bb};
-        }
-#pragma unroll
-        for (int ks = 0; ks < MF_KS2; ks++) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&hb[wave][li][ks * 32 + g * 8]);
-#pragma unroll
-            for (int t = 0; t < MF_NT; t++)
-                c2[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, f2[ks][t][l], c2[t], 0, 0, 0);
-        }
-        float sv[4];
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            float acc = 0.f;
-#pragma unroll
-            for (int t = 0; t < MF_NT; t++) acc += sw3[t * 16 + li] * relu(c2[t][v]);
-#pragma unroll
-            for (int o = 1; o < 16; o <<= 1) acc += __shfl_xor(acc, o);   // the 16 lanes of group g
-            sv[v] = acc;
-        }
-        if (li == 0)
-#pragma unroll
-            for (int v = 0; v < 4; v++) yb[wave][g * 4 + v] = sv[v];
-        wave_sync();
+            for (int e = 0; e < 8; e++) u[ks][e] = (__bf16)(float)(vq[ks * 8 + e] * inv);
+        const float y = net_tile(c, u, net, hb[wave], yb[wave], l);
         if (g == 0 && valid && sigma > 0) {
             const double diag = A.w[ell_at(m, A.S, r, 0)];
-            p[r] = p[r] + (TV)((double)(yb[wave][li] + b3) * sigma / diag);
+            p[r] = p[r] + (TV)((double)(y + b3) * sigma / diag);
+        }
+        wave_sync();   // hb and yb are rewritten by the next tile
+    }
+}
+
+// The same correction over a level in Morton blocks with LDS windows, one
+// workgroup per block: the block's residual window -- its own rows, then its
+// halo rows (A.win_*), plus one zero -- is loaded into LDS once, and each
+// stencil residual is read there through its 16-bit place instead of being
+// gathered from memory (about 70 gathers per node).  A lane reads its places
+// once (two per register) and the window twice (the RMS, then the scaled
+// operands), so it holds no residual values across the tile.  Tiles of 16 rows start at the block's
+// first row (a multiple of 4).  Same operations in the same order as
+// correct_kernel, so the corrections are bitwise the same.
+constexpr int CW_WAVES = 16;
+template <typename TV>
+__global__ void __launch_bounds__(64 * CW_WAVES)
+correct_window_kernel(Ell A, const uint8_t *__restrict__ btag, const float *__restrict__ G,
+                      const bf16 *__restrict__ W1rT, const bf16 *__restrict__ W2, const float *__restrict__ b2,
+                      const bf16 *__restrict__ w3, float b3, const TV *__restrict__ res, TV *__restrict__ p)
+{
+    __shared__ NetLds net;
+    __shared__ __attribute__((aligned(16))) __bf16 hb[CW_WAVES][16][MLP_H];
+    __shared__ float yb[CW_WAVES][16];
+    extern __shared__ unsigned char lds_raw[];
+    TV *win = reinterpret_cast<TV *>(lds_raw);
+    const int m = A.m;
+    const int b = (int)xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = A.win_row0[b], r1 = A.win_row0[b + 1];
+    const int h0 = A.win_hoff[b], nh = A.win_hoff[b + 1] - h0, nown = r1 - r0, zero = nown + nh;
+    for (int i = threadIdx.x; i < nown; i += 64 * CW_WAVES) win[i] = res[r0 + i];
+    for (int i = threadIdx.x; i < nh; i += 64 * CW_WAVES) win[nown + i] = res[A.win_halo[h0 + i]];
+    if (threadIdx.x == 0) win[zero] = TV(0);
+    load_net(net, m, W1rT, W2, b2, w3, threadIdx.x, 64 * CW_WAVES);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, li = l % 16, g = l / 16;
+    for (int nb = r0 + 16 * wave; nb < r1; nb += 16 * CW_WAVES) {
+        const int r = nb + li;
+        const bool valid = r < r1 && btag[r] == ROW_INTERIOR;
+        unsigned pl[MF_COLS / 2];   // two 16-bit places per register, column q in half q % 2
+        double s2 = 0;
+#pragma unroll
+        for (int ks = 0; ks < MF_KS1; ks++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int j = ks * 32 + g * 8 + e, q = ks * 8 + e;
+                const unsigned v = (valid && j < m) ? (unsigned)A.win_col[ell_at(m, A.S, r, j)] : (unsigned)zero;
+                pl[q / 2] = (q % 2) ? (pl[q / 2] | v << 16) : v;
+            }
+#pragma unroll
+        for (int q = 0; q < MF_COLS; q++) {
+            const double v = (double)win[(pl[q / 2] >> (16 * (q % 2))) & 0xffff];
+            s2 += v * v;
+        }
+        const double sigma = stencil_sigma(s2, m);
+        const double inv = sigma > 0 ? 1.0 / sigma : 0.0;
+        f32x4 c[MF_NT];
+        load_geo(c, G, nb + g * 4, nb + g * 4 < r1, li);   // r0, r1: multiples of 4
+        bf16x8 u[MF_KS1];
+#pragma unroll
+        for (int ks = 0; ks < MF_KS1; ks++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int q = ks * 8 + e;
+                u[ks][e] = (__bf16)(float)((double)win[(pl[q / 2] >> (16 * (q % 2))) & 0xffff] * inv);
+            }
+        const float y = net_tile(c, u, net, hb[wave], yb[wave], l);
+        if (g == 0 && valid && sigma > 0) {
+            const double diag = A.w[ell_at(m, A.S, r, 0)];
+            p[r] = p[r] + (TV)((double)(y + b3) * sigma / diag);
         }
         wave_sync();   // hb and yb are rewritten by the next tile
     }
@@ -543,6 +657,39 @@ int mlp_geometry(int Nd, Ell A, int dim, double h, const double *dx, const doubl
     return 0;
 }
 
+// LDS of the window form: the kernel's static arrays and the window.
+constexpr size_t kLdsPerCu = 160 * 1024;
+template <typename TV>
+static size_t correct_window_static()
+{
+    return sizeof(NetLds) + sizeof(__bf16) * CW_WAVES * 16 * MLP_H + sizeof(float) * CW_WAVES * 16;
+}
+template <typename TV>
+static bool correct_window_fits(const Ell &A)
+{
+    return A.win_blocks > 0 && A.S % 4 == 0 &&
+           correct_window_static<TV>() + ((size_t)A.win_max + 1) * sizeof(TV) <= kLdsPerCu;
+}
+// MGM_LEARNED_WINDOW=0 keeps the gathering form on every level.
+static bool window_learned_on()
+{
+    const char *e = getenv("MGM_LEARNED_WINDOW");
+    return !(e && e[0] == '0');
+}
+
+int mlp_device_init()
+{
+    HIP_TRY(hipFuncSetAttribute((const void *)correct_window_kernel<double>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(kLdsPerCu - correct_window_static<double>())));
+    HIP_TRY(hipFuncSetAttribute((const void *)correct_window_kernel<float>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(kLdsPerCu - correct_window_static<float>())));
+    return 0;
+}
+
+size_t mlp_geometry_size(int Nd) { return (size_t)((Nd + 15) / 16) * 16 * MLP_H; }
+
 template <typename TV>
 int mlp_correct(int Nd, Ell A, const uint8_t *btag, const float *G, const bf16 *W1rT, const bf16 *W2,
                 const float *b2, const bf16 *w3, float b3, const TV *res, TV *p, hipStream_t s)
@@ -550,6 +697,13 @@ int mlp_correct(int Nd, Ell A, const uint8_t *btag, const float *G, const bf16 *
     if (A.m > MLP_MAX_M) MGM_FAIL(-1, "learned smoother: stencil wider than supported");
     if (A.sorted) MGM_FAIL(-1, "learned smoother: needs the plain slab");
     if (Nd % 4) MGM_FAIL(-1, "learned smoother: device rows must be a multiple of 4");
+    if (correct_window_fits<TV>(A) && window_learned_on()) {
+        const size_t shm = ((size_t)A.win_max + 1) * sizeof(TV);
+        hipLaunchKernelGGL(correct_window_kernel<TV>, dim3(A.win_blocks), dim3(64 * CW_WAVES), shm, s, A, btag, G,
+                           W1rT, W2, b2, w3, b3, res, p);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     const long tiles = ((long)Nd + 15) / 16, waves = (tiles + MF_TPW - 1) / MF_TPW;
     const int blocks = (int)((waves + MF_WAVES - 1) / MF_WAVES);
     if (blocks > 0) {

@@ -29,12 +29,16 @@ int mlp_gather_batch(int n, int in, const int *idx, const float *X, const float 
 int mlp_features(int n, Ell A, int dim, double h, const double *dx, const double *dy, const double *dz,
                  const int *rows, const int *snap, int Nd, const double *res_snaps, const double *err_snaps,
                  float *X, float *t, hipStream_t s);
-// Geometric part of layer 1 per interior row (G: MLP_H x Nd, hidden-unit-major).
+// Geometric part of layer 1 per interior row: G holds mlp_geometry_size(Nd)
+// floats, 16-row tiles with the hidden units major inside a tile.
+size_t mlp_geometry_size(int Nd);
 int mlp_geometry(int Nd, Ell A, int dim, double h, const double *dx, const double *dy, const double *dz,
                  const uint8_t *btag, const float *W, float *G, hipStream_t s);
+// Raises the LDS limit of the window form of the correction (once per device).
+int mlp_device_init();
 // p += network correction on interior rows, from the residual res (W1rT: the
 // residual columns of W1 transposed, m x MLP_H); A supplies columns and the
-// FP64 diagonal.  BF16 matrix cores: BF16 weights and activations, FP32
+// FP64 diagonal, and its LDS-window tables, where set, select the window form.  BF16 matrix cores: BF16 weights and activations, FP32
 // accumulation.  Instantiated for FP64 and FP32 vectors.
 template <typename TV>
 int mlp_correct(int Nd, Ell A, const uint8_t *btag, const float *G, const bf16 *W1rT, const bf16 *W2,

@@ -19,6 +19,7 @@
 #include <numeric>
 #include <thread>
 
+#include "mlp.hpp"
 #include "problem.hpp"
 
 using namespace mgm;
@@ -848,6 +849,7 @@ static int level_slab(mgm_problem *pb, int l, int which)
 // 244 blocks, at most 9,871 window values, 151 KB of LDS with the staging;
 // the apply takes 167 us instead of 203 us gathering from memory.
 constexpr size_t kWindowLds = 159 * 1024;
+static_assert(kWindowLds / sizeof(float) <= 65536, "window places must fit 16 bits");
 static bool window_on()
 {
     const char *e = getenv("MGM_WINDOW");
@@ -863,7 +865,7 @@ static int level_window(Level &L)
     if (!L.blocked || !window_on() || L.blk_row0.size() < 2) return 0;
     const int nb = (int)L.blk_row0.size() - 1, m = L.m, S = L.S;
     std::vector<std::vector<int>> halo(nb);
-    std::vector<int> wcol(ell_size(L.Nd, m, S), 0);
+    std::vector<uint16_t> wcol(ell_size(L.Nd, m, S), 0);
     std::vector<int> wsize(nb, 0);
     const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::thread> th;
@@ -892,7 +894,7 @@ static int level_window(Level &L)
                                         ? c - r0
                                         : (r1 - r0) + (int)(std::lower_bound(h.begin(), h.end(), c) - h.begin());
                         }
-                        wcol[ell_at(m, S, d, j)] = local;
+                        wcol[ell_at(m, S, d, j)] = (uint16_t)local;
                     }
                 }
                 wsize[b] = (r1 - r0) + (int)h.size();
@@ -1015,6 +1017,7 @@ int mgm_init(int device)
     int rc = noalloc_init(device);   // the allocation-free entry point's scratch address
     if (rc) return rc;
     if ((rc = stencil_device_init())) return rc;
+    if ((rc = mlp_device_init())) return rc;
     if ((rc = knn_device_init())) return rc;
     if (g_stream) (void)hipStreamDestroy(g_stream);
     HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
@@ -1221,6 +1224,8 @@ int mgm_level_info_get(const mgm_problem *pb, int lev, mgm_level_info *o)
     o->num_colours = L.ncol; o->d_avg = L.d_avg; o->d_avg_bbox = L.d_avg_bbox; o->knn_ties = L.knn_ties;
     o->device_rows = L.Nd; o->slice_height = L.S; o->blocked = L.blocked; o->gather_sorted = L.gs;
     o->small_relax = relax_small_here(pb, L);
+    o->window_blocks = L.win_blocks;
+    o->window_max = L.win_max;
     return 0;
 }
 

@@ -83,7 +83,8 @@ struct Level {
 
     // ---- LDS windows of the full passes (levels in Morton blocks) ----
     std::vector<int> blk_row0;                 // device row where each Morton block starts (+ end)
-    DevBuf<int> win_col, win_row0, win_hoff, win_halo;
+    DevBuf<uint16_t> win_col;                  // 16-bit places in the block's window
+    DevBuf<int> win_row0, win_hoff, win_halo;
     int win_blocks = 0, win_max = 0;           // 0 blocks: no windows
     bool win_tried = false;                    // tables built (or found not to fit) once
     template <typename T>
@@ -141,6 +142,7 @@ struct LearnedState {
     std::vector<DevBuf<float>> G;     // per level: Nd x MLP_H
     std::vector<DevBuf<double>> dx, dy, dz;
     std::vector<char> lev_ok;         // level's stencils fit the network
+    std::vector<char> lev_use;        // levels the caller lets use it (empty: all)
     int fell_back = 0;                // the last learned solve switched to SOR
 };
 }  // namespace mgm

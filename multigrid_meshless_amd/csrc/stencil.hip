@@ -66,15 +66,6 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2.  Renumber them so XCD i takes a contiguous range of slices: neighbouring
-// rows (and the x entries their stencils gather) then stay in one L2.
-__device__ inline long xcd_block(long b, long nb)
-{
-    const long xcd = b % 8, q = nb / 8, rem = nb % 8, idx = b / 8;
-    return (xcd < rem) ? xcd * (q + 1) + idx : rem * (q + 1) + (xcd - rem) * q + idx;
-}
-
 // Non-temporal load (streamed slab entries, read once per pass); BF16 structs
 // load plainly.
 template <typename T> __device__ inline T ldnt(const T *p) { return __builtin_nontemporal_load(p); }
@@ -303,7 +294,7 @@ __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice,
     if (act && !skip) {
         const size_t base = (size_t)slice * S * m + (size_t)jo * S + ri;
         const TW *w = wp + base;
-        const int *c = A.win_col + base;
+        const uint16_t *c = A.win_col + base;
         TV *pp = P + jo * S + ri;
         const int nt = (m - jo + LPR - 1) / LPR;
         if (A.stream) {

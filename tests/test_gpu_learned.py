@@ -142,10 +142,20 @@ def test_adam_fit_matches_oracle(gpu, oracle, epochs, batch):
     assert l1 < l0
 
 
-@pytest.mark.parametrize("dim,sizes", [(2, (40, 20, 10)), (3, (16, 8))])
-def test_learned_sweep_matches_oracle(gpu, dim, sizes):
-    print(f"[gpu learned sweep] dim {dim} sizes {sizes}", flush=True)
+def windowed(monkeypatch, block_rows):
+    """Lay levels above 1,000 rows out in Morton blocks of block_rows rows, so
+    their full passes and learned corrections take the LDS-window form."""
+    monkeypatch.setenv("MGM_BLOCKED_ROWS", "1000")
+    monkeypatch.setenv("MGM_BLOCK_ROWS", str(block_rows))
+
+
+@pytest.mark.parametrize("dim,sizes,blocks", [(2, (40, 20, 10), 0), (3, (16, 8), 0), (3, (16, 8), 512)])
+def test_learned_sweep_matches_oracle(gpu, monkeypatch, dim, sizes, blocks):
+    print(f"[gpu learned sweep] dim {dim} sizes {sizes} blocks of {blocks or 'all'} rows", flush=True)
+    if blocks:
+        windowed(monkeypatch, blocks)
     P = problem(gpu, dim, sizes)
+    assert (P.info(0)["window_blocks"] > 0) == bool(blocks)
     n_in = P.mlp_inputs()
     w = he_weights(n_in, 3)
     P.set_mlp_weights(w)
@@ -171,6 +181,79 @@ def test_learned_sweep_matches_oracle(gpu, dim, sizes):
     err = np.abs(dp - want).max() / np.abs(want).max()
     print(f"  max |dp - oracle| / max |oracle| = {err:.2e}", flush=True)
     assert err < 1e-3
+
+
+@pytest.mark.parametrize("block_rows", [512, 1024])
+def test_learned_window_form_bitwise(gpu, monkeypatch, block_rows):
+    """The LDS-window form of the correction (levels in Morton blocks) equals
+    the gathering form bit for bit: one sweep from a random p, and whole
+    learned solves in FP64 and with the FP32 inner cycle."""
+    print(f"[gpu learned window] 3D 24/12, blocks of {block_rows} rows", flush=True)
+    windowed(monkeypatch, block_rows)
+    P = problem(gpu, 3, (24, 12))
+    assert P.info(0)["window_blocks"] > 1
+    P.set_mlp_weights(he_weights(P.mlp_inputs(), 3))
+    P.use_learned_smoother(True)
+    geo = P.geometry(0)
+    N = geo["x"].size
+    rng = np.random.default_rng(12)
+    p0 = rng.standard_normal(N)
+    s = np.where(geo["btag"] == 0, rng.standard_normal(N), 0.0)
+    out = {}
+    for form in ("1", "0"):
+        monkeypatch.setenv("MGM_LEARNED_WINDOW", form)
+        P.set_smoother(1.0, 1)
+        P.set_field(0, "p", p0)
+        P.set_field(0, "source", s)
+        P.relax(0)
+        sweep = P.field(0, "p")
+        P.set_smoother(1.0, 3)
+        solves = []
+        for prec in (0, 1):
+            P.zero_fields()
+            P.set_field(0, "source", s)
+            r = P.solve(tol=1e-8, max_cycles=100, precision=prec)
+            solves.append((r["history"], P.field(0, "p")))
+        out[form] = (sweep, solves)
+    assert not np.array_equal(out["1"][0], p0)
+    np.testing.assert_array_equal(out["1"][0], out["0"][0])
+    for (h1, p1), (h0, p0_) in zip(out["1"][1], out["0"][1]):
+        np.testing.assert_array_equal(h1, h0)
+        np.testing.assert_array_equal(p1, p0_)
+    print(f"  cycles FP64 {len(out['1'][1][0][0]) - 1}, FP32 inner {len(out['1'][1][1][0]) - 1}", flush=True)
+
+
+def test_learned_level_mask(gpu):
+    """set_learned_levels: a level left out relaxes exactly as plain SOR, a
+    level kept relaxes exactly as with no mask; an empty mask restores all."""
+    print("[gpu learned level mask] dim 2 sizes (40, 20, 10)", flush=True)
+    P = problem(gpu, 2, (40, 20, 10))
+    P.set_mlp_weights(he_weights(P.mlp_inputs(), 3))
+    P.set_smoother(1.0, 2)
+
+    def relax(l, learned, mask):
+        P.use_learned_smoother(learned)
+        P.set_learned_levels(mask)
+        N = P.geometry(l)["x"].size
+        r = np.random.default_rng(7 + l)
+        P.set_field(l, "p", r.standard_normal(N))
+        P.set_field(l, "source", r.standard_normal(N))
+        P.relax(l)
+        return P.field(l, "p")
+
+    for l in range(P.L):
+        plain = relax(l, False, None)
+        full = relax(l, True, None)
+        assert l > 0 or not np.array_equal(plain, full)   # level 0's stencils fit the network
+        off = [1] * P.L
+        off[l] = 0
+        on = [0] * P.L
+        on[l] = 1
+        assert np.array_equal(relax(l, True, off), plain)
+        assert np.array_equal(relax(l, True, on), full)
+        assert np.array_equal(relax(l, True, None), full)
+    with pytest.raises(Exception):
+        P.set_learned_levels([1] * (P.L + 1))
 
 
 def test_training_and_learned_solve(gpu):
