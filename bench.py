@@ -328,6 +328,11 @@ def main():
                     help="smoother factors of the cube solves: 'tune' (autotuned), one, or one per level (w0,w1,...)")
     ap.add_argument("--smoother", default=None,
                     help="with fixed --omega: smoother per level, s (SOR) or j (Jacobi), e.g. jjjjj")
+    ap.add_argument("--learned-omega", default=None,
+                    help="the learned solves' factors, one per level (default: tuned with --omega tune, "
+                         "else --omega)")
+    ap.add_argument("--learned-smoother", default=None,
+                    help="with --learned-omega: the learned solves' smoother per level (s/j)")
     ap.add_argument("--cpu-weight-sample", type=int, default=16384,
                     help="level-0 systems of the CPU weight-build sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -474,7 +479,13 @@ def main():
         del Pt
         P.use_learned_smoother(True)
         learned = {"train_s": round(t_train, 2), "train_loss": [round(l0, 4), round(l1, 4)], "omega": omega}
-        if args.omega == "tune":
+        if args.learned_omega:
+            lw = [float(np.float32(float(v))) for v in args.learned_omega.split(",")]
+            lk = [0] * P.L if not args.learned_smoother else ["sj".index(ch) for ch in args.learned_smoother]
+            P.set_smoother_levels(lw if len(lw) > 1 else lw * P.L)
+            P.set_smoother_kinds(lk)
+            learned["omega"], learned["smoother"] = P.smoother_levels(), "".join("sj"[k] for k in lk)
+        elif args.omega == "tune":
             reset(P, set_rhs(P, seed=3))
             tl = P.tune_smoother(args.tol, args.max_cycles, lo=0.5, hi=1.9, step=0.1, sweeps=3)
             learned["omega"], learned["smoother"] = tl["omega"], "".join("sj"[k] for k in tl["kind"])

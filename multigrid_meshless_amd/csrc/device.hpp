@@ -236,6 +236,13 @@ int launch_sor_slices(EllT<TW> A, const int *slices, int first, int nslices, con
 template <typename TW, typename TV>
 int launch_jacobi(EllT<TW> A, int N, const uint8_t *btag, const TV *src, const TV *x, TV *y, TV w, TV c,
                   hipStream_t s);
+// One pass over a level with LDS windows (has_windows) forming, from the same
+// products, the residual form 0 of x into res (bitwise launch_residual's) and
+// the Jacobi sweep of x into y (bitwise launch_jacobi's).  FP64.
+template <typename TW, typename TV>
+int launch_residual_jacobi(EllT<TW> A, const uint8_t *btag, const TV *src, const TV *x, TV *y, TV *res, TV w, TV c,
+                           hipStream_t s);
+bool has_windows(const EllT<double> &A);
 // The whole relaxation of a small level (all sweeps, all colour classes, and
 // for the T / F variants the pin and residual after each sweep, A2 being the
 // operator of the F residual) in one single-workgroup launch.  Weights TW

@@ -1504,10 +1504,18 @@ int op_relax(mgm_problem *pb, int l, int variant)
     if (pb->jacobi(l, variant)) {
         // weighted Jacobi: sweeps alternate between p and res (free until
         // the residual that follows the relaxation); an odd count ends in
-        // res and is copied back
+        // res and is copied back.  When the solve's fine residual already
+        // formed the first sweep (jac0), the rest alternate so that the last
+        // lands in p.
         const double *in = p;
         double *out = res;
-        for (int it = 0; it < nrelax; it++) {
+        int it = 0;
+        if (l == 0 && pb->jac0_ready && nrelax > 0) {
+            in = pb->jac0.p;
+            out = (nrelax - 1) % 2 == 0 ? res : p;
+            it = 1;
+        }
+        for (; it < nrelax; it++) {
             int rc = launch_jacobi(A, L.Nd, L.dbtag.p, src, in, out, w, c, g_stream);
             if (rc) return rc;
             in = out;

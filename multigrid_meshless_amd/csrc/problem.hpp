@@ -160,6 +160,11 @@ struct mgm_problem {
     int mp_built = 0;              // precision mode the FP32/BF16 copies hold (0: none / stale)
     bool mp_ns_built = false;      // FP32 lap copies of the time steps' pressure cycle are current
     bool res0_current = false;     // level 0's RES holds s - lap_Poison p (form 0) whenever a cycle starts
+    // Inside mgm_solve with weighted Jacobi on level 0: the fine residual pass
+    // also forms the first Jacobi sweep of p into jac0, and level 0's next
+    // relaxation starts from it (jac0_ready).
+    mgm::DevBuf<double> jac0;
+    bool jac0_ready = false;
     std::unique_ptr<mgm::LearnedState> ml;
     std::vector<float> omega_lv;   // SOR factor per level; empty: cfg.omega on every level
     std::vector<int> smoother_lv;  // MGM_SMOOTH_* per level; empty: SOR on every level

@@ -31,13 +31,22 @@ double wall()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// residual of level 0 into RES and its squared interior norm into scal[0]
-int enqueue_fine_residual(mgm_problem *pb, int variant)
+// residual of level 0 into RES and its squared interior norm into scal[0];
+// with fuse, the same pass forms the first Jacobi sweep of p into jac0
+int enqueue_fine_residual(mgm_problem *pb, int variant, bool fuse = false)
 {
     Level &L = pb->lv[0];
     int pf = variant == MGM_VAR_T ? MGM_F_PPRIME : MGM_F_P;
-    int rc = launch_residual(L.op(OP_LAPP), L.Nd, 0, L.dbtag.p, L.f[MGM_F_SOURCE].p, L.f[pf].p,
-                             L.f[MGM_F_RES].p, stream());
+    int rc;
+    if (fuse) {
+        const float wl = pb->omega(0);   // the factors op_relax passes to its sweeps
+        rc = launch_residual_jacobi(L.op(OP_LAPP), L.dbtag.p, (const double *)L.f[MGM_F_SOURCE].p,
+                                    (const double *)L.f[pf].p, pb->jac0.p, L.f[MGM_F_RES].p, (double)wl,
+                                    (double)(float)(1 - wl), stream());
+    } else {
+        rc = launch_residual(L.op(OP_LAPP), L.Nd, 0, L.dbtag.p, L.f[MGM_F_SOURCE].p, L.f[pf].p, L.f[MGM_F_RES].p,
+                             stream());
+    }
     if (rc) return rc;
     return launch_sqdiff_sum(L.Nd, L.dbtag.p, L.f[MGM_F_RES].p, nullptr, pb->partial.p, pb->scal.p, stream());
 }
@@ -76,16 +85,30 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
     if ((rc = read_scalar(pb, &fs))) return rc;
     double fnorm = sqrt(fs / L0.N);
     if (!(fnorm > 0)) fnorm = 1.0;
-    if ((rc = enqueue_fine_residual(pb, variant))) return rc;
-    double rs;
-    if ((rc = read_scalar(pb, &rs))) return rc;
-    if (history) history[0] = sqrt(rs / L0.N) / fnorm;
 
     // The learned smoother's cycles are checked against one plain cycle: if a
     // learned cycle reduces the residual less than the plain one did, the rest
     // of the solve falls back to plain SOR.
     LearnedState *ml = pb->ml.get();
     const bool learned = ml && ml->enabled && ml->prepared && variant == MGM_VAR_H;
+    // Weighted Jacobi on level 0 (FP64 H cycles, LDS windows, no learned
+    // sweep): the fine residual after each cycle and the first sweep of the
+    // next cycle's level-0 relaxation read the same iterate, so one pass forms
+    // both (the residual and the sweep are bitwise those of separate passes).
+    // MGM_FUSE_RESIDUAL=0 keeps them apart.
+    const char *fe = getenv("MGM_FUSE_RESIDUAL");
+    const bool fuse = !(fe && fe[0] == '0') && precision == 0 && variant == MGM_VAR_H && !learned &&
+                      pb->jacobi(0, variant) && pb->cfg.num_relax >= 1 && has_windows(L0.op(OP_LAPP));
+    struct Jac0Flag {
+        mgm_problem *pb;
+        ~Jac0Flag() { pb->jac0_ready = false; }
+    } jac0_flag{pb};
+    if (fuse && pb->jac0.n != (size_t)L0.Nd) HIP_TRY(pb->jac0.alloc(L0.Nd));
+    if ((rc = enqueue_fine_residual(pb, variant, fuse))) return rc;
+    pb->jac0_ready = fuse;
+    double rs;
+    if ((rc = read_scalar(pb, &rs))) return rc;
+    if (history) history[0] = sqrt(rs / L0.N) / fnorm;
     if (ml) ml->fell_back = 0;
     auto set_learned = [&](bool on) { if (ml) ml->enabled = on; };
     const bool ml_enabled = ml && ml->enabled;
@@ -116,7 +139,7 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
             set_learned(g == 1);
             HIP_TRY(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
             rc = cycle();
-            if (!rc) rc = enqueue_fine_residual(pb, variant);
+            if (!rc) rc = enqueue_fine_residual(pb, variant, fuse);
             hipError_t ce = hipStreamEndCapture(stream(), &graph[g]);
             if (rc) { cleanup(); return rc; }
             if (ce != hipSuccess) { cleanup(); MGM_FAIL(-10, std::string("graph capture failed: ") + hipGetErrorString(ce)); }
@@ -141,7 +164,7 @@ int mgm_solve(mgm_problem *pb, int variant, int precision, double tol, int max_c
         } else {
             set_learned(use_learned);
             if ((rc = cycle())) break;
-            if ((rc = enqueue_fine_residual(pb, variant))) break;
+            if ((rc = enqueue_fine_residual(pb, variant, fuse))) break;
         }
         if ((rc = read_scalar(pb, &rs))) break;
         ratio = sqrt(rs / L0.N) / fnorm;

@@ -49,6 +49,7 @@ enum Mode {
     M_RESTRICT = 6,  // interior sum, bnd 0 or kept    hf/heat_conduction.h:58-77 (+T, F)
     M_PROLONG = 7,   // interior p += sum              hf/heat_conduction.h:82-92
     M_JACOBI = 8,    // the SOR update of every row from the previous iterate x, into y
+    M_RESJAC = 9,    // M_RES0 into y2 and M_JACOBI into y, from one set of products
 };
 
 __device__ inline bool skips_boundary(int mode)
@@ -167,6 +168,32 @@ __device__ inline TV ordered_sum(const TV *P, int S, int ri, int j0, int m, TV a
     return acc;
 }
 
+// The sums ordered_sum<false> forms from +0 over slots 0..m-1 (*a) and over
+// slots 1..m-1 (*b), in one walk over the slots.
+template <typename TV>
+__device__ inline void ordered_sum2(const TV *P, int S, int ri, int m, TV *a, TV *b)
+{
+    const TV *q = P + ri;
+    TV sa = TV(0) + q[0], sb = TV(0);
+    int j = 1;
+    for (; j + 8 <= m; j += 8) {
+        TV v[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) v[t] = q[(j + t) * S];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            sa = sa + v[t];
+            sb = sb + v[t];
+        }
+    }
+    for (; j < m; j++) {
+        sa = sa + q[j * S];
+        sb = sb + q[j * S];
+    }
+    *a = sa;
+    *b = sb;
+}
+
 // One wave applies the rows of one slice: lanes (ri, jo) stage the products
 // of row ri (stage_products); after a wave barrier lane jo == 0 combines its
 // row in stencil order.  TW is the stored weight type, TV the type of
@@ -274,7 +301,7 @@ rows_kernel(EllT<TW> A, const int *__restrict__ slices, int first, int nslices, 
 template <int S, typename TW, typename TV>
 __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice, int mode,
                                     const uint8_t *__restrict__ btag, const TV *__restrict__ src, const TV *win,
-                                    int r0, TV *y, TV *P, TV wf, TV cf)
+                                    int r0, TV *y, TV *y2, TV *P, TV wf, TV cf)
 {
     constexpr int LPR = 64 / S;
     const int lane = threadIdx.x & 63;
@@ -287,7 +314,7 @@ __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice,
     const bool comb = jo == 0 && act;
     TV sv = 0, yv = 0, dg = 1;
     if (comb && mode != M_SPMV && !skip) sv = src[r];
-    if (comb && mode == M_JACOBI) {
+    if (comb && (mode == M_JACOBI || mode == M_RESJAC)) {
         yv = win[r - r0];   // the row's own previous value: own rows lead the window
         dg = wv(wp[(size_t)slice * S * m + ri]);
     }
@@ -332,6 +359,13 @@ __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice,
             y[r] = wf * ((sv - sum) / dg) + cf * yv;
             break;
         }
+        case M_RESJAC: {
+            TV all, off;
+            ordered_sum2(P, S, ri, m, &all, &off);
+            y[r] = wf * ((sv - off) / dg) + cf * yv;
+            y2[r] = bnd ? TV(0) : sv - all;
+            break;
+        }
         }
     }
     wave_sync();   // P is reused by the wave's next slice
@@ -347,7 +381,7 @@ __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice,
 template <int S, typename TW, typename TV>
 __global__ void __launch_bounds__(64 * kWindowWaves)
 window_rows_kernel(EllT<TW> A, int mode, const uint8_t *__restrict__ btag, const TV *__restrict__ src,
-                   const TV *__restrict__ x, TV *y, TV wf, TV cf)
+                   const TV *__restrict__ x, TV *y, TV *y2, TV wf, TV cf)
 {
     extern __shared__ unsigned char lds_raw[];
     constexpr int W = kWindowWaves;
@@ -363,7 +397,7 @@ window_rows_kernel(EllT<TW> A, int mode, const uint8_t *__restrict__ btag, const
     __syncthreads();
     const TW *wp = A.sorted ? A.w0 : A.w;
     for (int sl = r0 / S + wave; sl < r1 / S; sl += W)
-        window_slice<S>(A, wp, sl, mode, btag, src, win, r0, y, P, wf, cf);
+        window_slice<S>(A, wp, sl, mode, btag, src, win, r0, y, y2, P, wf, cf);
 }
 
 // Weights and columns of this lane's entries j = t*LPR + jo < MT of one
@@ -843,16 +877,16 @@ size_t window_lds(const EllT<TW> &A)
 }
 template <typename TW, typename TV>
 int launch_window(EllT<TW> A, int mode, const uint8_t *btag, const TV *src, const TV *x, TV *y, TV w, TV c,
-                  hipStream_t s)
+                  hipStream_t s, TV *y2 = nullptr)
 {
     const dim3 grid(A.win_blocks), block(64 * kWindowWaves);
     const size_t shm = window_lds<TW, TV>(A);
     if (A.S == 4)
-        hipLaunchKernelGGL((window_rows_kernel<4, TW, TV>), grid, block, shm, s, A, mode, btag, src, x, y, w, c);
+        hipLaunchKernelGGL((window_rows_kernel<4, TW, TV>), grid, block, shm, s, A, mode, btag, src, x, y, y2, w, c);
     else if (A.S == 8)
-        hipLaunchKernelGGL((window_rows_kernel<8, TW, TV>), grid, block, shm, s, A, mode, btag, src, x, y, w, c);
+        hipLaunchKernelGGL((window_rows_kernel<8, TW, TV>), grid, block, shm, s, A, mode, btag, src, x, y, y2, w, c);
     else if (A.S == 16)
-        hipLaunchKernelGGL((window_rows_kernel<16, TW, TV>), grid, block, shm, s, A, mode, btag, src, x, y, w, c);
+        hipLaunchKernelGGL((window_rows_kernel<16, TW, TV>), grid, block, shm, s, A, mode, btag, src, x, y, y2, w, c);
     else
         MGM_FAIL(-1, "unsupported slice height");
     HIP_TRY(hipGetLastError());
@@ -925,6 +959,15 @@ int launch_jacobi(EllT<TW> A, int N, const uint8_t *btag, const TV *src, const T
 {
     return launch_rows<TW, TV>(A, nullptr, 0, -1, N, M_JACOBI, btag, src, x, y, w, c, 0, s);
 }
+template <typename TW, typename TV>
+int launch_residual_jacobi(EllT<TW> A, const uint8_t *btag, const TV *src, const TV *x, TV *y, TV *res, TV w, TV c,
+                           hipStream_t s)
+{
+    if (!A.win_blocks || !btag || window_lds<TW, TV>(A) > kSmallLdsMax)
+        MGM_FAIL(-1, "launch_residual_jacobi: the level has no LDS windows");
+    return launch_window(A, M_RESJAC, btag, src, x, y, w, c, s, res);
+}
+bool has_windows(const Ell &A) { return A.win_blocks && window_lds<double, double>(A) <= kSmallLdsMax; }
 template <typename TW, typename TV>
 int launch_sor_slices(EllT<TW> A, const int *slices, int first, int nslices, const uint8_t *btag,
                       const TV *src, TV *p, TV w, TV c, hipStream_t s)
@@ -1118,6 +1161,8 @@ int launch_zero(int N, const uint8_t *btag, int interior_only, TV *p, hipStream_
                                          hipStream_t);                                               \
     template int launch_prolong<TW, TV>(EllT<TW>, int, const uint8_t *, const TV *, TV *, hipStream_t);
 MGM_INSTANTIATE(double, double)
+template int launch_residual_jacobi<double, double>(Ell, const uint8_t *, const double *, const double *, double *,
+                                                    double *, double, double, hipStream_t);
 MGM_INSTANTIATE(float, float)
 MGM_INSTANTIATE(bf16, float)
 #undef MGM_INSTANTIATE

@@ -387,3 +387,33 @@ def test_tune_smoother(case):
     finally:
         P.set_smoother_kinds(None)
         P.set_smoother(1.0, 3)
+
+
+@pytest.mark.parametrize("num_relax", [1, 2, 3])
+def test_fused_residual_jacobi_bitwise(case, monkeypatch, num_relax):
+    """With weighted Jacobi on level 0 the solve forms each fine residual and
+    the next cycle's first level-0 sweep in one window pass: residual history
+    and fields bit for bit those of separate passes, for 1, 2 and 3 sweeps
+    (the remaining sweeps end in p with no copy)."""
+    name, P = case
+    kinds, omega = JACOBI[0]
+    P.set_smoother(1.0, num_relax)
+    P.set_smoother_levels(list(omega))
+    P.set_smoother_kinds(list(kinds))
+    try:
+        _rows(P, True)
+        f = np.random.default_rng(19).standard_normal(P.info(0)["N"])
+        f[P.geometry(0)["btag"] != 0] = 0.0
+        out = []
+        for fuse in ("1", "0"):
+            monkeypatch.setenv("MGM_FUSE_RESIDUAL", fuse)
+            P.zero_fields()
+            P.set_field(0, "source", f)
+            r = P.solve(1e-300, 5)
+            out.append((np.asarray(r["history"]), P.field(0, "p"), P.field(1, "p")))
+        for a, b in zip(out[0], out[1]):
+            np.testing.assert_array_equal(a, b, err_msg=f"{name} num_relax {num_relax}")
+        assert out[0][0][-1] < out[0][0][0]
+    finally:
+        P.set_smoother_kinds(None)
+        P.set_smoother(1.0, 3)
