@@ -236,6 +236,10 @@ int launch_sor_slices(EllT<TW> A, const int *slices, int first, int nslices, con
 template <typename TW, typename TV>
 int launch_jacobi(EllT<TW> A, int N, const uint8_t *btag, const TV *src, const TV *x, TV *y, TV w, TV c,
                   hipStream_t s);
+// launch_jacobi for x = +0 on every row that is not padding (bitwise its
+// result): y = w * ((s - 0) / a_rr) + c * 0, no stencil read.
+template <typename TW, typename TV>
+int launch_jacobi_zero(EllT<TW> A, int N, const uint8_t *btag, const TV *src, TV *y, TV w, TV c, hipStream_t s);
 // One pass over a level with LDS windows (has_windows) forming, from the same
 // products, the residual form 0 of x into res (bitwise launch_residual's) and
 // the Jacobi sweep of x into y (bitwise launch_jacobi's).  FP64.

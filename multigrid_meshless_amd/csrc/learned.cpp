@@ -93,6 +93,8 @@ int prepare(mgm_problem *pb, LearnedState &S)
     if ((rc = to16(w3, MLP_H, S.w3))) return rc;
     HIP_TRY(S.b2.upload(b2, MLP_H, stream()));
     S.b3 = w3[MLP_H];
+    HIP_TRY(S.net.alloc(mlp_net_bytes()));
+    if ((rc = mlp_pack_net(m0, S.W1r.p, S.W2.p, S.b2.p, S.w3.p, S.net.p, stream()))) return rc;
     HIP_TRY(S.W.upload(S.w, stream()));
     S.G.clear();
     S.G.resize(pb->L);
@@ -212,7 +214,7 @@ int op_learned_sweep(mgm_problem *pb, int l)
                                  (const double *)L.f[MGM_F_P].p, L.f[MGM_F_RES].p, stream());
         if (rc) return rc;
     }
-    return mlp_correct(L.Nd, L.with_window(L.op_plain(OP_LAPP)), L.dbtag.p, S.G[l].p, S.W1r.p, S.W2.p, S.b2.p, S.w3.p, S.b3,
+    return mlp_correct(L.Nd, L.with_window(L.op_plain(OP_LAPP)), L.dbtag.p, S.G[l].p, S.net.p, S.b3,
                        L.f[MGM_F_RES].p, L.f[MGM_F_P].p, stream());
 }
 
@@ -231,7 +233,7 @@ int op_learned_sweep_mp(mgm_problem *pb, int l, int (*residual_mp)(Level &))
         if (rc) return rc;
         r = L.res32.p;
     }
-    return mlp_correct(L.Nd, L.with_window(L.op_plain(OP_LAPP)), L.dbtag.p, S.G[l].p, S.W1r.p, S.W2.p, S.b2.p, S.w3.p, S.b3,
+    return mlp_correct(L.Nd, L.with_window(L.op_plain(OP_LAPP)), L.dbtag.p, S.G[l].p, S.net.p, S.b3,
                        r, L.p32.p, stream());
 }
 

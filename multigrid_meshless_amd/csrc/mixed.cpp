@@ -65,7 +65,15 @@ int residual_mp(Level &L);
 
 // With the learned smoother active, its FP32 correction takes the place of
 // the first sweep.
-int relax_mp(mgm_problem *pb, int l)
+// from_zero: p32 is +0 on every row that is not padding (the descent of a
+// correction), so a first Jacobi sweep needs no stencil pass
+// (launch_jacobi_zero, bitwise the full pass; MGM_JACOBI_ZERO=0 runs the pass).
+static bool jacobi_zero_on()
+{
+    const char *e = getenv("MGM_JACOBI_ZERO");
+    return !(e && e[0] == '0');
+}
+int relax_mp(mgm_problem *pb, int l, bool from_zero)
 {
     Level &L = pb->lv[l];
     const float w = pb->omega(l);
@@ -80,7 +88,15 @@ int relax_mp(mgm_problem *pb, int l)
         return with_op(L, [&](auto A) {
             const float *in = L.p32.p;
             float *out = L.res32.p;
-            for (int it = it0; it < pb->cfg.num_relax; it++) {
+            int it = it0;
+            if (from_zero && it0 == 0 && pb->cfg.num_relax > 0 && jacobi_zero_on()) {
+                int rc = launch_jacobi_zero(A, L.Nd, L.dbtag.p, (const float *)L.src32.p, out, w, c, stream());
+                if (rc) return rc;
+                in = out;
+                out = L.p32.p;
+                it = 1;
+            }
+            for (; it < pb->cfg.num_relax; it++) {
                 int rc = launch_jacobi(A, L.Nd, L.dbtag.p, (const float *)L.src32.p, in, out, w, c, stream());
                 if (rc) return rc;
                 in = out;
@@ -253,13 +269,13 @@ int op_correction_mp(mgm_problem *pb)
         if ((rc = launch_zero(L.Nd, L.dbtag.p, 0, L.p32.p, stream()))) return rc;
     const int nl = pb->L;
     for (int l = 0; l < nl; l++) {
-        if ((rc = relax_mp(pb, l))) return rc;
+        if ((rc = relax_mp(pb, l, true))) return rc;   // every level's p32 was just zeroed
         if ((rc = residual_mp(pb->lv[l]))) return rc;
         if (l != nl - 1 && (rc = restrict_mp(pb->lv[l], pb->lv[l + 1]))) return rc;
     }
     for (int l = nl - 1; l > 0; l--) {
         if ((rc = prolong_mp(pb->lv[l - 1], pb->lv[l]))) return rc;
-        if (l != 1 && (rc = relax_mp(pb, l - 1))) return rc;
+        if (l != 1 && (rc = relax_mp(pb, l - 1, false))) return rc;
     }
     return launch_add_correction(L0.Nd, L0.dbtag.p, L0.p32.p, L0.f[MGM_F_P].p, stream());
 }

@@ -9,6 +9,7 @@
 //   [ (x_cj - x_i)/h for j < m, per axis d < dim ]  (m*dim values, h = level spacing)
 //   [ r_cj / sigma_i for j < m ]                     (m values, sigma_i = RMS of r over the stencil)
 // Output y; the correction is delta_i = y * sigma_i / a_ii.
+#include <algorithm>
 #include <cstdlib>
 
 #include "mlp.hpp"
@@ -323,7 +324,7 @@ constexpr int MF_KS1 = (MLP_MAX_M + 31) / 32;   // K steps of layer 1
 constexpr int MF_KS2 = MLP_H / 32;              // K steps of layer 2
 constexpr int MF_NT = MLP_H / 16;               // 16-wide column tiles of a hidden layer
 constexpr int MF_WAVES = BLK / 64;
-constexpr int MF_TPW = 4;                       // node tiles per wave
+constexpr int MF_TPW = 4;                       // most node tiles per wave
 constexpr int MF_COLS = MF_KS1 * 8;             // residual columns a lane holds per node
 
 __device__ inline __bf16 bits_bf(bf16 v) { return __builtin_bit_cast(__bf16, v.bits); }
@@ -337,17 +338,22 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The network's weights in LDS, B-operand layout (f1: residual columns of
-// W1, f2: W2), with b2 and BF16(w3).  Threads tid = 0 .. nthr-1 of the block
-// share the copy; the caller synchronises the block afterwards.
+// The network's weights as the correction reads them, B-operand layout (f1:
+// residual columns of W1, f2: W2), with b2 and BF16(w3).  Built once per
+// weight update in memory (pack_net_kernel); each correction block copies the
+// image into LDS with 16-byte loads (load_net).
 struct NetLds {
     bf16x8 f1[MF_KS1][MF_NT][64];
     bf16x8 f2[MF_KS2][MF_NT][64];
     float b2[MLP_H], w3[MLP_H];
 };
-__device__ inline void load_net(NetLds &n, int m, const bf16 *__restrict__ W1rT, const bf16 *__restrict__ W2,
-                                const float *__restrict__ b2, const bf16 *__restrict__ w3, int tid, int nthr)
+static_assert(sizeof(NetLds) % 16 == 0, "the image copies in 16-byte pieces");
+__global__ void __launch_bounds__(BLK)
+pack_net_kernel(NetLds *__restrict__ out, int m, const bf16 *__restrict__ W1rT, const bf16 *__restrict__ W2,
+                const float *__restrict__ b2, const bf16 *__restrict__ w3)
 {
+    NetLds &n = *out;
+    const int tid = threadIdx.x, nthr = BLK;
     for (int q = tid; q < MF_KS1 * MF_NT * 64; q += nthr) {
         const int l = q % 64, t = (q / 64) % MF_NT, ks = q / (64 * MF_NT);
         bf16x8 v;
@@ -372,6 +378,14 @@ __device__ inline void load_net(NetLds &n, int m, const bf16 *__restrict__ W1rT,
         n.b2[tid] = b2[tid];
         n.w3[tid] = bf(w3[tid]);
     }
+}
+// Threads tid = 0 .. nthr-1 of the block copy the packed image into LDS; the
+// caller synchronises the block afterwards.
+__device__ inline void load_net(NetLds &n, const NetLds *__restrict__ img, int tid, int nthr)
+{
+    const uint4 *s = reinterpret_cast<const uint4 *>(img);
+    uint4 *d = reinterpret_cast<uint4 *>(&n);
+    for (int q = tid; q < (int)(sizeof(NetLds) / 16); q += nthr) d[q] = s[q];
 }
 
 // The geometric part of layer 1 for the four rows n4 .. n4+3 (a multiple of
@@ -448,22 +462,32 @@ __device__ inline double stencil_sigma(double s2, int m)
 template <typename TV>
 __global__ void __launch_bounds__(BLK, 3)
 correct_kernel(int Nd, Ell A, const uint8_t *__restrict__ btag, const float *__restrict__ G,
-               const bf16 *__restrict__ W1rT, const bf16 *__restrict__ W2, const float *__restrict__ b2,
-               const bf16 *__restrict__ w3, float b3, const TV *__restrict__ res, TV *__restrict__ p)
+               const NetLds *__restrict__ img, float b3, const TV *__restrict__ res, TV *__restrict__ p, int tpw)
 {
     __shared__ NetLds net;
     __shared__ __attribute__((aligned(16))) __bf16 hb[MF_WAVES][16][MLP_H];
     __shared__ float yb[MF_WAVES][16];
     const int m = A.m;
-    load_net(net, m, W1rT, W2, b2, w3, threadIdx.x, BLK);
+    load_net(net, img, threadIdx.x, BLK);
     __syncthreads();
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, li = l % 16, g = l / 16;
     const long ntiles = ((long)Nd + 15) / 16;
-    for (int it = 0; it < MF_TPW; it++) {
-        const long tile = ((long)blockIdx.x * MF_WAVES + wave) * MF_TPW + it;
+    for (int it = 0; it < tpw; it++) {
+        const long tile = ((long)blockIdx.x * MF_WAVES + wave) * tpw + it;
         if (tile >= ntiles) break;
         const long nb = tile * 16, r = nb + li;
-        const bool valid = r < Nd && btag[r] == ROW_INTERIOR;
+        // No load waits on the row tag: rows that get no correction (boundary,
+        // padding: valid columns, weight 0) gather their stencils too, and
+        // since each row of the tile's products depends only on its own
+        // node's inputs, their values reach no other node's correction.
+        const bool inr = r < Nd;
+        const uint8_t tag = inr ? btag[r] : (uint8_t)ROW_PAD;
+        TV pold = 0;
+        double diag = 1;
+        if (g == 0 && inr) {
+            pold = p[r];
+            diag = A.w[ell_at(m, A.S, r, 0)];
+        }
         // this lane's residual columns of node li: j = ks*32 + g*8 + e
         double vq[MF_COLS];
         double s2 = 0;
@@ -473,10 +497,11 @@ correct_kernel(int Nd, Ell A, const uint8_t *__restrict__ btag, const float *__r
             for (int e = 0; e < 8; e++) {
                 const int j = ks * 32 + g * 8 + e;
                 double v = 0;
-                if (valid && j < m) v = (double)res[A.col[ell_at(m, A.S, r, j)]];
+                if (inr && j < m) v = (double)res[A.col[ell_at(m, A.S, r, j)]];
                 vq[ks * 8 + e] = v;
                 s2 += v * v;
             }
+        const bool valid = tag == ROW_INTERIOR;
         const double sigma = stencil_sigma(s2, m);
         const double inv = sigma > 0 ? 1.0 / sigma : 0.0;
         // Nd and nb are multiples of 4: the lane's four rows are all rows or all beyond Nd
@@ -488,10 +513,7 @@ correct_kernel(int Nd, Ell A, const uint8_t *__restrict__ btag, const float *__r
 #pragma unroll
             for (int e = 0; e < 8; e++) u[ks][e] = (__bf16)(float)(vq[ks * 8 + e] * inv);
         const float y = net_tile(c, u, net, hb[wave], yb[wave], l);
-        if (g == 0 && valid && sigma > 0) {
-            const double diag = A.w[ell_at(m, A.S, r, 0)];
-            p[r] = p[r] + (TV)((double)(y + b3) * sigma / diag);
-        }
+        if (g == 0 && valid && sigma > 0) p[r] = pold + (TV)((double)(y + b3) * sigma / diag);
         wave_sync();   // hb and yb are rewritten by the next tile
     }
 }
@@ -509,8 +531,7 @@ constexpr int CW_WAVES = 16;
 template <typename TV>
 __global__ void __launch_bounds__(64 * CW_WAVES)
 correct_window_kernel(Ell A, const uint8_t *__restrict__ btag, const float *__restrict__ G,
-                      const bf16 *__restrict__ W1rT, const bf16 *__restrict__ W2, const float *__restrict__ b2,
-                      const bf16 *__restrict__ w3, float b3, const TV *__restrict__ res, TV *__restrict__ p)
+                      const NetLds *__restrict__ img, float b3, const TV *__restrict__ res, TV *__restrict__ p)
 {
     __shared__ NetLds net;
     __shared__ __attribute__((aligned(16))) __bf16 hb[CW_WAVES][16][MLP_H];
@@ -524,12 +545,20 @@ correct_window_kernel(Ell A, const uint8_t *__restrict__ btag, const float *__re
     for (int i = threadIdx.x; i < nown; i += 64 * CW_WAVES) win[i] = res[r0 + i];
     for (int i = threadIdx.x; i < nh; i += 64 * CW_WAVES) win[nown + i] = res[A.win_halo[h0 + i]];
     if (threadIdx.x == 0) win[zero] = TV(0);
-    load_net(net, m, W1rT, W2, b2, w3, threadIdx.x, 64 * CW_WAVES);
+    load_net(net, img, threadIdx.x, 64 * CW_WAVES);
     __syncthreads();
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, li = l % 16, g = l / 16;
     for (int nb = r0 + 16 * wave; nb < r1; nb += 16 * CW_WAVES) {
         const int r = nb + li;
-        const bool valid = r < r1 && btag[r] == ROW_INTERIOR;
+        // no load waits on the row tag (as in correct_kernel)
+        const bool inr = r < r1;
+        const uint8_t tag = inr ? btag[r] : (uint8_t)ROW_PAD;
+        TV pold = 0;
+        double diag = 1;
+        if (g == 0 && inr) {
+            pold = p[r];
+            diag = A.w[ell_at(m, A.S, r, 0)];
+        }
         unsigned pl[MF_COLS / 2];   // two 16-bit places per register, column q in half q % 2
         double s2 = 0;
 #pragma unroll
@@ -537,9 +566,10 @@ correct_window_kernel(Ell A, const uint8_t *__restrict__ btag, const float *__re
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 const int j = ks * 32 + g * 8 + e, q = ks * 8 + e;
-                const unsigned v = (valid && j < m) ? (unsigned)A.win_col[ell_at(m, A.S, r, j)] : (unsigned)zero;
+                const unsigned v = (inr && j < m) ? (unsigned)A.win_col[ell_at(m, A.S, r, j)] : (unsigned)zero;
                 pl[q / 2] = (q % 2) ? (pl[q / 2] | v << 16) : v;
             }
+        const bool valid = tag == ROW_INTERIOR;
 #pragma unroll
         for (int q = 0; q < MF_COLS; q++) {
             const double v = (double)win[(pl[q / 2] >> (16 * (q % 2))) & 0xffff];
@@ -558,10 +588,7 @@ correct_window_kernel(Ell A, const uint8_t *__restrict__ btag, const float *__re
                 u[ks][e] = (__bf16)(float)((double)win[(pl[q / 2] >> (16 * (q % 2))) & 0xffff] * inv);
             }
         const float y = net_tile(c, u, net, hb[wave], yb[wave], l);
-        if (g == 0 && valid && sigma > 0) {
-            const double diag = A.w[ell_at(m, A.S, r, 0)];
-            p[r] = p[r] + (TV)((double)(y + b3) * sigma / diag);
-        }
+        if (g == 0 && valid && sigma > 0) p[r] = pold + (TV)((double)(y + b3) * sigma / diag);
         wave_sync();   // hb and yb are rewritten by the next tile
     }
 }
@@ -690,32 +717,47 @@ int mlp_device_init()
 
 size_t mlp_geometry_size(int Nd) { return (size_t)((Nd + 15) / 16) * 16 * MLP_H; }
 
+size_t mlp_net_bytes() { return sizeof(NetLds); }
+
+int mlp_pack_net(int m, const bf16 *W1rT, const bf16 *W2, const float *b2, const bf16 *w3, void *out, hipStream_t s)
+{
+    if (m > MLP_MAX_M) MGM_FAIL(-1, "learned smoother: stencil wider than supported");
+    hipLaunchKernelGGL(pack_net_kernel, dim3(1), dim3(BLK), 0, s, (NetLds *)out, m, W1rT, W2, b2, w3);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 template <typename TV>
-int mlp_correct(int Nd, Ell A, const uint8_t *btag, const float *G, const bf16 *W1rT, const bf16 *W2,
-                const float *b2, const bf16 *w3, float b3, const TV *res, TV *p, hipStream_t s)
+int mlp_correct(int Nd, Ell A, const uint8_t *btag, const float *G, const void *net, float b3, const TV *res, TV *p,
+                hipStream_t s)
 {
     if (A.m > MLP_MAX_M) MGM_FAIL(-1, "learned smoother: stencil wider than supported");
     if (A.sorted) MGM_FAIL(-1, "learned smoother: needs the plain slab");
     if (Nd % 4) MGM_FAIL(-1, "learned smoother: device rows must be a multiple of 4");
+    const NetLds *img = (const NetLds *)net;
     if (correct_window_fits<TV>(A) && window_learned_on()) {
         const size_t shm = ((size_t)A.win_max + 1) * sizeof(TV);
         hipLaunchKernelGGL(correct_window_kernel<TV>, dim3(A.win_blocks), dim3(64 * CW_WAVES), shm, s, A, btag, G,
-                           W1rT, W2, b2, w3, b3, res, p);
+                           img, b3, res, p);
         HIP_TRY(hipGetLastError());
         return 0;
     }
-    const long tiles = ((long)Nd + 15) / 16, waves = (tiles + MF_TPW - 1) / MF_TPW;
+    // Tiles per wave: one on small levels (every tile's latency chain runs at
+    // once), up to MF_TPW where the level has enough tiles to keep about
+    // 12k waves busy (each block copies the network image once).
+    const long tiles = ((long)Nd + 15) / 16;
+    const int tpw = (int)std::max(1L, std::min((long)MF_TPW, tiles / 12288));
+    const long waves = (tiles + tpw - 1) / tpw;
     const int blocks = (int)((waves + MF_WAVES - 1) / MF_WAVES);
     if (blocks > 0) {
-        hipLaunchKernelGGL(correct_kernel<TV>, dim3(blocks), dim3(BLK), 0, s, Nd, A, btag, G, W1rT, W2, b2, w3, b3,
-                           res, p);
+        hipLaunchKernelGGL(correct_kernel<TV>, dim3(blocks), dim3(BLK), 0, s, Nd, A, btag, G, img, b3, res, p, tpw);
         HIP_TRY(hipGetLastError());
     }
     return 0;
 }
-template int mlp_correct<double>(int, Ell, const uint8_t *, const float *, const bf16 *, const bf16 *,
-                                 const float *, const bf16 *, float, const double *, double *, hipStream_t);
-template int mlp_correct<float>(int, Ell, const uint8_t *, const float *, const bf16 *, const bf16 *,
-                                const float *, const bf16 *, float, const float *, float *, hipStream_t);
+template int mlp_correct<double>(int, Ell, const uint8_t *, const float *, const void *, float, const double *,
+                                 double *, hipStream_t);
+template int mlp_correct<float>(int, Ell, const uint8_t *, const float *, const void *, float, const float *, float *,
+                                hipStream_t);
 
 }  // namespace mgm

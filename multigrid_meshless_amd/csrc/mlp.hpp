@@ -36,12 +36,18 @@ int mlp_geometry(int Nd, Ell A, int dim, double h, const double *dx, const doubl
                  const uint8_t *btag, const float *W, float *G, hipStream_t s);
 // Raises the LDS limit of the window form of the correction (once per device).
 int mlp_device_init();
-// p += network correction on interior rows, from the residual res (W1rT: the
-// residual columns of W1 transposed, m x MLP_H); A supplies columns and the
-// FP64 diagonal, and its LDS-window tables, where set, select the window form.  BF16 matrix cores: BF16 weights and activations, FP32
-// accumulation.  Instantiated for FP64 and FP32 vectors.
+// The inference image of the network (mlp_net_bytes bytes): the BF16 weight
+// fragments in the matrix cores' operand layout, b2 and BF16(w3), built from
+// W1rT (the residual columns of W1 transposed, m x MLP_H), W2, b2 and w3.
+size_t mlp_net_bytes();
+int mlp_pack_net(int m, const bf16 *W1rT, const bf16 *W2, const float *b2, const bf16 *w3, void *out, hipStream_t s);
+// p += network correction on interior rows, from the residual res and the
+// packed network net; A supplies columns and the FP64 diagonal, and its
+// LDS-window tables, where set, select the window form.  BF16 matrix cores:
+// BF16 weights and activations, FP32 accumulation.  Instantiated for FP64 and
+// FP32 vectors.
 template <typename TV>
-int mlp_correct(int Nd, Ell A, const uint8_t *btag, const float *G, const bf16 *W1rT, const bf16 *W2,
-                const float *b2, const bf16 *w3, float b3, const TV *res, TV *p, hipStream_t s);
+int mlp_correct(int Nd, Ell A, const uint8_t *btag, const float *G, const void *net, float b3, const TV *res, TV *p,
+                hipStream_t s);
 
 }  // namespace mgm

@@ -138,6 +138,7 @@ struct LearnedState {
     bool prepared = false;            // inference copies and G match W
     DevBuf<bf16> W1r, W2, w3;            // W1r: residual columns of W1, [j][k]
     DevBuf<float> b2;
+    DevBuf<uint8_t> net;              // the correction's packed network image (mlp_pack_net)
     float b3 = 0;
     std::vector<DevBuf<float>> G;     // per level: Nd x MLP_H
     std::vector<DevBuf<double>> dx, dy, dz;

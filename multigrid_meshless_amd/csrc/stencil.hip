@@ -637,6 +637,24 @@ zero_kernel(int N, const uint8_t *__restrict__ btag, int interior_only, TV *p)
     if (!interior_only || !btag[r]) p[r] = 0;
 }
 
+// One weighted-Jacobi sweep from x = +0 on every row: each neighbour product
+// is a signed zero, so the ordered sum is +0, and y = w * ((s - 0) / a_rr) +
+// c * 0 -- the same operations in the same order as the full pass, hence its
+// result bit for bit -- on the rows it combines (all but padding), without
+// reading the stencils.
+template <typename TW, typename TV>
+__global__ void __launch_bounds__(BLK)
+jacobi_zero_kernel(EllT<TW> A, int N, const uint8_t *__restrict__ btag, const TV *__restrict__ src, TV *y, TV w,
+                   TV c)
+{
+    const int r = blockIdx.x * BLK + threadIdx.x;
+    if (r >= N || (btag && btag[r] == ROW_PAD)) return;
+    const TW *wp = A.sorted ? A.w0 : A.w;
+    const TV dg = wv(wp[ell_at(A.m, A.S, r, 0)]);
+    const TV sum = TV(0), yv = TV(0);
+    y[r] = w * ((src[r] - sum) / dg) + c * yv;
+}
+
 template <typename TV>
 __global__ void copy_scalar_kernel(const int *ref0, const TV *p, double *dst)
 {
@@ -960,6 +978,12 @@ int launch_jacobi(EllT<TW> A, int N, const uint8_t *btag, const TV *src, const T
     return launch_rows<TW, TV>(A, nullptr, 0, -1, N, M_JACOBI, btag, src, x, y, w, c, 0, s);
 }
 template <typename TW, typename TV>
+int launch_jacobi_zero(EllT<TW> A, int N, const uint8_t *btag, const TV *src, TV *y, TV w, TV c, hipStream_t s)
+{
+    LAUNCH((jacobi_zero_kernel<TW, TV>), N, A, N, btag, src, y, w, c);
+    return 0;
+}
+template <typename TW, typename TV>
 int launch_residual_jacobi(EllT<TW> A, const uint8_t *btag, const TV *src, const TV *x, TV *y, TV *res, TV w, TV c,
                            hipStream_t s)
 {
@@ -1155,6 +1179,8 @@ int launch_zero(int N, const uint8_t *btag, int interior_only, TV *p, hipStream_
                                            const TV *, TV *, TV, TV, hipStream_t);                   \
     template int launch_jacobi<TW, TV>(EllT<TW>, int, const uint8_t *, const TV *, const TV *, TV *, \
                                        TV, TV, hipStream_t);                                         \
+    template int launch_jacobi_zero<TW, TV>(EllT<TW>, int, const uint8_t *, const TV *, TV *, TV, TV, \
+                                            hipStream_t);                                            \
     template int launch_residual<TW, TV>(EllT<TW>, int, int, const uint8_t *, const TV *,            \
                                          const TV *, TV *, hipStream_t);                             \
     template int launch_restrict<TW, TV>(EllT<TW>, int, const uint8_t *, int, const TV *, TV *,      \

@@ -417,3 +417,32 @@ def test_fused_residual_jacobi_bitwise(case, monkeypatch, num_relax):
     finally:
         P.set_smoother_kinds(None)
         P.set_smoother(1.0, 3)
+
+
+def test_fp32_zero_start_jacobi_bitwise(case, monkeypatch):
+    """The FP32 inner cycle's first Jacobi sweep on each level starts from the
+    zeroed correction and runs without a stencil pass: residual history and
+    field bit for bit those of the full first pass (MGM_JACOBI_ZERO=0), with
+    FP32 and with BF16 coarse weights."""
+    name, P = case
+    kinds, omega = JACOBI[0]
+    P.set_smoother_levels(list(omega))
+    P.set_smoother_kinds(list(kinds))
+    try:
+        _rows(P, True)
+        f = np.random.default_rng(20).standard_normal(P.info(0)["N"])
+        f[P.geometry(0)["btag"] != 0] = 0.0
+        for prec in (1, 2):
+            out = []
+            for zero in ("1", "0"):
+                monkeypatch.setenv("MGM_JACOBI_ZERO", zero)
+                P.zero_fields()
+                P.set_field(0, "source", f)
+                r = P.solve(1e-300, 4, precision=prec)
+                out.append((np.asarray(r["history"]), P.field(0, "p")))
+            np.testing.assert_array_equal(out[0][0], out[1][0], err_msg=f"{name} precision {prec}")
+            np.testing.assert_array_equal(out[0][1], out[1][1], err_msg=f"{name} precision {prec}")
+            assert out[0][0][-1] < out[0][0][0]
+    finally:
+        P.set_smoother_kinds(None)
+        P.set_smoother(1.0, 3)
