@@ -87,20 +87,20 @@ int relax_mp(mgm_problem *pb, int l, bool from_zero)
     if (pb->jacobi(l, MGM_VAR_H))
         return with_op(L, [&](auto A) {
             const float *in = L.p32.p;
-            float *out = L.res32.p;
             int it = it0;
             if (from_zero && it0 == 0 && pb->cfg.num_relax > 0 && jacobi_zero_on()) {
-                int rc = launch_jacobi_zero(A, L.Nd, L.dbtag.p, (const float *)L.src32.p, out, w, c, stream());
+                int rc = launch_jacobi_zero(A, L.Nd, L.dbtag.p, (const float *)L.src32.p, L.res32.p, w, c, stream());
                 if (rc) return rc;
-                in = out;
-                out = L.p32.p;
+                in = L.res32.p;
                 it = 1;
             }
-            for (; it < pb->cfg.num_relax; it++) {
-                int rc = launch_jacobi(A, L.Nd, L.dbtag.p, (const float *)L.src32.p, in, out, w, c, stream());
+            const int n = pb->cfg.num_relax - it;
+            const bool from_p = in == L.p32.p;
+            float *const *seq = jacobi_targets(n, from_p, L.p32.p, L.res32.p, L.jtmp32.p);
+            for (int k = 0; k < n; k++) {
+                int rc = launch_jacobi(A, L.Nd, L.dbtag.p, (const float *)L.src32.p, in, seq[k], w, c, stream());
                 if (rc) return rc;
-                in = out;
-                out = out == L.res32.p ? L.p32.p : L.res32.p;
+                in = seq[k];
             }
             return in == L.p32.p ? 0 : launch_stream_copy((long)L.Nd * 4 / 16, in, L.p32.p, stream());
         });
@@ -246,7 +246,7 @@ int mp_prepare(mgm_problem *pb, int precision)
             if (L.gs && (rc = narrow(L.gslab[OP_LAPP], L.glapp32))) return rc;
             if (L.gsp && (rc = narrow(L.gpw, L.gpw32))) return rc;
         }
-        for (DevBuf<float> *f : {&L.p32, &L.src32, &L.res32}) {
+        for (DevBuf<float> *f : {&L.p32, &L.src32, &L.res32, &L.jtmp32}) {
             if (f->n != (size_t)L.Nd) {
                 HIP_TRY(f->alloc(L.Nd));
                 HIP_TRY(hipMemsetAsync(f->p, 0, sizeof(float) * L.Nd, stream()));

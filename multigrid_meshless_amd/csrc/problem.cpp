@@ -620,6 +620,7 @@ static int upload_level(Level &L)
         HIP_TRY(f.alloc(Nd));
         HIP_TRY(hipMemsetAsync(f.p, 0, sizeof(double) * Nd, g_stream));
     }
+    HIP_TRY(L.jtmp.alloc(Nd));   // allocated here: the sweeps may run under graph capture
     HIP_TRY(hipStreamSynchronize(g_stream));  // host staging vectors die here
     return 0;
 }
@@ -1486,6 +1487,30 @@ bool learned_active(const mgm_problem *pb, int l, int variant);
 
 static int pfield(int variant) { return variant == MGM_VAR_T ? MGM_F_PPRIME : MGM_F_P; }
 
+// Output buffers of n Jacobi sweeps (each reads the previous one's output):
+// p, res (free until the residual that follows the relaxation) and a third
+// buffer t.  The last sweep writes p where that takes no copy: from p (from_p)
+// an even count alternates res / p; an odd count alternates, then ends res ->
+// t -> p; a start outside p alternates so that the last lands in p.  Only
+// from p with n = 1 does the result end outside p (in res; the caller copies).
+template <typename T>
+T *const *jacobi_targets(int n, bool from_p, T *p, T *res, T *t)
+{
+    static thread_local T *seq[64];
+    for (int k = 0; k < n && k < 64; k++) {
+        const int left = n - k;   // sweeps still to write, this one included
+        if (from_p && n % 2 == 1 && n >= 3 && left <= 2)
+            seq[k] = left == 2 ? t : p;
+        else if (from_p)
+            seq[k] = (k % 2 == 0) ? res : p;
+        else
+            seq[k] = (left % 2 == 1) ? p : res;
+    }
+    return seq;
+}
+template double *const *jacobi_targets<double>(int, bool, double *, double *, double *);
+template float *const *jacobi_targets<float>(int, bool, float *, float *, float *);
+
 int op_relax(mgm_problem *pb, int l, int variant)
 {
     Level &L = pb->lv[l];
@@ -1508,18 +1533,16 @@ int op_relax(mgm_problem *pb, int l, int variant)
         // formed the first sweep (jac0), the rest alternate so that the last
         // lands in p.
         const double *in = p;
-        double *out = res;
         int it = 0;
         if (l == 0 && pb->jac0_ready && nrelax > 0) {
             in = pb->jac0.p;
-            out = (nrelax - 1) % 2 == 0 ? res : p;
             it = 1;
         }
-        for (; it < nrelax; it++) {
-            int rc = launch_jacobi(A, L.Nd, L.dbtag.p, src, in, out, w, c, g_stream);
+        double *const *seq = jacobi_targets(nrelax - it, in == p, p, res, L.jtmp.p);
+        for (int k = 0; it < nrelax; it++, k++) {
+            int rc = launch_jacobi(A, L.Nd, L.dbtag.p, src, in, seq[k], w, c, g_stream);
             if (rc) return rc;
-            in = out;
-            out = out == res ? p : res;
+            in = seq[k];
         }
         return in == p ? 0 : launch_stream_copy((long)L.Nd * 8 / 16, in, p, g_stream);
     }

@@ -71,6 +71,8 @@ struct Level {
     DevBuf<float> lapp32, rw32, pw32;
     DevBuf<bf16> lapp16, rw16, pw16;
     DevBuf<float> p32, src32, res32;
+    DevBuf<double> jtmp;                       // Jacobi sweeps: third buffer, so an odd count ends in p
+    DevBuf<float> jtmp32;
     DevBuf<float> lap32, srcl32;               // time steps: FP32 lap; level 0's source for lap residuals
     DevBuf<float> glapp32, glap32, gpw32;      // gather-sorted narrowed copies (levels with gs / gsp)
     DevBuf<bf16> glapp16, gpw16;
@@ -127,6 +129,10 @@ struct Level {
 }  // namespace mgm
 
 namespace mgm {
+// Output buffers of n chained Jacobi sweeps over p / res / a third buffer t,
+// ending in p without a copy wherever possible (problem.cpp).
+template <typename T>
+T *const *jacobi_targets(int n, bool from_p, T *p, T *res, T *t);
 // Learned smoother state: FP32 master weights, BF16 inference copies, the
 // per-level geometric part of layer 1 and device-order coordinates.
 struct LearnedState {

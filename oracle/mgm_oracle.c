@@ -818,13 +818,36 @@ static void jacobi_sweep(orc_level *L, float omega)
     memcpy(L->p, L->res, sizeof(double) * (size_t)N);
 }
 
+/* n weighted-Jacobi sweeps of a level (no reference counterpart).  The GPU
+ * stages the sweeps in the level's residual buffer and one more buffer, so
+ * that the last lands in p without a copy; res is left holding the output of
+ * the last sweep staged there -- sweep n - 2 of an odd count n >= 3, sweep
+ * n - 1 of an even count, the only sweep of n = 1 -- and so is it here. */
+static void jacobi_relax(orc_level *L, float omega, int n)
+{
+    const int keep = n == 1 ? 1 : (n % 2 == 0 ? n - 1 : n - 2);
+    const size_t bytes = sizeof(double) * (size_t)L->N;
+    double *kept = NULL;
+    for (int it = 1; it <= n; it++) {
+        jacobi_sweep(L, omega);
+        if (it == keep && it != n) {
+            kept = (double *)malloc(bytes);
+            memcpy(kept, L->res, bytes);
+        }
+    }
+    if (kept) {
+        memcpy(L->res, kept, bytes);
+        free(kept);
+    }
+}
+
 /* One V-cycle of the H variant with a smoother kind (0 SOR, 1 Jacobi) and a
  * factor per level (no reference counterpart; with SOR everywhere this is
  * orc_vcycle_levels). */
 void orc_vcycle_smooth(int nlev, orc_level *lv, const float *omega, const int *kind, int num_relax)
 {
     for (int l = 0; l < nlev; l++) {
-        if (kind[l]) for (int it = 0; it < num_relax; it++) jacobi_sweep(&lv[l], omega[l]);
+        if (kind[l]) jacobi_relax(&lv[l], omega[l], num_relax);
         else orc_relax(ORC_VAR_H, &lv[l], omega[l], num_relax);
         orc_residual(ORC_VAR_H, &lv[l]);
         if (l != nlev - 1) orc_restrict(ORC_VAR_H, &lv[l], &lv[l + 1]);
@@ -832,7 +855,7 @@ void orc_vcycle_smooth(int nlev, orc_level *lv, const float *omega, const int *k
     for (int l = nlev - 1; l > 0; l--) {
         orc_prolong(ORC_VAR_H, &lv[l - 1], &lv[l]);
         if (l != 1) {
-            if (kind[l - 1]) for (int it = 0; it < num_relax; it++) jacobi_sweep(&lv[l - 1], omega[l - 1]);
+            if (kind[l - 1]) jacobi_relax(&lv[l - 1], omega[l - 1], num_relax);
             else orc_relax(ORC_VAR_H, &lv[l - 1], omega[l - 1], num_relax);
         }
     }

@@ -389,7 +389,7 @@ def test_tune_smoother(case):
         P.set_smoother(1.0, 3)
 
 
-@pytest.mark.parametrize("num_relax", [1, 2, 3])
+@pytest.mark.parametrize("num_relax", [1, 2, 3, 5])
 def test_fused_residual_jacobi_bitwise(case, monkeypatch, num_relax):
     """With weighted Jacobi on level 0 the solve forms each fine residual and
     the next cycle's first level-0 sweep in one window pass: residual history
