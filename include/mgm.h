@@ -85,6 +85,8 @@ typedef struct {
     int small_relax;               /* relaxes in one workgroup (at the current MGM_SMALL_ROWS) */
     int window_blocks;             /* full passes read x from LDS windows of this many blocks (0: none) */
     int window_max;                /* largest window (values); places are stored as 16-bit */
+    int prolong_window_max;        /* the prolongation into this level reads LDS windows of at most this
+                                      many coarse values (0: it gathers from memory) */
 } mgm_level_info;
 
 typedef struct mgm_points mgm_points;

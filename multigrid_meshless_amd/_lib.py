@@ -26,7 +26,8 @@ class LevelInfo(C.Structure):
                 ("poly_degree", C.c_int), ("num_colours", C.c_int), ("d_avg", C.c_double),
                 ("d_avg_bbox", C.c_double), ("knn_ties", C.c_int), ("device_rows", C.c_int),
                 ("slice_height", C.c_int), ("blocked", C.c_int), ("gather_sorted", C.c_int),
-                ("small_relax", C.c_int), ("window_blocks", C.c_int), ("window_max", C.c_int)]
+                ("small_relax", C.c_int), ("window_blocks", C.c_int), ("window_max", C.c_int),
+                ("prolong_window_max", C.c_int)]
 
 
 class NsParams(C.Structure):

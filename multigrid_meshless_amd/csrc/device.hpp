@@ -180,6 +180,9 @@ struct EllT {
     const uint16_t *win_col = nullptr;
     const int *win_row0 = nullptr, *win_hoff = nullptr, *win_halo = nullptr;
     int win_blocks = 0, win_max = 0;
+    // 0: the window holds only the listed rows (win_halo), no own rows -- a
+    // transfer operator whose columns index another level
+    int win_self = 1;
 };
 // Waves per workgroup of the LDS-window full passes (one workgroup per block).
 constexpr int kWindowWaves = 16;

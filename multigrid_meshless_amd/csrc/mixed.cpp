@@ -35,8 +35,10 @@ template <typename T>
 EllT<T> narrowed_prol(const Level &F, const DevBuf<T> &plain, const DevBuf<T> &sorted)
 {
     const bool st = stream_slab(F.Nd, F.mp, sizeof(T));
+    // (no LDS window: at 4- and 2-byte weights the window pass is bound by its
+    // instructions, not bytes, and measured no faster than the gathering one)
     if (F.gsp && sorted.p) return EllT<T>{F.gpcol.p, sorted.p, F.mp, F.Sp, st, plain.p, true,
-                                                    32 - slot_bits(F.Sp, F.mp)};
+                                          32 - slot_bits(F.Sp, F.mp)};
     return EllT<T>{F.pcol.p, plain.p, F.mp, F.Sp, st};
 }
 

@@ -920,6 +920,74 @@ static int level_window(Level &L)
     return 0;
 }
 
+bool prol_window_on()
+{
+    const char *e = getenv("MGM_WINDOW_PROLONG");
+    return window_on() && !(e && e[0] == '0');
+}
+
+// The prolongation windows of fine level L (Morton blocks with windows) from
+// coarse level C: per block the sorted, distinct coarse device rows that its
+// prolongation rows reference (row d of L reads C.iperm[C.cloud[L.pp[i] * mc + j]],
+// i = L.perm[d], as the transfer slab composes them), and the plain slab's
+// columns as 16-bit places in that list (padding rows: place 0).
+static int level_prol_window(Level &L, const Level &C)
+{
+    L.pwin_max = 0;
+    if (!L.win_blocks || !prol_window_on()) return 0;
+    const int nb = L.win_blocks, mc = L.mp, S = L.Sp;
+    // the window pass walks slices of S rows from block start to block end
+    for (int b = 0; b <= nb; b++)
+        if (L.blk_row0[b] % S) return 0;
+    std::vector<std::vector<int>> lists(nb);
+    std::vector<uint16_t> wcol(ell_size(L.Nd, mc, S), 0);
+    std::vector<int> wsize(nb, 0);
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&, t] {
+            for (int b = t; b < nb; b += nt) {
+                const int r0 = L.blk_row0[b], r1 = L.blk_row0[b + 1];
+                std::vector<int> &h = lists[b];
+                for (int d = r0; d < r1; d++) {
+                    const int i = L.perm[d];
+                    if (i < 0) continue;
+                    const size_t k = (size_t)L.pp[i] * mc;
+                    for (int j = 0; j < mc; j++) h.push_back(C.iperm[C.cloud[k + j]]);
+                }
+                std::sort(h.begin(), h.end());
+                h.erase(std::unique(h.begin(), h.end()), h.end());
+                if (h.size() > 65535) { wsize[b] = (int)h.size(); continue; }
+                for (int d = r0; d < r1; d++) {
+                    const int i = L.perm[d];
+                    if (i < 0) continue;
+                    const size_t k = (size_t)L.pp[i] * mc;
+                    for (int j = 0; j < mc; j++) {
+                        const int c = C.iperm[C.cloud[k + j]];
+                        wcol[ell_at(mc, S, d, j)] =
+                            (uint16_t)(std::lower_bound(h.begin(), h.end(), c) - h.begin());
+                    }
+                }
+                wsize[b] = (int)h.size();
+            }
+        });
+    for (auto &x : th) x.join();
+    const int wmax = *std::max_element(wsize.begin(), wsize.end());
+    if (wmax > 65535 || (size_t)kWindowWaves * mc * S * 8 + (size_t)wmax * 8 > kWindowLds) return 0;
+    std::vector<int> hoff(nb + 1, 0), hall;
+    for (int b = 0; b < nb; b++) {
+        hoff[b + 1] = hoff[b] + (int)lists[b].size();
+        hall.insert(hall.end(), lists[b].begin(), lists[b].end());
+    }
+    if (hall.empty()) hall.push_back(0);
+    HIP_TRY(L.pwin_col.upload(wcol, g_stream));
+    HIP_TRY(L.pwin_hoff.upload(hoff, g_stream));
+    HIP_TRY(L.pwin_halo.upload(hall, g_stream));
+    HIP_TRY(hipStreamSynchronize(g_stream));   // host tables die here
+    L.pwin_max = std::max(wmax, 1);
+    return 0;
+}
+
 // (Re)build every slab of level l: stencil columns, the five operators and
 // the transfer rows living on this level.  Padding rows hold weight 0 and
 // column 0.
@@ -982,6 +1050,10 @@ static int level_slabs(mgm_problem *pb, int l)
                                          g_stream)))
                 return rc;
             if ((rc = launch_permute_slab((long)n, L.mp, L.Sp, L.gpcol.p, L.pw.p, L.gpw.p, g_stream))) return rc;
+        }
+        if (!L.pwin_tried) {
+            L.pwin_tried = true;
+            if ((rc = level_prol_window(L, C))) return rc;
         }
     }
     return 0;
@@ -1227,6 +1299,7 @@ int mgm_level_info_get(const mgm_problem *pb, int lev, mgm_level_info *o)
     o->small_relax = relax_small_here(pb, L);
     o->window_blocks = L.win_blocks;
     o->window_max = L.win_max;
+    o->prolong_window_max = L.pwin_max;
     return 0;
 }
 

@@ -13,6 +13,9 @@ struct mgm_points {
 
 namespace mgm {
 
+// MGM_WINDOW_PROLONG=0: prolongations gather from memory on blocked levels too.
+bool prol_window_on();
+
 enum { OP_DX = 0, OP_DY = 1, OP_DZ = 2, OP_LAP = 3, OP_LAPP = 4, NUM_OPS = 5 };
 
 struct Level {
@@ -89,6 +92,22 @@ struct Level {
     DevBuf<int> win_row0, win_hoff, win_halo;
     int win_blocks = 0, win_max = 0;           // 0 blocks: no windows
     bool win_tried = false;                    // tables built (or found not to fit) once
+    // the prolongation into this level in the window form: per block the
+    // sorted coarse rows its prolongation rows reference, and the plain
+    // prolongation slab's columns as 16-bit places in that list
+    DevBuf<uint16_t> pwin_col;
+    DevBuf<int> pwin_hoff, pwin_halo;
+    int pwin_max = 0;                          // 0: no prolongation windows
+    bool pwin_tried = false;
+    template <typename T>
+    EllT<T> with_prol_window(EllT<T> A) const
+    {
+        if (pwin_max && prol_window_on()) {
+            A.win_col = pwin_col.p; A.win_row0 = win_row0.p; A.win_hoff = pwin_hoff.p; A.win_halo = pwin_halo.p;
+            A.win_blocks = win_blocks; A.win_max = pwin_max; A.win_self = 0;
+        }
+        return A;
+    }
     template <typename T>
     EllT<T> with_window(EllT<T> A) const
     {
@@ -121,8 +140,9 @@ struct Level {
     Ell prol() const
     {
         if (gsp)
-            return Ell{gpcol.p, gpw.p, mp, Sp, stream_slab(Nd, mp, 8), pw.p, true, 32 - slot_bits(Sp, mp)};
-        return Ell{pcol.p, pw.p, mp, Sp, stream_slab(Nd, mp, 8)};
+            return with_prol_window(
+                Ell{gpcol.p, gpw.p, mp, Sp, stream_slab(Nd, mp, 8), pw.p, true, 32 - slot_bits(Sp, mp)});
+        return with_prol_window(Ell{pcol.p, pw.p, mp, Sp, stream_slab(Nd, mp, 8)});
     }
 };
 

@@ -313,7 +313,8 @@ __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice,
     const bool skip = bnd && skips_boundary(mode);
     const bool comb = jo == 0 && act;
     TV sv = 0, yv = 0, dg = 1;
-    if (comb && mode != M_SPMV && !skip) sv = src[r];
+    if (comb && mode != M_SPMV && mode != M_PROLONG && !skip) sv = src[r];
+    if (comb && mode == M_PROLONG && !bnd) yv = y[r];
     if (comb && (mode == M_JACOBI || mode == M_RESJAC)) {
         yv = win[r - r0];   // the row's own previous value: own rows lead the window
         dg = wv(wp[(size_t)slice * S * m + ri]);
@@ -366,6 +367,10 @@ __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice,
             y2[r] = bnd ? TV(0) : sv - all;
             break;
         }
+        case M_PROLONG:
+            if (bnd) break;
+            y[r] = yv + ordered_sum<false>(P, S, ri, 0, m, TV(0));
+            break;
         }
     }
     wave_sync();   // P is reused by the wave's next slice
@@ -391,7 +396,7 @@ window_rows_kernel(EllT<TW> A, int mode, const uint8_t *__restrict__ btag, const
     TV *win = reinterpret_cast<TV *>(lds_raw) + (size_t)W * m * S;
     const int b = (int)xcd_block(blockIdx.x, gridDim.x);
     const int r0 = A.win_row0[b], r1 = A.win_row0[b + 1];
-    const int h0 = A.win_hoff[b], nh = A.win_hoff[b + 1] - h0, nown = r1 - r0;
+    const int h0 = A.win_hoff[b], nh = A.win_hoff[b + 1] - h0, nown = A.win_self ? r1 - r0 : 0;
     for (int i = threadIdx.x; i < nown; i += 64 * W) win[i] = x[r0 + i];
     for (int i = threadIdx.x; i < nh; i += 64 * W) win[nown + i] = x[A.win_halo[h0 + i]];
     __syncthreads();
@@ -919,7 +924,7 @@ int launch_rows(EllT<TW> A, const int *slices, int first, int nslices, int nrows
     // full SpMV and residual passes over a level with LDS windows
     if (A.win_blocks && btag && !slices && first == 0 && nslices < 0 &&
         (mode == M_SPMV || mode == M_RES0 || mode == M_RES1 || mode == M_RES2 || mode == M_RES3 ||
-         mode == M_JACOBI) &&
+         mode == M_JACOBI || mode == M_PROLONG) &&
         window_lds<TW, TV>(A) <= kSmallLdsMax)
         return launch_window(A, mode, btag, src, x, y, w, c, s);
     if (nslices < 0) nslices = (nrows + A.S - 1) / A.S;  // full pass

@@ -446,3 +446,33 @@ def test_fp32_zero_start_jacobi_bitwise(case, monkeypatch):
     finally:
         P.set_smoother_kinds(None)
         P.set_smoother(1.0, 3)
+
+
+def test_window_prolongation_bitwise(case, monkeypatch):
+    """The prolongation into the blocked level 0 in the LDS-window form (the
+    coarse values its block references staged once, 16-bit places) equals the
+    gathering form bit for bit: residual history and field of solves in FP64,
+    FP32 and FP32 + BF16 coarse weights."""
+    name, P = case
+    assert P.info(0)["prolong_window_max"] > 0 and P.info(1)["prolong_window_max"] == 0
+    kinds, omega = JACOBI[0]
+    P.set_smoother_levels(list(omega))
+    P.set_smoother_kinds(list(kinds))
+    try:
+        _rows(P, True)
+        f = np.random.default_rng(21).standard_normal(P.info(0)["N"])
+        f[P.geometry(0)["btag"] != 0] = 0.0
+        for prec in (0, 1, 2):
+            out = []
+            for win in ("1", "0"):
+                monkeypatch.setenv("MGM_WINDOW_PROLONG", win)
+                P.zero_fields()
+                P.set_field(0, "source", f)
+                r = P.solve(1e-300, 4, precision=prec)
+                out.append((np.asarray(r["history"]), P.field(0, "p")))
+            np.testing.assert_array_equal(out[0][0], out[1][0], err_msg=f"{name} precision {prec}")
+            np.testing.assert_array_equal(out[0][1], out[1][1], err_msg=f"{name} precision {prec}")
+            assert out[0][0][-1] < out[0][0][0]
+    finally:
+        P.set_smoother_kinds(None)
+        P.set_smoother(1.0, 3)
