@@ -29,10 +29,13 @@ pytestmark = pytest.mark.gpu
 SIZES = (24, 12, 7)
 BENCH_LAYOUT = {"MGM_BLOCKED_ROWS": "8000", "MGM_GATHER_SORT_ROWS": "8000", "MGM_SMALL_ROWS": "1000"}
 PER_COLOUR = dict(BENCH_LAYOUT, MGM_SMALL_ROWS="0")
+# level 1 in small Morton blocks as well (the 1M cube's 125k-row level 1: 512-row blocks)
+MID_BLOCKED = dict(BENCH_LAYOUT, MGM_MID_BLOCKED_ROWS="1000", MGM_MID_BLOCK_ROWS="256")
 # id: (polynomial degree, multiplier, layout)
 CASES = {
     "m70_bench_layout": (3, 3.5, BENCH_LAYOUT),
     "m70_per_colour": (3, 3.5, PER_COLOUR),
+    "m70_mid_blocked": (3, 3.5, MID_BLOCKED),
     "m35_slice16": (2, 3.5, BENCH_LAYOUT),
     "m40_slice8": (2, 4.0, BENCH_LAYOUT),
 }
@@ -86,10 +89,12 @@ def test_layout_is_the_benchmarks(case):
     name, P = case
     i0, i1, i2 = (P.info(k) for k in range(3))
     assert i0["blocked"] and i0["gather_sorted"] and not i0["small_relax"]
-    assert not i1["blocked"] and not i1["gather_sorted"] and not i1["small_relax"]
+    mid = "MGM_MID_BLOCKED_ROWS" in CASES[name][2]
+    assert i1["blocked"] == mid and (i1["window_blocks"] > 1) == mid
+    assert not i1["gather_sorted"] and not i1["small_relax"]
     m = i0["m"]
     if m > 36:
-        assert (i0["slice_height"], i1["slice_height"]) == (8, 4)
+        assert (i0["slice_height"], i1["slice_height"]) == (8, 8 if mid else 4)
     else:
         assert (i0["slice_height"], i1["slice_height"]) == (16, 16)
     assert i2["small_relax"] == (CASES[name][2]["MGM_SMALL_ROWS"] != "0")
@@ -454,7 +459,9 @@ def test_window_prolongation_bitwise(case, monkeypatch):
     gathering form bit for bit: residual history and field of solves in FP64,
     FP32 and FP32 + BF16 coarse weights."""
     name, P = case
-    assert P.info(0)["prolong_window_max"] > 0 and P.info(1)["prolong_window_max"] == 0
+    assert P.info(0)["prolong_window_max"] > 0
+    if not P.info(1)["blocked"]:
+        assert P.info(1)["prolong_window_max"] == 0
     kinds, omega = JACOBI[0]
     P.set_smoother_levels(list(omega))
     P.set_smoother_kinds(list(kinds))
