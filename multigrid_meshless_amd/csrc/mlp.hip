@@ -528,7 +528,7 @@ correct_kernel(int Nd, Ell A, const uint8_t *__restrict__ btag, const float *__r
 // first row (a multiple of 4).  Same operations in the same order as
 // correct_kernel, so the corrections are bitwise the same.
 constexpr int CW_WAVES = 16;
-template <typename TV>
+template <int S, typename TV>
 __global__ void __launch_bounds__(64 * CW_WAVES)
 correct_window_kernel(Ell A, const uint8_t *__restrict__ btag, const float *__restrict__ G,
                       const NetLds *__restrict__ img, float b3, const TV *__restrict__ res, TV *__restrict__ p)
@@ -553,11 +553,15 @@ correct_window_kernel(Ell A, const uint8_t *__restrict__ btag, const float *__re
         // no load waits on the row tag (as in correct_kernel)
         const bool inr = r < r1;
         const uint8_t tag = inr ? btag[r] : (uint8_t)ROW_PAD;
+        // entry j of row r at base + j * S; this lane's columns j = ks*32 + g*8 + e
+        // sit at compile-time offsets from cp
+        const size_t base = (size_t)(r / S) * S * m + (size_t)(r % S);
+        const uint16_t *cp = A.win_col + base + (size_t)g * 8 * S;
         TV pold = 0;
         double diag = 1;
         if (g == 0 && inr) {
             pold = p[r];
-            diag = A.w[ell_at(m, A.S, r, 0)];
+            diag = A.w[base];
         }
         unsigned pl[MF_COLS / 2];   // two 16-bit places per register, column q in half q % 2
         double s2 = 0;
@@ -566,7 +570,7 @@ correct_window_kernel(Ell A, const uint8_t *__restrict__ btag, const float *__re
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 const int j = ks * 32 + g * 8 + e, q = ks * 8 + e;
-                const unsigned v = (inr && j < m) ? (unsigned)A.win_col[ell_at(m, A.S, r, j)] : (unsigned)zero;
+                const unsigned v = (inr && j < m) ? (unsigned)cp[(ks * 32 + e) * S] : (unsigned)zero;
                 pl[q / 2] = (q % 2) ? (pl[q / 2] | v << 16) : v;
             }
         const bool valid = tag == ROW_INTERIOR;
@@ -694,7 +698,7 @@ static size_t correct_window_static()
 template <typename TV>
 static bool correct_window_fits(const Ell &A)
 {
-    return A.win_blocks > 0 && A.S % 4 == 0 &&
+    return A.win_blocks > 0 && (A.S == 4 || A.S == 8 || A.S == 16) &&
            correct_window_static<TV>() + ((size_t)A.win_max + 1) * sizeof(TV) <= kLdsPerCu;
 }
 // MGM_LEARNED_WINDOW=0 keeps the gathering form on every level.
@@ -706,12 +710,14 @@ static bool window_learned_on()
 
 int mlp_device_init()
 {
-    HIP_TRY(hipFuncSetAttribute((const void *)correct_window_kernel<double>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)(kLdsPerCu - correct_window_static<double>())));
-    HIP_TRY(hipFuncSetAttribute((const void *)correct_window_kernel<float>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)(kLdsPerCu - correct_window_static<float>())));
+    const int dbl = (int)(kLdsPerCu - correct_window_static<double>());
+    const int flt = (int)(kLdsPerCu - correct_window_static<float>());
+    for (const void *f : {(const void *)correct_window_kernel<4, double>, (const void *)correct_window_kernel<8, double>,
+                          (const void *)correct_window_kernel<16, double>})
+        HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dbl));
+    for (const void *f : {(const void *)correct_window_kernel<4, float>, (const void *)correct_window_kernel<8, float>,
+                          (const void *)correct_window_kernel<16, float>})
+        HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, flt));
     return 0;
 }
 
@@ -737,8 +743,13 @@ int mlp_correct(int Nd, Ell A, const uint8_t *btag, const float *G, const void *
     const NetLds *img = (const NetLds *)net;
     if (correct_window_fits<TV>(A) && window_learned_on()) {
         const size_t shm = ((size_t)A.win_max + 1) * sizeof(TV);
-        hipLaunchKernelGGL(correct_window_kernel<TV>, dim3(A.win_blocks), dim3(64 * CW_WAVES), shm, s, A, btag, G,
-                           img, b3, res, p);
+        const dim3 grid(A.win_blocks), block(64 * CW_WAVES);
+        if (A.S == 4)
+            hipLaunchKernelGGL((correct_window_kernel<4, TV>), grid, block, shm, s, A, btag, G, img, b3, res, p);
+        else if (A.S == 8)
+            hipLaunchKernelGGL((correct_window_kernel<8, TV>), grid, block, shm, s, A, btag, G, img, b3, res, p);
+        else
+            hipLaunchKernelGGL((correct_window_kernel<16, TV>), grid, block, shm, s, A, btag, G, img, b3, res, p);
         HIP_TRY(hipGetLastError());
         return 0;
     }
