@@ -204,6 +204,10 @@ inline bool slab_sortable(int S, int m, long ncols)
 constexpr size_t kStreamSlabBytes = 256ull << 20;
 inline bool stream_slab(size_t rows, int m, size_t wbytes) { return rows * m * (wbytes + 4) > kStreamSlabBytes; }
 using Ell = EllT<double>;
+// Row stride of the products the slice kernels stage in LDS: a slice's row ri
+// holds its m products at [ri * prod_stride(m) + j], padded so a row starts on
+// 16 bytes and its sum reads four FP32 or two FP64 products per load.
+__host__ __device__ constexpr int prod_stride(int m) { return (m + 3) & ~3; }
 __host__ __device__ inline size_t ell_at(int m, int S, int r, int j)
 {
     return (size_t)(r / S) * (size_t)S * (size_t)m + (size_t)j * S + (size_t)(r % S);

@@ -923,7 +923,7 @@ static int level_window(Level &L)
         });
     for (auto &x : th) x.join();
     const int wmax = *std::max_element(wsize.begin(), wsize.end());
-    if ((size_t)kWindowWaves * m * S * 8 + (size_t)wmax * 8 > kWindowLds) return 0;
+    if ((size_t)kWindowWaves * prod_stride(m) * S * 8 + (size_t)wmax * 8 > kWindowLds) return 0;
     std::vector<int> hoff(nb + 1, 0), hall;
     for (int b = 0; b < nb; b++) {
         hoff[b + 1] = hoff[b] + (int)halo[b].size();
@@ -993,7 +993,7 @@ static int level_prol_window(Level &L, const Level &C)
         });
     for (auto &x : th) x.join();
     const int wmax = *std::max_element(wsize.begin(), wsize.end());
-    if (wmax > 65535 || (size_t)kWindowWaves * mc * S * 8 + (size_t)wmax * 8 > kWindowLds) return 0;
+    if (wmax > 65535 || (size_t)kWindowWaves * prod_stride(mc) * S * 8 + (size_t)wmax * 8 > kWindowLds) return 0;
     std::vector<int> hoff(nb + 1, 0), hall;
     for (int b = 0; b < nb; b++) {
         hoff[b + 1] = hoff[b] + (int)lists[b].size();

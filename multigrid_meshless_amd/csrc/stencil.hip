@@ -88,10 +88,11 @@ template <int S, int NIT, typename TW, typename TV>
 __device__ inline void stage_products(const EllT<TW> &A, size_t base, int ri, int jo, const TV *x, TV *P)
 {
     constexpr int LPR = 64 / S;
-    const int m = A.m;
+    const int m = A.m, ms = prod_stride(m);
     if (A.sorted) {
         // Gather-sorted slab: lane l takes the slice's entries l, l+64, ...
-        // in column order and writes each product to its plain slot.
+        // in column order and writes each product to the place of its plain
+        // slot j*S + ri: row ri, entry j.
         const int n = S * m;
         const size_t sb = base - ri;
         const int lane = jo * S + ri;
@@ -102,30 +103,31 @@ __device__ inline void stage_products(const EllT<TW> &A, size_t base, int ri, in
         if (A.stream) {
 #pragma unroll 4
             for (int q = lane; q < n; q += 64) {
-                const unsigned v = ldnt(cp + q);
-                P[v >> sh] = wv(ldnt(wp + q)) * x[v & mask];
+                const unsigned v = ldnt(cp + q), slot = v >> sh;
+                P[(slot % S) * ms + slot / S] = wv(ldnt(wp + q)) * x[v & mask];
             }
         } else {
 #pragma unroll 4
             for (int q = lane; q < n; q += 64) {
-                const unsigned v = cp[q];
-                P[v >> sh] = wv(wp[q]) * x[v & mask];
+                const unsigned v = cp[q], slot = v >> sh;
+                P[(slot % S) * ms + slot / S] = wv(wp[q]) * x[v & mask];
             }
         }
         return;
     }
-    // entry j = t*LPR + jo sits at base + j*S = lane base + t*64
+    // entry j = t*LPR + jo sits at base + j*S = lane base + t*64; its product
+    // at row ri, place j
     const TW *wp = A.w + base + (size_t)jo * S;
     const int *cp = A.col + base + (size_t)jo * S;
-    TV *pp = P + jo * S + ri;
+    TV *pp = P + ri * ms + jo;
     const int nt = (m - jo + LPR - 1) / LPR;   // entries of this lane
     if constexpr (NIT == 0) {
         if (A.stream) {
 #pragma unroll 4
-            for (int t = 0; t < nt; t++) pp[t * 64] = wv(ldnt(wp + t * 64)) * x[ldnt(cp + t * 64)];
+            for (int t = 0; t < nt; t++) pp[t * LPR] = wv(ldnt(wp + t * 64)) * x[ldnt(cp + t * 64)];
         } else {
 #pragma unroll 4
-            for (int t = 0; t < nt; t++) pp[t * 64] = wv(wp[t * 64]) * x[cp[t * 64]];
+            for (int t = 0; t < nt; t++) pp[t * LPR] = wv(wp[t * 64]) * x[cp[t * 64]];
         }
     } else {
         for (int t0 = 0; t0 < nt; t0 += NIT) {
@@ -143,7 +145,7 @@ __device__ inline void stage_products(const EllT<TW> &A, size_t base, int ri, in
                 if (t0 + t < nt) xq[t] = x[cq[t]];
 #pragma unroll
             for (int t = 0; t < NIT; t++)
-                if (t0 + t < nt) pp[(t0 + t) * 64] = wv(wq[t]) * xq[t];
+                if (t0 + t < nt) pp[(t0 + t) * LPR] = wv(wq[t]) * xq[t];
         }
     }
 }
@@ -153,33 +155,52 @@ __device__ inline void stage_products(const EllT<TW> &A, size_t base, int ri, in
 // (one LDS wait per eight dependent adds, no selects on the chain), the tail
 // one by one.
 template <bool SUBTRACT, typename TV>
-__device__ inline TV ordered_sum(const TV *P, int S, int ri, int j0, int m, TV acc)
+__device__ inline TV ordered_sum(const TV *P, int ri, int j0, int m, TV acc)
 {
-    const TV *q = P + ri;
+    // row ri's products, added (or subtracted) one after another in slot order
+    // from j0; read 16 bytes at a time where aligned, eight slots per wait
+    constexpr int NV = 16 / (int)sizeof(TV);
+    typedef TV vec __attribute__((ext_vector_type(NV)));
+    const TV *q = P + (size_t)ri * prod_stride(m);
     int j = j0;
+    for (; j < m && (j & (NV - 1)); j++) acc = SUBTRACT ? acc - q[j] : acc + q[j];
     for (; j + 8 <= m; j += 8) {
         TV v[8];
 #pragma unroll
-        for (int t = 0; t < 8; t++) v[t] = q[(j + t) * S];
+        for (int u = 0; u < 8 / NV; u++) {
+            const vec t = *reinterpret_cast<const vec *>(q + j + u * NV);
+#pragma unroll
+            for (int e = 0; e < NV; e++) v[u * NV + e] = t[e];
+        }
 #pragma unroll
         for (int t = 0; t < 8; t++) acc = SUBTRACT ? acc - v[t] : acc + v[t];
     }
-    for (; j < m; j++) acc = SUBTRACT ? acc - q[j * S] : acc + q[j * S];
+    for (; j < m; j++) acc = SUBTRACT ? acc - q[j] : acc + q[j];
     return acc;
 }
 
 // The sums ordered_sum<false> forms from +0 over slots 0..m-1 (*a) and over
 // slots 1..m-1 (*b), in one walk over the slots.
 template <typename TV>
-__device__ inline void ordered_sum2(const TV *P, int S, int ri, int m, TV *a, TV *b)
+__device__ inline void ordered_sum2(const TV *P, int ri, int m, TV *a, TV *b)
 {
-    const TV *q = P + ri;
+    constexpr int NV = 16 / (int)sizeof(TV);
+    typedef TV vec __attribute__((ext_vector_type(NV)));
+    const TV *q = P + (size_t)ri * prod_stride(m);
     TV sa = TV(0) + q[0], sb = TV(0);
     int j = 1;
+    for (; j < m && (j & (NV - 1)); j++) {
+        sa = sa + q[j];
+        sb = sb + q[j];
+    }
     for (; j + 8 <= m; j += 8) {
         TV v[8];
 #pragma unroll
-        for (int t = 0; t < 8; t++) v[t] = q[(j + t) * S];
+        for (int u = 0; u < 8 / NV; u++) {
+            const vec t = *reinterpret_cast<const vec *>(q + j + u * NV);
+#pragma unroll
+            for (int e = 0; e < NV; e++) v[u * NV + e] = t[e];
+        }
 #pragma unroll
         for (int t = 0; t < 8; t++) {
             sa = sa + v[t];
@@ -187,8 +208,8 @@ __device__ inline void ordered_sum2(const TV *P, int S, int ri, int m, TV *a, TV
         }
     }
     for (; j < m; j++) {
-        sa = sa + q[j * S];
-        sb = sb + q[j * S];
+        sa = sa + q[j];
+        sb = sb + q[j];
     }
     *a = sa;
     *b = sb;
@@ -230,11 +251,11 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
     if (jo == 0 && act) {
         switch (mode) {
         case M_SPMV:
-            y[r] = ordered_sum<false>(P, S, ri, 0, m, TV(0));
+            y[r] = ordered_sum<false>(P, ri, 0, m, TV(0));
             break;
         case M_SOR:
         case M_JACOBI: {
-            const TV sum = ordered_sum<false>(P, S, ri, 1, m, TV(0));
+            const TV sum = ordered_sum<false>(P, ri, 1, m, TV(0));
             y[r] = w * ((sv - sum) / dg) + c * yv;
             break;
         }
@@ -244,13 +265,13 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
                 if (mode == M_RES0) y[r] = 0;
                 break;
             }
-            y[r] = sv - ordered_sum<false>(P, S, ri, 0, m, TV(0));
+            y[r] = sv - ordered_sum<false>(P, ri, 0, m, TV(0));
             break;
         }
         case M_RES1:
         case M_RES2: {
             if (mode == M_RES1 && bnd) { y[r] = 0; break; }
-            y[r] = ordered_sum<true>(P, S, ri, 0, m, sv);
+            y[r] = ordered_sum<true>(P, ri, 0, m, sv);
             break;
         }
         case M_RESTRICT: {
@@ -258,12 +279,12 @@ __device__ inline void slice_rows(const EllT<TW> &A, long slice, int nrows, int 
                 if (!keep_boundary) y[r] = 0;
                 break;
             }
-            y[r] = ordered_sum<false>(P, S, ri, 0, m, TV(0));
+            y[r] = ordered_sum<false>(P, ri, 0, m, TV(0));
             break;
         }
         case M_PROLONG:
             if (bnd) break;
-            y[r] = yv + ordered_sum<false>(P, S, ri, 0, m, TV(0));
+            y[r] = yv + ordered_sum<false>(P, ri, 0, m, TV(0));
             break;
         }
     }
@@ -286,13 +307,13 @@ rows_kernel(EllT<TW> A, const int *__restrict__ slices, int first, int nslices, 
     // enough for this to show: 2.7% over a whole solve (measured).
     asm volatile("" ::"s"(A.col), "s"(A.w), "s"(x), "s"(y));
     extern __shared__ unsigned char lds_raw[];
-    TV *prod = reinterpret_cast<TV *>(lds_raw);   // [W][m][S]
+    TV *prod = reinterpret_cast<TV *>(lds_raw);   // [W][S][prod_stride(m)]
     const int wave = threadIdx.x >> 6;
     const long k = xcd_block(blockIdx.x, nblocks) * W + wave;   // nblocks = gridDim.x
     if (k >= nslices) return;
     const long slice = slices ? (long)slices[k] : (long)first + k;
     slice_rows<S, TW, TV>(A, slice, nrows, mode, btag, src, x, y, w, c, keep_boundary,
-                          prod + (size_t)wave * A.m * S);
+                          prod + (size_t)wave * prod_stride(A.m) * S);
 }
 
 // One slice of a full pass in the LDS-window form: as slice_rows on the
@@ -323,21 +344,21 @@ __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice,
         const size_t base = (size_t)slice * S * m + (size_t)jo * S + ri;
         const TW *w = wp + base;
         const uint16_t *c = A.win_col + base;
-        TV *pp = P + jo * S + ri;
+        TV *pp = P + ri * prod_stride(m) + jo;
         const int nt = (m - jo + LPR - 1) / LPR;
         if (A.stream) {
 #pragma unroll 4
-            for (int t = 0; t < nt; t++) pp[t * 64] = wv(ldnt(w + t * 64)) * win[ldnt(c + t * 64)];
+            for (int t = 0; t < nt; t++) pp[t * LPR] = wv(ldnt(w + t * 64)) * win[ldnt(c + t * 64)];
         } else {
 #pragma unroll 4
-            for (int t = 0; t < nt; t++) pp[t * 64] = wv(w[t * 64]) * win[c[t * 64]];
+            for (int t = 0; t < nt; t++) pp[t * LPR] = wv(w[t * 64]) * win[c[t * 64]];
         }
     }
     wave_sync();
     if (comb) {
         switch (mode) {
         case M_SPMV:
-            y[r] = ordered_sum<false>(P, S, ri, 0, m, TV(0));
+            y[r] = ordered_sum<false>(P, ri, 0, m, TV(0));
             break;
         case M_RES0:
         case M_RES3:
@@ -345,7 +366,7 @@ __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice,
                 if (mode == M_RES0) y[r] = 0;
                 break;
             }
-            y[r] = sv - ordered_sum<false>(P, S, ri, 0, m, TV(0));
+            y[r] = sv - ordered_sum<false>(P, ri, 0, m, TV(0));
             break;
         case M_RES1:
         case M_RES2:
@@ -353,23 +374,23 @@ __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice,
                 y[r] = 0;
                 break;
             }
-            y[r] = ordered_sum<true>(P, S, ri, 0, m, sv);
+            y[r] = ordered_sum<true>(P, ri, 0, m, sv);
             break;
         case M_JACOBI: {
-            const TV sum = ordered_sum<false>(P, S, ri, 1, m, TV(0));
+            const TV sum = ordered_sum<false>(P, ri, 1, m, TV(0));
             y[r] = wf * ((sv - sum) / dg) + cf * yv;
             break;
         }
         case M_RESJAC: {
             TV all, off;
-            ordered_sum2(P, S, ri, m, &all, &off);
+            ordered_sum2(P, ri, m, &all, &off);
             y[r] = wf * ((sv - off) / dg) + cf * yv;
             y2[r] = bnd ? TV(0) : sv - all;
             break;
         }
         case M_PROLONG:
             if (bnd) break;
-            y[r] = yv + ordered_sum<false>(P, S, ri, 0, m, TV(0));
+            y[r] = yv + ordered_sum<false>(P, ri, 0, m, TV(0));
             break;
         }
     }
@@ -392,8 +413,8 @@ window_rows_kernel(EllT<TW> A, int mode, const uint8_t *__restrict__ btag, const
     constexpr int W = kWindowWaves;
     const int m = A.m;
     const int wave = threadIdx.x >> 6;
-    TV *P = reinterpret_cast<TV *>(lds_raw) + (size_t)wave * m * S;
-    TV *win = reinterpret_cast<TV *>(lds_raw) + (size_t)W * m * S;
+    TV *P = reinterpret_cast<TV *>(lds_raw) + (size_t)wave * prod_stride(m) * S;
+    TV *win = reinterpret_cast<TV *>(lds_raw) + (size_t)W * prod_stride(m) * S;
     const int b = (int)xcd_block(blockIdx.x, gridDim.x);
     const int r0 = A.win_row0[b], r1 = A.win_row0[b + 1];
     const int h0 = A.win_hoff[b], nh = A.win_hoff[b + 1] - h0, nown = A.win_self ? r1 - r0 : 0;
@@ -871,7 +892,7 @@ int launch_rows_w(EllT<TW> A, const int *slices, int first, int nslices, int nro
                   const uint8_t *btag, const TV *src, const TV *x, TV *y, TV w, TV c, int keep, hipStream_t s)
 {
     const int blocks = (nslices + W - 1) / W;
-    const size_t shm = (size_t)W * A.m * A.S * sizeof(TV);
+    const size_t shm = (size_t)W * prod_stride(A.m) * A.S * sizeof(TV);
     const dim3 grid(blocks), block(64 * W);
     if (A.S == 4)
         hipLaunchKernelGGL((rows_kernel<4, TW, TV, W>), grid, block, shm, s, A, slices, first, nslices, nrows,
@@ -896,7 +917,7 @@ int launch_rows_w(EllT<TW> A, const int *slices, int first, int nslices, int nro
 template <typename TW, typename TV>
 size_t window_lds(const EllT<TW> &A)
 {
-    return ((size_t)kWindowWaves * A.m * A.S + (size_t)A.win_max) * sizeof(TV);
+    return ((size_t)kWindowWaves * prod_stride(A.m) * A.S + (size_t)A.win_max) * sizeof(TV);
 }
 template <typename TW, typename TV>
 int launch_window(EllT<TW> A, int mode, const uint8_t *btag, const TV *src, const TV *x, TV *y, TV w, TV c,
