@@ -457,10 +457,11 @@ __device__ inline double stencil_sigma(double s2, int m)
     return sqrt(s2 / m);
 }
 
-// Three waves per SIMD: the 24 residual gathers a lane keeps in flight need
-// the registers (a cap of four spills).
-template <typename TV>
-__global__ void __launch_bounds__(BLK, 3)
+// Four waves per SIMD (114 registers with the 24 residual gathers a lane
+// keeps in flight).  S: the slab's slice height (entry j of row r at the
+// row's base + j*S, so a lane's columns sit at compile-time offsets).
+template <int S, typename TV>
+__global__ void __launch_bounds__(BLK, 4)
 correct_kernel(int Nd, Ell A, const uint8_t *__restrict__ btag, const float *__restrict__ G,
                const NetLds *__restrict__ img, float b3, const TV *__restrict__ res, TV *__restrict__ p, int tpw)
 {
@@ -482,11 +483,13 @@ correct_kernel(int Nd, Ell A, const uint8_t *__restrict__ btag, const float *__r
         // node's inputs, their values reach no other node's correction.
         const bool inr = r < Nd;
         const uint8_t tag = inr ? btag[r] : (uint8_t)ROW_PAD;
+        const size_t base = (size_t)(r / S) * S * m + (size_t)(r % S);
+        const int *cp = A.col + base + (size_t)g * 8 * S;
         TV pold = 0;
         double diag = 1;
         if (g == 0 && inr) {
             pold = p[r];
-            diag = A.w[ell_at(m, A.S, r, 0)];
+            diag = A.w[base];
         }
         // this lane's residual columns of node li: j = ks*32 + g*8 + e
         double vq[MF_COLS];
@@ -497,7 +500,7 @@ correct_kernel(int Nd, Ell A, const uint8_t *__restrict__ btag, const float *__r
             for (int e = 0; e < 8; e++) {
                 const int j = ks * 32 + g * 8 + e;
                 double v = 0;
-                if (inr && j < m) v = (double)res[A.col[ell_at(m, A.S, r, j)]];
+                if (inr && j < m) v = (double)res[cp[(ks * 32 + e) * S]];
                 vq[ks * 8 + e] = v;
                 s2 += v * v;
             }
@@ -761,7 +764,17 @@ int mlp_correct(int Nd, Ell A, const uint8_t *btag, const float *G, const void *
     const long waves = (tiles + tpw - 1) / tpw;
     const int blocks = (int)((waves + MF_WAVES - 1) / MF_WAVES);
     if (blocks > 0) {
-        hipLaunchKernelGGL(correct_kernel<TV>, dim3(blocks), dim3(BLK), 0, s, Nd, A, btag, G, img, b3, res, p, tpw);
+        if (A.S == 4)
+            hipLaunchKernelGGL((correct_kernel<4, TV>), dim3(blocks), dim3(BLK), 0, s, Nd, A, btag, G, img, b3, res, p,
+                               tpw);
+        else if (A.S == 8)
+            hipLaunchKernelGGL((correct_kernel<8, TV>), dim3(blocks), dim3(BLK), 0, s, Nd, A, btag, G, img, b3, res, p,
+                               tpw);
+        else if (A.S == 16)
+            hipLaunchKernelGGL((correct_kernel<16, TV>), dim3(blocks), dim3(BLK), 0, s, Nd, A, btag, G, img, b3, res,
+                               p, tpw);
+        else
+            MGM_FAIL(-1, "learned smoother: unsupported slice height");
         HIP_TRY(hipGetLastError());
     }
     return 0;
