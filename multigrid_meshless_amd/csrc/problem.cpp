@@ -946,21 +946,25 @@ bool prol_window_on()
     return window_on() && !(e && e[0] == '0');
 }
 
-// The prolongation windows of fine level L (Morton blocks with windows) from
-// coarse level C: per block the sorted, distinct coarse device rows that its
-// prolongation rows reference (row d of L reads C.iperm[C.cloud[L.pp[i] * mc + j]],
-// i = L.perm[d], as the transfer slab composes them), and the plain slab's
-// columns as 16-bit places in that list (padding rows: place 0).
-static int level_prol_window(Level &L, const Level &C)
+// List windows of a transfer operator whose rows are level L's device rows
+// (Morton blocks with windows) and whose columns index another level: per
+// block the sorted, distinct columns its rows reference (col(i, j) for the
+// block's real rows, i = L.perm[d]) and the plain slab's columns (width mt,
+// slice height St) as 16-bit places in that list (padding rows: place 0).
+// Returns the largest list, 0 when the windows do not fit or blocks are not
+// whole slices of St.
+template <typename ColFn>
+static int level_list_window(const Level &L, int mt, int St, ColFn col, DevBuf<uint16_t> &wcol_d,
+                             DevBuf<int> &hoff_d, DevBuf<int> &list_d, int *wmax_out)
 {
-    L.pwin_max = 0;
-    if (!L.win_blocks || !prol_window_on()) return 0;
-    const int nb = L.win_blocks, mc = L.mp, S = L.Sp;
-    // the window pass walks slices of S rows from block start to block end
+    *wmax_out = 0;
+    if (!L.win_blocks || !window_on()) return 0;
+    const int nb = L.win_blocks;
+    // the window pass walks slices of St rows from block start to block end
     for (int b = 0; b <= nb; b++)
-        if (L.blk_row0[b] % S) return 0;
+        if (L.blk_row0[b] % St) return 0;
     std::vector<std::vector<int>> lists(nb);
-    std::vector<uint16_t> wcol(ell_size(L.Nd, mc, S), 0);
+    std::vector<uint16_t> wcol(ell_size(L.Nd, mt, St), 0);
     std::vector<int> wsize(nb, 0);
     const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::thread> th;
@@ -972,40 +976,58 @@ static int level_prol_window(Level &L, const Level &C)
                 for (int d = r0; d < r1; d++) {
                     const int i = L.perm[d];
                     if (i < 0) continue;
-                    const size_t k = (size_t)L.pp[i] * mc;
-                    for (int j = 0; j < mc; j++) h.push_back(C.iperm[C.cloud[k + j]]);
+                    for (int j = 0; j < mt; j++) h.push_back(col(i, j));
                 }
                 std::sort(h.begin(), h.end());
                 h.erase(std::unique(h.begin(), h.end()), h.end());
-                if (h.size() > 65535) { wsize[b] = (int)h.size(); continue; }
+                wsize[b] = (int)h.size();
+                if (h.size() > 65535) continue;
                 for (int d = r0; d < r1; d++) {
                     const int i = L.perm[d];
                     if (i < 0) continue;
-                    const size_t k = (size_t)L.pp[i] * mc;
-                    for (int j = 0; j < mc; j++) {
-                        const int c = C.iperm[C.cloud[k + j]];
-                        wcol[ell_at(mc, S, d, j)] =
-                            (uint16_t)(std::lower_bound(h.begin(), h.end(), c) - h.begin());
-                    }
+                    for (int j = 0; j < mt; j++)
+                        wcol[ell_at(mt, St, d, j)] =
+                            (uint16_t)(std::lower_bound(h.begin(), h.end(), col(i, j)) - h.begin());
                 }
-                wsize[b] = (int)h.size();
             }
         });
     for (auto &x : th) x.join();
     const int wmax = *std::max_element(wsize.begin(), wsize.end());
-    if (wmax > 65535 || (size_t)kWindowWaves * prod_stride(mc) * S * 8 + (size_t)wmax * 8 > kWindowLds) return 0;
+    if (wmax > 65535 || (size_t)kWindowWaves * prod_stride(mt) * St * 8 + (size_t)wmax * 8 > kWindowLds) return 0;
     std::vector<int> hoff(nb + 1, 0), hall;
     for (int b = 0; b < nb; b++) {
         hoff[b + 1] = hoff[b] + (int)lists[b].size();
         hall.insert(hall.end(), lists[b].begin(), lists[b].end());
     }
     if (hall.empty()) hall.push_back(0);
-    HIP_TRY(L.pwin_col.upload(wcol, g_stream));
-    HIP_TRY(L.pwin_hoff.upload(hoff, g_stream));
-    HIP_TRY(L.pwin_halo.upload(hall, g_stream));
+    HIP_TRY(wcol_d.upload(wcol, g_stream));
+    HIP_TRY(hoff_d.upload(hoff, g_stream));
+    HIP_TRY(list_d.upload(hall, g_stream));
     HIP_TRY(hipStreamSynchronize(g_stream));   // host tables die here
-    L.pwin_max = std::max(wmax, 1);
+    *wmax_out = std::max(wmax, 1);
     return 0;
+}
+
+// The prolongation into fine level L from coarse level C: row d of L reads
+// C.iperm[C.cloud[L.pp[i] * mc + j]] (as the transfer slab composes it).
+static int level_prol_window(Level &L, const Level &C)
+{
+    L.pwin_max = 0;
+    if (!prol_window_on()) return 0;
+    const int mc = L.mp;
+    return level_list_window(L, mc, L.Sp, [&](int i, int j) { return C.iperm[C.cloud[(size_t)L.pp[i] * mc + j]]; },
+                             L.pwin_col, L.pwin_hoff, L.pwin_halo, &L.pwin_max);
+}
+
+// The restriction into coarse level C from fine level F: row d of C reads
+// F.iperm[F.cloud[C.rp[i] * mf + j]].
+static int level_restr_window(Level &C, const Level &F)
+{
+    C.rwin_max = 0;
+    if (!prol_window_on()) return 0;
+    const int mf = C.mr;
+    return level_list_window(C, mf, C.Sr, [&](int i, int j) { return F.iperm[F.cloud[(size_t)C.rp[i] * mf + j]]; },
+                             C.rwin_col, C.rwin_hoff, C.rwin_halo, &C.rwin_max);
 }
 
 // (Re)build every slab of level l: stencil columns, the five operators and
@@ -1049,6 +1071,10 @@ static int level_slabs(mgm_problem *pb, int l)
         rc = launch_transfer_slab(L.N, L.mr, L.Sr, L.Rrows.p, L.d_iperm.p, L.d_rp.p, F.rcloud.p, F.d_iperm.p,
                                   L.rw.p, L.rcol.p, g_stream);
         if (rc) return rc;
+        if (!L.rwin_tried) {
+            L.rwin_tried = true;
+            if ((rc = level_restr_window(L, F))) return rc;
+        }
     }
     if (l + 1 < pb->L) {
         Level &C = pb->lv[l + 1];

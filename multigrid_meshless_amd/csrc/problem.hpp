@@ -13,7 +13,8 @@ struct mgm_points {
 
 namespace mgm {
 
-// MGM_WINDOW_PROLONG=0: prolongations gather from memory on blocked levels too.
+// MGM_WINDOW_PROLONG=0: FP64 prolongations and restrictions gather from memory
+// on blocked levels too.
 bool prol_window_on();
 
 enum { OP_DX = 0, OP_DY = 1, OP_DZ = 2, OP_LAP = 3, OP_LAPP = 4, NUM_OPS = 5 };
@@ -99,6 +100,11 @@ struct Level {
     DevBuf<int> pwin_hoff, pwin_halo;
     int pwin_max = 0;                          // 0: no prolongation windows
     bool pwin_tried = false;
+    // the restriction into this level (its rows read the finer level) likewise
+    DevBuf<uint16_t> rwin_col;
+    DevBuf<int> rwin_hoff, rwin_halo;
+    int rwin_max = 0;
+    bool rwin_tried = false;
     template <typename T>
     EllT<T> with_prol_window(EllT<T> A) const
     {
@@ -136,7 +142,15 @@ struct Level {
         *list = col_contig ? nullptr : d_col_slices.p + col_soff[k];
         *first = (col_contig && *n) ? col_slices[col_soff[k]] : 0;
     }
-    Ell restr() const { return Ell{rcol.p, rw.p, mr, Sr, stream_slab(Nd, mr, 8)}; }
+    Ell restr() const
+    {
+        Ell A{rcol.p, rw.p, mr, Sr, stream_slab(Nd, mr, 8)};
+        if (rwin_max && prol_window_on()) {
+            A.win_col = rwin_col.p; A.win_row0 = win_row0.p; A.win_hoff = rwin_hoff.p; A.win_halo = rwin_halo.p;
+            A.win_blocks = win_blocks; A.win_max = rwin_max; A.win_self = 0;
+        }
+        return A;
+    }
     Ell prol() const
     {
         if (gsp)

@@ -334,7 +334,7 @@ __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice,
     const bool skip = bnd && skips_boundary(mode);
     const bool comb = jo == 0 && act;
     TV sv = 0, yv = 0, dg = 1;
-    if (comb && mode != M_SPMV && mode != M_PROLONG && !skip) sv = src[r];
+    if (comb && mode != M_SPMV && mode != M_PROLONG && mode != M_RESTRICT && !skip) sv = src[r];
     if (comb && mode == M_PROLONG && !bnd) yv = y[r];
     if (comb && (mode == M_JACOBI || mode == M_RESJAC)) {
         yv = win[r - r0];   // the row's own previous value: own rows lead the window
@@ -391,6 +391,13 @@ __device__ inline void window_slice(const EllT<TW> &A, const TW *wp, long slice,
         case M_PROLONG:
             if (bnd) break;
             y[r] = yv + ordered_sum<false>(P, ri, 0, m, TV(0));
+            break;
+        case M_RESTRICT:   // wf != 0: keep boundary rows (launch_rows passes the flag there)
+            if (bnd) {
+                if (wf == TV(0)) y[r] = 0;
+                break;
+            }
+            y[r] = ordered_sum<false>(P, ri, 0, m, TV(0));
             break;
         }
     }
@@ -945,9 +952,9 @@ int launch_rows(EllT<TW> A, const int *slices, int first, int nslices, int nrows
     // full SpMV and residual passes over a level with LDS windows
     if (A.win_blocks && btag && !slices && first == 0 && nslices < 0 &&
         (mode == M_SPMV || mode == M_RES0 || mode == M_RES1 || mode == M_RES2 || mode == M_RES3 ||
-         mode == M_JACOBI || mode == M_PROLONG) &&
+         mode == M_JACOBI || mode == M_PROLONG || mode == M_RESTRICT) &&
         window_lds<TW, TV>(A) <= kSmallLdsMax)
-        return launch_window(A, mode, btag, src, x, y, w, c, s);
+        return launch_window(A, mode, btag, src, x, y, mode == M_RESTRICT ? TV(keep ? 1 : 0) : w, c, s);
     if (nslices < 0) nslices = (nrows + A.S - 1) / A.S;  // full pass
     if (nslices <= 0 || nrows <= 0) return 0;
     if (mode == M_SOR)
